@@ -1,0 +1,232 @@
+/*
+ * rbtp.h — C-ABI of the MI355X (gfx950) D3Q27 thermal lattice-Boltzmann solver with
+ * finite-size thermal spheres.
+ *
+ * The reference solver (Summer536/RB-ThermoParticles) has no plugin layer: its hot path is a set
+ * of free host functions over global state, called from main() in a fixed order
+ * (main.cu:32-86).  This header is the drop-in boundary that replaces them.  Every entry point
+ * takes plain pointers and sizes, returns 0 on success and a negative RBTP_E* code on failure
+ * (the message is kept in rbtp_last_error()), and never calls exit().  Device memory is owned by
+ * the rbtp_sim handle; host buffers passed in are owned by the caller.  A handle is not
+ * thread-safe; distinct handles are independent and each runs on its own HIP stream.
+ *
+ * Array layouts follow the reference: a scalar field is N = lx*ly*lz doubles indexed
+ * idx = iz*lx*ly + iy*lx + ix; a population field is 27*N doubles indexed ip*N + idx with the
+ * D3Q27 ordering of initial.cu:276-279; particle arrays are P doubles indexed by particle id.
+ */
+#ifndef RBTP_H
+#define RBTP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RBTP_ABI_VERSION 1
+
+/* Status codes. */
+#define RBTP_OK 0
+#define RBTP_E_ARG (-1)      /* invalid argument or case description */
+#define RBTP_E_HIP (-2)      /* HIP runtime error (message names the call) */
+#define RBTP_E_NOMEM (-3)    /* device or host allocation failed */
+#define RBTP_E_IO (-4)       /* file missing, short or malformed */
+#define RBTP_E_STATE (-5)    /* call not valid in the current state */
+#define RBTP_E_CAPACITY (-6) /* a device-side capacity bound was exceeded */
+#define RBTP_E_PLACEMENT (-7)/* particle placement gave up (pinit.cu:126-131) */
+
+/* Storage/arithmetic forms of the lattice. */
+#define RBTP_PREC_FP64 0   /* FP64 storage and arithmetic, operation order of the reference */
+#define RBTP_PREC_FP32 1   /* FP32 storage and arithmetic */
+#define RBTP_PREC_FP16S 2  /* FP32 arithmetic, populations stored as FP16 (shifted, 1/w scaled) */
+#define RBTP_PREC_BF16S 3  /* FP32 arithmetic, populations stored as BF16 (shifted, 1/w scaled) */
+
+/*
+ * Case description: every macro of parameters.h:1-55 as a runtime field (same names, lower
+ * case), plus the extensions at the end.  rbtp_case_default() fills the values parameters.h
+ * ships with.  BLOCK_X/Y/Z (parameters.h:51-53) have no counterpart: launch shapes are chosen by
+ * the library.
+ */
+typedef struct rbtp_case {
+    /* parameters.h:5 and :8-20 */
+    int continuous_steps;
+    int lx, ly, lz;
+    int nend, ndiag, nflowout, nnuout, nout_p;
+    /* parameters.h:23-30 */
+    double rayleigh, prandtl, t_hot, t_cold, t_ref, beta, visc, rho0;
+    /* parameters.h:33-48 */
+    int activate_particles;
+    double particle_fraction, particle_diam_frac, cp_particle_ratio, cp_fluid;
+    double rho_particle_ratio, repforce_scale, particle_init_temp;
+    int particle_rand_seed;
+    double rep_min_gap, rep_min_gap_w, rep_stf0, rep_stf1, rep_stf0_w, rep_stf1_w;
+    /* Extensions. */
+    int npart_override; /* >=0: use this particle count instead of the fraction rule */
+    int precision;      /* RBTP_PREC_* */
+    int reserved[6];    /* zero */
+} rbtp_case;
+
+/* Constants derived from the case (globals.cu:18-33, globals.cu:126-169). */
+typedef struct rbtp_derived {
+    double grav0, tau, u0, diff, tauc, tauci;
+    double particle_radius, particle_mass, particle_inertia, particle_heat_capacity;
+    int npart;
+    int reserved;
+} rbtp_derived;
+
+typedef struct rbtp_sim rbtp_sim;
+
+/* Last error message of the calling thread ("" if none). */
+const char *rbtp_last_error(void);
+int rbtp_abi_version(void);
+
+/* parameters.h defaults. */
+int rbtp_case_default(rbtp_case *c);
+/* globals.cu:18-33 (grav0, tau, u0, diff, tauc, tauci), globals.cu:126-148 (particle constants)
+ * and update_particle_count (globals.cu:150-169). */
+int rbtp_case_derive(const rbtp_case *c, rbtp_derived *out);
+
+/* Allocation part of initialize() (initial.cu:240-263) and allocate_particle_arrays
+ * (pinit.cu:8-95).  Fails with RBTP_E_ARG for an invalid case. */
+int rbtp_create(const rbtp_case *c, rbtp_sim **out);
+void rbtp_destroy(rbtp_sim *s);
+int rbtp_get_case(const rbtp_sim *s, rbtp_case *c, rbtp_derived *d);
+
+/*
+ * Initial state: init_f/init_g (initial.cu:335-397, called at initial.cu:318-327), particle
+ * placement (pinit.cu:97-144, glibc srand/rand) and the first build_links (initial.cu:330).
+ *   temperature: N values of the initial temperature field phi, or NULL for the reference's
+ *                uniform t_ref.  g is set from phi by init_g's formula with rho = rho0, u = 0.
+ *   ppos:        3*P values (x[P], y[P], z[P]) or NULL for the reference's rejection placement.
+ *   ptemp:       P initial particle temperatures, or NULL for particle_init_temp.
+ */
+int rbtp_init(rbtp_sim *s, const double *temperature, const double *ppos, const double *ptemp);
+
+/* Host helper: the initial temperature used by the benchmark cases, T(ix) linear from t_hot at
+ * x=0 to t_cold at x=lx at cell centres, plus amp * U[-1,1) from splitmix64(seed) in idx order. */
+int rbtp_linear_temperature(const rbtp_case *c, uint64_t seed, double amp, double *out_n);
+
+/*
+ * Advance nsteps time steps of main.cu:32-86 (macrovar -> forcing -> collision_BGK -> streaming
+ * -> bounce_back_particles -> collision_BGK_scalar -> streaming_scalar ->
+ * bounce_back_thermal_particles -> prepare_particle_forces_step -> apply_repulsive_forces ->
+ * compute_particle_forces -> compute_particle_heat -> update_particles -> build_links ->
+ * refill_nodes).  Work is enqueued on the handle's stream; the call returns after the stream
+ * has drained and device-side error flags were checked.
+ *   flags: RBTP_STEP_MACRO_LAST makes the last step leave rho/ux/uy/uz/phi as macrovar (and
+ *          refill) left them, as the reference's output functions read them.
+ */
+#define RBTP_STEP_MACRO_LAST 1u
+int rbtp_step(rbtp_sim *s, int nsteps, unsigned flags);
+/* Same, without waiting for the stream (errors are reported by the next synchronising call). */
+int rbtp_step_async(rbtp_sim *s, int nsteps, unsigned flags);
+int rbtp_synchronize(rbtp_sim *s);
+/* Number of completed time steps since rbtp_init (restart_step counts). */
+int rbtp_current_step(const rbtp_sim *s, int *step);
+
+/*
+ * Parity accessors (reference semantics at a step boundary).  Each copies to caller-owned host
+ * memory.  Fields: which = RBTP_FIELD_*.  Populations: which = RBTP_POP_* (27*N doubles).
+ */
+#define RBTP_FIELD_RHO 0
+#define RBTP_FIELD_UX 1
+#define RBTP_FIELD_UY 2
+#define RBTP_FIELD_UZ 3
+#define RBTP_FIELD_PHI 4
+#define RBTP_FIELD_FX 5
+#define RBTP_FIELD_FY 6
+#define RBTP_FIELD_FZ 7
+int rbtp_get_field(rbtp_sim *s, int which, double *out_n);
+#define RBTP_POP_F 0          /* f after streaming, IBB and refill (d_f) */
+#define RBTP_POP_G 1          /* g after streaming, IBB and refill (d_g) */
+#define RBTP_POP_F_COLLIDE 2  /* post-collision f of the last step (d_f_collide) */
+#define RBTP_POP_G_COLLIDE 3  /* post-collision g of the last step (d_g_collide) */
+int rbtp_get_populations(rbtp_sim *s, int which, double *out_27n);
+/* Overwrite f and g (post-stream state, as restart loads them: initial.cu:7-30). */
+int rbtp_set_populations(rbtp_sim *s, const double *f_27n, const double *g_27n);
+
+/* ibnode / ibnode_owner after the last build_links (pinit.cu:146-179, links.cu:143-191). */
+int rbtp_get_flags(rbtp_sim *s, int32_t *ibnode_n, int32_t *owner_n);
+
+/* Particle state (P doubles each).  which = RBTP_P_*. */
+#define RBTP_P_POS_X 0
+#define RBTP_P_POS_Y 1
+#define RBTP_P_POS_Z 2
+#define RBTP_P_VEL_X 3
+#define RBTP_P_VEL_Y 4
+#define RBTP_P_VEL_Z 5
+#define RBTP_P_OMEGA_X 6
+#define RBTP_P_OMEGA_Y 7
+#define RBTP_P_OMEGA_Z 8
+#define RBTP_P_THETA_X 9
+#define RBTP_P_THETA_Y 10
+#define RBTP_P_THETA_Z 11
+#define RBTP_P_FORCE_X 12   /* d_particle_force_accum_* */
+#define RBTP_P_FORCE_Y 13
+#define RBTP_P_FORCE_Z 14
+#define RBTP_P_TORQUE_X 15  /* d_particle_torque_accum_* */
+#define RBTP_P_TORQUE_Y 16
+#define RBTP_P_TORQUE_Z 17
+#define RBTP_P_FORCE_PREV_X 18
+#define RBTP_P_FORCE_PREV_Y 19
+#define RBTP_P_FORCE_PREV_Z 20
+#define RBTP_P_FORCE_PREV2_X 21
+#define RBTP_P_FORCE_PREV2_Y 22
+#define RBTP_P_FORCE_PREV2_Z 23
+#define RBTP_P_TORQUE_PREV_X 24
+#define RBTP_P_TORQUE_PREV_Y 25
+#define RBTP_P_TORQUE_PREV_Z 26
+#define RBTP_P_TORQUE_PREV2_X 27
+#define RBTP_P_TORQUE_PREV2_Y 28
+#define RBTP_P_TORQUE_PREV2_Z 29
+#define RBTP_P_TEMP 30
+#define RBTP_P_HEAT 31
+#define RBTP_P_HEAT_PREV 32
+#define RBTP_P_COUNT 33
+int rbtp_get_particles(rbtp_sim *s, int which, double *out_p);
+int rbtp_set_particles(rbtp_sim *s, int which, const double *in_p);
+
+/*
+ * The IBB link list of the last build_links in the reference's record layout
+ * (particle.h:19-29: int cell_i, cell_j, cell_k, dir, pid; double q, rx, ry, rz; 56 bytes),
+ * sorted by (idx, dir) as link_fill_kernel emits it (links.cu:89-139).
+ * Pass out=NULL to query the count.
+ */
+typedef struct rbtp_link {
+    int32_t cell_i, cell_j, cell_k, dir, pid;
+    int32_t pad;
+    double q, rx, ry, rz;
+} rbtp_link;
+int rbtp_get_links(rbtp_sim *s, rbtp_link *out, int64_t capacity, int64_t *count);
+/* New-fluid nodes and their previous owners of the last refill_nodes (refill.cu:244-302),
+ * sorted by idx.  Pass NULL buffers to query the count. */
+int rbtp_get_new_fluid(rbtp_sim *s, int32_t *nodes, int32_t *pids, int64_t capacity,
+                       int64_t *count);
+
+/*
+ * Diagnostics computed on the device in FP64 (output_statis.cu).  They read the macroscopic
+ * fields left by the last step, so call them after rbtp_step(..., RBTP_STEP_MACRO_LAST).
+ *   rbtp_nu:    output_nu (output_statis.cu:111-223): out[7] = {step, volume_fraction, Nu_bulk,
+ *               Nu_top, Nu_bottom, Nu_bulk_fluid, Nu_bulk_part}.
+ *   rbtp_means: diag_flow (output_statis.cu:64-109) after overwrite_particle_nodes:
+ *               out[4] = {umean, vmean, wmean, tmean}.
+ *   rbtp_kinetic_energy: mean over all nodes of 0.5*(ux^2+uy^2+uz^2) (no reference counterpart).
+ */
+int rbtp_nu(rbtp_sim *s, double out[7]);
+int rbtp_means(rbtp_sim *s, double out[4]);
+int rbtp_kinetic_energy(rbtp_sim *s, double *out);
+
+/* Device time of the kernels enqueued by the last rbtp_step* call, from HIP events on the
+ * handle's stream (milliseconds). */
+int rbtp_last_step_ms(rbtp_sim *s, double *ms);
+/* Average duration (ms) of the fused lattice kernel over the last rbtp_step* call, measured
+ * with HIP events around every launch when enabled (0 disables, the default). */
+int rbtp_set_kernel_timing(rbtp_sim *s, int enable);
+int rbtp_kernel_ms(rbtp_sim *s, double *avg_ms, int64_t *launches);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* RBTP_H */

@@ -1,0 +1,85 @@
+/*
+ * oracle.h — FP64 host restatement of the reference kernels.  TEST INFRASTRUCTURE ONLY: loaded
+ * by tests/, __graft_entry__.smoke() and the cpu_baseline leg of bench.py, never by the product.
+ *
+ * How each function is pinned independently of its own formula (tests in tests/test_oracle.py
+ * unless noted).  The reference ships no tests, fixtures or golden vectors and cannot be built
+ * here (CUDA + CUB, no nvcc), so every pin below comes from mathematics or brute force:
+ *
+ *   or_case_derive        closed forms of tau, tauc, grav0, u0, radius, mass, npart (hand values
+ *                         for parameters.h defaults and the benchmark cases).
+ *   D3Q27 tables          opp∘opp = id, c_opp = -c, sum w = 1, sum w c c = I/3, 4th-order
+ *                         isotropy, 27 distinct velocities.
+ *   or_macrovar           moments of random populations against numpy sums (rho, u+F/2, phi).
+ *   or_collision_f        mass conservation, momentum change = Guo source first moment, rest
+ *                         state with u=F=0 is a fixed point (f=0), feq moments.
+ *   or_collision_g        zeroth moment of g after collision equals sum of geqnew (= phi) plus
+ *                         sum of the source; rest state g = w*T (uniform T, u=F=0) stays put
+ *                         bitwise over many full steps.  The source term's coefficients
+ *                         (collision.cu:119-125) have no independent closed form beyond this:
+ *                         PARITY UNPINNED for the Ralph term's value at nonzero gradients.
+ *   or_stream_f/g         single-population delta moves to x+c, reflects into opp at walls, the
+ *                         x-wall anti-bounce-back value 2 w T_wall (1+rho) - g_opp, z/y/x
+ *                         precedence at edges; mass conservation of f over a full step.
+ *   init_fluid            uniform T: g_0 = T(2 w_0 - 1), g_i = 2 w_i T; phi = T recovered.
+ *   compute_link_q        axis-aligned links against the analytic intersection; brute-force
+ *                         sampling of the segment on random spheres.
+ *   or_build_links        brute-force enumeration on tiny grids; sphere centred on a cell centre
+ *                         gives a symmetric link set; lowest pid keeps shared nodes.
+ *   or_ibb_f / or_ibb_g   a sphere at rest in fluid at rest: zero force and torque by symmetry;
+ *                         q<0.5 / q>=0.5 branches against the Bouzidi formulas evaluated in
+ *                         numpy on the same inputs (link-by-link).
+ *   or_repulsion          two spheres and a sphere at a wall at known gaps (closed form).
+ *   or_move               constant force and heat: closed-form trapezoid sequences; free fall.
+ *   or_refill             an isolated new-fluid node with no prev-fluid neighbour gives the
+ *                         closed-form fallback equilibrium.  The extrapolation average
+ *                         (refill.cu:121-153) is checked against a numpy restatement on random
+ *                         populations; no closed form exists: PARITY UNPINNED beyond that.
+ *   or_nu                 exact linear conduction profile with u=0: wall Nu = 1 and
+ *                         Nu_bulk = 1 - phi_p.
+ *   place_particles       non-overlap and window constraints; same glibc rand sequence as the
+ *                         library (same call, same glibc).
+ *
+ * Summation order: see the header of oracle.cpp (deterministic per-particle reductions).
+ */
+#ifndef RBTP_ORACLE_H
+#define RBTP_ORACLE_H
+
+#include "../include/rbtp.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct OrState OrState;
+
+int or_abi_version(void);
+void or_case_default(rbtp_case *c);
+void or_case_derive(const rbtp_case *c, rbtp_derived *d);
+OrState *or_create(const rbtp_case *c);
+void or_destroy(OrState *s);
+int or_init(OrState *s, const double *temperature, const double *ppos, const double *ptemp);
+int or_stage(OrState *s, int stage);
+int or_step(OrState *s, int nsteps);
+int or_npart(const OrState *s);
+long or_nodes(const OrState *s);
+int or_current_step(const OrState *s);
+void or_get_derived(const OrState *s, rbtp_derived *d);
+long or_unhandled_links(const OrState *s, int thermal);
+double *or_array(OrState *s, int which);
+int *or_iarray(OrState *s, int which);
+long or_nlinks(const OrState *s);
+const rbtp_link *or_links(const OrState *s);
+long or_nnew_fluid(const OrState *s);
+const int *or_new_fluid_nodes(const OrState *s);
+const int *or_new_fluid_pids(const OrState *s);
+void or_linear_temperature(const rbtp_case *c, uint64_t seed, double amp, double *out);
+void or_nu(const OrState *s, double out[7]);
+void or_means(const OrState *s, double out[4]);
+int or_threads(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

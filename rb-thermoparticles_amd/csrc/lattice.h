@@ -1,0 +1,27 @@
+// lattice.h — host launchers of lattice.hip.
+#pragma once
+
+#include "rbtp_internal.h"
+
+namespace rbtp {
+
+template <class S>
+hipError_t launch_lattice_step(const StepArgs<S> &a, bool strict, bool nostream, hipStream_t st);
+template <class S>
+hipError_t launch_special_collide(const StepArgs<S> &a, const SpecialBuf &sp, int max_count,
+                                  bool strict, hipStream_t st);
+template <class S>
+hipError_t launch_stream_only(const S *fin, const S *gin, const typename Enc<S>::A *rho_wall,
+                              double *fout, double *gout, Dims d, double t0, double thot,
+                              double tcold, bool nostream, hipStream_t st);
+template <class S>
+hipError_t launch_decode(const S *in, double *out, size_t n, int is_g, double t0, hipStream_t st);
+template <class S>
+hipError_t launch_encode(const double *in, S *out, size_t n, int is_g, double t0, hipStream_t st);
+template <class S>
+hipError_t launch_init(S *f, S *g, typename Enc<S>::A *fx, typename Enc<S>::A *rho_wall,
+                       const double *temperature, double rho0, double t0, double grav0,
+                       double beta, Dims d, double *mr, double *mux, double *muy, double *muz,
+                       double *mphi, hipStream_t st);
+
+}  // namespace rbtp

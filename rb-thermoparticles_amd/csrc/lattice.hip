@@ -1,0 +1,474 @@
+// lattice.hip — fused stream + macrovar + forcing + BGK(f) + BGK(g) kernels for gfx950.
+//
+// State between time steps is the post-collision populations of both distributions (what the
+// reference copies into d_f_collide / d_g_collide, main.cu:45,61).  One launch per time step
+// pulls them across the lattice with the wall rules of streaming (streaming.cu:3-23) and
+// streaming_scalar (streaming.cu:25-63), takes the moments of macrovar (macrovar.cu:3-34),
+// applies forcing (macrovar.cu:36-47), collides f (collision.cu:3-33) and g
+// (collision.cu:35-89), and writes the next post-collision state into the other buffer of a
+// ping-pong pair.  Every population is read once and written once per step (one thread per
+// node, 54 coalesced read and 54 coalesced write streams), which is the HBM roofline of the
+// method: 864 B/node/step in FP64 plus the F_x auxiliary field.
+//
+// Two arithmetic forms:
+//   * strict FP64 (collide_strict): the reference's operation order, true divisions by RT, tau
+//     and tauc, no fused multiply-add (the translation unit is compiled with
+//     -ffp-contract=off).  Terms whose velocity component is zero are skipped; adding or
+//     multiplying an exact zero changes at most the sign of a zero, never a value.
+//   * fast (collide_fast<T>): the same model with divisions replaced by reciprocal
+//     multiplications, in FP64 or FP32, contraction allowed.  Used by the FP32 and
+//     FP16/BF16-storage forms.
+#include "lattice.h"
+#include "lattice_device.h"
+
+#include <type_traits>
+
+namespace rbtp {
+
+namespace {
+
+constexpr int BLOCK = 256;
+constexpr int NUM_XCD = 8;
+
+// Workgroups are dispatched round-robin over the 8 XCDs.  Remap the hardware workgroup id so
+// that each XCD sweeps one contiguous range of the lattice: the partially shared cache lines at
+// workgroup seams then stay in one XCD's L2.
+__device__ inline int xcd_block(int nblocks) {
+    const int bid = blockIdx.x;
+    const int q = nblocks / NUM_XCD, r = nblocks % NUM_XCD;
+    const int xcd = bid % NUM_XCD, local = bid / NUM_XCD;
+    return xcd * q + (xcd < r ? xcd : r) + local;
+}
+
+// Node outputs shared by the lattice kernel and the special-node kernel.
+template <class S> struct NodeOut {
+    S *fout;
+    S *gout;
+    size_t n;
+};
+
+// Strict FP64 macrovar + forcing + collision_BGK + collision_BGK_scalar for one node.
+// Returns the new F_x; writes the post-collision populations.
+template <class S>
+__device__ inline double collide_strict(double (&f)[NPOP], double (&g)[NPOP], double fx_old,
+                                        int idx, const NodeOut<S> &o, const FluidK &k,
+                                        double &rho, double &ux, double &uy, double &uz,
+                                        double &phi) {
+    moments<double>(f, g, rho, ux, uy, uz, phi);
+    ux = ux + 0.5 * fx_old;  // macrovar.cu:30; F_y = F_z = 0 always (macrovar.cu:45-46)
+    const double fx = k.gb * (phi - k.t0);  // forcing, macrovar.cu:44
+    const double RT = 1.0 / 3.0;
+    const double u2 = ux * ux + uy * uy + uz * uz;
+    const double uv = u2 / RT;
+    double X[NPOP];  // (1 + eu) + 0.5 (eu^2 - uv), shared by f's Guo term and g's equilibrium
+    // collision_BGK (collision.cu:16-32)
+#pragma unroll
+    for (int i = 0; i < NPOP; i++) {
+        const double tp = weight(i);
+        const double eu = (i == 0) ? 0.0 : cdot<double>(i, ux, uy, uz) / RT;
+        const double h = 0.5 * (eu * eu - uv);
+        const double feq = tp * ((rho + eu) + h);
+        const double fpop = f[i];
+        double fchange = -(fpop - feq) / k.tau;
+        X[i] = (1.0 + eu) + h;
+        const double feq_rho0 = tp * X[i];
+        const double cuF = ((double)D3Q27::cx[i] - ux) * fx;
+        fchange += k.force_coeff * cuF / RT * feq_rho0;
+        Enc<S>::st(o.fout + (size_t)i * o.n + idx, fpop + fchange);
+    }
+    // collision_BGK_scalar pass 1 (collision.cu:57-70)
+    double gx = 0.0, gy = 0.0, gz = 0.0;
+#pragma unroll
+    for (int i = 0; i < NPOP; i++) {
+        const double tp = weight(i);
+        const double geq = tp * phi * X[i];
+        const double geqnew = (i == 0) ? (geq - k.lambda0 * phi * rho) : (geq + tp * phi * rho);
+        X[i] = geqnew;
+        const double diff = g[i] - geqnew;
+        if (D3Q27::cx[i] == 1) gx += diff;
+        if (D3Q27::cx[i] == -1) gx -= diff;
+        if (D3Q27::cy[i] == 1) gy += diff;
+        if (D3Q27::cy[i] == -1) gy -= diff;
+        if (D3Q27::cz[i] == 1) gz += diff;
+        if (D3Q27::cz[i] == -1) gz -= diff;
+    }
+    // pass 2 (collision.cu:72-87)
+    const double grad_denom = rho / 3.0 + k.c2t;
+#pragma unroll
+    for (int i = 0; i < NPOP; i++) {
+        const double tp = weight(i);
+        const double gpop = g[i];
+        double gchange;
+        if (i == 0) {
+            gchange = -(gpop - X[i]) / k.tauc;
+        } else {
+            const int cx = D3Q27::cx[i];
+            const double gneq = cdot<double>(i, gx, gy, gz);
+            double gradt, src;
+            if (cx != 0) {
+                const double eb = cx > 0 ? fx : -fx;
+                gradt = -(2.0 * gneq + eb * phi) / grad_denom;
+                src = gradt * rho + 3.0 * phi * eb;
+            } else {
+                gradt = -(2.0 * gneq) / grad_denom;
+                src = gradt * rho;
+            }
+            const double Ralph = tp * k.ralph_coeff * src;
+            gchange = -(gpop - X[i]) / k.tauc + Ralph;
+        }
+        Enc<S>::st(o.gout + (size_t)i * o.n + idx, gpop + gchange);
+    }
+    return fx;
+}
+
+// Fast form of the same node update (reciprocals instead of divisions), in T = float or double.
+template <class S, class T>
+__device__ inline T collide_fast(T (&f)[NPOP], T (&g)[NPOP], T fx_old, int idx,
+                                 const NodeOut<S> &o, const FluidK &k, T &rho, T &ux, T &uy,
+                                 T &uz, T &phi) {
+#pragma clang fp contract(fast)
+    moments<T>(f, g, rho, ux, uy, uz, phi);
+    ux = ux + T(0.5) * fx_old;
+    const T t0 = T(k.t0);
+    const T fx = T(k.gb) * (phi - t0);
+    const T inv_tau = T(k.inv_tau), inv_tauc = T(k.inv_tauc);
+    const T uv = T(3) * (ux * ux + uy * uy + uz * uz);
+    const T fc3 = T(3) * T(k.force_coeff) * fx;
+    T gx = T(0), gy = T(0), gz = T(0);
+    const T lam = T(k.lambda0) * phi * rho;
+#pragma unroll
+    for (int i = 0; i < NPOP; i++) {
+        const T tp = T(weight(i));
+        const T eu = (i == 0) ? T(0) : T(3) * cdot<T>(i, ux, uy, uz);
+        const T h = T(0.5) * (eu * eu - uv);
+        const T Xi = (T(1) + eu) + h;
+        const T feq = tp * ((rho + eu) + h);
+        const T fpop = f[i];
+        const T fnew = fpop + (feq - fpop) * inv_tau + (T(D3Q27::cx[i]) - ux) * fc3 * tp * Xi;
+        enc_f<S, T>(o.fout + (size_t)i * o.n + idx, fnew, i);
+        const T geq = tp * phi * Xi;
+        const T geqnew = (i == 0) ? (geq - lam) : (geq + tp * phi * rho);
+        f[i] = geqnew;  // reuse the register for g's equilibrium
+        const T diff = g[i] - geqnew;
+        if (D3Q27::cx[i] == 1) gx += diff;
+        if (D3Q27::cx[i] == -1) gx -= diff;
+        if (D3Q27::cy[i] == 1) gy += diff;
+        if (D3Q27::cy[i] == -1) gy -= diff;
+        if (D3Q27::cz[i] == 1) gz += diff;
+        if (D3Q27::cz[i] == -1) gz -= diff;
+    }
+    const T inv_gd = T(1) / (rho * T(1.0 / 3.0) + T(k.c2t));
+    const T rc = T(k.ralph_coeff);
+#pragma unroll
+    for (int i = 0; i < NPOP; i++) {
+        const T tp = T(weight(i));
+        const T gpop = g[i];
+        T gnew = gpop + (f[i] - gpop) * inv_tauc;
+        if (i != 0) {
+            const int cx = D3Q27::cx[i];
+            const T gneq = cdot<T>(i, gx, gy, gz);
+            const T eb = cx == 0 ? T(0) : (cx > 0 ? fx : -fx);
+            const T gradt = -(T(2) * gneq + eb * phi) * inv_gd;
+            gnew += tp * rc * (gradt * rho + T(3) * phi * eb);
+        }
+        enc_g<S, T>(o.gout + (size_t)i * o.n + idx, gnew, i, t0);
+    }
+    return fx;
+}
+
+template <class T> struct Arith;
+template <> struct Arith<double> { static constexpr bool strict_ok = true; };
+template <> struct Arith<float> { static constexpr bool strict_ok = false; };
+
+template <class S, class T, bool STRICT>
+__device__ inline T collide_node(T (&f)[NPOP], T (&g)[NPOP], T fx_old, int idx,
+                                 const NodeOut<S> &o, const FluidK &k, T &rho, T &ux, T &uy,
+                                 T &uz, T &phi) {
+    if constexpr (STRICT) {
+        static_assert(Arith<T>::strict_ok, "strict form needs FP64 arithmetic");
+        return collide_strict<S>(f, g, fx_old, idx, o, k, rho, ux, uy, uz, phi);
+    } else {
+        return collide_fast<S, T>(f, g, fx_old, idx, o, k, rho, ux, uy, uz, phi);
+    }
+}
+
+// Writes shared by both kernels: F_x, wall rho, optional rho field and macro fields.
+template <class S, class T>
+__device__ inline void write_node_aux(const StepArgs<S> &a, int idx, int ix, int iy, int iz,
+                                      T fx, T rho, T ux, T uy, T uz, T phi) {
+    using A = typename Enc<S>::A;
+    a.fx[idx] = A(fx);
+    if (ix == 0) a.rho_wall[iz * a.d.ly + iy] = A(rho);
+    if (ix == a.d.lx - 1) a.rho_wall[a.d.ly * a.d.lz + iz * a.d.ly + iy] = A(rho);
+    if (a.rho_full) a.rho_full[idx] = double(rho);
+    if (a.macro_rho) {
+        a.macro_rho[idx] = double(rho);
+        a.macro_ux[idx] = double(ux);
+        a.macro_uy[idx] = double(uy);
+        a.macro_uz[idx] = double(uz);
+        a.macro_phi[idx] = double(phi);
+    }
+}
+
+// One time step for every node.
+template <class S, class T, bool STRICT, bool NOSTREAM>
+__global__ __launch_bounds__(BLOCK) void k_lattice_step(const StepArgs<S> a) {
+    const int nblocks = (a.d.n + BLOCK - 1) / BLOCK;
+    const int idx = xcd_block(nblocks) * BLOCK + threadIdx.x;
+    if (idx >= a.d.n) return;
+    const int iz = idx / a.d.lxy;
+    const int rem = idx - iz * a.d.lxy;
+    const int iy = rem / a.d.lx;
+    const int ix = rem - iy * a.d.lx;
+    T rho_w = T(0);
+    if (!NOSTREAM) {
+        if (ix == 0) rho_w = T(a.rho_wall[iz * a.d.ly + iy]);
+        else if (ix == a.d.lx - 1) rho_w = T(a.rho_wall[a.d.ly * a.d.lz + iz * a.d.ly + iy]);
+    }
+    T f[NPOP], g[NPOP];
+    gather<S, T, NOSTREAM>(a.fin, a.gin, (size_t)a.d.n, idx, ix, iy, iz, a.d, rho_w,
+                           T(a.k.t0), T(a.k.thot), T(a.k.tcold), f, g);
+    const T fx_old = T(a.fx[idx]);
+    NodeOut<S> o{a.fout, a.gout, (size_t)a.d.n};
+    T rho, ux, uy, uz, phi;
+    const T fx = collide_node<S, T, STRICT>(f, g, fx_old, idx, o, a.k, rho, ux, uy, uz, phi);
+    write_node_aux<S, T>(a, idx, ix, iy, iz, fx, rho, ux, uy, uz, phi);
+}
+
+// Post-stream populations only (for the parity accessors and the fg restart writer), decoded to
+// FP64.  Special nodes are overwritten afterwards from the special-node buffer.
+template <class S>
+__global__ __launch_bounds__(BLOCK) void k_stream_only(const S *__restrict__ fin,
+                                                       const S *__restrict__ gin,
+                                                       const typename Enc<S>::A *rho_wall,
+                                                       double *fout, double *gout, Dims d,
+                                                       double t0, double thot, double tcold,
+                                                       int nostream) {
+    const int idx = blockIdx.x * BLOCK + threadIdx.x;
+    if (idx >= d.n) return;
+    const int iz = idx / d.lxy;
+    const int rem = idx - iz * d.lxy;
+    const int iy = rem / d.lx;
+    const int ix = rem - iy * d.lx;
+    using T = typename std::conditional<std::is_same<S, double>::value, double, float>::type;
+    T rho_w = T(0);
+    if (ix == 0) rho_w = T(rho_wall[iz * d.ly + iy]);
+    else if (ix == d.lx - 1) rho_w = T(rho_wall[d.ly * d.lz + iz * d.ly + iy]);
+    T f[NPOP], g[NPOP];
+    if (nostream)
+        gather<S, T, true>(fin, gin, (size_t)d.n, idx, ix, iy, iz, d, rho_w, T(t0), T(thot),
+                           T(tcold), f, g);
+    else
+        gather<S, T, false>(fin, gin, (size_t)d.n, idx, ix, iy, iz, d, rho_w, T(t0), T(thot),
+                            T(tcold), f, g);
+#pragma unroll
+    for (int i = 0; i < NPOP; i++) {
+        fout[(size_t)i * d.n + idx] = double(f[i]);
+        gout[(size_t)i * d.n + idx] = double(g[i]);
+    }
+}
+
+// Decode a population buffer to FP64 (post-collision accessor).
+template <class S>
+__global__ __launch_bounds__(BLOCK) void k_decode(const S *__restrict__ in, double *out, size_t n,
+                                                  int is_g, double t0) {
+    const size_t j = (size_t)blockIdx.x * BLOCK + threadIdx.x;
+    if (j >= (size_t)NPOP * n) return;
+    const int i = (int)(j / n);
+    using T = typename std::conditional<std::is_same<S, double>::value, double, float>::type;
+    // weight(i) with a runtime index: evaluate the same constant expression.
+    const double w = i == 0 ? 8.0 / 27.0 : (i <= 6 ? 2.0 / 27.0 : (i <= 18 ? 1.0 / 54.0 : 1.0 / 216.0));
+    const T s = T(Enc<S>::ld(in + j));
+    if constexpr (Enc<S>::scaled) {
+        out[j] = is_g ? double((s + T(t0)) * T(w)) : double(s * T(w));
+    } else {
+        out[j] = double(s);
+    }
+}
+
+// Encode FP64 populations into the storage form (restart load, set_populations).
+template <class S>
+__global__ __launch_bounds__(BLOCK) void k_encode(const double *__restrict__ in, S *out, size_t n,
+                                                  int is_g, double t0) {
+    const size_t j = (size_t)blockIdx.x * BLOCK + threadIdx.x;
+    if (j >= (size_t)NPOP * n) return;
+    const int i = (int)(j / n);
+    const double w = i == 0 ? 8.0 / 27.0 : (i <= 6 ? 2.0 / 27.0 : (i <= 18 ? 1.0 / 54.0 : 1.0 / 216.0));
+    const double v = in[j];
+    if constexpr (Enc<S>::scaled) {
+        Enc<S>::st(out + j, is_g ? float(v * (1.0 / w) - t0) : float(v * (1.0 / w)));
+    } else {
+        Enc<S>::st(out + j, (S)v);
+    }
+}
+
+// init_f (initial.cu:335-360) and init_g (initial.cu:362-397).  temperature may be null
+// (uniform t0, the reference) or a field (benchmark initial state).  Writes the post-stream
+// state (f, g), phi, F_x and the wall rho; the rho field of init_f is RHO0.
+template <class S>
+__global__ __launch_bounds__(BLOCK) void k_init(S *f, S *g, typename Enc<S>::A *fx_field,
+                                                typename Enc<S>::A *rho_wall,
+                                                const double *temperature, double rho0,
+                                                double t0, double grav0, double beta, Dims d,
+                                                double *macro_rho, double *macro_ux,
+                                                double *macro_uy, double *macro_uz,
+                                                double *macro_phi) {
+    const int idx = blockIdx.x * BLOCK + threadIdx.x;
+    if (idx >= d.n) return;
+    const int iz = idx / d.lxy;
+    const int rem = idx - iz * d.lxy;
+    const int iy = rem / d.lx;
+    const int ix = rem - iy * d.lx;
+    const size_t n = (size_t)d.n;
+    // init_f: r = u = v = w = 0, so eu = 0 and feq = w_i (0 + 0 + 0.5 (0 - 0)) = 0.
+    const double RT = 1.0 / 3.0;
+    const double uv = (0.0 * 0.0 + 0.0 * 0.0 + 0.0 * 0.0) / RT;
+#pragma unroll
+    for (int i = 0; i < NPOP; i++) {
+        const double eu = 0.0 / RT;
+        const double feq = weight(i) * (0.0 + eu + 0.5 * (eu * eu - uv));
+        enc_f<S, double>(f + (size_t)i * n + idx, feq, i);
+    }
+    // init_g with r = rho = RHO0 (written by init_f, initial.cu:350) and u = 0.
+    const double r = rho0;
+    const double tempT = temperature ? temperature[idx] : t0;
+    const double lambda0 = 1.0 - weight(0);
+#pragma unroll
+    for (int i = 0; i < NPOP; i++) {
+        const double eu = 0.0 / RT;
+        double geq = weight(i) * tempT * (1.0 + eu + 0.5 * (eu * eu - uv));
+        if (i == 0) geq -= lambda0 * tempT * r;
+        else geq += weight(i) * tempT * r;
+        enc_g<S, double>(g + (size_t)i * n + idx, geq, i, t0);
+    }
+    using A = typename Enc<S>::A;
+    fx_field[idx] = A(grav0 * beta * (tempT - t0));
+    if (ix == 0) rho_wall[iz * d.ly + iy] = A(rho0);
+    if (ix == d.lx - 1) rho_wall[d.ly * d.lz + iz * d.ly + iy] = A(rho0);
+    if (macro_rho) {
+        macro_rho[idx] = rho0;
+        macro_ux[idx] = 0.0;
+        macro_uy[idx] = 0.0;
+        macro_uz[idx] = 0.0;
+        macro_phi[idx] = tempT;
+    }
+}
+
+// Re-collide the special nodes from their buffered post-stream populations (IBB- and
+// refill-corrected), overwriting what the lattice kernel wrote there.
+template <class S, class T, bool STRICT>
+__global__ __launch_bounds__(BLOCK) void k_special_collide(const StepArgs<S> a, SpecialBuf sp) {
+    const int j = blockIdx.x * BLOCK + threadIdx.x;
+    const int count = *sp.count;
+    if (j >= count || j >= sp.cap) return;
+    const int idx = sp.node[j];
+    const int iz = idx / a.d.lxy;
+    const int rem = idx - iz * a.d.lxy;
+    const int iy = rem / a.d.lx;
+    const int ix = rem - iy * a.d.lx;
+    T f[NPOP], g[NPOP];
+    const size_t cap = (size_t)sp.cap;
+#pragma unroll
+    for (int i = 0; i < NPOP; i++) {
+        f[i] = T(sp.pop[(size_t)i * cap + j]);
+        g[i] = T(sp.pop[(size_t)(NPOP + i) * cap + j]);
+    }
+    NodeOut<S> o{a.fout, a.gout, (size_t)a.d.n};
+    T rho, ux, uy, uz, phi;
+    const T fx = collide_node<S, T, STRICT>(f, g, T(sp.fx_old[j]), idx, o, a.k, rho, ux, uy, uz,
+                                            phi);
+    write_node_aux<S, T>(a, idx, ix, iy, iz, fx, rho, ux, uy, uz, phi);
+}
+
+inline int blocks_for(size_t n) { return (int)((n + BLOCK - 1) / BLOCK); }
+
+}  // namespace
+
+// ---- host launchers --------------------------------------------------------------------------
+
+template <class S>
+hipError_t launch_lattice_step(const StepArgs<S> &a, bool strict, bool nostream,
+                               hipStream_t st) {
+    const int nb = blocks_for((size_t)a.d.n);
+    if constexpr (std::is_same<S, double>::value) {
+        if (strict) {
+            if (nostream) hipLaunchKernelGGL((k_lattice_step<S, double, true, true>), nb, BLOCK, 0, st, a);
+            else hipLaunchKernelGGL((k_lattice_step<S, double, true, false>), nb, BLOCK, 0, st, a);
+        } else {
+            if (nostream) hipLaunchKernelGGL((k_lattice_step<S, double, false, true>), nb, BLOCK, 0, st, a);
+            else hipLaunchKernelGGL((k_lattice_step<S, double, false, false>), nb, BLOCK, 0, st, a);
+        }
+    } else {
+        (void)strict;
+        if (nostream) hipLaunchKernelGGL((k_lattice_step<S, float, false, true>), nb, BLOCK, 0, st, a);
+        else hipLaunchKernelGGL((k_lattice_step<S, float, false, false>), nb, BLOCK, 0, st, a);
+    }
+    return hipGetLastError();
+}
+
+template <class S>
+hipError_t launch_special_collide(const StepArgs<S> &a, const SpecialBuf &sp, int max_count,
+                                  bool strict, hipStream_t st) {
+    if (max_count <= 0) return hipSuccess;
+    const int nb = blocks_for((size_t)max_count);
+    if constexpr (std::is_same<S, double>::value) {
+        if (strict) hipLaunchKernelGGL((k_special_collide<S, double, true>), nb, BLOCK, 0, st, a, sp);
+        else hipLaunchKernelGGL((k_special_collide<S, double, false>), nb, BLOCK, 0, st, a, sp);
+    } else {
+        (void)strict;
+        hipLaunchKernelGGL((k_special_collide<S, float, false>), nb, BLOCK, 0, st, a, sp);
+    }
+    return hipGetLastError();
+}
+
+template <class S>
+hipError_t launch_stream_only(const S *fin, const S *gin, const typename Enc<S>::A *rho_wall,
+                              double *fout, double *gout, Dims d, double t0, double thot,
+                              double tcold, bool nostream, hipStream_t st) {
+    hipLaunchKernelGGL((k_stream_only<S>), blocks_for((size_t)d.n), BLOCK, 0, st, fin, gin,
+                       rho_wall, fout, gout, d, t0, thot, tcold, nostream ? 1 : 0);
+    return hipGetLastError();
+}
+
+template <class S>
+hipError_t launch_decode(const S *in, double *out, size_t n, int is_g, double t0, hipStream_t st) {
+    hipLaunchKernelGGL((k_decode<S>), blocks_for((size_t)NPOP * n), BLOCK, 0, st, in, out, n, is_g, t0);
+    return hipGetLastError();
+}
+
+template <class S>
+hipError_t launch_encode(const double *in, S *out, size_t n, int is_g, double t0, hipStream_t st) {
+    hipLaunchKernelGGL((k_encode<S>), blocks_for((size_t)NPOP * n), BLOCK, 0, st, in, out, n, is_g, t0);
+    return hipGetLastError();
+}
+
+template <class S>
+hipError_t launch_init(S *f, S *g, typename Enc<S>::A *fx, typename Enc<S>::A *rho_wall,
+                       const double *temperature, double rho0, double t0, double grav0,
+                       double beta, Dims d, double *mr, double *mux, double *muy, double *muz,
+                       double *mphi, hipStream_t st) {
+    hipLaunchKernelGGL((k_init<S>), blocks_for((size_t)d.n), BLOCK, 0, st, f, g, fx, rho_wall,
+                       temperature, rho0, t0, grav0, beta, d, mr, mux, muy, muz, mphi);
+    return hipGetLastError();
+}
+
+#define RBTP_INSTANTIATE(S)                                                                     \
+    template hipError_t launch_lattice_step<S>(const StepArgs<S> &, bool, bool, hipStream_t);   \
+    template hipError_t launch_special_collide<S>(const StepArgs<S> &, const SpecialBuf &, int, \
+                                                  bool, hipStream_t);                           \
+    template hipError_t launch_stream_only<S>(const S *, const S *,                             \
+                                              const typename Enc<S>::A *, double *, double *,   \
+                                              Dims, double, double, double, bool, hipStream_t); \
+    template hipError_t launch_decode<S>(const S *, double *, size_t, int, double, hipStream_t); \
+    template hipError_t launch_encode<S>(const double *, S *, size_t, int, double, hipStream_t); \
+    template hipError_t launch_init<S>(S *, S *, typename Enc<S>::A *, typename Enc<S>::A *,    \
+                                       const double *, double, double, double, double, Dims,    \
+                                       double *, double *, double *, double *, double *,        \
+                                       hipStream_t);
+
+RBTP_INSTANTIATE(double)
+RBTP_INSTANTIATE(float)
+RBTP_INSTANTIATE(__half)
+RBTP_INSTANTIATE(__hip_bfloat16)
+
+}  // namespace rbtp

@@ -1,0 +1,99 @@
+// lattice_device.h — device helpers shared by the lattice and particle kernels: population
+// encodings, the pull gather with the wall rules of streaming/streaming_scalar, moments.
+#pragma once
+
+#include "rbtp_internal.h"
+
+namespace rbtp {
+
+// ---- populations in and out ------------------------------------------------------------------
+
+template <class S, class T>
+__device__ inline T dec_f(S s, int i) {
+    if constexpr (Enc<S>::scaled) return T(s) * T(weight(i));
+    else return T(s);
+}
+template <class S, class T>
+__device__ inline T dec_g(S s, int i, T t0) {
+    if constexpr (Enc<S>::scaled) return (T(s) + t0) * T(weight(i));
+    else return T(s);
+}
+template <class S, class T>
+__device__ inline void enc_f(S *p, T v, int i) {
+    if constexpr (Enc<S>::scaled) Enc<S>::st(p, float(v * T(1.0 / weight(i))));
+    else Enc<S>::st(p, v);
+}
+template <class S, class T>
+__device__ inline void enc_g(S *p, T v, int i, T t0) {
+    if constexpr (Enc<S>::scaled) Enc<S>::st(p, float(v * T(1.0 / weight(i)) - t0));
+    else Enc<S>::st(p, v);
+}
+
+// Pull the post-stream populations of node (ix,iy,iz) from the post-collision state.
+//   f: f_i(x) = f*_i(x - c_i), or f*_opp(x) when x - c_i is outside (streaming.cu:10-21).
+//   g: as f inside; outside, the z then y walls reflect (adiabatic) and the x walls apply
+//      -g*_opp(x) + 2 w_i T_wall (1 + rho(x)) (streaming.cu:37-61).
+template <class S, class T, bool NOSTREAM>
+__device__ inline void gather(const S *__restrict__ fin, const S *__restrict__ gin, size_t n,
+                              int idx, int ix, int iy, int iz, const Dims &d, T rho_w, T t0,
+                              T thot, T tcold, T (&f)[NPOP], T (&g)[NPOP]) {
+#pragma unroll
+    for (int i = 0; i < NPOP; i++) {
+        const int cx = D3Q27::cx[i], cy = D3Q27::cy[i], cz = D3Q27::cz[i];
+        const int op = D3Q27::opp[i];
+        if (NOSTREAM) {
+            f[i] = dec_f<S, T>(Enc<S>::ld(fin + (size_t)i * n + idx), i);
+            g[i] = dec_g<S, T>(Enc<S>::ld(gin + (size_t)i * n + idx), i, t0);
+            continue;
+        }
+        const bool outx = (cx == 1 && ix == 0) || (cx == -1 && ix == d.lx - 1);
+        const bool outy = (cy == 1 && iy == 0) || (cy == -1 && iy == d.ly - 1);
+        const bool outz = (cz == 1 && iz == 0) || (cz == -1 && iz == d.lz - 1);
+        const bool inside = !(outx || outy || outz);
+        const int off = cx + cy * d.lx + cz * d.lxy;
+        const size_t src = inside ? (size_t)i * n + (size_t)(idx - off) : (size_t)op * n + idx;
+        const T fv = dec_f<S, T>(Enc<S>::ld(fin + src), i);
+        const T gv = dec_g<S, T>(Enc<S>::ld(gin + src), i, t0);
+        f[i] = fv;
+        if (inside || outz || outy) {
+            g[i] = gv;
+        } else {
+            // x wall only: hot plate at x=0 (c_x=+1), cold plate at x=LX (c_x=-1).
+            const T twall = (cx == 1) ? thot : tcold;
+            g[i] = -gv + T(2.0) * T(weight(i)) * twall * (T(1.0) + rho_w);
+        }
+    }
+}
+
+// Moments of macrovar (macrovar.cu:18-33): sequential sums in population order; terms with a
+// zero velocity component are skipped (they add an exact zero).
+template <class T>
+__device__ inline void moments(const T (&f)[NPOP], const T (&g)[NPOP], T &rho, T &ux, T &uy,
+                               T &uz, T &phi) {
+    rho = T(0); ux = T(0); uy = T(0); uz = T(0); phi = T(0);
+#pragma unroll
+    for (int i = 0; i < NPOP; i++) {
+        rho += f[i];
+        if (D3Q27::cx[i] == 1) ux += f[i];
+        if (D3Q27::cx[i] == -1) ux -= f[i];
+        if (D3Q27::cy[i] == 1) uy += f[i];
+        if (D3Q27::cy[i] == -1) uy -= f[i];
+        if (D3Q27::cz[i] == 1) uz += f[i];
+        if (D3Q27::cz[i] == -1) uz -= f[i];
+        phi += g[i];
+    }
+}
+
+// c_i . v with the reference's (x + y) + z order and exact-zero terms skipped.
+template <class T>
+__device__ inline T cdot(int i, T vx, T vy, T vz) {
+    const int cx = D3Q27::cx[i], cy = D3Q27::cy[i], cz = D3Q27::cz[i];
+    T s = T(0);
+    bool first = true;
+    if (cx != 0) { s = cx > 0 ? vx : -vx; first = false; }
+    if (cy != 0) { s = first ? (cy > 0 ? vy : -vy) : (cy > 0 ? s + vy : s - vy); first = false; }
+    if (cz != 0) { s = first ? (cz > 0 ? vz : -vz) : (cz > 0 ? s + vz : s - vz); first = false; }
+    return s;
+}
+
+}  // namespace rbtp

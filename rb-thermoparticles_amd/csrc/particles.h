@@ -1,0 +1,59 @@
+// particles.h — particle coupling and diagnostics (particles.hip, diag.hip).
+#pragma once
+
+#include <utility>
+#include <vector>
+
+#include "../../include/rbtp.h"
+#include "rbtp_internal.h"
+
+namespace rbtp {
+
+// Indices of the snapshot of the particle state before the motion of the current step.
+enum { OLD_PX = 0, OLD_PY, OLD_PZ, OLD_VX, OLD_VY, OLD_VZ, OLD_WX, OLD_WY, OLD_WZ, OLD_T, OLD_COUNT };
+
+// Error flags raised by device code (checked after every synchronising call).
+enum { ERR_SPECIAL_CAP = 0, ERR_CONTACT_CAP = 1, ERR_UNHANDLED_F = 2, ERR_UNHANDLED_G = 3, ERR_COUNT = 4 };
+
+struct PartDev {
+    int np;
+    // Case constants (globals.cu:126-148, pforces.cu:121-124).
+    double radius, radius_sq, mass, inertia, heat_capacity, rho_ratio, grav0, coeff;
+    double min_gap, min_gap_w, stf0, stf1, stf0_w, stf1_w, rho0, t0, thot, tcold;
+    double *base;            // one allocation for every P-length FP64 array
+    double *p[RBTP_P_COUNT]; // particle state, indexed by RBTP_P_*
+    double *old[OLD_COUNT];  // state before this step's motion
+    int32_t *owner[2];       // solid-node owner (-1: fluid), two geometry generations
+    int32_t *box[2];         // marking box per particle and generation: lo xyz, hi xyz
+    int geo_cur;             // generation holding the newest geometry
+    SpecialBuf sp;
+    int32_t *sp_kind;        // -1: IBB link node; >= 0: refilled node, value = previous owner
+    bool special_pending;    // sp holds corrected populations for the next lattice step
+    int32_t *err;            // ERR_COUNT device flags
+    int box_w;               // upper bound of marking-box nodes per axis
+};
+
+int report(int code, const char *msg);
+int report_hip(hipError_t e, const char *what);
+
+int part_alloc(PartDev &pd, const rbtp_case &c, const rbtp_derived &d, const Dims &dims);
+void part_free(PartDev &pd);
+int part_init(PartDev &pd, const rbtp_case &c, const rbtp_derived &d, const Dims &dims,
+              const double *ppos, const double *ptemp, hipStream_t st);
+int part_phase(PartDev &pd, const rbtp_case &c, const rbtp_derived &d, const Dims &dims,
+               int prec, const void *fcur, const void *gcur, void *fx, void *rho_wall,
+               double *const *macro, hipStream_t st);
+int part_check_errors(PartDev &pd);
+hipError_t part_overwrite_special(PartDev &pd, double *f, double *g, const Dims &dims,
+                                  hipStream_t st);
+int part_collect_links(PartDev &pd, const Dims &dims, hipStream_t st, std::vector<rbtp_link> &out);
+int part_collect_new_fluid(PartDev &pd, hipStream_t st,
+                           std::vector<std::pair<int32_t, int32_t>> &out);
+
+int diag_nu(const rbtp_case &c, const rbtp_derived &d, const Dims &dims, const double *ux,
+            const double *phi, const int32_t *owner, int step, hipStream_t st, double out[7]);
+int diag_means(const Dims &dims, double *const *macro, const PartDev *pd, hipStream_t st,
+               double out[4]);
+int diag_kinetic_energy(const Dims &dims, double *const *macro, hipStream_t st, double *out);
+
+}  // namespace rbtp

@@ -1,0 +1,1004 @@
+// particles.hip — particle coupling on gfx950: geometry, IBB, force/torque/heat sums, contacts,
+// motion, new-fluid detection and refill (src_part/*.cu of the reference).
+//
+// Links are never stored.  A link (x, i, p) exists when x is a fluid node, x + c_i is a solid
+// node owned by p, and the ray from x's centre along c_i meets sphere p at q in [0, 1]
+// (link_count_kernel, links.cu:50-87).  Each consumer re-derives the links it needs from the two
+// owner maps with the same bit-exact test:
+//   * k_link_sums: one workgroup per particle walks the particle's link box and sums
+//     force, torque and heat in a fixed order (256 lane partials + pairwise tree), replacing the
+//     reference's FP64 atomics (ibb.cu:182-187,204) with a reproducible reduction;
+//   * k_special_links / k_special_refill: compact list of the nodes whose post-stream
+//     populations the coupling replaces (IBB link nodes, refilled nodes);
+//   * k_prefixup: those nodes' corrected post-stream populations (pull + IBB, or refill),
+//     buffered for the next lattice step.
+// Geometry is two owner maps (previous and current generation); each generation is cleared only
+// inside the marking boxes it was built from, so a step touches O(P R^3) nodes instead of the
+// reference's full-lattice memsets, scans and host round trips (links.cu:147-256,
+// refill.cu:244-329).  Nothing is read back to the host inside a step.
+//
+// The translation unit is compiled with -ffp-contract=off: every expression is evaluated in the
+// reference's operation order.
+#include "particles.h"
+
+#include <algorithm>
+#include <climits>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+#include "lattice_device.h"
+
+namespace rbtp {
+
+namespace {
+
+constexpr int PB = 256;  // threads per particle workgroup (also the reduction width)
+
+__constant__ int c_cx[NPOP] = {0, 1, 0, -1, 0, 0, 0, 1, -1, -1, 1, 1, 0, -1,
+                               0, 1, 0, -1, 0, 1, -1, -1, 1, 1, -1, -1, 1};
+__constant__ int c_cy[NPOP] = {0, 0, 1, 0, -1, 0, 0, 1, 1, -1, -1, 0, 1, 0,
+                               -1, 0, 1, 0, -1, 1, 1, -1, -1, 1, 1, -1, -1};
+__constant__ int c_cz[NPOP] = {0, 0, 0, 0, 0, 1, -1, 0, 0, 0, 0, 1, 1, 1,
+                               1, -1, -1, -1, -1, 1, 1, 1, 1, -1, -1, -1, -1};
+__constant__ int c_opp[NPOP] = {0, 3, 4, 1, 2, 6, 5, 9, 10, 7, 8, 17, 18, 15,
+                                16, 13, 14, 11, 12, 25, 26, 23, 24, 21, 22, 19, 20};
+
+__device__ inline double wt(int i) {
+    return i == 0 ? 8.0 / 27.0 : (i <= 6 ? 2.0 / 27.0 : (i <= 18 ? 1.0 / 54.0 : 1.0 / 216.0));
+}
+
+// compute_link_q (links.cu:7-23)
+__device__ inline bool link_q(int ip, double x0, double y0, double z0, double px, double py,
+                              double pz, double radius_sq, double &q) {
+    const double cx_dir = static_cast<double>(c_cx[ip]);
+    const double cy_dir = static_cast<double>(c_cy[ip]);
+    const double cz_dir = static_cast<double>(c_cz[ip]);
+    const double dx = x0 - px;
+    const double dy = y0 - py;
+    const double dz = z0 - pz;
+    const double a = cx_dir * cx_dir + cy_dir * cy_dir + cz_dir * cz_dir;
+    const double b = 2.0 * (cx_dir * dx + cy_dir * dy + cz_dir * dz);
+    const double c = dx * dx + dy * dy + dz * dz - radius_sq;
+    const double disc = b * b - 4.0 * a * c;
+    if (disc <= 0.0) return false;
+    q = (-b - sqrt(disc)) / (2.0 * a);
+    return (q >= 0.0 && q <= 1.0);
+}
+
+// Marking box of a particle (links.cu:164-177); false when empty.
+__host__ __device__ inline bool marking_box(double px, double py, double pz, double R,
+                                            const Dims &d, int lo[3], int hi[3]) {
+    lo[0] = static_cast<int>(floor(px - R - 0.5));
+    hi[0] = static_cast<int>(floor(px + R - 0.5));
+    lo[1] = static_cast<int>(floor(py - R - 0.5));
+    hi[1] = static_cast<int>(floor(py + R - 0.5));
+    lo[2] = static_cast<int>(floor(pz - R - 0.5));
+    hi[2] = static_cast<int>(floor(pz + R - 0.5));
+    const int L[3] = {d.lx, d.ly, d.lz};
+    for (int a = 0; a < 3; a++) {
+        if (lo[a] < 0) lo[a] = 0;
+        if (hi[a] > L[a] - 1) hi[a] = L[a] - 1;
+    }
+    return !(lo[0] > hi[0] || lo[1] > hi[1] || lo[2] > hi[2]);
+}
+
+__device__ inline bool load_box(const int32_t *box, int p, int lo[3], int hi[3]) {
+    for (int a = 0; a < 3; a++) {
+        lo[a] = box[p * 6 + a];
+        hi[a] = box[p * 6 + 3 + a];
+    }
+    return !(lo[0] > hi[0] || lo[1] > hi[1] || lo[2] > hi[2]);
+}
+
+// The link box: marking box grown by one node per side, clamped.
+__device__ inline void grow_box(const Dims &d, int lo[3], int hi[3]) {
+    const int L[3] = {d.lx, d.ly, d.lz};
+    for (int a = 0; a < 3; a++) {
+        lo[a] = lo[a] - 1 < 0 ? 0 : lo[a] - 1;
+        hi[a] = hi[a] + 1 > L[a] - 1 ? L[a] - 1 : hi[a] + 1;
+    }
+}
+
+__device__ inline int box_node(const Dims &d, const int lo[3], const int hi[3], int k, int &ix,
+                               int &iy, int &iz) {
+    const int nx = hi[0] - lo[0] + 1, ny = hi[1] - lo[1] + 1;
+    const int kz = k / (nx * ny);
+    const int r = k - kz * nx * ny;
+    const int ky = r / nx;
+    const int kx = r - ky * nx;
+    ix = lo[0] + kx;
+    iy = lo[1] + ky;
+    iz = lo[2] + kz;
+    return iz * d.lxy + iy * d.lx + ix;
+}
+
+__device__ inline int box_count(const int lo[3], const int hi[3]) {
+    return (hi[0] - lo[0] + 1) * (hi[1] - lo[1] + 1) * (hi[2] - lo[2] + 1);
+}
+
+__device__ inline bool in_domain(const Dims &d, int x, int y, int z) {
+    return x >= 0 && x < d.lx && y >= 0 && y < d.ly && z >= 0 && z < d.lz;
+}
+
+// ---- geometry --------------------------------------------------------------------------------
+
+__global__ __launch_bounds__(PB) void k_geo_clear(int32_t *owner, const int32_t *box, Dims d) {
+    const int p = blockIdx.x;
+    int lo[3], hi[3];
+    if (!load_box(box, p, lo, hi)) return;
+    const int cnt = box_count(lo, hi);
+    for (int k = threadIdx.x; k < cnt; k += PB) {
+        int ix, iy, iz;
+        owner[box_node(d, lo, hi, k, ix, iy, iz)] = -1;
+    }
+}
+
+// mark_ibnode_kernel (links.cu:25-48) for every particle at once.  The reference launches the
+// particles one after another so the lowest pid claims a shared node; atomicMin on the owner
+// (-1 read as the largest unsigned value) gives the same result in one launch.
+__global__ __launch_bounds__(PB) void k_geo_mark(int32_t *owner, int32_t *box, const double *px,
+                                                 const double *py, const double *pz, double R,
+                                                 double radius_sq, Dims d) {
+    const int p = blockIdx.x;
+    int lo[3], hi[3];
+    const double X = px[p], Y = py[p], Z = pz[p];
+    const bool ok = marking_box(X, Y, Z, R, d, lo, hi);
+    if (threadIdx.x == 0) {
+        for (int a = 0; a < 3; a++) {
+            box[p * 6 + a] = ok ? lo[a] : 1;
+            box[p * 6 + 3 + a] = ok ? hi[a] : 0;
+        }
+    }
+    if (!ok) return;
+    const int cnt = box_count(lo, hi);
+    for (int k = threadIdx.x; k < cnt; k += PB) {
+        int ix, iy, iz;
+        const int idx = box_node(d, lo, hi, k, ix, iy, iz);
+        const double cx = static_cast<double>(ix) + 0.5;
+        const double cy = static_cast<double>(iy) + 0.5;
+        const double cz = static_cast<double>(iz) + 0.5;
+        const double dx = cx - X, dy = cy - Y, dz = cz - Z;
+        if (dx * dx + dy * dy + dz * dz > radius_sq) continue;
+        atomicMin(reinterpret_cast<unsigned int *>(owner + idx), static_cast<unsigned int>(p));
+    }
+}
+
+// ---- contacts and force history ----------------------------------------------------------------
+
+// prepare_particle_forces_step (pforces.cu:80-116), the heat history of compute_particle_heat
+// (ibb.cu:260-261), and apply_repulsive_forces (pforces.cu:118-148).  Pair terms are summed per
+// particle in ascending partner order; the term on the higher pid of a pair is evaluated from its
+// own side, which is bit-for-bit the negation the reference applies (pforces.cu:35-37).
+__global__ __launch_bounds__(128) void k_contacts(PartDev pd, Dims d) {
+    const int i = blockIdx.x * 128 + threadIdx.x;
+    if (i >= pd.np) return;
+    double **P = pd.p;
+    for (int a = 0; a < 3; a++) {
+        P[RBTP_P_FORCE_PREV2_X + a][i] = P[RBTP_P_FORCE_PREV_X + a][i];
+        P[RBTP_P_TORQUE_PREV2_X + a][i] = P[RBTP_P_TORQUE_PREV_X + a][i];
+        P[RBTP_P_FORCE_PREV_X + a][i] = P[RBTP_P_FORCE_X + a][i];
+        P[RBTP_P_TORQUE_PREV_X + a][i] = P[RBTP_P_TORQUE_X + a][i];
+        P[RBTP_P_TORQUE_X + a][i] = 0.0;
+    }
+    P[RBTP_P_HEAT_PREV][i] = P[RBTP_P_HEAT][i];
+    P[RBTP_P_HEAT][i] = 0.0;
+    const double *X = P[RBTP_P_POS_X], *Y = P[RBTP_P_POS_Y], *Z = P[RBTP_P_POS_Z];
+    const double xi = X[i], yi = Y[i], zi = Z[i];
+    double FX = 0.0, FY = 0.0, FZ = 0.0;
+    const double radius = pd.radius, coeff = pd.coeff, min_gap = pd.min_gap;
+    if (pd.np > 1) {
+        for (int j = 0; j < pd.np; j++) {
+            if (j == i) continue;
+            const double dx = xi - X[j], dy = yi - Y[j], dz = zi - Z[j];
+            const double dist_sq = dx * dx + dy * dy + dz * dz;
+            if (dist_sq <= 1.0e-12) continue;
+            const double dist = sqrt(dist_sq);
+            const double hgap = dist - 2.0 * radius;
+            if (hgap >= min_gap) continue;
+            const double overlap = (hgap < 0.0) ? -hgap : 0.0;
+            const double factor = (min_gap - hgap) / min_gap;
+            const double lubforce = coeff * (factor * factor / pd.stf0 + overlap / (min_gap * pd.stf1));
+            const double nx = dx / dist, ny = dy / dist, nz = dz / dist;
+            FX += lubforce * nx;
+            FY += lubforce * ny;
+            FZ += lubforce * nz;
+        }
+    }
+    // repulsive_wall_kernel (pforces.cu:40-76)
+    double fx = 0.0, fy = 0.0, fz = 0.0;
+    const double mgw = pd.min_gap_w;
+    auto wall = [&](double gap, double nx, double ny, double nz) {
+        if (gap >= mgw) return;
+        const double overlap = (gap < 0.0) ? -gap : 0.0;
+        const double factor = (mgw - gap) / mgw;
+        const double lubforce = coeff * (factor * factor / pd.stf0_w + overlap / (mgw * pd.stf1_w));
+        fx += lubforce * nx;
+        fy += lubforce * ny;
+        fz += lubforce * nz;
+    };
+    wall(xi - radius, 1.0, 0.0, 0.0);
+    wall((static_cast<double>(d.lx) - xi) - radius, -1.0, 0.0, 0.0);
+    wall(yi - radius, 0.0, 1.0, 0.0);
+    wall((static_cast<double>(d.ly) - yi) - radius, 0.0, -1.0, 0.0);
+    wall(zi - radius, 0.0, 0.0, 1.0);
+    wall((static_cast<double>(d.lz) - zi) - radius, 0.0, 0.0, -1.0);
+    P[RBTP_P_FORCE_X][i] = FX + fx;
+    P[RBTP_P_FORCE_Y][i] = FY + fy;
+    P[RBTP_P_FORCE_Z][i] = FZ + fz;
+}
+
+// ---- link evaluation -----------------------------------------------------------------------------
+
+// Particle state used by the IBB of a link.
+struct PView {
+    const double *px, *py, *pz, *vx, *vy, *vz, *wx, *wy, *wz, *temp;
+};
+
+template <class S>
+struct PopView {
+    const S *f, *g;
+    size_t n;
+    double t0;
+    __device__ inline double F(int i, int idx) const {
+        return dec_f<S, double>(Enc<S>::ld(f + (size_t)i * n + idx), i);
+    }
+    __device__ inline double G(int i, int idx) const {
+        return dec_g<S, double>(Enc<S>::ld(g + (size_t)i * n + idx), i, t0);
+    }
+};
+
+struct LinkOut {
+    double f_in, f_out, g_in, g_out;
+    double relx, rely, relz, uwx, uwy, uwz;
+    bool unhandled_f, unhandled_g;
+};
+
+// bounce_back_particles_kernel (ibb.cu:19-81) and bounce_back_thermal_kernel (ibb.cu:94-135)
+// for one link, reading the post-collision populations; returns the values they write into
+// f[opp][idx] and g[opp][idx] plus what the force and heat sums read (ibb.cu:160-180,202-203).
+template <class S>
+__device__ inline LinkOut eval_link(int ip, int ix, int iy, int iz, double q, int pid,
+                                    const PView &pv, const PopView<S> &pop,
+                                    const int32_t *owner_prev, double density, double rho0,
+                                    const Dims &d) {
+    LinkOut o;
+    const int opp = c_opp[ip];
+    const int idx = iz * d.lxy + iy * d.lx + ix;
+    const int ix1 = ix - c_cx[ip], iy1 = iy - c_cy[ip], iz1 = iz - c_cz[ip];
+    const bool up_in = in_domain(d, ix1, iy1, iz1);
+    const int idx1 = up_in ? iz1 * d.lxy + iy1 * d.lx + ix1 : idx;
+    const bool up_fluid = up_in && owner_prev[idx1] == -1;
+    const bool here_fluid = owner_prev[idx] == -1;
+    const double tp = wt(ip);
+    // wall velocity at the boundary point (ibb.cu:33-45)
+    const double cx = static_cast<double>(ix) + 0.5 + q * c_cx[ip];
+    const double cy = static_cast<double>(iy) + 0.5 + q * c_cy[ip];
+    const double cz = static_cast<double>(iz) + 0.5 + q * c_cz[ip];
+    const double relx = cx - pv.px[pid], rely = cy - pv.py[pid], relz = cz - pv.pz[pid];
+    const double wx = pv.wx[pid], wy = pv.wy[pid], wz = pv.wz[pid];
+    const double uwx = pv.vx[pid] + (relz * wy - rely * wz);
+    const double uwy = pv.vy[pid] + (relx * wz - relz * wx);
+    const double uwz = pv.vz[pid] + (rely * wx - relx * wy);
+    o.relx = relx; o.rely = rely; o.relz = relz;
+    o.uwx = uwx; o.uwy = uwy; o.uwz = uwz;
+    const double evel = c_cx[ip] * uwx + c_cy[ip] * uwy + c_cz[ip] * uwz;
+    const double RT = 1.0 / 3.0;
+    const double momentum_term = 2.0 * tp * rho0 * evel / RT;
+    const double f_here = pop.F(ip, idx);
+    o.f_in = f_here;
+    o.unhandled_f = false;
+    if (up_fluid) {
+        if (q < 0.5) {
+            const double f_upstream = pop.F(ip, idx1);
+            o.f_out = 2.0 * q * f_here + (1.0 - 2.0 * q) * f_upstream - momentum_term;
+        } else {
+            const double f_opposite_collide = pop.F(opp, idx);
+            const double inv_2q = 0.5 / q;
+            o.f_out = (f_here - momentum_term) * inv_2q + (2.0 * q - 1.0) * inv_2q * f_opposite_collide;
+        }
+    } else if (here_fluid) {
+        o.f_out = f_here - momentum_term;
+    } else {
+        o.f_out = f_here;
+        o.unhandled_f = true;
+    }
+    // thermal (ibb.cu:109-135)
+    const double Tp = pv.temp[pid];
+    const double amp = 2.0 * tp * Tp * (1.0 + density);
+    const double g_here = pop.G(ip, idx);
+    o.g_in = g_here;
+    o.unhandled_g = false;
+    if (up_fluid) {
+        if (q <= 0.5) {
+            const double g_upstream = pop.G(ip, idx1);
+            o.g_out = -(2.0 * q * g_here + (1.0 - 2.0 * q) * g_upstream) + amp;
+        } else {
+            const double g_opp_here = pop.G(opp, idx);
+            const double inv_2q = 0.5 / q;
+            o.g_out = -g_here * inv_2q + amp * inv_2q + (2.0 * q - 1.0) * inv_2q * g_opp_here;
+        }
+    } else if (here_fluid) {
+        o.g_out = -g_here + amp;
+    } else {
+        // No write: g[opp][idx] keeps its streamed value, g*_opp(x + c_ip).
+        o.g_out = pop.G(opp, (iz + c_cz[ip]) * d.lxy + (iy + c_cy[ip]) * d.lx + (ix + c_cx[ip]));
+        o.unhandled_g = true;
+    }
+    return o;
+}
+
+// Hydrodynamic force, torque and heat of every particle (compute_particle_forces_kernel,
+// ibb.cu:138-188; compute_particle_heat_kernel, ibb.cu:190-205).  Lane t of particle p's
+// workgroup takes the nodes k = t, t+256, ... of p's link box, each node's links in direction
+// order; the 256 partials are combined pairwise (stride 128 ... 1) and added to the sums left
+// by k_contacts.
+template <class S>
+__global__ __launch_bounds__(PB) void k_link_sums(PartDev pd, PopView<S> pop,
+                                                  const int32_t *owner_prev,
+                                                  const int32_t *box_prev, const double *rho,
+                                                  Dims d) {
+    __shared__ double red[7][PB];
+    const int p = blockIdx.x;
+    const int t = threadIdx.x;
+    double s[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    int lo[3], hi[3];
+    int nunh_f = 0, nunh_g = 0;
+    if (load_box(box_prev, p, lo, hi)) {
+        grow_box(d, lo, hi);
+        const PView pv{pd.p[RBTP_P_POS_X], pd.p[RBTP_P_POS_Y], pd.p[RBTP_P_POS_Z],
+                       pd.p[RBTP_P_VEL_X], pd.p[RBTP_P_VEL_Y], pd.p[RBTP_P_VEL_Z],
+                       pd.p[RBTP_P_OMEGA_X], pd.p[RBTP_P_OMEGA_Y], pd.p[RBTP_P_OMEGA_Z],
+                       pd.p[RBTP_P_TEMP]};
+        const double X = pv.px[p], Y = pv.py[p], Z = pv.pz[p];
+        const int cnt = box_count(lo, hi);
+        for (int k = t; k < cnt; k += PB) {
+            int ix, iy, iz;
+            const int idx = box_node(d, lo, hi, k, ix, iy, iz);
+            if (owner_prev[idx] != -1) continue;
+            const double x0 = static_cast<double>(ix) + 0.5;
+            const double y0 = static_cast<double>(iy) + 0.5;
+            const double z0 = static_cast<double>(iz) + 0.5;
+            for (int ip = 1; ip < NPOP; ip++) {
+                const int nx = ix + c_cx[ip], ny = iy + c_cy[ip], nz = iz + c_cz[ip];
+                if (!in_domain(d, nx, ny, nz)) continue;
+                if (owner_prev[nz * d.lxy + ny * d.lx + nx] != p) continue;
+                double q = 0.0;
+                if (!link_q(ip, x0, y0, z0, X, Y, Z, pd.radius_sq, q)) continue;
+                const LinkOut L = eval_link<S>(ip, ix, iy, iz, q, p, pv, pop, owner_prev,
+                                               rho[idx], pd.rho0, d);
+                nunh_f += L.unhandled_f;
+                nunh_g += L.unhandled_g;
+                const int opp = c_opp[ip];
+                // Force uses the link record's r = q c (links.cu:134-136, ibb.cu:160-168).
+                const double rx = q * static_cast<double>(c_cx[ip]);
+                const double ry = q * static_cast<double>(c_cy[ip]);
+                const double rz = q * static_cast<double>(c_cz[ip]);
+                const double relx = ix + 0.5 + rx - X;
+                const double rely = iy + 0.5 + ry - Y;
+                const double relz = iz + 0.5 + rz - Z;
+                const double Fx = (c_cx[ip] - L.uwx) * L.f_in - (c_cx[opp] - L.uwx) * L.f_out;
+                const double Fy = (c_cy[ip] - L.uwy) * L.f_in - (c_cy[opp] - L.uwy) * L.f_out;
+                const double Fz = (c_cz[ip] - L.uwz) * L.f_in - (c_cz[opp] - L.uwz) * L.f_out;
+                s[0] += Fx;
+                s[1] += Fy;
+                s[2] += Fz;
+                s[3] += rely * Fz - relz * Fy;
+                s[4] += relz * Fx - relx * Fz;
+                s[5] += relx * Fy - rely * Fx;
+                s[6] += L.g_in - L.g_out;
+            }
+        }
+    }
+    if (nunh_f) atomicAdd(pd.err + ERR_UNHANDLED_F, nunh_f);
+    if (nunh_g) atomicAdd(pd.err + ERR_UNHANDLED_G, nunh_g);
+    for (int c = 0; c < 7; c++) red[c][t] = s[c];
+    __syncthreads();
+    for (int stride = PB / 2; stride >= 1; stride >>= 1) {
+        if (t < stride)
+            for (int c = 0; c < 7; c++) red[c][t] = red[c][t] + red[c][t + stride];
+        __syncthreads();
+    }
+    if (t == 0) {
+        pd.p[RBTP_P_FORCE_X][p] += red[0][0];
+        pd.p[RBTP_P_FORCE_Y][p] += red[1][0];
+        pd.p[RBTP_P_FORCE_Z][p] += red[2][0];
+        pd.p[RBTP_P_TORQUE_X][p] += red[3][0];
+        pd.p[RBTP_P_TORQUE_Y][p] += red[4][0];
+        pd.p[RBTP_P_TORQUE_Z][p] += red[5][0];
+        pd.p[RBTP_P_HEAT][p] += red[6][0];
+    }
+}
+
+// ---- motion -------------------------------------------------------------------------------------
+
+// update_particles_kernel (move.cu:5-103), after saving the state the IBB of this step used.
+__global__ __launch_bounds__(128) void k_move(PartDev pd) {
+    const int pid = blockIdx.x * 128 + threadIdx.x;
+    if (pid >= pd.np) return;
+    double **P = pd.p;
+    const int src[OLD_COUNT] = {RBTP_P_POS_X, RBTP_P_POS_Y, RBTP_P_POS_Z, RBTP_P_VEL_X,
+                                RBTP_P_VEL_Y, RBTP_P_VEL_Z, RBTP_P_OMEGA_X, RBTP_P_OMEGA_Y,
+                                RBTP_P_OMEGA_Z, RBTP_P_TEMP};
+    for (int k = 0; k < OLD_COUNT; k++) pd.old[k][pid] = P[src[k]][pid];
+    const double dt = 1.0;
+    const double mass = pd.mass, inertia = pd.inertia, heat_capacity = pd.heat_capacity;
+    const double forceX = P[RBTP_P_FORCE_X][pid], forceY = P[RBTP_P_FORCE_Y][pid], forceZ = P[RBTP_P_FORCE_Z][pid];
+    const double torqueX = P[RBTP_P_TORQUE_X][pid], torqueY = P[RBTP_P_TORQUE_Y][pid], torqueZ = P[RBTP_P_TORQUE_Z][pid];
+    const double fx_prev = P[RBTP_P_FORCE_PREV_X][pid], fy_prev = P[RBTP_P_FORCE_PREV_Y][pid], fz_prev = P[RBTP_P_FORCE_PREV_Z][pid];
+    const double fx_prev2 = P[RBTP_P_FORCE_PREV2_X][pid], fy_prev2 = P[RBTP_P_FORCE_PREV2_Y][pid], fz_prev2 = P[RBTP_P_FORCE_PREV2_Z][pid];
+    const double tpx = P[RBTP_P_TORQUE_PREV_X][pid], tpy = P[RBTP_P_TORQUE_PREV_Y][pid], tpz = P[RBTP_P_TORQUE_PREV_Z][pid];
+    const double tp2x = P[RBTP_P_TORQUE_PREV2_X][pid], tp2y = P[RBTP_P_TORQUE_PREV2_Y][pid], tp2z = P[RBTP_P_TORQUE_PREV2_Z][pid];
+    const double buoyancy = (1.0 - 1.0 / pd.rho_ratio) * pd.grav0;
+    const double vx_old = P[RBTP_P_VEL_X][pid], vy_old = P[RBTP_P_VEL_Y][pid], vz_old = P[RBTP_P_VEL_Z][pid];
+    const double omega_x_old = P[RBTP_P_OMEGA_X][pid], omega_y_old = P[RBTP_P_OMEGA_Y][pid], omega_z_old = P[RBTP_P_OMEGA_Z][pid];
+    const double combo_fx = 0.5 * forceX + fx_prev + 0.5 * fx_prev2;
+    const double combo_fy = 0.5 * forceY + fy_prev + 0.5 * fy_prev2;
+    const double combo_fz = 0.5 * forceZ + fz_prev + 0.5 * fz_prev2;
+    const double combo_torque_x = 0.5 * torqueX + tpx + 0.5 * tp2x;
+    const double combo_torque_y = 0.5 * torqueY + tpy + 0.5 * tp2y;
+    const double combo_torque_z = 0.5 * torqueZ + tpz + 0.5 * tp2z;
+    const double vx_new = vx_old + 0.5 * dt * combo_fx / mass + dt * buoyancy;
+    const double vy_new = vy_old + 0.5 * dt * combo_fy / mass;
+    const double vz_new = vz_old + 0.5 * dt * combo_fz / mass;
+    const double omega_x_new = omega_x_old + 0.5 * dt * combo_torque_x / inertia;
+    const double omega_y_new = omega_y_old + 0.5 * dt * combo_torque_y / inertia;
+    const double omega_z_new = omega_z_old + 0.5 * dt * combo_torque_z / inertia;
+    P[RBTP_P_POS_X][pid] = P[RBTP_P_POS_X][pid] + 0.5 * dt * (vx_new + vx_old);
+    P[RBTP_P_POS_Y][pid] = P[RBTP_P_POS_Y][pid] + 0.5 * dt * (vy_new + vy_old);
+    P[RBTP_P_POS_Z][pid] = P[RBTP_P_POS_Z][pid] + 0.5 * dt * (vz_new + vz_old);
+    P[RBTP_P_VEL_X][pid] = vx_new;
+    P[RBTP_P_VEL_Y][pid] = vy_new;
+    P[RBTP_P_VEL_Z][pid] = vz_new;
+    P[RBTP_P_OMEGA_X][pid] = omega_x_new;
+    P[RBTP_P_OMEGA_Y][pid] = omega_y_new;
+    P[RBTP_P_OMEGA_Z][pid] = omega_z_new;
+    double theta_x = P[RBTP_P_THETA_X][pid], theta_y = P[RBTP_P_THETA_Y][pid], theta_z = P[RBTP_P_THETA_Z][pid];
+    theta_x += 0.5 * dt * (omega_x_new + omega_x_old);
+    theta_y += 0.5 * dt * (omega_y_new + omega_y_old);
+    theta_z += 0.5 * dt * (omega_z_new + omega_z_old);
+    P[RBTP_P_THETA_X][pid] = theta_x;
+    P[RBTP_P_THETA_Y][pid] = theta_y;
+    P[RBTP_P_THETA_Z][pid] = theta_z;
+    const double heat = P[RBTP_P_HEAT][pid], heat_prev = P[RBTP_P_HEAT_PREV][pid];
+    double T = P[RBTP_P_TEMP][pid];
+    T += 0.5 * dt * (heat + heat_prev) / heat_capacity;
+    P[RBTP_P_TEMP][pid] = T;
+}
+
+// ---- special nodes ------------------------------------------------------------------------------
+
+__global__ void k_zero_count(int *count) { *count = 0; }
+
+__device__ inline void push_special(const PartDev &pd, int idx, int kind) {
+    const int j = atomicAdd(pd.sp.count, 1);
+    if (j < pd.sp.cap) {
+        pd.sp.node[j] = idx;
+        pd.sp_kind[j] = kind;
+    } else {
+        atomicOr(pd.err + ERR_SPECIAL_CAP, 1);
+    }
+}
+
+// Link nodes of the previous geometry (the nodes bounce_back_particles writes).  A node with
+// links to several particles is listed once, by the lowest linked pid.
+__global__ __launch_bounds__(PB) void k_special_links(PartDev pd, const int32_t *owner_prev,
+                                                      const int32_t *box_prev, Dims d) {
+    const int p = blockIdx.x;
+    int lo[3], hi[3];
+    if (!load_box(box_prev, p, lo, hi)) return;
+    grow_box(d, lo, hi);
+    const double *OX = pd.old[OLD_PX], *OY = pd.old[OLD_PY], *OZ = pd.old[OLD_PZ];
+    const int cnt = box_count(lo, hi);
+    for (int k = threadIdx.x; k < cnt; k += PB) {
+        int ix, iy, iz;
+        const int idx = box_node(d, lo, hi, k, ix, iy, iz);
+        if (owner_prev[idx] != -1) continue;
+        const double x0 = static_cast<double>(ix) + 0.5;
+        const double y0 = static_cast<double>(iy) + 0.5;
+        const double z0 = static_cast<double>(iz) + 0.5;
+        int minpid = INT_MAX;
+        for (int ip = 1; ip < NPOP; ip++) {
+            const int nx = ix + c_cx[ip], ny = iy + c_cy[ip], nz = iz + c_cz[ip];
+            if (!in_domain(d, nx, ny, nz)) continue;
+            const int q_owner = owner_prev[nz * d.lxy + ny * d.lx + nx];
+            if (q_owner < 0 || q_owner >= minpid) continue;
+            double q = 0.0;
+            if (link_q(ip, x0, y0, z0, OX[q_owner], OY[q_owner], OZ[q_owner], pd.radius_sq, q))
+                minpid = q_owner;
+        }
+        if (minpid == p) push_special(pd, idx, -1);
+    }
+}
+
+// New-fluid nodes (new_fluid_count_kernel, refill.cu:8-41): solid in the previous geometry,
+// fluid now, previous owner valid, and within close_thresh of that owner's new surface.
+__global__ __launch_bounds__(PB) void k_special_refill(PartDev pd, const int32_t *owner_prev,
+                                                       const int32_t *box_prev,
+                                                       const int32_t *owner_curr, Dims d) {
+    const int p = blockIdx.x;
+    int lo[3], hi[3];
+    if (!load_box(box_prev, p, lo, hi)) return;
+    const double PXv = pd.p[RBTP_P_POS_X][p], PYv = pd.p[RBTP_P_POS_Y][p], PZv = pd.p[RBTP_P_POS_Z][p];
+    const int cnt = box_count(lo, hi);
+    for (int k = threadIdx.x; k < cnt; k += PB) {
+        int ix, iy, iz;
+        const int idx = box_node(d, lo, hi, k, ix, iy, iz);
+        if (owner_prev[idx] != p || owner_curr[idx] != -1) continue;
+        const double x0 = static_cast<double>(ix) + 0.5;
+        const double y0 = static_cast<double>(iy) + 0.5;
+        const double z0 = static_cast<double>(iz) + 0.5;
+        const double dx = x0 - PXv, dy = y0 - PYv, dz = z0 - PZv;
+        const double dist = sqrt(dx * dx + dy * dy + dz * dz);
+        const double close_thresh = 1.7606816861659007;
+        if (dist - pd.radius <= close_thresh) push_special(pd, idx, p);
+    }
+}
+
+// Corrected post-stream populations of the special nodes.
+//   link node: pull from the post-collision state (streaming rules), then replace f_opp and
+//              g_opp of every link by the IBB values (bounce_back_particles /
+//              bounce_back_thermal_particles, with the state before this step's motion);
+//   new fluid: refill_new_fluid_kernel (refill.cu:78-240) from the post-collision state at
+//              neighbours that were fluid in the previous geometry, with the moved state for
+//              the fallback equilibrium; it also writes rho, u, phi and F = 0 at the node.
+template <class S>
+__global__ __launch_bounds__(128) void k_prefixup(PartDev pd, PopView<S> pop,
+                                                  const int32_t *owner_prev,
+                                                  typename Enc<S>::A *fx_field,
+                                                  const typename Enc<S>::A *rho_wall,
+                                                  double *macro_rho, double *macro_ux,
+                                                  double *macro_uy, double *macro_uz,
+                                                  double *macro_phi, Dims d) {
+    const int count = min(*pd.sp.count, pd.sp.cap);
+    const size_t cap = (size_t)pd.sp.cap;
+    for (int j = blockIdx.x * 128 + threadIdx.x; j < count; j += gridDim.x * 128) {
+        const int idx = pd.sp.node[j];
+        const int kind = pd.sp_kind[j];
+        const int iz = idx / d.lxy;
+        const int rem = idx - iz * d.lxy;
+        const int iy = rem / d.lx;
+        const int ix = rem - iy * d.lx;
+        double *out = pd.sp.pop;
+        if (kind < 0) {
+            double rho_w = 0.0;
+            if (ix == 0) rho_w = double(rho_wall[iz * d.ly + iy]);
+            else if (ix == d.lx - 1) rho_w = double(rho_wall[d.ly * d.lz + iz * d.ly + iy]);
+            double f[NPOP], g[NPOP];
+            gather<S, double, false>(pop.f, pop.g, pop.n, idx, ix, iy, iz, d, rho_w, pop.t0,
+                                     pd.thot, pd.tcold, f, g);
+            const PView pv{pd.old[OLD_PX], pd.old[OLD_PY], pd.old[OLD_PZ], pd.old[OLD_VX],
+                           pd.old[OLD_VY], pd.old[OLD_VZ], pd.old[OLD_WX], pd.old[OLD_WY],
+                           pd.old[OLD_WZ], pd.old[OLD_T]};
+            const double x0 = static_cast<double>(ix) + 0.5;
+            const double y0 = static_cast<double>(iy) + 0.5;
+            const double z0 = static_cast<double>(iz) + 0.5;
+            const double density = macro_rho[idx];
+            for (int ip = 1; ip < NPOP; ip++) {
+                const int nx = ix + c_cx[ip], ny = iy + c_cy[ip], nz = iz + c_cz[ip];
+                if (!in_domain(d, nx, ny, nz)) continue;
+                const int q_owner = owner_prev[nz * d.lxy + ny * d.lx + nx];
+                if (q_owner < 0) continue;
+                double q = 0.0;
+                if (!link_q(ip, x0, y0, z0, pv.px[q_owner], pv.py[q_owner], pv.pz[q_owner],
+                            pd.radius_sq, q))
+                    continue;
+                const LinkOut L = eval_link<S>(ip, ix, iy, iz, q, q_owner, pv, pop, owner_prev,
+                                               density, pd.rho0, d);
+                const int opp = c_opp[ip];
+                f[opp] = L.f_out;
+                g[opp] = L.g_out;
+            }
+            for (int i = 0; i < NPOP; i++) {
+                out[(size_t)i * cap + j] = f[i];
+                out[(size_t)(NPOP + i) * cap + j] = g[i];
+            }
+            pd.sp.fx_old[j] = double(fx_field[idx]);
+        } else {
+            const int pid = kind;
+            const double lambda0 = 1.0 - wt(0);
+            const double RT = 1.0 / 3.0;
+            double rho_loc = 0.0, mom_x = 0.0, mom_y = 0.0, mom_z = 0.0, phi_loc = 0.0;
+            for (int ip = 0; ip < NPOP; ++ip) {
+                double temp_f = 0.0, temp_g = 0.0;
+                int sample_count = 0;
+                for (int ip2 = 1; ip2 < NPOP; ++ip2) {
+                    const int ix1 = ix + c_cx[ip2], iy1 = iy + c_cy[ip2], iz1 = iz + c_cz[ip2];
+                    if (!in_domain(d, ix1, iy1, iz1)) continue;
+                    const int idx1 = iz1 * d.lxy + iy1 * d.lx + ix1;
+                    if (owner_prev[idx1] != -1) continue;
+                    const int ix2 = ix + 2 * c_cx[ip2], iy2 = iy + 2 * c_cy[ip2], iz2 = iz + 2 * c_cz[ip2];
+                    if (in_domain(d, ix2, iy2, iz2)) {
+                        const int idx2 = iz2 * d.lxy + iy2 * d.lx + ix2;
+                        if (owner_prev[idx2] == -1) {
+                            temp_f += 2.0 * pop.F(ip, idx1) - pop.F(ip, idx2);
+                            temp_g += 2.0 * pop.G(ip, idx1) - pop.G(ip, idx2);
+                            ++sample_count;
+                            continue;
+                        }
+                    }
+                    temp_f += pop.F(ip, idx1);
+                    temp_g += pop.G(ip, idx1);
+                    ++sample_count;
+                }
+                if (sample_count == 0) {
+                    const double px = pd.p[RBTP_P_POS_X][pid], py = pd.p[RBTP_P_POS_Y][pid], pz = pd.p[RBTP_P_POS_Z][pid];
+                    const double vx = pd.p[RBTP_P_VEL_X][pid], vy = pd.p[RBTP_P_VEL_Y][pid], vz = pd.p[RBTP_P_VEL_Z][pid];
+                    const double omega_x = pd.p[RBTP_P_OMEGA_X][pid], omega_y = pd.p[RBTP_P_OMEGA_Y][pid], omega_z = pd.p[RBTP_P_OMEGA_Z][pid];
+                    const double tempT = pd.p[RBTP_P_TEMP][pid];
+                    const double x = static_cast<double>(ix) + 0.5;
+                    const double y = static_cast<double>(iy) + 0.5;
+                    const double z = static_cast<double>(iz) + 0.5;
+                    const double relx = x - px, rely = y - py, relz = z - pz;
+                    const double u = vx + (relz * omega_y - rely * omega_z);
+                    const double v = vy + (relx * omega_z - relz * omega_x);
+                    const double w = vz + (rely * omega_x - relx * omega_y);
+                    const double r = pd.rho0;
+                    const double eu = c_cx[ip] * u + c_cy[ip] * v + c_cz[ip] * w;
+                    const double uv = (u * u + v * v + w * w) / RT;
+                    temp_f = wt(ip) * (r + eu + 0.5 * (eu * eu - uv));
+                    temp_g = wt(ip) * tempT * (1.0 + eu + 0.5 * (eu * eu - uv));
+                    if (ip == 0) temp_g -= lambda0 * tempT * r;
+                    else temp_g += wt(ip) * tempT * r;
+                    sample_count = 1;
+                }
+                const double inv_count = 1.0 / static_cast<double>(sample_count);
+                const double f_val = temp_f * inv_count;
+                const double g_val = temp_g * inv_count;
+                out[(size_t)ip * cap + j] = f_val;
+                out[(size_t)(NPOP + ip) * cap + j] = g_val;
+                rho_loc += f_val;
+                mom_x += c_cx[ip] * f_val;
+                mom_y += c_cy[ip] * f_val;
+                mom_z += c_cz[ip] * f_val;
+                phi_loc += g_val;
+            }
+            macro_rho[idx] = rho_loc;
+            if (rho_loc > 1e-12) {
+                const double inv_rho = 1.0 / rho_loc;
+                macro_ux[idx] = mom_x * inv_rho;
+                macro_uy[idx] = mom_y * inv_rho;
+                macro_uz[idx] = mom_z * inv_rho;
+            } else {
+                macro_ux[idx] = 0.0;
+                macro_uy[idx] = 0.0;
+                macro_uz[idx] = 0.0;
+            }
+            macro_phi[idx] = phi_loc;
+            fx_field[idx] = typename Enc<S>::A(0.0);
+            pd.sp.fx_old[j] = 0.0;
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_overwrite_special(PartDev pd, double *f, double *g, Dims d) {
+    const int count = min(*pd.sp.count, pd.sp.cap);
+    const size_t cap = (size_t)pd.sp.cap, n = (size_t)d.n;
+    for (int j = blockIdx.x * 256 + threadIdx.x; j < count; j += gridDim.x * 256) {
+        const int idx = pd.sp.node[j];
+        for (int i = 0; i < NPOP; i++) {
+            f[(size_t)i * n + idx] = pd.sp.pop[(size_t)i * cap + j];
+            g[(size_t)i * n + idx] = pd.sp.pop[(size_t)(NPOP + i) * cap + j];
+        }
+    }
+}
+
+// Link records of the newest geometry in the reference layout (links.cu:89-139), unordered.
+__global__ __launch_bounds__(PB) void k_collect_links(PartDev pd, const int32_t *owner,
+                                                      const int32_t *box, Dims d,
+                                                      rbtp_link *out, int cap, int *count) {
+    const int p = blockIdx.x;
+    int lo[3], hi[3];
+    if (!load_box(box, p, lo, hi)) return;
+    grow_box(d, lo, hi);
+    const double X = pd.p[RBTP_P_POS_X][p], Y = pd.p[RBTP_P_POS_Y][p], Z = pd.p[RBTP_P_POS_Z][p];
+    const int cnt = box_count(lo, hi);
+    for (int k = threadIdx.x; k < cnt; k += PB) {
+        int ix, iy, iz;
+        const int idx = box_node(d, lo, hi, k, ix, iy, iz);
+        if (owner[idx] != -1) continue;
+        const double x0 = static_cast<double>(ix) + 0.5;
+        const double y0 = static_cast<double>(iy) + 0.5;
+        const double z0 = static_cast<double>(iz) + 0.5;
+        for (int ip = 1; ip < NPOP; ip++) {
+            const int nx = ix + c_cx[ip], ny = iy + c_cy[ip], nz = iz + c_cz[ip];
+            if (!in_domain(d, nx, ny, nz)) continue;
+            if (owner[nz * d.lxy + ny * d.lx + nx] != p) continue;
+            double q = 0.0;
+            if (!link_q(ip, x0, y0, z0, X, Y, Z, pd.radius_sq, q)) continue;
+            const int j = atomicAdd(count, 1);
+            if (j >= cap) continue;
+            rbtp_link L;
+            L.cell_i = ix; L.cell_j = iy; L.cell_k = iz; L.dir = ip; L.pid = p; L.pad = 0;
+            L.q = q;
+            L.rx = q * static_cast<double>(c_cx[ip]);
+            L.ry = q * static_cast<double>(c_cy[ip]);
+            L.rz = q * static_cast<double>(c_cz[ip]);
+            out[j] = L;
+        }
+    }
+}
+
+template <class F>
+hipError_t dispatch_prec(int prec, F &&fn) {
+    switch (prec) {
+        case RBTP_PREC_FP64: return fn((double *)nullptr);
+        case RBTP_PREC_FP32: return fn((float *)nullptr);
+        case RBTP_PREC_FP16S: return fn((__half *)nullptr);
+        default: return fn((__hip_bfloat16 *)nullptr);
+    }
+}
+
+#define PHIP(call)                                   \
+    do {                                             \
+        hipError_t e_ = (call);                      \
+        if (e_ != hipSuccess) return report_hip(e_, #call); \
+    } while (0)
+
+int grid_for(int count, int block, int maxg) {
+    int g = (count + block - 1) / block;
+    if (g < 1) g = 1;
+    return g > maxg ? maxg : g;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------------------------
+// Host side
+// ---------------------------------------------------------------------------------------------
+
+int part_alloc(PartDev &pd, const rbtp_case &c, const rbtp_derived &d, const Dims &dims) {
+    std::memset(&pd, 0, sizeof(pd));
+    pd.np = d.npart;
+    pd.radius = d.particle_radius;
+    pd.radius_sq = d.particle_radius * d.particle_radius;
+    pd.mass = d.particle_mass;
+    pd.inertia = d.particle_inertia;
+    pd.heat_capacity = d.particle_heat_capacity;
+    pd.rho_ratio = c.rho_particle_ratio;
+    pd.grav0 = d.grav0;
+    pd.coeff = d.particle_mass * c.repforce_scale;
+    pd.min_gap = c.rep_min_gap;
+    pd.min_gap_w = c.rep_min_gap_w;
+    pd.stf0 = c.rep_stf0;
+    pd.stf1 = c.rep_stf1;
+    pd.stf0_w = c.rep_stf0_w;
+    pd.stf1_w = c.rep_stf1_w;
+    pd.rho0 = c.rho0;
+    pd.t0 = c.t_ref;
+    pd.thot = c.t_hot;
+    pd.tcold = c.t_cold;
+    const size_t np = (size_t)pd.np;
+    PHIP(hipMalloc(&pd.base, (RBTP_P_COUNT + OLD_COUNT) * np * sizeof(double)));
+    PHIP(hipMemset(pd.base, 0, (RBTP_P_COUNT + OLD_COUNT) * np * sizeof(double)));
+    for (int k = 0; k < RBTP_P_COUNT; k++) pd.p[k] = pd.base + (size_t)k * np;
+    for (int k = 0; k < OLD_COUNT; k++) pd.old[k] = pd.base + (size_t)(RBTP_P_COUNT + k) * np;
+    for (int b = 0; b < 2; b++) {
+        PHIP(hipMalloc(&pd.owner[b], (size_t)dims.n * sizeof(int32_t)));
+        PHIP(hipMalloc(&pd.box[b], 6 * np * sizeof(int32_t)));
+    }
+    // Marking boxes span at most ceil(2R)+1 nodes per axis; link boxes two more.
+    pd.box_w = (int)std::ceil(2.0 * pd.radius) + 1;
+    const long long lb = (long long)(pd.box_w + 2) * (pd.box_w + 2) * (pd.box_w + 2);
+    const long long mb = (long long)pd.box_w * pd.box_w * pd.box_w;
+    long long cap = (long long)np * (lb + mb);
+    if (cap > dims.n) cap = dims.n;
+    if (cap < 1) cap = 1;
+    pd.sp.cap = (int)cap;
+    PHIP(hipMalloc(&pd.sp.node, cap * sizeof(int32_t)));
+    PHIP(hipMalloc(&pd.sp_kind, cap * sizeof(int32_t)));
+    PHIP(hipMalloc(&pd.sp.pop, (size_t)2 * NPOP * cap * sizeof(double)));
+    PHIP(hipMalloc(&pd.sp.fx_old, cap * sizeof(double)));
+    PHIP(hipMalloc(&pd.sp.count, sizeof(int)));
+    PHIP(hipMemset(pd.sp.count, 0, sizeof(int)));
+    PHIP(hipMalloc(&pd.err, ERR_COUNT * sizeof(int32_t)));
+    PHIP(hipMemset(pd.err, 0, ERR_COUNT * sizeof(int32_t)));
+    return RBTP_OK;
+}
+
+void part_free(PartDev &pd) {
+    (void)hipFree(pd.base);
+    for (int b = 0; b < 2; b++) {
+        (void)hipFree(pd.owner[b]);
+        (void)hipFree(pd.box[b]);
+    }
+    (void)hipFree(pd.sp.node);
+    (void)hipFree(pd.sp_kind);
+    (void)hipFree(pd.sp.pop);
+    (void)hipFree(pd.sp.fx_old);
+    (void)hipFree(pd.sp.count);
+    (void)hipFree(pd.err);
+    std::memset(&pd, 0, sizeof(pd));
+}
+
+// generate_initial_positions (pinit.cu:97-144): glibc srand/rand rejection sampling.
+static int place_particles(const rbtp_case &c, const rbtp_derived &d, std::vector<double> &X,
+                           std::vector<double> &Y, std::vector<double> &Z) {
+    const int np = d.npart;
+    const double radius = d.particle_radius;
+    std::srand(c.particle_rand_seed);
+    const double min_pos_x = 1.5 * radius, min_pos_y = 1.5 * radius, min_pos_z = 1.5 * radius;
+    const double span_x = static_cast<double>(c.lx) - 3.0 * radius;
+    const double span_y = static_cast<double>(c.ly) - 3.0 * radius;
+    const double span_z = static_cast<double>(c.lz) - 3.0 * radius;
+    long counter = 0;
+    for (int p = 0; p < np; ++p) {
+        while (true) {
+            const double rx = static_cast<double>(std::rand()) / static_cast<double>(RAND_MAX);
+            const double ry = static_cast<double>(std::rand()) / static_cast<double>(RAND_MAX);
+            const double rz = static_cast<double>(std::rand()) / static_cast<double>(RAND_MAX);
+            const double x = min_pos_x + rx * span_x;
+            const double y = min_pos_y + ry * span_y;
+            const double z = min_pos_z + rz * span_z;
+            bool ok = true;
+            for (int q = 0; q < p; ++q) {
+                const double dx = x - X[q], dy = y - Y[q], dz = z - Z[q];
+                const double dist = std::sqrt(dx * dx + dy * dy + dz * dz);
+                if (dist - 2.0 * radius < radius / 3.0) {
+                    counter++;
+                    if (counter > 1000000)
+                        return report(RBTP_E_PLACEMENT,
+                                      "Can't distribute the particles, iter. limit=1000000");
+                    ok = false;
+                    break;
+                }
+            }
+            if (ok) {
+                X[p] = x; Y[p] = y; Z[p] = z;
+                break;
+            }
+        }
+    }
+    return RBTP_OK;
+}
+
+// init_particles (pinit.cu:183-295) and build_initial_ibnode (pinit.cu:146-179); both geometry
+// generations start as the initial one.
+int part_init(PartDev &pd, const rbtp_case &c, const rbtp_derived &d, const Dims &dims,
+              const double *ppos, const double *ptemp, hipStream_t st) {
+    const int np = pd.np;
+    std::vector<double> host((size_t)(RBTP_P_COUNT + OLD_COUNT) * np, 0.0);
+    std::vector<double> X(np), Y(np), Z(np);
+    if (ppos) {
+        for (int p = 0; p < np; p++) {
+            X[p] = ppos[p];
+            Y[p] = ppos[np + p];
+            Z[p] = ppos[2 * np + p];
+        }
+    } else {
+        const int r = place_particles(c, d, X, Y, Z);
+        if (r != RBTP_OK) return r;
+    }
+    for (int p = 0; p < np; p++) {
+        host[(size_t)RBTP_P_POS_X * np + p] = X[p];
+        host[(size_t)RBTP_P_POS_Y * np + p] = Y[p];
+        host[(size_t)RBTP_P_POS_Z * np + p] = Z[p];
+        host[(size_t)RBTP_P_TEMP * np + p] = ptemp ? ptemp[p] : c.particle_init_temp;
+    }
+    PHIP(hipMemcpyAsync(pd.base, host.data(), host.size() * sizeof(double), hipMemcpyHostToDevice, st));
+    PHIP(hipMemsetAsync(pd.owner[0], 0xFF, (size_t)dims.n * sizeof(int32_t), st));
+    hipLaunchKernelGGL(k_geo_mark, np, PB, 0, st, pd.owner[0], pd.box[0], pd.p[RBTP_P_POS_X],
+                       pd.p[RBTP_P_POS_Y], pd.p[RBTP_P_POS_Z], pd.radius, pd.radius_sq, dims);
+    PHIP(hipGetLastError());
+    PHIP(hipMemcpyAsync(pd.owner[1], pd.owner[0], (size_t)dims.n * sizeof(int32_t), hipMemcpyDeviceToDevice, st));
+    PHIP(hipMemcpyAsync(pd.box[1], pd.box[0], 6 * (size_t)np * sizeof(int32_t), hipMemcpyDeviceToDevice, st));
+    PHIP(hipMemsetAsync(pd.err, 0, ERR_COUNT * sizeof(int32_t), st));
+    PHIP(hipMemsetAsync(pd.sp.count, 0, sizeof(int), st));
+    pd.geo_cur = 0;
+    pd.special_pending = false;
+    return RBTP_OK;
+}
+
+// One particle phase (main.cu:52-84 minus the lattice passes), enqueued on st.
+int part_phase(PartDev &pd, const rbtp_case &c, const rbtp_derived &d, const Dims &dims,
+               int prec, const void *fcur, const void *gcur, void *fx, void *rho_wall,
+               double *const *macro, hipStream_t st) {
+    (void)c;
+    (void)d;
+    const int np = pd.np;
+    const int prev = pd.geo_cur, curr = 1 - pd.geo_cur;
+    hipLaunchKernelGGL(k_contacts, (np + 127) / 128, 128, 0, st, pd, dims);
+    PHIP(hipGetLastError());
+    PHIP(dispatch_prec(prec, [&](auto *tag) {
+        using S = std::remove_pointer_t<decltype(tag)>;
+        const PopView<S> pop{(const S *)fcur, (const S *)gcur, (size_t)dims.n, pd.t0};
+        hipLaunchKernelGGL((k_link_sums<S>), np, PB, 0, st, pd, pop, pd.owner[prev],
+                           pd.box[prev], macro[0], dims);
+        return hipGetLastError();
+    }));
+    hipLaunchKernelGGL(k_move, (np + 127) / 128, 128, 0, st, pd);
+    PHIP(hipGetLastError());
+    hipLaunchKernelGGL(k_geo_clear, np, PB, 0, st, pd.owner[curr], pd.box[curr], dims);
+    PHIP(hipGetLastError());
+    hipLaunchKernelGGL(k_geo_mark, np, PB, 0, st, pd.owner[curr], pd.box[curr], pd.p[RBTP_P_POS_X],
+                       pd.p[RBTP_P_POS_Y], pd.p[RBTP_P_POS_Z], pd.radius, pd.radius_sq, dims);
+    PHIP(hipGetLastError());
+    hipLaunchKernelGGL(k_zero_count, 1, 1, 0, st, pd.sp.count);
+    PHIP(hipGetLastError());
+    hipLaunchKernelGGL(k_special_links, np, PB, 0, st, pd, pd.owner[prev], pd.box[prev], dims);
+    PHIP(hipGetLastError());
+    hipLaunchKernelGGL(k_special_refill, np, PB, 0, st, pd, pd.owner[prev], pd.box[prev],
+                       pd.owner[curr], dims);
+    PHIP(hipGetLastError());
+    PHIP(dispatch_prec(prec, [&](auto *tag) {
+        using S = std::remove_pointer_t<decltype(tag)>;
+        using A = typename Enc<S>::A;
+        const PopView<S> pop{(const S *)fcur, (const S *)gcur, (size_t)dims.n, pd.t0};
+        hipLaunchKernelGGL((k_prefixup<S>), grid_for(pd.sp.cap, 128, 2048), 128, 0, st, pd, pop,
+                           pd.owner[prev], (A *)fx, (const A *)rho_wall, macro[0], macro[1],
+                           macro[2], macro[3], macro[4], dims);
+        return hipGetLastError();
+    }));
+    pd.geo_cur = curr;
+    pd.special_pending = true;
+    return RBTP_OK;
+}
+
+int part_check_errors(PartDev &pd) {
+    int32_t h[ERR_COUNT];
+    PHIP(hipMemcpy(h, pd.err, sizeof(h), hipMemcpyDeviceToHost));
+    if (h[ERR_SPECIAL_CAP])
+        return report(RBTP_E_CAPACITY, "special-node list overflowed its capacity");
+    if (h[ERR_CONTACT_CAP])
+        return report(RBTP_E_CAPACITY, "contact list overflowed its capacity");
+    return RBTP_OK;
+}
+
+hipError_t part_overwrite_special(PartDev &pd, double *f, double *g, const Dims &dims,
+                                  hipStream_t st) {
+    if (!pd.special_pending) return hipSuccess;
+    hipLaunchKernelGGL(k_overwrite_special, grid_for(pd.sp.cap, 256, 1024), 256, 0, st, pd, f, g, dims);
+    return hipGetLastError();
+}
+
+int part_collect_links(PartDev &pd, const Dims &dims, hipStream_t st, std::vector<rbtp_link> &out) {
+    int *count = nullptr;
+    PHIP(hipMalloc(&count, sizeof(int)));
+    int total = 0;
+    hipError_t e = hipMemsetAsync(count, 0, sizeof(int), st);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_collect_links, pd.np, PB, 0, st, pd, pd.owner[pd.geo_cur],
+                           pd.box[pd.geo_cur], dims, (rbtp_link *)nullptr, 0, count);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(&total, count, sizeof(int), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    rbtp_link *buf = nullptr;
+    if (e == hipSuccess && total > 0) e = hipMalloc(&buf, (size_t)total * sizeof(rbtp_link));
+    if (e == hipSuccess && total > 0) {
+        e = hipMemsetAsync(count, 0, sizeof(int), st);
+        if (e == hipSuccess) {
+            hipLaunchKernelGGL(k_collect_links, pd.np, PB, 0, st, pd, pd.owner[pd.geo_cur],
+                               pd.box[pd.geo_cur], dims, buf, total, count);
+            e = hipGetLastError();
+        }
+        out.resize(total);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(out.data(), buf, (size_t)total * sizeof(rbtp_link), hipMemcpyDeviceToHost, st);
+        if (e == hipSuccess) e = hipStreamSynchronize(st);
+    }
+    (void)hipFree(buf);
+    (void)hipFree(count);
+    if (e != hipSuccess) return report_hip(e, "collect links");
+    return RBTP_OK;
+}
+
+int part_collect_new_fluid(PartDev &pd, hipStream_t st, std::vector<std::pair<int32_t, int32_t>> &out) {
+    out.clear();
+    if (!pd.special_pending) return RBTP_OK;
+    int count = 0;
+    PHIP(hipMemcpyAsync(&count, pd.sp.count, sizeof(int), hipMemcpyDeviceToHost, st));
+    PHIP(hipStreamSynchronize(st));
+    count = std::min(count, pd.sp.cap);
+    std::vector<int32_t> nodes(count), kinds(count);
+    if (count > 0) {
+        PHIP(hipMemcpy(nodes.data(), pd.sp.node, count * sizeof(int32_t), hipMemcpyDeviceToHost));
+        PHIP(hipMemcpy(kinds.data(), pd.sp_kind, count * sizeof(int32_t), hipMemcpyDeviceToHost));
+    }
+    for (int j = 0; j < count; j++)
+        if (kinds[j] >= 0) out.emplace_back(nodes[j], kinds[j]);
+    return RBTP_OK;
+}
+
+}  // namespace rbtp

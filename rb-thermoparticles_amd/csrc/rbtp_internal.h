@@ -1,0 +1,87 @@
+// rbtp_internal.h — types shared by the HIP translation units of librbtp.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <hip/hip_fp16.h>
+#include <hip/hip_bf16.h>
+#include <cstdint>
+#include <cstddef>
+
+#include "d3q27.h"
+
+namespace rbtp {
+
+// Lattice shape.  n = lx*ly*lz must stay below 2^31 (checked by rbtp_create).
+struct Dims {
+    int lx, ly, lz, lxy;
+    int n;
+};
+
+// Per-case fluid constants, evaluated on the host in the reference's operation order.
+struct FluidK {
+    double tau;          // 3*visc + 0.5                       (globals.cu:29)
+    double tauc;         // 3*diff + 0.5                       (globals.cu:32)
+    double gb;           // grav0 * beta, the first product of forcing (macrovar.cu:44)
+    double t0;           // T_REF
+    double thot, tcold;  // wall temperatures
+    double force_coeff;  // 1.0 - 0.5/tau                      (collision.cu:18)
+    double lambda0;      // 1.0 - w0                           (collision.cu:55)
+    double ralph_coeff;  // 1.0 - 1.0/(2.0*tauc)               (collision.cu:72)
+    double c2t;          // (2.0*tauc)/3.0, second term of grad_denom (collision.cu:73)
+    double inv_tau, inv_tauc;
+};
+
+// Storage encodings of the populations.
+//   double/float: the reference's value.
+//   half/bf16:    value/w_i for f; value/w_i - t0 for g (shifted and scaled).
+template <class S> struct Enc;
+template <> struct Enc<double> {
+    using A = double;  // auxiliary per-node field type (F_x, wall rho)
+    static constexpr bool scaled = false;
+    __device__ static inline double ld(const double *p) { return *p; }
+    __device__ static inline void st(double *p, double v) { *p = v; }
+};
+template <> struct Enc<float> {
+    using A = float;
+    static constexpr bool scaled = false;
+    __device__ static inline float ld(const float *p) { return *p; }
+    __device__ static inline void st(float *p, float v) { *p = v; }
+};
+template <> struct Enc<__half> {
+    using A = float;
+    static constexpr bool scaled = true;
+    __device__ static inline float ld(const __half *p) { return __half2float(*p); }
+    __device__ static inline void st(__half *p, float v) { *p = __float2half_rn(v); }
+};
+template <> struct Enc<__hip_bfloat16> {
+    using A = float;
+    static constexpr bool scaled = true;
+    __device__ static inline float ld(const __hip_bfloat16 *p) { return __bfloat162float(*p); }
+    __device__ static inline void st(__hip_bfloat16 *p, float v) { *p = __float2bfloat16(v); }
+};
+
+// Arguments of the fused lattice kernels.  Populations are SoA: pop[i*n + idx].
+template <class S> struct StepArgs {
+    const S *fin;
+    const S *gin;
+    S *fout;
+    S *gout;
+    typename Enc<S>::A *fx;        // F_x of the previous forcing, updated in place
+    typename Enc<S>::A *rho_wall;  // rho at ix=0 (first ly*lz) and ix=lx-1 (next ly*lz), in place
+    double *rho_full;              // rho of this step at every node (particle coupling), or null
+    double *macro_rho, *macro_ux, *macro_uy, *macro_uz, *macro_phi;  // output fields, or null
+    Dims d;
+    FluidK k;
+};
+
+// Post-stream populations of the special nodes (IBB link nodes and refilled nodes), computed
+// before the lattice kernel from the post-collision state, and re-collided after it.
+struct SpecialBuf {
+    int *node;       // lattice index, [cap]
+    double *pop;     // [54][cap]: f_0..f_26 then g_0..g_26 (post-stream, post-IBB/refill)
+    double *fx_old;  // F_x entering macrovar at this node, [cap]
+    int *count;      // device counter
+    int cap;
+};
+
+}  // namespace rbtp

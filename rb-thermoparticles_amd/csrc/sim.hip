@@ -1,0 +1,688 @@
+// sim.hip — the rbtp C-ABI (include/rbtp.h): handle, device buffers, the time loop and the
+// parity accessors.
+//
+// Time-step organisation.  The reference runs ~15 separate passes per step over global arrays
+// (main.cu:35-85) and keeps three copies of f and of g.  Here the state between steps is the
+// post-collision pair (f*, g*) in one buffer of a ping-pong pair, and one step is
+//   1. the fused lattice kernel: pull-stream of the previous step (wall rules included), then
+//      macrovar, forcing and both collisions of this step, into the other buffer;
+//   2. the special-node re-collision: nodes whose post-stream populations the particle coupling
+//      replaced (IBB link nodes, refilled nodes) are collided again from their corrected values;
+//   3. the particle phase (particles.hip): force/torque/heat sums, contacts, motion, new
+//      geometry, new-fluid detection, and the corrected post-stream populations of the next
+//      special nodes.
+// Every reference observable at a step boundary is reconstructed by the accessors.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/rbtp.h"
+#include "lattice.h"
+#include "particles.h"
+
+using namespace rbtp;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int set_error(int code, const char *fmt, ...) {
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_last_error = buf;
+    return code;
+}
+
+#define HIP_TRY(call)                                                                         \
+    do {                                                                                      \
+        hipError_t e_ = (call);                                                               \
+        if (e_ != hipSuccess)                                                                 \
+            return set_error(e_ == hipErrorOutOfMemory ? RBTP_E_NOMEM : RBTP_E_HIP,           \
+                             "%s failed: %s (%s:%d)", #call, hipGetErrorString(e_), __FILE__, \
+                             __LINE__);                                                       \
+    } while (0)
+
+#define RB_TRY(call)              \
+    do {                          \
+        int r_ = (call);          \
+        if (r_ != RBTP_OK) return r_; \
+    } while (0)
+
+size_t storage_bytes(int prec) {
+    switch (prec) {
+        case RBTP_PREC_FP64: return 8;
+        case RBTP_PREC_FP32: return 4;
+        default: return 2;
+    }
+}
+size_t aux_bytes(int prec) { return prec == RBTP_PREC_FP64 ? 8 : 4; }
+
+}  // namespace
+
+struct rbtp_sim {
+    rbtp_case c;
+    rbtp_derived d;
+    Dims dims;
+    FluidK k;
+    int prec;
+    bool strict;
+    int device;
+    hipStream_t stream;
+    hipEvent_t ev_start, ev_stop, ev_k0, ev_k1;
+    bool kernel_timing;
+    double kernel_ms_sum;
+    int64_t kernel_launches;
+    double last_step_ms;
+    // lattice state
+    void *f[2], *g[2];
+    void *fx;        // F_x (aux type)
+    void *rho_wall;  // 2*ly*lz (aux type)
+    double *macro[5];  // rho, ux, uy, uz, phi
+    double *temp_dev;  // initial temperature upload
+    int cur;
+    bool poststream;   // cur holds post-stream populations (after init / restart / set)
+    bool macro_valid;
+    int step;
+    bool initialised;
+    // particles
+    PartDev pd;
+    bool particles;
+};
+
+// ---------------------------------------------------------------------------------------------
+// Case description
+// ---------------------------------------------------------------------------------------------
+
+extern "C" const char *rbtp_last_error(void) { return g_last_error.c_str(); }
+extern "C" int rbtp_abi_version(void) { return RBTP_ABI_VERSION; }
+
+// parameters.h:1-55
+extern "C" int rbtp_case_default(rbtp_case *c) {
+    if (!c) return set_error(RBTP_E_ARG, "rbtp_case_default: null case");
+    std::memset(c, 0, sizeof(*c));
+    c->continuous_steps = 0;
+    c->lx = 320; c->ly = 320; c->lz = 160;
+    c->nend = 1000000; c->ndiag = 125; c->nflowout = 25000; c->nnuout = 125; c->nout_p = 125;
+    c->rayleigh = 1.0e7; c->prandtl = 0.71;
+    c->t_hot = 1.0; c->t_cold = 0.0; c->t_ref = 0.5;
+    c->beta = 0.0000625; c->visc = 0.01206; c->rho0 = 1.0;
+    c->activate_particles = 1;
+    c->particle_fraction = 0.05;
+    c->particle_diam_frac = (1.0 / 20.0);
+    c->cp_particle_ratio = 10.0;
+    c->cp_fluid = 1.0;
+    c->rho_particle_ratio = 1.0;
+    c->repforce_scale = 2.0e-4;
+    c->particle_init_temp = (0.0 + 1.0) / 2;
+    c->particle_rand_seed = 86143;
+    c->rep_min_gap = 2.0; c->rep_min_gap_w = 2.0;
+    c->rep_stf0 = 0.025; c->rep_stf1 = 0.002;
+    c->rep_stf0_w = 0.025; c->rep_stf1_w = 0.002;
+    c->npart_override = -1;
+    c->precision = RBTP_PREC_FP64;
+    return RBTP_OK;
+}
+
+// globals.cu:18-33, init_particle_constants (globals.cu:126-148), update_particle_count
+// (globals.cu:150-169).
+extern "C" int rbtp_case_derive(const rbtp_case *c, rbtp_derived *d) {
+    if (!c || !d) return set_error(RBTP_E_ARG, "rbtp_case_derive: null argument");
+    std::memset(d, 0, sizeof(*d));
+    const double rayl = c->rayleigh, prand = c->prandtl, beta = c->beta, visc = c->visc;
+    const double tHot = c->t_hot, tCold = c->t_cold;
+    const int LX = c->lx;
+    d->grav0 = (beta != 0.0 && (tHot - tCold) != 0.0)
+                   ? rayl * visc * visc /
+                         (beta * (tHot - tCold) * (double)LX * (double)LX * (double)LX * prand)
+                   : 0.0;
+    d->tau = 3.0 * visc + 0.5;
+    d->u0 = std::sqrt(std::fabs(d->grav0 * beta * (tHot - tCold) * (double)LX));
+    d->diff = visc / prand;
+    d->tauc = 3.0 * d->diff + 0.5;
+    d->tauci = 1.0 / d->tauc;
+    constexpr double pi = 3.14159265358979323846;
+    const double radius = 0.5 * c->particle_diam_frac * static_cast<double>(LX);
+    const double volume = 4.0 * pi * radius * radius * radius / 3.0;
+    const double mass = c->rho0 * c->rho_particle_ratio * volume;
+    d->particle_radius = radius;
+    d->particle_mass = mass;
+    d->particle_inertia = 0.4 * mass * radius * radius;
+    d->particle_heat_capacity = mass * c->cp_particle_ratio * c->cp_fluid;
+    if (!c->activate_particles) {
+        d->npart = 0;
+    } else if (c->npart_override >= 0) {
+        d->npart = c->npart_override;
+    } else if (c->particle_fraction <= 0.0 || volume <= 0.0) {
+        d->npart = 0;
+    } else {
+        const int lxyz = c->lx * c->ly * c->lz;
+        const double raw = lxyz * c->particle_fraction / volume;
+        d->npart = static_cast<int>(std::floor(raw < 0.0 ? 0.0 : raw));
+    }
+    return RBTP_OK;
+}
+
+static int validate_case(const rbtp_case *c) {
+    if (c->lx < 3 || c->ly < 3 || c->lz < 3)
+        return set_error(RBTP_E_ARG, "lattice %dx%dx%d: every extent must be >= 3", c->lx, c->ly, c->lz);
+    const long long n = (long long)c->lx * c->ly * c->lz;
+    if (n >= (1ll << 31) / 1)
+        return set_error(RBTP_E_ARG, "lattice of %lld nodes exceeds 2^31", n);
+    if (c->precision < RBTP_PREC_FP64 || c->precision > RBTP_PREC_BF16S)
+        return set_error(RBTP_E_ARG, "unknown precision %d", c->precision);
+    if (!(c->visc > 0.0) || !(c->prandtl > 0.0))
+        return set_error(RBTP_E_ARG, "visc and prandtl must be positive");
+    if (c->activate_particles && c->particle_diam_frac < 0.0)
+        return set_error(RBTP_E_ARG, "particle_diam_frac must be >= 0");
+    return RBTP_OK;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Create / destroy
+// ---------------------------------------------------------------------------------------------
+
+extern "C" void rbtp_destroy(rbtp_sim *s) {
+    if (!s) return;
+    (void)hipSetDevice(s->device);
+    if (s->stream) (void)hipStreamSynchronize(s->stream);
+    for (int b = 0; b < 2; b++) {
+        (void)hipFree(s->f[b]);
+        (void)hipFree(s->g[b]);
+    }
+    (void)hipFree(s->fx);
+    (void)hipFree(s->rho_wall);
+    for (double *m : s->macro) (void)hipFree(m);
+    (void)hipFree(s->temp_dev);
+    part_free(s->pd);
+    if (s->ev_start) (void)hipEventDestroy(s->ev_start);
+    if (s->ev_stop) (void)hipEventDestroy(s->ev_stop);
+    if (s->ev_k0) (void)hipEventDestroy(s->ev_k0);
+    if (s->ev_k1) (void)hipEventDestroy(s->ev_k1);
+    if (s->stream) (void)hipStreamDestroy(s->stream);
+    delete s;
+}
+
+static int create_impl(const rbtp_case *c, rbtp_sim *s) {
+    s->c = *c;
+    rbtp_case_derive(c, &s->d);
+    s->dims = Dims{c->lx, c->ly, c->lz, c->lx * c->ly, c->lx * c->ly * c->lz};
+    const double tau = s->d.tau, tauc = s->d.tauc;
+    s->k.tau = tau;
+    s->k.tauc = tauc;
+    s->k.gb = s->d.grav0 * c->beta;
+    s->k.t0 = c->t_ref;
+    s->k.thot = c->t_hot;
+    s->k.tcold = c->t_cold;
+    s->k.force_coeff = (1.0 - 0.5 / tau);
+    s->k.lambda0 = 1.0 - weight(0);
+    s->k.ralph_coeff = (1.0 - 1.0 / (2.0 * tauc));
+    s->k.c2t = 2.0 * tauc / 3.0;
+    s->k.inv_tau = 1.0 / tau;
+    s->k.inv_tauc = 1.0 / tauc;
+    s->prec = c->precision;
+    s->strict = (c->precision == RBTP_PREC_FP64);
+    HIP_TRY(hipGetDevice(&s->device));
+    HIP_TRY(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+    HIP_TRY(hipEventCreate(&s->ev_start));
+    HIP_TRY(hipEventCreate(&s->ev_stop));
+    HIP_TRY(hipEventCreate(&s->ev_k0));
+    HIP_TRY(hipEventCreate(&s->ev_k1));
+    const size_t n = (size_t)s->dims.n;
+    const size_t pop_bytes = 27 * n * storage_bytes(s->prec);
+    for (int b = 0; b < 2; b++) {
+        HIP_TRY(hipMalloc(&s->f[b], pop_bytes));
+        HIP_TRY(hipMalloc(&s->g[b], pop_bytes));
+    }
+    HIP_TRY(hipMalloc(&s->fx, n * aux_bytes(s->prec)));
+    HIP_TRY(hipMalloc(&s->rho_wall, 2 * (size_t)c->ly * c->lz * aux_bytes(s->prec)));
+    for (double *&m : s->macro) HIP_TRY(hipMalloc(&m, n * sizeof(double)));
+    s->particles = c->activate_particles && s->d.npart > 0;
+    if (s->particles) RB_TRY(part_alloc(s->pd, *c, s->d, s->dims));
+    return RBTP_OK;
+}
+
+extern "C" int rbtp_create(const rbtp_case *c, rbtp_sim **out) {
+    if (!c || !out) return set_error(RBTP_E_ARG, "rbtp_create: null argument");
+    *out = nullptr;
+    RB_TRY(validate_case(c));
+    rbtp_sim *s = new rbtp_sim();
+    std::memset((void *)s, 0, sizeof(*s));
+    const int r = create_impl(c, s);
+    if (r != RBTP_OK) {
+        std::string msg = g_last_error;
+        rbtp_destroy(s);
+        g_last_error = msg;
+        return r;
+    }
+    *out = s;
+    return RBTP_OK;
+}
+
+extern "C" int rbtp_get_case(const rbtp_sim *s, rbtp_case *c, rbtp_derived *d) {
+    if (!s) return set_error(RBTP_E_ARG, "null handle");
+    if (c) *c = s->c;
+    if (d) *d = s->d;
+    return RBTP_OK;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Typed dispatch helpers
+// ---------------------------------------------------------------------------------------------
+
+template <class F>
+static hipError_t dispatch(int prec, F &&fn) {
+    switch (prec) {
+        case RBTP_PREC_FP64: return fn((double *)nullptr);
+        case RBTP_PREC_FP32: return fn((float *)nullptr);
+        case RBTP_PREC_FP16S: return fn((__half *)nullptr);
+        default: return fn((__hip_bfloat16 *)nullptr);
+    }
+}
+
+template <class S>
+static StepArgs<S> step_args(rbtp_sim *s, bool macro_out) {
+    StepArgs<S> a;
+    a.fin = (const S *)s->f[s->cur];
+    a.gin = (const S *)s->g[s->cur];
+    a.fout = (S *)s->f[1 - s->cur];
+    a.gout = (S *)s->g[1 - s->cur];
+    a.fx = (typename Enc<S>::A *)s->fx;
+    a.rho_wall = (typename Enc<S>::A *)s->rho_wall;
+    a.rho_full = s->particles ? s->macro[0] : nullptr;
+    if (macro_out) {
+        a.macro_rho = s->macro[0];
+        a.macro_ux = s->macro[1];
+        a.macro_uy = s->macro[2];
+        a.macro_uz = s->macro[3];
+        a.macro_phi = s->macro[4];
+    } else {
+        a.macro_rho = a.macro_ux = a.macro_uy = a.macro_uz = a.macro_phi = nullptr;
+    }
+    a.d = s->dims;
+    a.k = s->k;
+    return a;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Initial state
+// ---------------------------------------------------------------------------------------------
+
+static uint64_t splitmix64(uint64_t &x) {
+    uint64_t z = (x += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+extern "C" int rbtp_linear_temperature(const rbtp_case *c, uint64_t seed, double amp, double *out) {
+    if (!c || !out) return set_error(RBTP_E_ARG, "rbtp_linear_temperature: null argument");
+    const int LX = c->lx;
+    const long n = (long)c->lx * c->ly * c->lz;
+    uint64_t st = seed;
+    for (long idx = 0; idx < n; idx++) {
+        const int ix = (int)(idx % LX);
+        const double xi = (static_cast<double>(ix) + 0.5) / static_cast<double>(LX);
+        const double lin = c->t_hot + (c->t_cold - c->t_hot) * xi;
+        const double u = static_cast<double>(splitmix64(st) >> 11) * 0x1.0p-53;
+        out[idx] = lin + amp * (2.0 * u - 1.0);
+    }
+    return RBTP_OK;
+}
+
+extern "C" int rbtp_init(rbtp_sim *s, const double *temperature, const double *ppos,
+                         const double *ptemp) {
+    if (!s) return set_error(RBTP_E_ARG, "null handle");
+    HIP_TRY(hipSetDevice(s->device));
+    const size_t n = (size_t)s->dims.n;
+    const double *tdev = nullptr;
+    if (temperature) {
+        if (!s->temp_dev) HIP_TRY(hipMalloc(&s->temp_dev, n * sizeof(double)));
+        HIP_TRY(hipMemcpyAsync(s->temp_dev, temperature, n * sizeof(double), hipMemcpyHostToDevice,
+                               s->stream));
+        tdev = s->temp_dev;
+    }
+    s->cur = 0;
+    HIP_TRY(dispatch(s->prec, [&](auto *tag) {
+        using S = std::remove_pointer_t<decltype(tag)>;
+        return launch_init<S>((S *)s->f[0], (S *)s->g[0], (typename Enc<S>::A *)s->fx,
+                              (typename Enc<S>::A *)s->rho_wall, tdev, s->c.rho0, s->c.t_ref,
+                              s->d.grav0, s->c.beta, s->dims, s->macro[0], s->macro[1],
+                              s->macro[2], s->macro[3], s->macro[4], s->stream);
+    }));
+    if (s->particles) {
+        // init_particles (pinit.cu:183-295) and the first build_links (initial.cu:330).
+        RB_TRY(part_init(s->pd, s->c, s->d, s->dims, ppos, ptemp, s->stream));
+    }
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    s->poststream = true;
+    s->macro_valid = true;
+    s->step = 0;
+    s->initialised = true;
+    return RBTP_OK;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Time loop
+// ---------------------------------------------------------------------------------------------
+
+static int one_step(rbtp_sim *s, bool macro_out) {
+    const bool nostream = s->poststream;
+    HIP_TRY(dispatch(s->prec, [&](auto *tag) -> hipError_t {
+        using S = std::remove_pointer_t<decltype(tag)>;
+        const StepArgs<S> a = step_args<S>(s, macro_out);
+        if (s->kernel_timing) {
+            hipError_t e = hipEventRecord(s->ev_k0, s->stream);
+            if (e != hipSuccess) return e;
+        }
+        hipError_t e = launch_lattice_step<S>(a, s->strict, nostream, s->stream);
+        if (e != hipSuccess) return e;
+        if (s->kernel_timing) {
+            e = hipEventRecord(s->ev_k1, s->stream);
+            if (e != hipSuccess) return e;
+        }
+        if (s->particles && s->pd.special_pending)
+            e = launch_special_collide<S>(a, s->pd.sp, s->pd.sp.cap, s->strict, s->stream);
+        return e;
+    }));
+    if (s->kernel_timing) {
+        HIP_TRY(hipEventSynchronize(s->ev_k1));
+        float ms = 0.f;
+        HIP_TRY(hipEventElapsedTime(&ms, s->ev_k0, s->ev_k1));
+        s->kernel_ms_sum += ms;
+        s->kernel_launches++;
+    }
+    s->cur = 1 - s->cur;
+    s->poststream = false;
+    if (s->particles) {
+        RB_TRY(part_phase(s->pd, s->c, s->d, s->dims, s->prec, s->f[s->cur], s->g[s->cur],
+                          s->fx, s->rho_wall, s->macro, s->stream));
+    }
+    s->step++;
+    return RBTP_OK;
+}
+
+static int step_impl(rbtp_sim *s, int nsteps, unsigned flags, bool sync) {
+    if (!s) return set_error(RBTP_E_ARG, "null handle");
+    if (!s->initialised) return set_error(RBTP_E_STATE, "rbtp_step before rbtp_init");
+    if (nsteps < 0) return set_error(RBTP_E_ARG, "nsteps < 0");
+    HIP_TRY(hipSetDevice(s->device));
+    if (s->kernel_timing) {
+        s->kernel_ms_sum = 0.0;
+        s->kernel_launches = 0;
+    }
+    HIP_TRY(hipEventRecord(s->ev_start, s->stream));
+    for (int n = 0; n < nsteps; n++) {
+        const bool macro_out = (flags & RBTP_STEP_MACRO_LAST) && n == nsteps - 1;
+        RB_TRY(one_step(s, macro_out));
+        if (nsteps > 0) s->macro_valid = macro_out;
+    }
+    HIP_TRY(hipEventRecord(s->ev_stop, s->stream));
+    if (sync) {
+        HIP_TRY(hipEventSynchronize(s->ev_stop));
+        float ms = 0.f;
+        HIP_TRY(hipEventElapsedTime(&ms, s->ev_start, s->ev_stop));
+        s->last_step_ms = ms;
+        if (s->particles) RB_TRY(part_check_errors(s->pd));
+    }
+    return RBTP_OK;
+}
+
+extern "C" int rbtp_step(rbtp_sim *s, int nsteps, unsigned flags) {
+    return step_impl(s, nsteps, flags, true);
+}
+extern "C" int rbtp_step_async(rbtp_sim *s, int nsteps, unsigned flags) {
+    return step_impl(s, nsteps, flags, false);
+}
+extern "C" int rbtp_synchronize(rbtp_sim *s) {
+    if (!s) return set_error(RBTP_E_ARG, "null handle");
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, s->ev_start, s->ev_stop) == hipSuccess) s->last_step_ms = ms;
+    if (s->particles) RB_TRY(part_check_errors(s->pd));
+    return RBTP_OK;
+}
+extern "C" int rbtp_current_step(const rbtp_sim *s, int *step) {
+    if (!s || !step) return set_error(RBTP_E_ARG, "null argument");
+    *step = s->step;
+    return RBTP_OK;
+}
+extern "C" int rbtp_last_step_ms(rbtp_sim *s, double *ms) {
+    if (!s || !ms) return set_error(RBTP_E_ARG, "null argument");
+    *ms = s->last_step_ms;
+    return RBTP_OK;
+}
+extern "C" int rbtp_set_kernel_timing(rbtp_sim *s, int enable) {
+    if (!s) return set_error(RBTP_E_ARG, "null handle");
+    s->kernel_timing = enable != 0;
+    return RBTP_OK;
+}
+extern "C" int rbtp_kernel_ms(rbtp_sim *s, double *avg_ms, int64_t *launches) {
+    if (!s) return set_error(RBTP_E_ARG, "null handle");
+    if (avg_ms) *avg_ms = s->kernel_launches ? s->kernel_ms_sum / s->kernel_launches : 0.0;
+    if (launches) *launches = s->kernel_launches;
+    return RBTP_OK;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Accessors
+// ---------------------------------------------------------------------------------------------
+
+extern "C" int rbtp_get_field(rbtp_sim *s, int which, double *out) {
+    if (!s || !out) return set_error(RBTP_E_ARG, "null argument");
+    if (!s->initialised) return set_error(RBTP_E_STATE, "not initialised");
+    HIP_TRY(hipSetDevice(s->device));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    const size_t n = (size_t)s->dims.n;
+    if (which >= RBTP_FIELD_RHO && which <= RBTP_FIELD_PHI) {
+        if (!s->macro_valid)
+            return set_error(RBTP_E_STATE,
+                             "macroscopic fields are only kept when the last step ran with "
+                             "RBTP_STEP_MACRO_LAST");
+        HIP_TRY(hipMemcpy(out, s->macro[which], n * sizeof(double), hipMemcpyDeviceToHost));
+        return RBTP_OK;
+    }
+    if (which == RBTP_FIELD_FX) {
+        if (s->prec == RBTP_PREC_FP64) {
+            HIP_TRY(hipMemcpy(out, s->fx, n * sizeof(double), hipMemcpyDeviceToHost));
+        } else {
+            std::vector<float> tmp(n);
+            HIP_TRY(hipMemcpy(tmp.data(), s->fx, n * sizeof(float), hipMemcpyDeviceToHost));
+            for (size_t i = 0; i < n; i++) out[i] = tmp[i];
+        }
+        return RBTP_OK;
+    }
+    if (which == RBTP_FIELD_FY || which == RBTP_FIELD_FZ) {
+        std::fill(out, out + n, 0.0);  // forcing writes 0 (macrovar.cu:45-46)
+        return RBTP_OK;
+    }
+    return set_error(RBTP_E_ARG, "unknown field %d", which);
+}
+
+extern "C" int rbtp_get_populations(rbtp_sim *s, int which, double *out) {
+    if (!s || !out) return set_error(RBTP_E_ARG, "null argument");
+    if (!s->initialised) return set_error(RBTP_E_STATE, "not initialised");
+    if (which < RBTP_POP_F || which > RBTP_POP_G_COLLIDE) return set_error(RBTP_E_ARG, "unknown population set %d", which);
+    HIP_TRY(hipSetDevice(s->device));
+    const size_t n = (size_t)s->dims.n;
+    double *dev = nullptr;
+    HIP_TRY(hipMalloc(&dev, 2 * 27 * n * sizeof(double)));
+    int rc = RBTP_OK;
+    hipError_t e = hipSuccess;
+    const bool want_g = (which == RBTP_POP_G || which == RBTP_POP_G_COLLIDE);
+    if (which == RBTP_POP_F_COLLIDE || which == RBTP_POP_G_COLLIDE || s->poststream) {
+        // d_f_collide holds f after init (initial.cu:326) and the last collision otherwise.
+        e = dispatch(s->prec, [&](auto *tag) {
+            using S = std::remove_pointer_t<decltype(tag)>;
+            const void *src = want_g ? s->g[s->cur] : s->f[s->cur];
+            return launch_decode<S>((const S *)src, dev, n, want_g ? 1 : 0, s->c.t_ref, s->stream);
+        });
+    } else {
+        e = dispatch(s->prec, [&](auto *tag) {
+            using S = std::remove_pointer_t<decltype(tag)>;
+            return launch_stream_only<S>((const S *)s->f[s->cur], (const S *)s->g[s->cur],
+                                         (const typename Enc<S>::A *)s->rho_wall, dev,
+                                         dev + 27 * n, s->dims, s->c.t_ref, s->c.t_hot,
+                                         s->c.t_cold, false, s->stream);
+        });
+        if (e == hipSuccess && s->particles)
+            e = part_overwrite_special(s->pd, dev, dev + 27 * n, s->dims, s->stream);
+        if (want_g && e == hipSuccess) {
+            e = hipMemcpyAsync(dev, dev + 27 * n, 27 * n * sizeof(double), hipMemcpyDeviceToDevice, s->stream);
+        }
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(out, dev, 27 * n * sizeof(double), hipMemcpyDeviceToHost, s->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+    if (e != hipSuccess) rc = set_error(RBTP_E_HIP, "rbtp_get_populations: %s", hipGetErrorString(e));
+    (void)hipFree(dev);
+    return rc;
+}
+
+extern "C" int rbtp_set_populations(rbtp_sim *s, const double *f, const double *g) {
+    if (!s || !f || !g) return set_error(RBTP_E_ARG, "null argument");
+    if (!s->initialised) return set_error(RBTP_E_STATE, "not initialised");
+    HIP_TRY(hipSetDevice(s->device));
+    const size_t n = (size_t)s->dims.n;
+    double *dev = nullptr;
+    HIP_TRY(hipMalloc(&dev, 27 * n * sizeof(double)));
+    hipError_t e = hipSuccess;
+    for (int which = 0; which < 2 && e == hipSuccess; which++) {
+        e = hipMemcpyAsync(dev, which ? g : f, 27 * n * sizeof(double), hipMemcpyHostToDevice, s->stream);
+        if (e != hipSuccess) break;
+        e = dispatch(s->prec, [&](auto *tag) {
+            using S = std::remove_pointer_t<decltype(tag)>;
+            return launch_encode<S>(dev, (S *)(which ? s->g[s->cur] : s->f[s->cur]), n, which,
+                                    s->c.t_ref, s->stream);
+        });
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+    (void)hipFree(dev);
+    if (e != hipSuccess) return set_error(RBTP_E_HIP, "rbtp_set_populations: %s", hipGetErrorString(e));
+    s->poststream = true;
+    if (s->particles) s->pd.special_pending = false;
+    return RBTP_OK;
+}
+
+extern "C" int rbtp_get_flags(rbtp_sim *s, int32_t *ibnode, int32_t *owner) {
+    if (!s) return set_error(RBTP_E_ARG, "null handle");
+    if (!s->initialised) return set_error(RBTP_E_STATE, "not initialised");
+    HIP_TRY(hipSetDevice(s->device));
+    const size_t n = (size_t)s->dims.n;
+    std::vector<int32_t> own(n, -1);
+    if (s->particles) {
+        HIP_TRY(hipStreamSynchronize(s->stream));
+        HIP_TRY(hipMemcpy(own.data(), s->pd.owner[s->pd.geo_cur], n * sizeof(int32_t), hipMemcpyDeviceToHost));
+    }
+    if (owner) std::copy(own.begin(), own.end(), owner);
+    if (ibnode)
+        for (size_t i = 0; i < n; i++) ibnode[i] = own[i] >= 0 ? 1 : 0;
+    return RBTP_OK;
+}
+
+extern "C" int rbtp_get_particles(rbtp_sim *s, int which, double *out) {
+    if (!s || !out) return set_error(RBTP_E_ARG, "null argument");
+    if (which < 0 || which >= RBTP_P_COUNT) return set_error(RBTP_E_ARG, "unknown particle array %d", which);
+    if (!s->particles) return RBTP_OK;
+    HIP_TRY(hipSetDevice(s->device));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    HIP_TRY(hipMemcpy(out, s->pd.p[which], s->pd.np * sizeof(double), hipMemcpyDeviceToHost));
+    return RBTP_OK;
+}
+
+extern "C" int rbtp_set_particles(rbtp_sim *s, int which, const double *in) {
+    if (!s || !in) return set_error(RBTP_E_ARG, "null argument");
+    if (which < 0 || which >= RBTP_P_COUNT) return set_error(RBTP_E_ARG, "unknown particle array %d", which);
+    if (!s->particles) return RBTP_OK;
+    HIP_TRY(hipSetDevice(s->device));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    HIP_TRY(hipMemcpy(s->pd.p[which], in, s->pd.np * sizeof(double), hipMemcpyHostToDevice));
+    return RBTP_OK;
+}
+
+extern "C" int rbtp_get_links(rbtp_sim *s, rbtp_link *out, int64_t capacity, int64_t *count) {
+    if (!s || !count) return set_error(RBTP_E_ARG, "null argument");
+    *count = 0;
+    if (!s->particles) return RBTP_OK;
+    HIP_TRY(hipSetDevice(s->device));
+    std::vector<rbtp_link> links;
+    RB_TRY(part_collect_links(s->pd, s->dims, s->stream, links));
+    std::sort(links.begin(), links.end(), [](const rbtp_link &a, const rbtp_link &b) {
+        if (a.cell_k != b.cell_k) return a.cell_k < b.cell_k;
+        if (a.cell_j != b.cell_j) return a.cell_j < b.cell_j;
+        if (a.cell_i != b.cell_i) return a.cell_i < b.cell_i;
+        return a.dir < b.dir;
+    });
+    *count = (int64_t)links.size();
+    if (out) {
+        if (capacity < *count) return set_error(RBTP_E_ARG, "link buffer too small (%lld < %lld)", (long long)capacity, (long long)*count);
+        std::copy(links.begin(), links.end(), out);
+    }
+    return RBTP_OK;
+}
+
+extern "C" int rbtp_get_new_fluid(rbtp_sim *s, int32_t *nodes, int32_t *pids, int64_t capacity,
+                                  int64_t *count) {
+    if (!s || !count) return set_error(RBTP_E_ARG, "null argument");
+    *count = 0;
+    if (!s->particles) return RBTP_OK;
+    HIP_TRY(hipSetDevice(s->device));
+    std::vector<std::pair<int32_t, int32_t>> nf;
+    RB_TRY(part_collect_new_fluid(s->pd, s->stream, nf));
+    std::sort(nf.begin(), nf.end());
+    *count = (int64_t)nf.size();
+    if (nodes || pids) {
+        if (capacity < *count) return set_error(RBTP_E_ARG, "new-fluid buffer too small");
+        for (size_t i = 0; i < nf.size(); i++) {
+            if (nodes) nodes[i] = nf[i].first;
+            if (pids) pids[i] = nf[i].second;
+        }
+    }
+    return RBTP_OK;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Diagnostics
+// ---------------------------------------------------------------------------------------------
+
+extern "C" int rbtp_nu(rbtp_sim *s, double out[7]) {
+    if (!s || !out) return set_error(RBTP_E_ARG, "null argument");
+    if (!s->macro_valid) return set_error(RBTP_E_STATE, "run the last step with RBTP_STEP_MACRO_LAST first");
+    HIP_TRY(hipSetDevice(s->device));
+    const int32_t *owner = s->particles ? s->pd.owner[s->pd.geo_cur] : nullptr;
+    RB_TRY(diag_nu(s->c, s->d, s->dims, s->macro[1], s->macro[4], owner, s->step, s->stream, out));
+    return RBTP_OK;
+}
+
+extern "C" int rbtp_means(rbtp_sim *s, double out[4]) {
+    if (!s || !out) return set_error(RBTP_E_ARG, "null argument");
+    if (!s->macro_valid) return set_error(RBTP_E_STATE, "run the last step with RBTP_STEP_MACRO_LAST first");
+    HIP_TRY(hipSetDevice(s->device));
+    RB_TRY(diag_means(s->dims, s->macro, s->particles ? &s->pd : nullptr, s->stream, out));
+    return RBTP_OK;
+}
+
+extern "C" int rbtp_kinetic_energy(rbtp_sim *s, double *out) {
+    if (!s || !out) return set_error(RBTP_E_ARG, "null argument");
+    if (!s->macro_valid) return set_error(RBTP_E_STATE, "run the last step with RBTP_STEP_MACRO_LAST first");
+    HIP_TRY(hipSetDevice(s->device));
+    RB_TRY(diag_kinetic_energy(s->dims, s->macro, s->stream, out));
+    return RBTP_OK;
+}
+
+// Error helper for the other translation units.
+namespace rbtp {
+int report(int code, const char *msg) { return set_error(code, "%s", msg); }
+int report_hip(hipError_t e, const char *what) {
+    return set_error(e == hipErrorOutOfMemory ? RBTP_E_NOMEM : RBTP_E_HIP, "%s: %s", what,
+                     hipGetErrorString(e));
+}
+}  // namespace rbtp
