@@ -1,0 +1,166 @@
+#!/usr/bin/env python3
+"""Benchmark of the hot path: D3Q27 f+g thermal LBM time steps on one MI355X.
+
+Workload: single-phase Rayleigh-Benard at the reference's production shape, 320x320x160 nodes,
+Ra=1e8, Pr=0.71, FP64 (the plain drop-in form), synthetic initial state (linear temperature plus
+seeded noise).  A "step" is one full time step of main.cu's sequence over all nodes.
+
+Prints one JSON line.  value = lattice updates per second of the whole job (MLUPS), with the
+state resident in HBM before the timed region starts.  The line also carries:
+  roofline      the fused lattice kernel's algorithmic HBM bytes per launch (880 B/node: 54 FP64
+                populations read and written plus F_x read and written) over its average launch
+                time measured with HIP events on the library's stream;
+  cpu_baseline  the FP64 host oracle (OpenMP) timed on this machine's cores on a bounded sample;
+  forms         the FP32-math/FP16-storage form on the same workload, with its Nu deviation.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "tests"))
+
+HBM_PEAK_GBS = 8000.0
+BYTES_PER_NODE = {0: 880.0, 1: 440.0, 2: 224.0, 3: 224.0}
+
+
+def load_rbtp():
+    import importlib.util
+    if "rbtp_amd" in sys.modules:
+        return sys.modules["rbtp_amd"]
+    spec = importlib.util.spec_from_file_location(
+        "rbtp_amd", os.path.join(ROOT, "rb-thermoparticles_amd", "__init__.py"))
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules["rbtp_amd"] = mod
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def run_form(rb, kw, steps, warmup, timing_steps, log):
+    case = rb.default_case(**kw)
+    sim = rb.Sim(case)
+    T = rb.linear_temperature(case, 2024, 1e-3)
+    sim.init(temperature=T)
+    n = sim.n
+    sim.step(warmup, macro_last=False)
+    sim.synchronize()
+    t0 = time.perf_counter()
+    sim.step(steps, macro_last=True)
+    t1 = time.perf_counter()
+    wall_ms = (t1 - t0) * 1e3
+    dev_ms = sim.timing().step_ms
+    nu = sim.nu()
+    ke = sim.kinetic_energy()
+    # kernel-only timing (events around every lattice launch)
+    sim.set_kernel_timing(True)
+    sim.step(timing_steps, macro_last=False)
+    tm = sim.timing()
+    sim.close()
+    log(f"form precision={kw.get('precision', 0)}: {steps} steps in {wall_ms:.1f} ms wall "
+        f"({dev_ms:.1f} ms device), kernel avg {tm.kernel_ms:.4f} ms over {tm.launches}")
+    return dict(n=n, wall_ms=wall_ms, dev_ms=dev_ms, kernel_ms=tm.kernel_ms,
+                kernel_launches=tm.launches, nu=nu.tolist(), ke=ke)
+
+
+def cpu_baseline(kw, log, budget_s=20.0):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    case = O.default_case(**kw)
+    ora = O.Oracle(case)
+    T = O.linear_temperature(case, 2024, 1e-3)
+    ora.init(temperature=T)
+    ora.step(1)  # warm-up (page faults, thread start)
+    t0 = time.perf_counter()
+    steps = 0
+    while True:
+        ora.step(1)
+        steps += 1
+        if time.perf_counter() - t0 > budget_s or steps >= 50:
+            break
+    dt = time.perf_counter() - t0
+    n = ora.n
+    ora.close()
+    mlups = n * steps / dt / 1e6
+    log(f"cpu oracle: {steps} steps of {n} nodes in {dt:.2f} s -> {mlups:.2f} MLUPS "
+        f"on {O.threads()} threads")
+    return dict(value=mlups, unit="MLUPS", cores=O.threads(), kind="port",
+                sample=f"FP64 oracle, {steps} steps (after 1 warm-up) of the same "
+                       f"{kw['lx']}x{kw['ly']}x{kw['lz']} Ra=1e8 workload")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3000)
+    ap.add_argument("--warmup", type=int, default=300)
+    ap.add_argument("--timing-steps", type=int, default=100)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-forms", action="store_true", help="skip the FP16-storage form")
+    ap.add_argument("--quiet", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(the multi-GPU slab decomposition is a design, not code)", file=sys.stderr)
+        sys.exit(2)
+
+    def log(msg):
+        if not args.quiet:
+            print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+    from cases import config2
+    rb = load_rbtp()
+    kw64 = config2(precision=0)
+    r64 = run_form(rb, kw64, args.steps, args.warmup, args.timing_steps, log)
+    n = r64["n"]
+    mlups = n * args.steps / (r64["wall_ms"] * 1e-3) / 1e6
+    achieved = BYTES_PER_NODE[0] * n / (r64["kernel_ms"] * 1e-3) / 1e9
+    forms = {}
+    if not args.no_forms:
+        r16 = run_form(rb, config2(precision=2), args.steps, args.warmup, args.timing_steps, log)
+        m16 = n * args.steps / (r16["wall_ms"] * 1e-3) / 1e6
+        forms["fp16s"] = dict(
+            mlups=m16, ratio_vs_fp64=m16 / mlups, kernel_ms=r16["kernel_ms"],
+            kernel_gbs=BYTES_PER_NODE[2] * n / (r16["kernel_ms"] * 1e-3) / 1e9,
+            nu_bulk=r16["nu"][2], nu_hot=r16["nu"][4],
+            nu_hot_rel_err_vs_fp64=abs(r16["nu"][4] - r64["nu"][4]) / abs(r64["nu"][4]),
+            ke_rel_err_vs_fp64=abs(r16["ke"] - r64["ke"]) / abs(r64["ke"]),
+            bytes_per_node=BYTES_PER_NODE[2])
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "lattice_fp64_320x320x160_traffic.json")
+    if os.path.exists(tpath):
+        with open(tpath) as fh:
+            traffic = json.load(fh).get("bytes_per_launch")
+    cpu = None if args.no_cpu else cpu_baseline(kw64, log)
+    out = {
+        "metric": "MLUPS (D3Q27 f+g), FP64, 320x320x160 Ra=1e8",
+        "value": mlups,
+        "unit": "MLUPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": r64["wall_ms"] / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (linear temperature + 1e-3 seeded noise, fluid at rest)",
+        "config": {"workload": "rb_single_phase_320x320x160_ra1e8_pr0.71_fp64",
+                   "lx": 320, "ly": 320, "lz": 160, "rayleigh": 1e8, "prandtl": 0.71,
+                   "visc": kw64["visc"], "precision": "fp64", "parallelism": "single-gpu"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": "k_lattice_step<double,double,strict>",
+                     "kernel_ms": r64["kernel_ms"], "bytes_per_node": BYTES_PER_NODE[0]},
+        "cpu_baseline": cpu,
+        "forms": forms,
+        "nu_fp64": r64["nu"],
+        "ke_fp64": r64["ke"],
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

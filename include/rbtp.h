@@ -83,6 +83,10 @@ int rbtp_abi_version(void);
 
 /* parameters.h defaults. */
 int rbtp_case_default(rbtp_case *c);
+/* Read a parameters.h file at run time (parameters.h:1-55): every #define is evaluated with C
+ * arithmetic (integer literals stay integers), known macro names fill the case, the rest of the
+ * fields keep their defaults.  Two optional extra macros: NPART_OVERRIDE and PRECISION. */
+int rbtp_case_from_parameters_h(const char *path, rbtp_case *out);
 /* globals.cu:18-33 (grav0, tau, u0, diff, tauc, tauci), globals.cu:126-148 (particle constants)
  * and update_particle_count (globals.cu:150-169). */
 int rbtp_case_derive(const rbtp_case *c, rbtp_derived *out);
@@ -216,6 +220,28 @@ int rbtp_get_new_fluid(rbtp_sim *s, int32_t *nodes, int32_t *pids, int64_t capac
 int rbtp_nu(rbtp_sim *s, double out[7]);
 int rbtp_means(rbtp_sim *s, double out[4]);
 int rbtp_kinetic_energy(rbtp_sim *s, double *out);
+
+/*
+ * Output-side fields and files (output/output_flow.cu, output/output_part.cu,
+ * output/output_statis.cu).  They read the state left by rbtp_step(..., RBTP_STEP_MACRO_LAST).
+ *   rbtp_get_output_fields: the seven fields of %09d.dat in file order (rho, ux, uy, uz, phi,
+ *       ted, Ked; 7*N doubles) after overwrite_particle_nodes (output_flow.cu:64-110), with ted
+ *       and Ked from compute_ted / compute_Ked (collision.cu:91-198) on the device.
+ *   rbtp_profile: output_profile's per-x-plane means of ux, uy, uz, phi (4*lx doubles,
+ *       output_statis.cu:225-285).
+ *   rbtp_write_fg: output_fg, f then g after streaming/IBB/refill (output_flow.cu:112-127).
+ *   rbtp_write_flow: output_flow's %09d.dat (output_flow.cu:97-109).
+ *   rbtp_write_particles: output_particles' pos_%09d.txt lines (output_part.cu:91-115).
+ *   rbtp_restart_load: load_restart_state (initial.cu:194-236) from <dir>/fg%09d.dat,
+ *       <dir>/pos/pos_%09d.txt and <dir>/%09d.dat; fails with RBTP_E_IO on a missing, short or
+ *       malformed file (the reference does not check the %09d.dat size).
+ */
+int rbtp_get_output_fields(rbtp_sim *s, double *out_7n);
+int rbtp_profile(rbtp_sim *s, double *out_4lx);
+int rbtp_write_fg(rbtp_sim *s, const char *path);
+int rbtp_write_flow(rbtp_sim *s, const char *path);
+int rbtp_write_particles(rbtp_sim *s, const char *path);
+int rbtp_restart_load(rbtp_sim *s, const char *dir, int step);
 
 /* Device time of the kernels enqueued by the last rbtp_step* call, from HIP events on the
  * handle's stream (milliseconds). */

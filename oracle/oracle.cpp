@@ -1283,6 +1283,112 @@ void or_means(const OrState *s, double out[4]) {
     out[0] = umean / N; out[1] = vmean / N; out[2] = wmean / N; out[3] = tmean / N;
 }
 
+
+// The seven %09d.dat fields of output_flow (output_flow.cu:64-110): compute_Ked
+// (collision.cu:91-152) on f_collide, compute_ted (collision.cu:154-198) on g_collide, then
+// overwrite_particle_nodes (output_flow.cu:5-60).  out: 7*N doubles in file order.
+void or_output_fields(const OrState *s, double *out) {
+    const long N = s->N;
+    const int LX = s->LX, LXY = s->LXY;
+    const double tau = s->d.tau, tauc = s->d.tauc, visc = s->c.visc, diff = s->d.diff;
+#pragma omp parallel for schedule(static)
+    for (long idx = 0; idx < N; idx++) {
+        const double fxv = s->fx[idx], fyv = s->fy[idx], fzv = s->fz[idx];
+        const double rho_local = s->rho[idx], phi_local = s->phi[idx];
+        const double ux_local = s->ux[idx], uy_local = s->uy[idx], uz_local = s->uz[idx];
+        const double RT = 1.0 / 3.0;
+        const double u2 = ux_local * ux_local + uy_local * uy_local + uz_local * uz_local;
+        const double uv = u2 / RT;
+        const double bu = fxv * ux_local + fyv * uy_local + fzv * uz_local;
+        double tau_xx = 0.0, tau_xy = 0.0, tau_xz = 0.0, tau_yy = 0.0, tau_yz = 0.0, tau_zz = 0.0;
+        for (int ip = 0; ip < NPOP; ip++) {
+            double eu = (CIX[ip] * ux_local + CIY[ip] * uy_local + CIZ[ip] * uz_local) / RT;
+            double feq = s->tp[ip] * (rho_local + eu + 0.5 * (eu * eu - uv));
+            double fneq = s->f_collide[ip * N + idx] - feq;
+            double cidotF = CIX[ip] * fxv + CIY[ip] * fyv + CIZ[ip] * fzv;
+            double feq_term = 0.5 * (cidotF - bu) * feq / RT;
+            tau_xx += CIX[ip] * CIX[ip] * (fneq + feq_term);
+            tau_xy += CIX[ip] * CIY[ip] * (fneq + feq_term);
+            tau_xz += CIX[ip] * CIZ[ip] * (fneq + feq_term);
+            tau_yy += CIY[ip] * CIY[ip] * (fneq + feq_term);
+            tau_yz += CIY[ip] * CIZ[ip] * (fneq + feq_term);
+            tau_zz += CIZ[ip] * CIZ[ip] * (fneq + feq_term);
+        }
+        double coeff = -(1.0 - 0.5 / tau);
+        tau_xx *= coeff; tau_xy *= coeff; tau_xz *= coeff;
+        tau_yy *= coeff; tau_yz *= coeff; tau_zz *= coeff;
+        double S_xx = tau_xx / (2.0 * visc), S_xy = tau_xy / (2.0 * visc), S_xz = tau_xz / (2.0 * visc);
+        double S_yy = tau_yy / (2.0 * visc), S_yz = tau_yz / (2.0 * visc), S_zz = tau_zz / (2.0 * visc);
+        double Ked = 0.5 * visc * (4.0 * (S_xx * S_xx + S_yy * S_yy + S_zz * S_zz) +
+                                   8.0 * (S_xy * S_xy + S_xz * S_xz + S_yz * S_yz));
+        const double lambda0 = 1.0 - s->tp[0];
+        double gneqx = 0.0, gneqy = 0.0, gneqz = 0.0;
+        for (int ip = 0; ip < NPOP; ip++) {
+            double eu = (CIX[ip] * ux_local + CIY[ip] * uy_local + CIZ[ip] * uz_local) / RT;
+            double geq = s->tp[ip] * phi_local * (1.0 + eu + 0.5 * (eu * eu - uv));
+            double geqnew = (ip == 0) ? (geq - lambda0 * phi_local * rho_local)
+                                      : (geq + s->tp[ip] * phi_local * rho_local);
+            double dg = s->g_collide[ip * N + idx] - geqnew;
+            gneqx += CIX[ip] * dg;
+            gneqy += CIY[ip] * dg;
+            gneqz += CIZ[ip] * dg;
+        }
+        double p = rho_local * RT;
+        double denominator = 2.0 * tauc * RT + p;
+        double tx = -(2.0 * gneqx + phi_local * fxv) / denominator;
+        double ty = -(2.0 * gneqy + phi_local * fyv) / denominator;
+        double tz = -(2.0 * gneqz + phi_local * fzv) / denominator;
+        double ted = (diff > 0.0) ? diff * (tx * tx + ty * ty + tz * tz) : 0.0;
+        double r = rho_local, u = ux_local, v = uy_local, w = uz_local, T = phi_local;
+        if (s->particles() && s->ibnode[idx] != 0) {
+            const int pid = s->owner[idx];
+            if (pid >= 0 && pid < s->np) {
+                const int iz = (int)(idx / LXY);
+                const int iy = (int)((idx - (long)iz * LXY) / LX);
+                const int ix = (int)(idx - (long)iz * LXY - (long)iy * LX);
+                const double relx = (static_cast<double>(ix) + 0.5) - s->p[RBTP_P_POS_X][pid];
+                const double rely = (static_cast<double>(iy) + 0.5) - s->p[RBTP_P_POS_Y][pid];
+                const double relz = (static_cast<double>(iz) + 0.5) - s->p[RBTP_P_POS_Z][pid];
+                const double wx = s->p[RBTP_P_OMEGA_X][pid], wy = s->p[RBTP_P_OMEGA_Y][pid],
+                             wz = s->p[RBTP_P_OMEGA_Z][pid];
+                r = s->c.rho0;
+                u = s->p[RBTP_P_VEL_X][pid] + (relz * wy - rely * wz);
+                v = s->p[RBTP_P_VEL_Y][pid] + (relx * wz - relz * wx);
+                w = s->p[RBTP_P_VEL_Z][pid] + (rely * wx - relx * wy);
+                T = s->p[RBTP_P_TEMP][pid];
+                ted = 0.0;
+                Ked = 0.0;
+            }
+        }
+        out[idx] = r; out[N + idx] = u; out[2 * N + idx] = v; out[3 * N + idx] = w;
+        out[4 * N + idx] = T; out[5 * N + idx] = ted; out[6 * N + idx] = Ked;
+    }
+}
+
+// output_profile (output_statis.cu:251-281) on the output fields: out[4*LX].
+void or_profile(const OrState *s, const double *fields, double *out) {
+    const long N = s->N;
+    const int LX = s->LX, LY = s->LY, LZ = s->LZ, LXY = s->LXY;
+    for (int ix = 0; ix < LX; ix++) {
+        double sum_ux = 0.0, sum_uy = 0.0, sum_uz = 0.0, sum_phi = 0.0;
+        int count = 0;
+        for (int iz = 0; iz < LZ; iz++)
+            for (int iy = 0; iy < LY; iy++) {
+                const long idx = (long)iz * LXY + (long)iy * LX + ix;
+                sum_ux += fields[N + idx];
+                sum_uy += fields[2 * N + idx];
+                sum_uz += fields[3 * N + idx];
+                sum_phi += fields[4 * N + idx];
+                count++;
+            }
+        const double inv_count = (count > 0) ? (1.0 / static_cast<double>(count)) : 0.0;
+        out[4 * ix] = sum_ux * inv_count;
+        out[4 * ix + 1] = sum_uy * inv_count;
+        out[4 * ix + 2] = sum_uz * inv_count;
+        out[4 * ix + 3] = sum_phi * inv_count;
+    }
+}
+
 int or_threads(void) {
 #ifdef _OPENMP
     return omp_get_max_threads();

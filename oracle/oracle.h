@@ -76,6 +76,8 @@ const int *or_new_fluid_pids(const OrState *s);
 void or_linear_temperature(const rbtp_case *c, uint64_t seed, double amp, double *out);
 void or_nu(const OrState *s, double out[7]);
 void or_means(const OrState *s, double out[4]);
+void or_output_fields(const OrState *s, double *out);
+void or_profile(const OrState *s, const double *fields, double *out);
 int or_threads(void);
 
 #ifdef __cplusplus

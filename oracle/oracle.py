@@ -7,7 +7,6 @@ operation order (see oracle/oracle.cpp).
 from __future__ import annotations
 
 import ctypes as C
-import importlib.util
 import os
 
 import numpy as np
@@ -15,19 +14,6 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liboracle.so")
 ROOT = os.path.dirname(HERE)
-
-
-def _pkg():
-    """The product's Python module (for the shared rbtp_case layout only)."""
-    import sys
-    if "rbtp_amd" in sys.modules:
-        return sys.modules["rbtp_amd"]
-    spec = importlib.util.spec_from_file_location(
-        "rbtp_amd", os.path.join(ROOT, "rb-thermoparticles_amd", "__init__.py"))
-    mod = importlib.util.module_from_spec(spec)
-    sys.modules["rbtp_amd"] = mod
-    spec.loader.exec_module(mod)
-    return mod
 
 
 # Case/Link layouts are plain data; define them here again so the oracle does not need the GPU
@@ -116,6 +102,8 @@ def _load():
         "or_nu": (None, [P, P]),
         "or_means": (None, [P, P]),
         "or_threads": (I, []),
+        "or_output_fields": (None, [P, P]),
+        "or_profile": (None, [P, P, P]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -257,6 +245,17 @@ class Oracle:
         out = np.zeros(7)
         lib.or_nu(self._h, out.ctypes.data_as(C.c_void_p))
         return out
+
+    def output_fields(self) -> np.ndarray:
+        out = np.zeros(7 * self.n)
+        lib.or_output_fields(self._h, out.ctypes.data_as(C.c_void_p))
+        return out.reshape(7, self.n)
+
+    def profile(self) -> np.ndarray:
+        fields = np.ascontiguousarray(self.output_fields().reshape(-1))
+        out = np.zeros(4 * self.case.lx)
+        lib.or_profile(self._h, fields.ctypes.data_as(C.c_void_p), out.ctypes.data_as(C.c_void_p))
+        return out.reshape(self.case.lx, 4)
 
     def means(self) -> np.ndarray:
         out = np.zeros(4)

@@ -135,6 +135,13 @@ def _load():
         "rbtp_nu": (I, [P, dp]),
         "rbtp_means": (I, [P, dp]),
         "rbtp_kinetic_energy": (I, [P, C.POINTER(D)]),
+        "rbtp_case_from_parameters_h": (I, [C.c_char_p, C.POINTER(Case)]),
+        "rbtp_get_output_fields": (I, [P, dp]),
+        "rbtp_profile": (I, [P, dp]),
+        "rbtp_write_fg": (I, [P, C.c_char_p]),
+        "rbtp_write_flow": (I, [P, C.c_char_p]),
+        "rbtp_write_particles": (I, [P, C.c_char_p]),
+        "rbtp_restart_load": (I, [P, C.c_char_p, I]),
         "rbtp_last_step_ms": (I, [P, C.POINTER(D)]),
         "rbtp_set_kernel_timing": (I, [P, I]),
         "rbtp_kernel_ms": (I, [P, C.POINTER(D), C.POINTER(C.c_int64)]),
@@ -158,6 +165,13 @@ def default_case(**kw) -> Case:
     c = Case()
     _check(lib.rbtp_case_default(C.byref(c)))
     return c.update(**kw)
+
+
+def case_from_parameters_h(path: str) -> Case:
+    """Case from a parameters.h file (same macro names as the reference)."""
+    c = Case()
+    _check(lib.rbtp_case_from_parameters_h(os.fsencode(path), C.byref(c)))
+    return c
 
 
 def derive(case: Case) -> Derived:
@@ -302,6 +316,31 @@ class Sim:
         v = C.c_double()
         _check(lib.rbtp_kinetic_energy(self._h, C.byref(v)))
         return v.value
+
+    def output_fields(self) -> np.ndarray:
+        """The seven %09d.dat fields (rho, ux, uy, uz, phi, ted, Ked) after the particle-node
+        overwrite, shape (7, n)."""
+        out = np.empty(7 * self.n, dtype=np.float64)
+        _check(lib.rbtp_get_output_fields(self._h, out))
+        return out.reshape(7, self.n)
+
+    def profile(self) -> np.ndarray:
+        out = np.empty(4 * self.case.lx, dtype=np.float64)
+        _check(lib.rbtp_profile(self._h, out))
+        return out.reshape(self.case.lx, 4)
+
+    # -- files (fg%09d.dat, %09d.dat, pos_%09d.txt) and restart ----------------------------
+    def write_fg(self, path: str) -> None:
+        _check(lib.rbtp_write_fg(self._h, os.fsencode(path)))
+
+    def write_flow(self, path: str) -> None:
+        _check(lib.rbtp_write_flow(self._h, os.fsencode(path)))
+
+    def write_particles(self, path: str) -> None:
+        _check(lib.rbtp_write_particles(self._h, os.fsencode(path)))
+
+    def restart_load(self, directory: str, step: int) -> None:
+        _check(lib.rbtp_restart_load(self._h, os.fsencode(directory), int(step)))
 
     # -- timing ------------------------------------------------------------------------------------
     def set_kernel_timing(self, enable: bool) -> None:

@@ -9,6 +9,7 @@
 #include <vector>
 
 #include "particles.h"
+#include "lattice_device.h"
 
 namespace rbtp {
 
@@ -111,6 +112,112 @@ __global__ __launch_bounds__(DB) void k_ke(const double *ux, const double *uy, c
     if (t == 0) out[blockIdx.x] = red[0][0];
 }
 
+
+// Output fields of %09d.dat (output_flow.cu:64-110): rho, ux, uy, uz, phi from the last
+// macrovar (and refill), ted and Ked from the post-collision populations (compute_ted,
+// collision.cu:154-198; compute_Ked, collision.cu:91-152), then overwrite_particle_nodes
+// (output_flow.cu:5-60) at solid nodes.  out is 7*n doubles in file order.
+template <class S>
+__global__ __launch_bounds__(DB) void k_output_fields(const S *fc, const S *gc,
+                                                      const typename Enc<S>::A *fxf,
+                                                      const double *rho, const double *ux,
+                                                      const double *uy, const double *uz,
+                                                      const double *phi, const int32_t *owner,
+                                                      PartDev pd, OutK k, Dims d, double *out) {
+    const int idx = blockIdx.x * DB + threadIdx.x;
+    if (idx >= d.n) return;
+    const size_t n = (size_t)d.n;
+    const double fx = double(fxf[idx]), fy = 0.0, fz = 0.0;
+    const double rho_local = rho[idx], ux_local = ux[idx], uy_local = uy[idx], uz_local = uz[idx];
+    const double phi_local = phi[idx];
+    const double RT = 1.0 / 3.0;
+    const double u2 = ux_local * ux_local + uy_local * uy_local + uz_local * uz_local;
+    const double uv = u2 / RT;
+    // compute_Ked
+    const double bu = fx * ux_local + fy * uy_local + fz * uz_local;
+    double txx = 0.0, txy = 0.0, txz = 0.0, tyy = 0.0, tyz = 0.0, tzz = 0.0;
+    // compute_ted
+    const double lambda0 = 1.0 - weight(0);
+    double gneqx = 0.0, gneqy = 0.0, gneqz = 0.0;
+#pragma unroll
+    for (int ip = 0; ip < NPOP; ip++) {
+        const int cx = D3Q27::cx[ip], cy = D3Q27::cy[ip], cz = D3Q27::cz[ip];
+        const double tp = weight(ip);
+        const double eu = (cx * ux_local + cy * uy_local + cz * uz_local) / RT;
+        const double feq = tp * (rho_local + eu + 0.5 * (eu * eu - uv));
+        const double fneq = dec_f<S, double>(Enc<S>::ld(fc + (size_t)ip * n + idx), ip) - feq;
+        const double cidotF = cx * fx + cy * fy + cz * fz;
+        const double feq_term = 0.5 * (cidotF - bu) * feq / RT;
+        txx += cx * cx * (fneq + feq_term);
+        txy += cx * cy * (fneq + feq_term);
+        txz += cx * cz * (fneq + feq_term);
+        tyy += cy * cy * (fneq + feq_term);
+        tyz += cy * cz * (fneq + feq_term);
+        tzz += cz * cz * (fneq + feq_term);
+        const double geq = tp * phi_local * (1.0 + eu + 0.5 * (eu * eu - uv));
+        const double geqnew = (ip == 0) ? (geq - lambda0 * phi_local * rho_local)
+                                        : (geq + tp * phi_local * rho_local);
+        const double diff = dec_g<S, double>(Enc<S>::ld(gc + (size_t)ip * n + idx), ip, k.t0) - geqnew;
+        gneqx += cx * diff;
+        gneqy += cy * diff;
+        gneqz += cz * diff;
+    }
+    const double coeff = -(1.0 - 0.5 / k.tau);
+    txx *= coeff; txy *= coeff; txz *= coeff; tyy *= coeff; tyz *= coeff; tzz *= coeff;
+    const double Sxx = txx / (2.0 * k.visc), Sxy = txy / (2.0 * k.visc), Sxz = txz / (2.0 * k.visc);
+    const double Syy = tyy / (2.0 * k.visc), Syz = tyz / (2.0 * k.visc), Szz = tzz / (2.0 * k.visc);
+    double ked = 0.5 * k.visc * (4.0 * (Sxx * Sxx + Syy * Syy + Szz * Szz) +
+                                 8.0 * (Sxy * Sxy + Sxz * Sxz + Syz * Syz));
+    const double p = rho_local * RT;
+    const double denominator = 2.0 * k.tauc * RT + p;
+    const double tx = -(2.0 * gneqx + phi_local * fx) / denominator;
+    const double ty = -(2.0 * gneqy + phi_local * fy) / denominator;
+    const double tz = -(2.0 * gneqz + phi_local * fz) / denominator;
+    double ted = (k.diff > 0.0) ? k.diff * (tx * tx + ty * ty + tz * tz) : 0.0;
+    double r = rho_local, u = ux_local, v = uy_local, w = uz_local, T = phi_local;
+    const int pid = owner ? owner[idx] : -1;
+    if (pid >= 0 && pid < pd.np) {
+        const int iz = idx / d.lxy, rem = idx - iz * d.lxy, iy = rem / d.lx, ix = rem - iy * d.lx;
+        const double relx = (static_cast<double>(ix) + 0.5) - pd.p[RBTP_P_POS_X][pid];
+        const double rely = (static_cast<double>(iy) + 0.5) - pd.p[RBTP_P_POS_Y][pid];
+        const double relz = (static_cast<double>(iz) + 0.5) - pd.p[RBTP_P_POS_Z][pid];
+        const double wx = pd.p[RBTP_P_OMEGA_X][pid], wy = pd.p[RBTP_P_OMEGA_Y][pid], wz = pd.p[RBTP_P_OMEGA_Z][pid];
+        r = k.rho0;
+        u = pd.p[RBTP_P_VEL_X][pid] + (relz * wy - rely * wz);
+        v = pd.p[RBTP_P_VEL_Y][pid] + (relx * wz - relz * wx);
+        w = pd.p[RBTP_P_VEL_Z][pid] + (rely * wx - relx * wy);
+        T = pd.p[RBTP_P_TEMP][pid];
+        ted = 0.0;
+        ked = 0.0;
+    }
+    out[idx] = r;
+    out[n + idx] = u;
+    out[2 * n + idx] = v;
+    out[3 * n + idx] = w;
+    out[4 * n + idx] = T;
+    out[5 * n + idx] = ted;
+    out[6 * n + idx] = ked;
+}
+
+// output_profile (output_statis.cu:225-285): per x-plane sums of ux, uy, uz, phi after
+// overwrite_particle_nodes, from the seven output fields.
+__global__ __launch_bounds__(DB) void k_profile(const double *fields, Dims d, double *out) {
+    __shared__ double red[4][DB];
+    const int ix = blockIdx.x, t = threadIdx.x;
+    const size_t n = (size_t)d.n;
+    double s[4] = {0.0, 0.0, 0.0, 0.0};
+    const int m = d.ly * d.lz;
+    for (int j = t; j < m; j += DB) {
+        const int iz = j / d.ly, iy = j - iz * d.ly;
+        const size_t idx = (size_t)iz * d.lxy + (size_t)iy * d.lx + ix;
+        for (int c = 0; c < 4; c++) s[c] += fields[(size_t)(c + 1) * n + idx];
+    }
+    for (int c = 0; c < 4; c++) red[c][t] = s[c];
+    block_tree<4>(red, t);
+    if (t == 0)
+        for (int c = 0; c < 4; c++) out[ix * 4 + c] = red[c][0];
+}
+
 constexpr int RED_BLOCKS = 1024;
 
 #define DHIP(call)                                         \
@@ -206,6 +313,47 @@ int diag_kinetic_energy(const Dims &dims, double *const *macro, hipStream_t st, 
     double s = 0.0;
     for (int b = 0; b < RED_BLOCKS; b++) s += h[b];
     *out = s / (double)dims.n;
+    return RBTP_OK;
+}
+
+
+int diag_output_fields(int prec, const void *fc, const void *gc, const void *fx, double *const *macro,
+                       const PartDev *pd, const OutK &k, const Dims &dims, hipStream_t st,
+                       double *dev_out) {
+    PartDev empty;
+    std::memset(&empty, 0, sizeof(empty));
+    const PartDev &p = pd ? *pd : empty;
+    const int32_t *owner = pd ? pd->owner[pd->geo_cur] : nullptr;
+    const int nb = (dims.n + DB - 1) / DB;
+    hipError_t e;
+    switch (prec) {
+#define OUTF(S)                                                                                   \
+    hipLaunchKernelGGL((k_output_fields<S>), nb, DB, 0, st, (const S *)fc, (const S *)gc,         \
+                       (const typename Enc<S>::A *)fx, macro[0], macro[1], macro[2], macro[3],   \
+                       macro[4], owner, p, k, dims, dev_out);                                     \
+    e = hipGetLastError();                                                                        \
+    break;
+        case RBTP_PREC_FP64: OUTF(double)
+        case RBTP_PREC_FP32: OUTF(float)
+        case RBTP_PREC_FP16S: OUTF(__half)
+        default: OUTF(__hip_bfloat16)
+#undef OUTF
+    }
+    if (e != hipSuccess) return report_hip(e, "output fields");
+    return RBTP_OK;
+}
+
+int diag_profile(const double *dev_fields, const Dims &dims, hipStream_t st, double *out) {
+    DevBuf buf;
+    DHIP(hipMalloc(&buf.p, (size_t)4 * dims.lx * sizeof(double)));
+    hipLaunchKernelGGL(k_profile, dims.lx, DB, 0, st, dev_fields, dims, buf.p);
+    DHIP(hipGetLastError());
+    std::vector<double> h((size_t)4 * dims.lx);
+    DHIP(hipMemcpyAsync(h.data(), buf.p, h.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+    DHIP(hipStreamSynchronize(st));
+    const double count = (double)dims.ly * (double)dims.lz;
+    const double inv_count = (count > 0) ? (1.0 / count) : 0.0;
+    for (size_t i = 0; i < h.size(); i++) out[i] = h[i] * inv_count;
     return RBTP_OK;
 }
 

@@ -24,4 +24,9 @@ hipError_t launch_init(S *f, S *g, typename Enc<S>::A *fx, typename Enc<S>::A *r
                        double beta, Dims d, double *mr, double *mux, double *muy, double *muz,
                        double *mphi, hipStream_t st);
 
+template <class S>
+hipError_t launch_forcing_field(typename Enc<S>::A *fx, typename Enc<S>::A *rho_wall,
+                                const double *rho, const double *phi, double gb, double t0,
+                                Dims d, hipStream_t st);
+
 }  // namespace rbtp

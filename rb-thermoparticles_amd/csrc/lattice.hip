@@ -380,6 +380,22 @@ __global__ __launch_bounds__(BLOCK) void k_special_collide(const StepArgs<S> a, 
     write_node_aux<S, T>(a, idx, ix, iy, iz, fx, rho, ux, uy, uz, phi);
 }
 
+// forcing (macrovar.cu:36-47) on a loaded phi, and the wall rho from a loaded rho field
+// (restart, initial.cu:206-209).
+template <class S>
+__global__ __launch_bounds__(BLOCK) void k_forcing_field(typename Enc<S>::A *fx,
+                                                         typename Enc<S>::A *rho_wall,
+                                                         const double *rho, const double *phi,
+                                                         double gb, double t0, Dims d) {
+    const int idx = blockIdx.x * BLOCK + threadIdx.x;
+    if (idx >= d.n) return;
+    using A = typename Enc<S>::A;
+    fx[idx] = A(gb * (phi[idx] - t0));
+    const int iz = idx / d.lxy, rem = idx - iz * d.lxy, iy = rem / d.lx, ix = rem - iy * d.lx;
+    if (ix == 0) rho_wall[iz * d.ly + iy] = A(rho[idx]);
+    if (ix == d.lx - 1) rho_wall[d.ly * d.lz + iz * d.ly + iy] = A(rho[idx]);
+}
+
 inline int blocks_for(size_t n) { return (int)((n + BLOCK - 1) / BLOCK); }
 
 }  // namespace
@@ -452,6 +468,15 @@ hipError_t launch_init(S *f, S *g, typename Enc<S>::A *fx, typename Enc<S>::A *r
     return hipGetLastError();
 }
 
+template <class S>
+hipError_t launch_forcing_field(typename Enc<S>::A *fx, typename Enc<S>::A *rho_wall,
+                                const double *rho, const double *phi, double gb, double t0,
+                                Dims d, hipStream_t st) {
+    hipLaunchKernelGGL((k_forcing_field<S>), blocks_for((size_t)d.n), BLOCK, 0, st, fx, rho_wall,
+                       rho, phi, gb, t0, d);
+    return hipGetLastError();
+}
+
 #define RBTP_INSTANTIATE(S)                                                                     \
     template hipError_t launch_lattice_step<S>(const StepArgs<S> &, bool, bool, hipStream_t);   \
     template hipError_t launch_special_collide<S>(const StepArgs<S> &, const SpecialBuf &, int, \
@@ -464,7 +489,10 @@ hipError_t launch_init(S *f, S *g, typename Enc<S>::A *fx, typename Enc<S>::A *r
     template hipError_t launch_init<S>(S *, S *, typename Enc<S>::A *, typename Enc<S>::A *,    \
                                        const double *, double, double, double, double, Dims,    \
                                        double *, double *, double *, double *, double *,        \
-                                       hipStream_t);
+                                       hipStream_t);                                            \
+    template hipError_t launch_forcing_field<S>(typename Enc<S>::A *, typename Enc<S>::A *,     \
+                                                const double *, const double *, double, double, \
+                                                Dims, hipStream_t);
 
 RBTP_INSTANTIATE(double)
 RBTP_INSTANTIATE(float)

@@ -40,6 +40,7 @@ int part_alloc(PartDev &pd, const rbtp_case &c, const rbtp_derived &d, const Dim
 void part_free(PartDev &pd);
 int part_init(PartDev &pd, const rbtp_case &c, const rbtp_derived &d, const Dims &dims,
               const double *ppos, const double *ptemp, hipStream_t st);
+int part_load_state(PartDev &pd, const double *host, const Dims &dims, hipStream_t st);
 int part_phase(PartDev &pd, const rbtp_case &c, const rbtp_derived &d, const Dims &dims,
                int prec, const void *fcur, const void *gcur, void *fx, void *rho_wall,
                double *const *macro, hipStream_t st);
@@ -50,6 +51,15 @@ int part_collect_links(PartDev &pd, const Dims &dims, hipStream_t st, std::vecto
 int part_collect_new_fluid(PartDev &pd, hipStream_t st,
                            std::vector<std::pair<int32_t, int32_t>> &out);
 
+// Constants of the output-field kernels (compute_Ked / compute_ted, overwrite_particle_nodes).
+struct OutK {
+    double tau, tauc, visc, diff, t0, rho0;
+};
+
+int diag_output_fields(int prec, const void *fc, const void *gc, const void *fx,
+                       double *const *macro, const PartDev *pd, const OutK &k, const Dims &dims,
+                       hipStream_t st, double *dev_out);
+int diag_profile(const double *dev_fields, const Dims &dims, hipStream_t st, double *out);
 int diag_nu(const rbtp_case &c, const rbtp_derived &d, const Dims &dims, const double *ux,
             const double *phi, const int32_t *owner, int step, hipStream_t st, double out[7]);
 int diag_means(const Dims &dims, double *const *macro, const PartDev *pd, hipStream_t st,

@@ -853,12 +853,32 @@ static int place_particles(const rbtp_case &c, const rbtp_derived &d, std::vecto
     return RBTP_OK;
 }
 
-// init_particles (pinit.cu:183-295) and build_initial_ibnode (pinit.cu:146-179); both geometry
-// generations start as the initial one.
+// Upload a full particle state (RBTP_P_COUNT arrays of np doubles) and build both geometry
+// generations from its positions (build_initial_ibnode, pinit.cu:146-179, and build_links'
+// marking, links.cu:143-191).
+int part_load_state(PartDev &pd, const double *host, const Dims &dims, hipStream_t st) {
+    const int np = pd.np;
+    PHIP(hipMemsetAsync(pd.base, 0, (size_t)(RBTP_P_COUNT + OLD_COUNT) * np * sizeof(double), st));
+    PHIP(hipMemcpyAsync(pd.base, host, (size_t)RBTP_P_COUNT * np * sizeof(double), hipMemcpyHostToDevice, st));
+    PHIP(hipMemsetAsync(pd.owner[0], 0xFF, (size_t)dims.n * sizeof(int32_t), st));
+    hipLaunchKernelGGL(k_geo_mark, np, PB, 0, st, pd.owner[0], pd.box[0], pd.p[RBTP_P_POS_X],
+                       pd.p[RBTP_P_POS_Y], pd.p[RBTP_P_POS_Z], pd.radius, pd.radius_sq, dims);
+    PHIP(hipGetLastError());
+    PHIP(hipMemcpyAsync(pd.owner[1], pd.owner[0], (size_t)dims.n * sizeof(int32_t), hipMemcpyDeviceToDevice, st));
+    PHIP(hipMemcpyAsync(pd.box[1], pd.box[0], 6 * (size_t)np * sizeof(int32_t), hipMemcpyDeviceToDevice, st));
+    PHIP(hipMemsetAsync(pd.err, 0, ERR_COUNT * sizeof(int32_t), st));
+    PHIP(hipMemsetAsync(pd.sp.count, 0, sizeof(int), st));
+    PHIP(hipStreamSynchronize(st));
+    pd.geo_cur = 0;
+    pd.special_pending = false;
+    return RBTP_OK;
+}
+
+// init_particles (pinit.cu:183-295): zero state, positions (given or placed), temperatures.
 int part_init(PartDev &pd, const rbtp_case &c, const rbtp_derived &d, const Dims &dims,
               const double *ppos, const double *ptemp, hipStream_t st) {
     const int np = pd.np;
-    std::vector<double> host((size_t)(RBTP_P_COUNT + OLD_COUNT) * np, 0.0);
+    std::vector<double> host((size_t)RBTP_P_COUNT * np, 0.0);
     std::vector<double> X(np), Y(np), Z(np);
     if (ppos) {
         for (int p = 0; p < np; p++) {
@@ -876,18 +896,7 @@ int part_init(PartDev &pd, const rbtp_case &c, const rbtp_derived &d, const Dims
         host[(size_t)RBTP_P_POS_Z * np + p] = Z[p];
         host[(size_t)RBTP_P_TEMP * np + p] = ptemp ? ptemp[p] : c.particle_init_temp;
     }
-    PHIP(hipMemcpyAsync(pd.base, host.data(), host.size() * sizeof(double), hipMemcpyHostToDevice, st));
-    PHIP(hipMemsetAsync(pd.owner[0], 0xFF, (size_t)dims.n * sizeof(int32_t), st));
-    hipLaunchKernelGGL(k_geo_mark, np, PB, 0, st, pd.owner[0], pd.box[0], pd.p[RBTP_P_POS_X],
-                       pd.p[RBTP_P_POS_Y], pd.p[RBTP_P_POS_Z], pd.radius, pd.radius_sq, dims);
-    PHIP(hipGetLastError());
-    PHIP(hipMemcpyAsync(pd.owner[1], pd.owner[0], (size_t)dims.n * sizeof(int32_t), hipMemcpyDeviceToDevice, st));
-    PHIP(hipMemcpyAsync(pd.box[1], pd.box[0], 6 * (size_t)np * sizeof(int32_t), hipMemcpyDeviceToDevice, st));
-    PHIP(hipMemsetAsync(pd.err, 0, ERR_COUNT * sizeof(int32_t), st));
-    PHIP(hipMemsetAsync(pd.sp.count, 0, sizeof(int), st));
-    pd.geo_cur = 0;
-    pd.special_pending = false;
-    return RBTP_OK;
+    return part_load_state(pd, host.data(), dims, st);
 }
 
 // One particle phase (main.cu:52-84 minus the lattice passes), enqueued on st.

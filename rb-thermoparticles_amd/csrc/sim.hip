@@ -678,6 +678,194 @@ extern "C" int rbtp_kinetic_energy(rbtp_sim *s, double *out) {
     return RBTP_OK;
 }
 
+
+// ---------------------------------------------------------------------------------------------
+// Output fields and files
+// ---------------------------------------------------------------------------------------------
+
+static OutK out_consts(const rbtp_sim *s) {
+    OutK k;
+    k.tau = s->d.tau;
+    k.tauc = s->d.tauc;
+    k.visc = s->c.visc;
+    k.diff = s->d.diff;
+    k.t0 = s->c.t_ref;
+    k.rho0 = s->c.rho0;
+    return k;
+}
+
+// The seven %09d.dat fields on the device (caller frees *dev).
+static int output_fields_dev(rbtp_sim *s, double **dev) {
+    if (!s->macro_valid)
+        return set_error(RBTP_E_STATE, "run the last step with RBTP_STEP_MACRO_LAST first");
+    const size_t n = (size_t)s->dims.n;
+    HIP_TRY(hipMalloc(dev, 7 * n * sizeof(double)));
+    const int r = diag_output_fields(s->prec, s->f[s->cur], s->g[s->cur], s->fx, s->macro,
+                                     s->particles ? &s->pd : nullptr, out_consts(s), s->dims,
+                                     s->stream, *dev);
+    if (r != RBTP_OK) {
+        (void)hipFree(*dev);
+        *dev = nullptr;
+    }
+    return r;
+}
+
+extern "C" int rbtp_get_output_fields(rbtp_sim *s, double *out) {
+    if (!s || !out) return set_error(RBTP_E_ARG, "null argument");
+    HIP_TRY(hipSetDevice(s->device));
+    double *dev = nullptr;
+    RB_TRY(output_fields_dev(s, &dev));
+    hipError_t e = hipMemcpyAsync(out, dev, 7 * (size_t)s->dims.n * sizeof(double),
+                                  hipMemcpyDeviceToHost, s->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+    (void)hipFree(dev);
+    if (e != hipSuccess) return set_error(RBTP_E_HIP, "rbtp_get_output_fields: %s", hipGetErrorString(e));
+    return RBTP_OK;
+}
+
+extern "C" int rbtp_profile(rbtp_sim *s, double *out) {
+    if (!s || !out) return set_error(RBTP_E_ARG, "null argument");
+    HIP_TRY(hipSetDevice(s->device));
+    double *dev = nullptr;
+    RB_TRY(output_fields_dev(s, &dev));
+    const int r = diag_profile(dev, s->dims, s->stream, out);
+    (void)hipFree(dev);
+    return r;
+}
+
+static int write_all(FILE *fp, const double *p, size_t count, const char *path) {
+    if (fwrite(p, sizeof(double), count, fp) != count)
+        return set_error(RBTP_E_IO, "short write to %s", path);
+    return RBTP_OK;
+}
+
+extern "C" int rbtp_write_fg(rbtp_sim *s, const char *path) {
+    if (!s || !path) return set_error(RBTP_E_ARG, "null argument");
+    const size_t n = 27 * (size_t)s->dims.n;
+    std::vector<double> buf(n);
+    FILE *fp = fopen(path, "wb");
+    if (!fp) return set_error(RBTP_E_IO, "cannot open %s", path);
+    int r = rbtp_get_populations(s, RBTP_POP_F, buf.data());
+    if (r == RBTP_OK) r = write_all(fp, buf.data(), n, path);
+    if (r == RBTP_OK) r = rbtp_get_populations(s, RBTP_POP_G, buf.data());
+    if (r == RBTP_OK) r = write_all(fp, buf.data(), n, path);
+    if (fclose(fp) != 0 && r == RBTP_OK) r = set_error(RBTP_E_IO, "close failed on %s", path);
+    return r;
+}
+
+extern "C" int rbtp_write_flow(rbtp_sim *s, const char *path) {
+    if (!s || !path) return set_error(RBTP_E_ARG, "null argument");
+    const size_t n = (size_t)s->dims.n;
+    std::vector<double> buf(7 * n);
+    RB_TRY(rbtp_get_output_fields(s, buf.data()));
+    FILE *fp = fopen(path, "wb");
+    if (!fp) return set_error(RBTP_E_IO, "cannot open %s", path);
+    int r = write_all(fp, buf.data(), 7 * n, path);
+    if (fclose(fp) != 0 && r == RBTP_OK) r = set_error(RBTP_E_IO, "close failed on %s", path);
+    return r;
+}
+
+// Column order of pos_%09d.txt (output_part.cu:101-113).
+static const int kPosColumns[33] = {
+    RBTP_P_POS_X, RBTP_P_POS_Y, RBTP_P_POS_Z, RBTP_P_VEL_X, RBTP_P_VEL_Y, RBTP_P_VEL_Z,
+    RBTP_P_FORCE_X, RBTP_P_FORCE_Y, RBTP_P_FORCE_Z, RBTP_P_FORCE_PREV_X, RBTP_P_FORCE_PREV_Y,
+    RBTP_P_FORCE_PREV_Z, RBTP_P_FORCE_PREV2_X, RBTP_P_FORCE_PREV2_Y, RBTP_P_FORCE_PREV2_Z,
+    RBTP_P_THETA_X, RBTP_P_THETA_Y, RBTP_P_THETA_Z, RBTP_P_OMEGA_X, RBTP_P_OMEGA_Y,
+    RBTP_P_OMEGA_Z, RBTP_P_TORQUE_X, RBTP_P_TORQUE_Y, RBTP_P_TORQUE_Z, RBTP_P_TORQUE_PREV_X,
+    RBTP_P_TORQUE_PREV_Y, RBTP_P_TORQUE_PREV_Z, RBTP_P_TORQUE_PREV2_X, RBTP_P_TORQUE_PREV2_Y,
+    RBTP_P_TORQUE_PREV2_Z, RBTP_P_TEMP, RBTP_P_HEAT, RBTP_P_HEAT_PREV};
+
+extern "C" int rbtp_write_particles(rbtp_sim *s, const char *path) {
+    if (!s || !path) return set_error(RBTP_E_ARG, "null argument");
+    if (!s->particles) return RBTP_OK;
+    const int np = s->pd.np;
+    HIP_TRY(hipSetDevice(s->device));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    std::vector<double> h((size_t)RBTP_P_COUNT * np);
+    HIP_TRY(hipMemcpy(h.data(), s->pd.base, h.size() * sizeof(double), hipMemcpyDeviceToHost));
+    FILE *fp = fopen(path, "w");
+    if (!fp) return set_error(RBTP_E_IO, "cannot open %s", path);
+    for (int pid = 0; pid < np; ++pid) {
+        fprintf(fp, "%d", pid);
+        for (int c = 0; c < 33; c++)
+            fprintf(fp, c == 32 ? " %14.6e\n" : " %14.6e", h[(size_t)kPosColumns[c] * np + pid]);
+    }
+    if (fclose(fp) != 0) return set_error(RBTP_E_IO, "close failed on %s", path);
+    return RBTP_OK;
+}
+
+// load_restart_state (initial.cu:194-236).
+extern "C" int rbtp_restart_load(rbtp_sim *s, const char *dir, int step) {
+    if (!s || !dir) return set_error(RBTP_E_ARG, "null argument");
+    if (step < 0) return set_error(RBTP_E_ARG, "negative restart step");
+    if (!s->initialised) {
+        // allocate the state the way rbtp_init would, then replace it
+        RB_TRY(rbtp_init(s, nullptr, nullptr, nullptr));
+    }
+    HIP_TRY(hipSetDevice(s->device));
+    const size_t n = (size_t)s->dims.n;
+    char path[4096];
+    // load_distributions_from_restart (initial.cu:7-30)
+    snprintf(path, sizeof(path), "%s/fg%09d.dat", dir, step);
+    std::vector<double> f(27 * n), g(27 * n);
+    {
+        FILE *fp = fopen(path, "rb");
+        if (!fp) return set_error(RBTP_E_IO, "Failed to open restart distribution file %s", path);
+        const size_t rf = fread(f.data(), sizeof(double), f.size(), fp);
+        const size_t rg = fread(g.data(), sizeof(double), g.size(), fp);
+        fclose(fp);
+        if (rf != f.size() || rg != g.size())
+            return set_error(RBTP_E_IO, "Restart distribution file %s is corrupted (read %zu/%zu for f, %zu/%zu for g)",
+                             path, rf, f.size(), rg, g.size());
+    }
+    // load_particles_from_restart (initial.cu:57-192)
+    if (s->particles) {
+        const int np = s->pd.np;
+        snprintf(path, sizeof(path), "%s/pos/pos_%09d.txt", dir, step);
+        FILE *fp = fopen(path, "r");
+        if (!fp) return set_error(RBTP_E_IO, "Failed to open restart particle file %s", path);
+        std::vector<double> h((size_t)RBTP_P_COUNT * np, 0.0);
+        for (int line = 0; line < np; line++) {
+            int pid = -1;
+            double v[33];
+            bool ok = fscanf(fp, "%d", &pid) == 1;
+            for (int c = 0; ok && c < 33; c++) ok = fscanf(fp, "%lf", &v[c]) == 1;
+            if (!ok || pid < 0 || pid >= np) {
+                fclose(fp);
+                return set_error(RBTP_E_IO, "Failed to read particle restart data from %s", path);
+            }
+            for (int c = 0; c < 33; c++) h[(size_t)kPosColumns[c] * np + pid] = v[c];
+        }
+        fclose(fp);
+        RB_TRY(part_load_state(s->pd, h.data(), s->dims, s->stream));
+    }
+    RB_TRY(rbtp_set_populations(s, f.data(), g.data()));
+    // load_flow_fields (initial.cu:32-55) and forcing on the loaded phi (initial.cu:207)
+    snprintf(path, sizeof(path), "%s/%09d.dat", dir, step);
+    {
+        std::vector<double> fields(7 * n);
+        FILE *fp = fopen(path, "rb");
+        if (!fp) return set_error(RBTP_E_IO, "Failed to open flow restart file %s", path);
+        const size_t rd = fread(fields.data(), sizeof(double), fields.size(), fp);
+        fclose(fp);
+        if (rd != fields.size())
+            return set_error(RBTP_E_IO, "flow restart file %s is short (%zu of %zu values)", path, rd, fields.size());
+        for (int k = 0; k < 5; k++)
+            HIP_TRY(hipMemcpy(s->macro[k], fields.data() + k * n, n * sizeof(double), hipMemcpyHostToDevice));
+        HIP_TRY(dispatch(s->prec, [&](auto *tag) {
+            using S = std::remove_pointer_t<decltype(tag)>;
+            return launch_forcing_field<S>((typename Enc<S>::A *)s->fx,
+                                           (typename Enc<S>::A *)s->rho_wall, s->macro[0],
+                                           s->macro[4], s->k.gb, s->c.t_ref, s->dims, s->stream);
+        }));
+        HIP_TRY(hipStreamSynchronize(s->stream));
+    }
+    s->poststream = true;
+    s->macro_valid = true;
+    s->step = step;
+    return RBTP_OK;
+}
+
 // Error helper for the other translation units.
 namespace rbtp {
 int report(int code, const char *msg) { return set_error(code, "%s", msg); }

@@ -1,0 +1,41 @@
+"""Case descriptions used by the tests and the benchmark (parameters.h fields by name).
+
+The benchmark cases run with T_HOT=+0.5, T_COLD=-0.5, T_REF=0 and an initial temperature that
+is linear between the walls plus seeded noise.  At Ra=1e8 the reference's default VISC would give
+a free-fall velocity of 0.45 lattice units (globals.cu:26-30), so those cases set
+VISC = 0.1414*LX*sqrt(Pr/Ra), the free-fall velocity of the reference's own case (0.1414).
+"""
+import math
+
+RB_BASE = dict(t_hot=0.5, t_cold=-0.5, t_ref=0.0, activate_particles=0, particle_fraction=0.0)
+
+
+def visc_for(lx, ra, pr):
+    return 0.1414 * lx * math.sqrt(pr / ra)
+
+
+def single_phase(lx, ly, lz, ra=1e5, pr=0.71, visc=None, **kw):
+    d = dict(RB_BASE, lx=lx, ly=ly, lz=lz, rayleigh=ra, prandtl=pr)
+    if visc is not None:
+        d["visc"] = visc
+    d.update(kw)
+    return d
+
+
+def config1():
+    """64x64x32, Ra=1e5, Pr=0.71, default BETA and VISC (g0=1.2503, u0=0.0707)."""
+    return single_phase(64, 64, 32, ra=1e5)
+
+
+def config2(precision=0):
+    """320x320x160, Ra=1e8, Pr=0.71."""
+    return single_phase(320, 320, 160, ra=1e8, visc=visc_for(320, 1e8, 0.71), precision=precision)
+
+
+def particle_case(lx, ly, lz, npart, diam, ra=1e5, pr=0.71, beta=None, **kw):
+    d = single_phase(lx, ly, lz, ra=ra, pr=pr, **kw)
+    d.update(activate_particles=1, npart_override=npart, particle_diam_frac=diam / lx,
+             rho_particle_ratio=2.0, cp_particle_ratio=1.0, particle_fraction=0.0)
+    if beta is not None:
+        d["beta"] = beta
+    return d

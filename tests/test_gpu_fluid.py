@@ -1,0 +1,117 @@
+"""GPU parity of the single-phase lattice path against the FP64 oracle.
+
+The FP64 form evaluates every operation in the reference's order with no fused multiply-add, so
+the expected agreement is bitwise; the tolerance written in each assertion is the bar
+(fields within 1e-12 of the field's scale, Nu within 1e-10) and the bitwise checks say where
+equality is expected.
+"""
+import numpy as np
+import pytest
+
+from cases import config1, single_phase
+
+pytestmark = pytest.mark.gpu
+
+
+def rel_err(a, b):
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    scale = max(np.max(np.abs(b)), 1e-300)
+    return float(np.max(np.abs(a - b)) / scale)
+
+
+def make_pair(rb, O, case_kw, seed=11, amp=1e-2, temperature=True):
+    case = rb.default_case(**case_kw)
+    sim = rb.Sim(case)
+    ora = O.Oracle(case)
+    T = rb.linear_temperature(case, seed, amp) if temperature else None
+    sim.init(temperature=T)
+    ora.init(temperature=T)
+    return sim, ora
+
+
+def test_init_state_bitwise(rb, O):
+    sim, ora = make_pair(rb, O, single_phase(24, 20, 16))
+    for name in ("f", "g", "f_collide", "g_collide"):
+        assert np.array_equal(sim.populations(name), ora.populations(name)), name
+    for name in ("rho", "ux", "phi", "fx"):
+        assert np.array_equal(sim.field(name), ora.field(name)), name
+
+
+@pytest.mark.parametrize("shape", [(24, 20, 16), (17, 9, 5), (3, 3, 3)])
+def test_one_step_bitwise(rb, O, shape):
+    sim, ora = make_pair(rb, O, single_phase(*shape, ra=1e5))
+    sim.step(1)
+    ora.step(1)
+    for name in ("f_collide", "g_collide", "f", "g"):
+        a, b = sim.populations(name), ora.populations(name)
+        assert rel_err(a, b) <= 1e-12, name
+        assert np.array_equal(a, b), f"{name}: not bitwise (max diff {np.max(np.abs(a - b))})"
+    for name in ("rho", "ux", "uy", "uz", "phi", "fx"):
+        a, b = sim.field(name), ora.field(name)
+        assert np.array_equal(a, b), name
+
+
+def test_multi_step_bitwise(rb, O):
+    sim, ora = make_pair(rb, O, single_phase(32, 24, 16, ra=1e6))
+    for n in (1, 4, 25):
+        sim.step(n)
+        ora.step(n)
+        for name in ("f", "g"):
+            a, b = sim.populations(name), ora.populations(name)
+            assert rel_err(a, b) <= 1e-12, (n, name)
+            assert np.array_equal(a, b), (n, name)
+        assert np.array_equal(sim.field("phi"), ora.field("phi"))
+
+
+def test_config1_1000_steps_fields_and_nu(rb, O):
+    """64x64x32, Ra=1e5, Pr=0.71: fields to 1e-12 relative and Nu to 1e-10 after 1000 steps."""
+    sim, ora = make_pair(rb, O, config1(), seed=2024, amp=1e-3)
+    sim.step(1000)
+    ora.step(1000)
+    for name in ("f", "g", "f_collide", "g_collide"):
+        assert rel_err(sim.populations(name), ora.populations(name)) <= 1e-12, name
+    for name in ("rho", "ux", "uy", "uz", "phi", "fx"):
+        assert rel_err(sim.field(name), ora.field(name)) <= 1e-12, name
+    nu_gpu, nu_cpu = sim.nu(), ora.nu()
+    assert np.all(np.abs(nu_gpu - nu_cpu) <= 1e-10 * np.maximum(1.0, np.abs(nu_cpu))), (nu_gpu, nu_cpu)
+    m_gpu, m_cpu = sim.means(), ora.means()
+    assert np.all(np.abs(m_gpu - m_cpu) <= 1e-12 + 1e-10 * np.abs(m_cpu))
+
+
+def test_uniform_temperature_rest_state(rb, O):
+    """f = 0, g = w T with T_hot = T_cold = T (so no gravity) stays at rest: f bitwise zero,
+    g within rounding of w T (the weights sum to 1 - 2^-52 in FP64), and the GPU agrees with
+    the oracle bitwise from the same start."""
+    case = rb.default_case(**single_phase(16, 12, 10, t_hot=0.25, t_cold=0.25, t_ref=0.25))
+    sim, ora = rb.Sim(case), O.Oracle(case)
+    sim.init()
+    ora.init()
+    w = np.array([8 / 27] + [2 / 27] * 6 + [1 / 54] * 12 + [1 / 216] * 8)
+    g0 = np.repeat((w * 0.25)[:, None], sim.n, axis=1)
+    f0 = np.zeros_like(g0)
+    sim.set_populations(f0, g0)
+    for name, v in (("f", f0), ("g", g0), ("f_collide", f0), ("g_collide", g0)):
+        ora.set_populations(name, v)
+    sim.step(50)
+    ora.step(50)
+    assert np.all(sim.populations("f") == 0.0)
+    assert rel_err(sim.populations("g"), g0) <= 1e-14
+    assert np.array_equal(sim.populations("g"), ora.populations("g"))
+
+
+@pytest.mark.parametrize("prec", [1, 2, 3])
+def test_low_precision_forms_track_fp64(rb, prec):
+    """FP32 math with FP32/FP16/BF16 storage stays close to FP64 over a short run."""
+    kw = single_phase(32, 32, 16, ra=1e5)
+    ref = rb.Sim(rb.default_case(**kw))
+    low = rb.Sim(rb.default_case(**kw, precision=prec))
+    T = rb.linear_temperature(ref.case, 5, 1e-3)
+    ref.init(temperature=T)
+    low.init(temperature=T)
+    ref.step(200)
+    low.step(200)
+    tol = {1: 1e-5, 2: 2e-3, 3: 2e-2}[prec]
+    assert rel_err(low.field("phi"), ref.field("phi")) <= tol
+    nu_r, nu_l = ref.nu(), low.nu()
+    assert abs(nu_l[4] - nu_r[4]) <= 10 * tol * abs(nu_r[4])

@@ -1,0 +1,178 @@
+"""GPU tests of the output side and the restart layout.
+
+Output fields (%09d.dat order: rho, ux, uy, uz, phi, ted, Ked, with particle nodes overwritten)
+and the plane profile are compared with the oracle; files are checked for their byte layout;
+a restart from fg%09d.dat + %09d.dat continues a single-phase run bit for bit, because f, g and
+F_x (recomputed from the stored phi, initial.cu:207) are all exact.
+"""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from cases import particle_case, single_phase
+
+pytestmark = pytest.mark.gpu
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def rel_err(a, b):
+    scale = max(np.max(np.abs(b)), 1e-300)
+    return float(np.max(np.abs(np.asarray(a) - np.asarray(b))) / scale)
+
+
+def pair(rb, O, kw, ppos=None, steps=7):
+    case = rb.default_case(**kw)
+    sim, ora = rb.Sim(case), O.Oracle(case)
+    T = rb.linear_temperature(case, 8, 1e-2)
+    p = None if ppos is None else np.asarray(ppos, float).T.reshape(-1)
+    sim.init(temperature=T, ppos=p)
+    ora.init(temperature=T, ppos=p)
+    sim.step(steps)
+    ora.step(steps)
+    return sim, ora
+
+
+@pytest.mark.parametrize("particles", [False, True])
+def test_output_fields_and_profile(rb, O, particles):
+    if particles:
+        kw = particle_case(24, 22, 20, npart=2, diam=6.0, ra=1e5, beta=0.05)
+        sim, ora = pair(rb, O, kw, ppos=[(8.2, 10.1, 9.7), (15.9, 12.4, 10.3)])
+    else:
+        sim, ora = pair(rb, O, single_phase(20, 18, 12, ra=1e6))
+    a, b = sim.output_fields(), ora.output_fields()
+    for k, name in enumerate(("rho", "ux", "uy", "uz", "phi", "ted", "Ked")):
+        assert rel_err(a[k], b[k]) <= 1e-12, name
+    assert rel_err(sim.profile(), ora.profile()) <= 1e-12
+
+
+def test_file_layouts(rb, O, tmp_path):
+    kw = particle_case(20, 16, 14, npart=2, diam=5.0, ra=1e5, beta=0.05)
+    sim, ora = pair(rb, O, kw, ppos=[(6.2, 8.1, 7.7), (13.9, 8.4, 6.3)])
+    n = sim.n
+    fg = tmp_path / "fg000000007.dat"
+    sim.write_fg(str(fg))
+    raw = np.fromfile(fg, dtype="<f8")
+    assert raw.size == 2 * 27 * n
+    assert np.array_equal(raw[:27 * n], sim.populations("f").reshape(-1))
+    assert np.array_equal(raw[27 * n:], sim.populations("g").reshape(-1))
+    flow = tmp_path / "000000007.dat"
+    sim.write_flow(str(flow))
+    raw = np.fromfile(flow, dtype="<f8")
+    assert raw.size == 7 * n
+    assert np.array_equal(raw, sim.output_fields().reshape(-1))
+    pos = tmp_path / "pos_000000007.txt"
+    sim.write_particles(str(pos))
+    lines = pos.read_text().splitlines()
+    assert len(lines) == 2
+    for pid, line in enumerate(lines):
+        cols = line.split()
+        assert len(cols) == 34 and int(cols[0]) == pid
+        assert all(len(c) <= 14 and "e" in c for c in cols[1:])
+        assert float(cols[1]) == pytest.approx(sim.particles("pos_x")[pid], rel=1e-6)
+        assert float(cols[31]) == pytest.approx(sim.particles("temp")[pid], rel=1e-6, abs=1e-12)
+
+
+def test_single_phase_restart_is_bit_continuous(rb, tmp_path):
+    kw = single_phase(24, 20, 16, ra=1e6)
+    case = rb.default_case(**kw)
+    T = rb.linear_temperature(case, 5, 1e-2)
+    full = rb.Sim(case)
+    full.init(temperature=T)
+    full.step(30)
+    part = rb.Sim(case)
+    part.init(temperature=T)
+    part.step(12)
+    d = tmp_path / "Ra"
+    d.mkdir()
+    part.write_fg(str(d / "fg000000012.dat"))
+    part.write_flow(str(d / "000000012.dat"))
+    again = rb.Sim(rb.default_case(**kw, continuous_steps=12))
+    again.restart_load(str(d), 12)
+    assert again.current_step == 12
+    again.step(18)
+    for name in ("f", "g"):
+        assert np.array_equal(again.populations(name), full.populations(name)), name
+    assert np.array_equal(again.field("phi"), full.field("phi"))
+
+
+def test_restart_with_particles_reads_pos_file(rb, tmp_path):
+    kw = particle_case(20, 16, 14, npart=2, diam=5.0, ra=1e5, beta=0.05)
+    case = rb.default_case(**kw)
+    sim = rb.Sim(case)
+    sim.init(temperature=rb.linear_temperature(case, 1, 1e-2),
+             ppos=np.array([6.2, 13.9, 8.1, 8.4, 7.7, 6.3]))
+    sim.step(9)
+    d = tmp_path / "dir"
+    (d / "pos").mkdir(parents=True)
+    sim.write_fg(str(d / "fg000000009.dat"))
+    sim.write_flow(str(d / "000000009.dat"))
+    sim.write_particles(str(d / "pos" / "pos_000000009.txt"))
+    again = rb.Sim(case)
+    again.restart_load(str(d), 9)
+    txt = np.loadtxt(d / "pos" / "pos_000000009.txt")
+    assert np.array_equal(again.particles("pos_x"), txt[:, 1])
+    assert np.array_equal(again.particles("temp"), txt[:, 31])
+    ib_a, ow_a = again.flags()
+    ib_b, ow_b = sim.flags()
+    assert abs(int(ib_a.sum()) - int(ib_b.sum())) <= 4  # positions kept to 7 digits
+    again.step(5)
+    assert np.all(np.isfinite(again.populations("f")))
+
+
+def test_restart_missing_or_short_files(rb, tmp_path):
+    kw = single_phase(12, 10, 8)
+    sim = rb.Sim(rb.default_case(**kw))
+    with pytest.raises(rb.RbtpError) as e:
+        sim.restart_load(str(tmp_path), 5)
+    assert e.value.code == -4
+    np.zeros(27 * 960 + 5).tofile(tmp_path / "fg000000005.dat")
+    with pytest.raises(rb.RbtpError) as e:
+        sim.restart_load(str(tmp_path), 5)
+    assert "corrupted" in str(e.value)
+
+
+def test_rb3d_driver_outputs_and_restart(tmp_path):
+    exe = os.path.join(ROOT, "rb-thermoparticles_amd", "rb3d")
+    params = tmp_path / "parameters.h"
+    body = """#define CONTINUOUS_STEPS {cs}
+#define LX 24
+#define LY 20
+#define LZ 16
+#define NEND {nend}
+#define NDIAG 10
+#define NFLOWOUT 20
+#define NNUOUT 10
+#define NOUT_P 10
+#define RAYLEIGH 1.0e5
+#define PRANDTL 0.71
+#define T_HOT 0.5
+#define T_COLD -0.5
+#define T_REF 0.0
+#define ACTIVATE_PARTICLES 1
+#define PARTICLE_FRACTION 0.01
+#define PARTICLE_DIAM_FRAC (6.0/24.0)
+#define RHO_PARTICLE_RATIO 2.0
+#define BETA 0.05
+"""
+    params.write_text(body.format(cs=0, nend=20))
+    r = subprocess.run([exe, str(params), "--temperature", "linear:3:0.01"], cwd=tmp_path,
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert "Step 10" in r.stdout and "Simulation completed." in r.stdout
+    d = tmp_path / "Ra1.0e+05Pr0.71"
+    n = 24 * 20 * 16
+    assert (d / "fg000000020.dat").stat().st_size == 2 * 27 * n * 8
+    assert (d / "000000020.dat").stat().st_size == 7 * n * 8
+    assert len((d / "statistics.dat").read_text().splitlines()) == 2
+    nu = np.loadtxt(d / "Nu_1.00e+05.txt")
+    assert nu.shape == (2, 7)
+    prof = (d / "Profile_1.00e+05.txt").read_text().splitlines()
+    assert len(prof) == 2 * (1 + 24)
+    assert (d / "pos" / "pos_000000000.txt").exists() and (d / "pos" / "pos_000000020.txt").exists()
+    params.write_text(body.format(cs=20, nend=30))
+    r = subprocess.run([exe, str(params)], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert "Restarted from step: 20" in r.stdout
+    assert (d / "fg000000030.dat").exists()

@@ -1,0 +1,144 @@
+"""GPU parity of the particle coupling against the FP64 oracle.
+
+Bars: solid/fluid flags, owners, link sets (sorted by idx, dir; q and r bitwise) and new-fluid
+masks bit-exact; particle state and fields within 1e-12 relative (bitwise expected, since both
+sides sum link contributions in the same fixed order).
+"""
+import numpy as np
+import pytest
+
+from cases import particle_case
+
+pytestmark = pytest.mark.gpu
+
+P_ARRAYS = ["pos_x", "pos_y", "pos_z", "vel_x", "vel_y", "vel_z", "omega_x", "omega_y",
+            "omega_z", "theta_x", "theta_y", "theta_z", "force_x", "force_y", "force_z",
+            "torque_x", "torque_y", "torque_z", "force_prev_x", "force_prev_y", "force_prev_z",
+            "force_prev2_x", "force_prev2_y", "force_prev2_z", "torque_prev_x", "torque_prev_y",
+            "torque_prev_z", "torque_prev2_x", "torque_prev2_y", "torque_prev2_z", "temp",
+            "heat", "heat_prev"]
+
+
+def rel_err(a, b):
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    if a.size == 0 and b.size == 0:
+        return 0.0
+    scale = max(np.max(np.abs(b)), 1e-300)
+    return float(np.max(np.abs(a - b)) / scale)
+
+
+def sorted_links(L):
+    idx = (L["cell_k"].astype(np.int64) * 100000 + L["cell_j"]) * 100000 + L["cell_i"]
+    order = np.lexsort((L["dir"], idx))
+    return L[order]
+
+
+def make_pair(rb, O, kw, ppos, ptemp=None, seed=3, amp=1e-2):
+    case = rb.default_case(**kw)
+    sim, ora = rb.Sim(case), O.Oracle(case)
+    T = rb.linear_temperature(case, seed, amp)
+    ppos = np.asarray(ppos, dtype=np.float64).T.reshape(-1)  # (P,3) -> x[], y[], z[]
+    sim.init(temperature=T, ppos=ppos, ptemp=ptemp)
+    ora.init(temperature=T, ppos=ppos, ptemp=ptemp)
+    return sim, ora
+
+
+def check_geometry(sim, ora, where):
+    ib_g, ow_g = sim.flags()
+    ib_c, ow_c = ora.flags()
+    assert np.array_equal(ib_g, ib_c), f"{where}: ibnode differs"
+    assert np.array_equal(ow_g, ow_c), f"{where}: owner differs"
+    lg, lc = sim.links(), sorted_links(ora.links())
+    assert len(lg) == len(lc), f"{where}: {len(lg)} links vs {len(lc)}"
+    for k in ("cell_i", "cell_j", "cell_k", "dir", "pid"):
+        assert np.array_equal(lg[k], lc[k]), f"{where}: link {k} differs"
+    for k in ("q", "rx", "ry", "rz"):
+        assert np.array_equal(lg[k].view(np.uint64), lc[k].view(np.uint64)), f"{where}: link {k}"
+
+
+def check_particles(sim, ora, where, tol=1e-12):
+    for name in P_ARRAYS:
+        a, b = sim.particles(name), ora.particles(name)
+        assert rel_err(a, b) <= tol, f"{where}: particle {name} rel err {rel_err(a, b):.3e}"
+
+
+def check_fields(sim, ora, where, tol=1e-12):
+    for name in ("f", "g", "f_collide", "g_collide"):
+        e = rel_err(sim.populations(name), ora.populations(name))
+        assert e <= tol, f"{where}: {name} rel err {e:.3e}"
+
+
+TWO = [(8.3, 9.1, 12.4), (15.6, 14.2, 11.7)]
+
+
+def test_initial_geometry(rb, O):
+    sim, ora = make_pair(rb, O, particle_case(24, 24, 24, npart=2, diam=6.0, ra=1e4, beta=0.05), TWO)
+    check_geometry(sim, ora, "init")
+
+
+def test_overlapping_spheres_lowest_pid_owns(rb, O):
+    """Two overlapping spheres: shared nodes belong to the lower pid (links.cu:45-47)."""
+    pos = [(12.0, 12.2, 11.9), (14.5, 12.6, 12.3), (12.7, 14.9, 12.1)]
+    sim, ora = make_pair(rb, O, particle_case(24, 24, 24, npart=3, diam=6.0, ra=1e4, beta=0.05), pos)
+    check_geometry(sim, ora, "init")
+    ib, ow = sim.flags()
+    assert (ow == 0).sum() > (ow == 1).sum()
+
+
+@pytest.mark.parametrize("steps", [1, 3, 20])
+def test_two_spheres_step_parity(rb, O, steps):
+    sim, ora = make_pair(rb, O, particle_case(24, 24, 24, npart=2, diam=6.0, ra=1e4, beta=0.05), TWO)
+    sim.step(steps)
+    ora.step(steps)
+    where = f"after {steps} steps"
+    check_particles(sim, ora, where)
+    check_geometry(sim, ora, where)
+    nf_g, pid_g = sim.new_fluid()
+    nf_c, pid_c = ora.new_fluid()
+    assert np.array_equal(nf_g, nf_c) and np.array_equal(pid_g, pid_c), f"{where}: new fluid"
+    check_fields(sim, ora, where)
+    for name in ("rho", "ux", "uy", "uz", "phi", "fx"):
+        assert rel_err(sim.field(name), ora.field(name)) <= 1e-12, f"{where}: {name}"
+
+
+def test_contacts_and_walls(rb, O):
+    """Spheres inside the repulsion range of each other and of the walls."""
+    pos = [(4.5, 10.0, 10.0), (11.5, 10.3, 9.8), (20.2, 3.6, 10.0), (10.0, 20.0, 20.6)]
+    sim, ora = make_pair(rb, O, particle_case(26, 24, 24, npart=4, diam=6.0, ra=1e4, beta=0.05), pos)
+    sim.step(10)
+    ora.step(10)
+    check_particles(sim, ora, "contacts")
+    check_geometry(sim, ora, "contacts")
+    check_fields(sim, ora, "contacts")
+
+
+def test_moving_spheres_many_steps(rb, O):
+    """Fast heavy spheres cross many nodes: refill and link rebuild every few steps."""
+    kw = particle_case(32, 24, 24, npart=3, diam=6.0, ra=1e5, beta=0.5)
+    pos = [(8.0, 8.0, 8.0), (8.5, 16.0, 15.0), (20.0, 12.0, 12.0)]
+    sim, ora = make_pair(rb, O, kw, pos, ptemp=np.array([0.3, -0.2, 0.0]))
+    total_nf = 0
+    for k in range(6):
+        sim.step(10)
+        ora.step(10)
+        nf_g, _ = sim.new_fluid()
+        nf_c, _ = ora.new_fluid()
+        assert np.array_equal(nf_g, nf_c)
+        total_nf += len(nf_c)
+        check_particles(sim, ora, f"block {k}")
+        check_geometry(sim, ora, f"block {k}")
+    check_fields(sim, ora, "end")
+    assert ora.unhandled_links(False) == 0 and ora.unhandled_links(True) == 0
+
+
+def test_reference_placement(rb, O):
+    """glibc srand/rand rejection placement (pinit.cu:97-144) gives the oracle's positions."""
+    kw = particle_case(40, 40, 40, npart=12, diam=6.0, ra=1e4, beta=0.05)
+    case = rb.default_case(**kw)
+    sim, ora = rb.Sim(case), O.Oracle(case)
+    sim.init()
+    ora.init()
+    for name in ("pos_x", "pos_y", "pos_z", "temp"):
+        assert np.array_equal(sim.particles(name), ora.particles(name))
+    check_geometry(sim, ora, "placed")
