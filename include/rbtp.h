@@ -243,6 +243,34 @@ int rbtp_write_flow(rbtp_sim *s, const char *path);
 int rbtp_write_particles(rbtp_sim *s, const char *path);
 int rbtp_restart_load(rbtp_sim *s, const char *dir, int step);
 
+/*
+ * Serving (no counterpart in the reference, which runs one case per process): a batch of
+ * independent cavities of any size, Ra, Pr and particle count, all in one precision form,
+ * resident in HBM and advanced together.  Each admitted case is a slot; rbtp_batch_slot gives
+ * an rbtp_sim view for the accessors and diagnostics above (do not step it directly).
+ *   rbtp_batch_create: max_slots concurrent cases; admission keeps the batch under hbm_fraction
+ *       of the device memory and leaves 1 GiB free.
+ *   rbtp_batch_admit: initialise a case (as rbtp_init) and run its first step; RBTP_E_CAPACITY
+ *       when no slot or not enough HBM is free (the caller retries after a release).
+ *   rbtp_batch_step: nsteps steps of every active slot: one batched persistent lattice launch per
+ *       step plus each slot's particle work, replayed from a hipGraph captured per composition.
+ *   rbtp_batch_tune: choose the lattice kernel's tile size and persistent grid by timing the
+ *       candidates on the current composition (no slot's state changes).
+ *   rbtp_batch_set_launch: fix tile size (64/128/256), grid and graph length (even; 0 = no graph).
+ */
+typedef struct rbtp_batch rbtp_batch;
+int rbtp_batch_create(int precision, int max_slots, double hbm_fraction, rbtp_batch **out);
+void rbtp_batch_destroy(rbtp_batch *b);
+int rbtp_batch_admit(rbtp_batch *b, const rbtp_case *c, const double *temperature,
+                     const double *ppos, const double *ptemp, int *slot);
+int rbtp_batch_release(rbtp_batch *b, int slot);
+int rbtp_batch_slot(rbtp_batch *b, int slot, rbtp_sim **sim);
+int rbtp_batch_step(rbtp_batch *b, int nsteps, unsigned flags);
+int rbtp_batch_tune(rbtp_batch *b, int *block, int *grid, double *ms);
+int rbtp_batch_set_launch(rbtp_batch *b, int block, int grid, int graph_steps);
+int rbtp_batch_info(rbtp_batch *b, int *active, int64_t *used_bytes, int64_t *free_bytes,
+                    double *last_ms, int *block, int *grid);
+
 /* Device time of the kernels enqueued by the last rbtp_step* call, from HIP events on the
  * handle's stream (milliseconds). */
 int rbtp_last_step_ms(rbtp_sim *s, double *ms);

@@ -142,6 +142,16 @@ def _load():
         "rbtp_write_flow": (I, [P, C.c_char_p]),
         "rbtp_write_particles": (I, [P, C.c_char_p]),
         "rbtp_restart_load": (I, [P, C.c_char_p, I]),
+        "rbtp_batch_create": (I, [I, I, D, C.POINTER(P)]),
+        "rbtp_batch_destroy": (None, [P]),
+        "rbtp_batch_admit": (I, [P, C.POINTER(Case), P, P, P, C.POINTER(I)]),
+        "rbtp_batch_release": (I, [P, I]),
+        "rbtp_batch_slot": (I, [P, I, C.POINTER(P)]),
+        "rbtp_batch_step": (I, [P, I, U]),
+        "rbtp_batch_tune": (I, [P, C.POINTER(I), C.POINTER(I), C.POINTER(D)]),
+        "rbtp_batch_set_launch": (I, [P, I, I, I]),
+        "rbtp_batch_info": (I, [P, C.POINTER(I), C.POINTER(C.c_int64), C.POINTER(C.c_int64),
+                                C.POINTER(D), C.POINTER(I), C.POINTER(I)]),
         "rbtp_last_step_ms": (I, [P, C.POINTER(D)]),
         "rbtp_set_kernel_timing": (I, [P, I]),
         "rbtp_kernel_ms": (I, [P, C.POINTER(D), C.POINTER(C.c_int64)]),
@@ -196,10 +206,14 @@ class StepTiming:
 class Sim:
     """One solver instance (rbtp_sim) on the current HIP device."""
 
-    def __init__(self, case: Case):
+    def __init__(self, case: Case, _handle=None):
         self.case = Case.from_buffer_copy(case)
-        h = C.c_void_p()
-        _check(lib.rbtp_create(C.byref(self.case), C.byref(h)))
+        self._owned = _handle is None
+        if _handle is None:
+            h = C.c_void_p()
+            _check(lib.rbtp_create(C.byref(self.case), C.byref(h)))
+        else:
+            h = _handle
         self._h = h
         self.derived = Derived()
         _check(lib.rbtp_get_case(self._h, None, C.byref(self.derived)))
@@ -208,7 +222,8 @@ class Sim:
 
     def close(self) -> None:
         if getattr(self, "_h", None):
-            lib.rbtp_destroy(self._h)
+            if self._owned:
+                lib.rbtp_destroy(self._h)
             self._h = None
 
     def __del__(self):
@@ -353,3 +368,85 @@ class Sim:
         n = C.c_int64()
         _check(lib.rbtp_kernel_ms(self._h, C.byref(kms), C.byref(n)))
         return StepTiming(ms.value, kms.value, n.value)
+
+
+class CapacityError(RbtpError):
+    """No free slot or not enough free HBM for an admission."""
+
+
+class Batch:
+    """A serving batch (rbtp_batch): independent cavities resident in HBM, stepped together by
+    one batched persistent lattice launch per step, replayed from a captured hipGraph."""
+
+    def __init__(self, precision: int = PREC_FP16S, max_slots: int = 64,
+                 hbm_fraction: float = 0.8):
+        h = C.c_void_p()
+        _check(lib.rbtp_batch_create(int(precision), int(max_slots), float(hbm_fraction),
+                                     C.byref(h)))
+        self._h = h
+        self.precision = precision
+        self._slots = {}
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            for v in self._slots.values():
+                v._h = None
+            lib.rbtp_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def admit(self, case: Case, temperature=None, ppos=None, ptemp=None) -> int:
+        """Admit a case; returns its slot id.  Raises CapacityError when full."""
+        case = Case.from_buffer_copy(case)
+        case.precision = self.precision
+        keep = []
+
+        def ptr(a):
+            if a is None:
+                return None
+            a = np.ascontiguousarray(a, dtype=np.float64).reshape(-1)
+            keep.append(a)
+            return a.ctypes.data_as(C.c_void_p)
+
+        slot = C.c_int(-1)
+        rc = lib.rbtp_batch_admit(self._h, C.byref(case), ptr(temperature), ptr(ppos),
+                                  ptr(ptemp), C.byref(slot))
+        if rc == -6:
+            raise CapacityError(rc, lib.rbtp_last_error().decode(errors="replace"))
+        _check(rc)
+        h = C.c_void_p()
+        _check(lib.rbtp_batch_slot(self._h, slot.value, C.byref(h)))
+        self._slots[slot.value] = Sim(case, _handle=h)
+        return slot.value
+
+    def release(self, slot: int) -> None:
+        v = self._slots.pop(slot)
+        v._h = None
+        _check(lib.rbtp_batch_release(self._h, int(slot)))
+
+    def sim(self, slot: int) -> Sim:
+        return self._slots[slot]
+
+    @property
+    def slots(self):
+        return list(self._slots)
+
+    def step(self, nsteps: int, macro_last: bool = True) -> None:
+        _check(lib.rbtp_batch_step(self._h, int(nsteps), STEP_MACRO_LAST if macro_last else 0))
+
+    def tune(self):
+        b, g, ms = C.c_int(), C.c_int(), C.c_double()
+        _check(lib.rbtp_batch_tune(self._h, C.byref(b), C.byref(g), C.byref(ms)))
+        return b.value, g.value, ms.value
+
+    def set_launch(self, block: int, grid: int, graph_steps: int) -> None:
+        _check(lib.rbtp_batch_set_launch(self._h, int(block), int(grid), int(graph_steps)))
+
+    def info(self) -> dict:
+        a, u, f, ms, b, g = C.c_int(), C.c_int64(), C.c_int64(), C.c_double(), C.c_int(), C.c_int()
+        _check(lib.rbtp_batch_info(self._h, C.byref(a), C.byref(u), C.byref(f), C.byref(ms),
+                                   C.byref(b), C.byref(g)))
+        return dict(active=a.value, used_bytes=u.value, free_bytes=f.value, last_ms=ms.value,
+                    block=b.value, grid=g.value)

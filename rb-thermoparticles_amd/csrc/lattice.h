@@ -7,6 +7,13 @@ namespace rbtp {
 
 template <class S>
 hipError_t launch_lattice_step(const StepArgs<S> &a, bool strict, bool nostream, hipStream_t st);
+// Batched step over several lattices; tile_start has nslots entries (first tile of each slot,
+// tiles of `block` nodes).  block is 64, 128 or 256; grid is the number of persistent
+// workgroups.
+template <class S>
+hipError_t launch_lattice_batched(const StepArgs<S> *slots_dev, const int *tile_start_dev,
+                                  int nslots, int total_tiles, int block, int grid, bool strict,
+                                  hipStream_t st);
 template <class S>
 hipError_t launch_special_collide(const StepArgs<S> &a, const SpecialBuf &sp, int max_count,
                                   bool strict, hipStream_t st);

@@ -210,12 +210,9 @@ __device__ inline void write_node_aux(const StepArgs<S> &a, int idx, int ix, int
     }
 }
 
-// One time step for every node.
+// One node's time step (shared by the single-lattice and batched kernels).
 template <class S, class T, bool STRICT, bool NOSTREAM>
-__global__ __launch_bounds__(BLOCK) void k_lattice_step(const StepArgs<S> a) {
-    const int nblocks = (a.d.n + BLOCK - 1) / BLOCK;
-    const int idx = xcd_block(nblocks) * BLOCK + threadIdx.x;
-    if (idx >= a.d.n) return;
+__device__ inline void node_step(const StepArgs<S> &a, int idx) {
     const int iz = idx / a.d.lxy;
     const int rem = idx - iz * a.d.lxy;
     const int iy = rem / a.d.lx;
@@ -233,6 +230,35 @@ __global__ __launch_bounds__(BLOCK) void k_lattice_step(const StepArgs<S> a) {
     T rho, ux, uy, uz, phi;
     const T fx = collide_node<S, T, STRICT>(f, g, fx_old, idx, o, a.k, rho, ux, uy, uz, phi);
     write_node_aux<S, T>(a, idx, ix, iy, iz, fx, rho, ux, uy, uz, phi);
+}
+
+// One time step for every node.
+template <class S, class T, bool STRICT, bool NOSTREAM>
+__global__ __launch_bounds__(BLOCK) void k_lattice_step(const StepArgs<S> a) {
+    const int nblocks = (a.d.n + BLOCK - 1) / BLOCK;
+    const int idx = xcd_block(nblocks) * BLOCK + threadIdx.x;
+    if (idx >= a.d.n) return;
+    node_step<S, T, STRICT, NOSTREAM>(a, idx);
+}
+
+// One time step for every node of several independent lattices (serving slots) in one launch.
+// Tiles of BLK nodes are numbered across slots (tile_start[s] is slot s's first tile); a
+// persistent grid walks them, so the launch shape is independent of the batch composition.
+template <class S, class T, bool STRICT, int BLK>
+__global__ __launch_bounds__(BLK) void k_lattice_batched(const StepArgs<S> *__restrict__ slots,
+                                                         const int *__restrict__ tile_start,
+                                                         int nslots, int total_tiles) {
+    for (int b = blockIdx.x; b < total_tiles; b += gridDim.x) {
+        int lo = 0, hi = nslots - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (tile_start[mid] <= b) lo = mid;
+            else hi = mid - 1;
+        }
+        const StepArgs<S> a = slots[lo];
+        const int idx = (b - tile_start[lo]) * BLK + threadIdx.x;
+        if (idx < a.d.n) node_step<S, T, STRICT, false>(a, idx);
+    }
 }
 
 // Post-stream populations only (for the parity accessors and the fg restart writer), decoded to
@@ -477,8 +503,42 @@ hipError_t launch_forcing_field(typename Enc<S>::A *fx, typename Enc<S>::A *rho_
     return hipGetLastError();
 }
 
+template <class S>
+hipError_t launch_lattice_batched(const StepArgs<S> *slots_dev, const int *tile_start_dev,
+                                  int nslots, int total_tiles, int block, int grid, bool strict,
+                                  hipStream_t st) {
+    if (nslots <= 0 || total_tiles <= 0) return hipSuccess;
+    const bool dbl = std::is_same<S, double>::value;
+#define RBTP_BATCHED(T, STRICT_, BLK)                                                          \
+    hipLaunchKernelGGL((k_lattice_batched<S, T, STRICT_, BLK>), grid, BLK, 0, st, slots_dev,   \
+                       tile_start_dev, nslots, total_tiles)
+    if (dbl && strict) {
+        switch (block) {
+            case 64: RBTP_BATCHED(double, true, 64); break;
+            case 128: RBTP_BATCHED(double, true, 128); break;
+            default: RBTP_BATCHED(double, true, 256); break;
+        }
+    } else if (dbl) {
+        switch (block) {
+            case 64: RBTP_BATCHED(double, false, 64); break;
+            case 128: RBTP_BATCHED(double, false, 128); break;
+            default: RBTP_BATCHED(double, false, 256); break;
+        }
+    } else {
+        switch (block) {
+            case 64: RBTP_BATCHED(float, false, 64); break;
+            case 128: RBTP_BATCHED(float, false, 128); break;
+            default: RBTP_BATCHED(float, false, 256); break;
+        }
+    }
+#undef RBTP_BATCHED
+    return hipGetLastError();
+}
+
 #define RBTP_INSTANTIATE(S)                                                                     \
     template hipError_t launch_lattice_step<S>(const StepArgs<S> &, bool, bool, hipStream_t);   \
+    template hipError_t launch_lattice_batched<S>(const StepArgs<S> *, const int *, int, int, int, \
+                                                  int, bool, hipStream_t);                        \
     template hipError_t launch_special_collide<S>(const StepArgs<S> &, const SpecialBuf &, int, \
                                                   bool, hipStream_t);                           \
     template hipError_t launch_stream_only<S>(const S *, const S *,                             \

@@ -78,6 +78,7 @@ struct rbtp_sim {
     bool strict;
     int device;
     hipStream_t stream;
+    bool own_stream;
     hipEvent_t ev_start, ev_stop, ev_k0, ev_k1;
     bool kernel_timing;
     double kernel_ms_sum;
@@ -208,11 +209,11 @@ extern "C" void rbtp_destroy(rbtp_sim *s) {
     if (s->ev_stop) (void)hipEventDestroy(s->ev_stop);
     if (s->ev_k0) (void)hipEventDestroy(s->ev_k0);
     if (s->ev_k1) (void)hipEventDestroy(s->ev_k1);
-    if (s->stream) (void)hipStreamDestroy(s->stream);
+    if (s->stream && s->own_stream) (void)hipStreamDestroy(s->stream);
     delete s;
 }
 
-static int create_impl(const rbtp_case *c, rbtp_sim *s) {
+static int create_impl(const rbtp_case *c, rbtp_sim *s, hipStream_t stream) {
     s->c = *c;
     rbtp_case_derive(c, &s->d);
     s->dims = Dims{c->lx, c->ly, c->lz, c->lx * c->ly, c->lx * c->ly * c->lz};
@@ -232,7 +233,13 @@ static int create_impl(const rbtp_case *c, rbtp_sim *s) {
     s->prec = c->precision;
     s->strict = (c->precision == RBTP_PREC_FP64);
     HIP_TRY(hipGetDevice(&s->device));
-    HIP_TRY(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+    if (stream) {
+        s->stream = stream;
+        s->own_stream = false;
+    } else {
+        HIP_TRY(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+        s->own_stream = true;
+    }
     HIP_TRY(hipEventCreate(&s->ev_start));
     HIP_TRY(hipEventCreate(&s->ev_stop));
     HIP_TRY(hipEventCreate(&s->ev_k0));
@@ -251,13 +258,13 @@ static int create_impl(const rbtp_case *c, rbtp_sim *s) {
     return RBTP_OK;
 }
 
-extern "C" int rbtp_create(const rbtp_case *c, rbtp_sim **out) {
+static int create_on_stream(const rbtp_case *c, rbtp_sim **out, hipStream_t stream) {
     if (!c || !out) return set_error(RBTP_E_ARG, "rbtp_create: null argument");
     *out = nullptr;
     RB_TRY(validate_case(c));
     rbtp_sim *s = new rbtp_sim();
     std::memset((void *)s, 0, sizeof(*s));
-    const int r = create_impl(c, s);
+    const int r = create_impl(c, s, stream);
     if (r != RBTP_OK) {
         std::string msg = g_last_error;
         rbtp_destroy(s);
@@ -266,6 +273,10 @@ extern "C" int rbtp_create(const rbtp_case *c, rbtp_sim **out) {
     }
     *out = s;
     return RBTP_OK;
+}
+
+extern "C" int rbtp_create(const rbtp_case *c, rbtp_sim **out) {
+    return create_on_stream(c, out, nullptr);
 }
 
 extern "C" int rbtp_get_case(const rbtp_sim *s, rbtp_case *c, rbtp_derived *d) {
@@ -375,6 +386,26 @@ extern "C" int rbtp_init(rbtp_sim *s, const double *temperature, const double *p
 // Time loop
 // ---------------------------------------------------------------------------------------------
 
+// Everything of a time step after the lattice kernel: the special-node re-collision (reads the
+// same StepArgs as the lattice kernel), the buffer flip and the particle phase.
+static int after_lattice(rbtp_sim *s, bool macro_out) {
+    if (s->particles && s->pd.special_pending) {
+        HIP_TRY(dispatch(s->prec, [&](auto *tag) -> hipError_t {
+            using S = std::remove_pointer_t<decltype(tag)>;
+            const StepArgs<S> a = step_args<S>(s, macro_out);
+            return launch_special_collide<S>(a, s->pd.sp, s->pd.sp.cap, s->strict, s->stream);
+        }));
+    }
+    s->cur = 1 - s->cur;
+    s->poststream = false;
+    if (s->particles) {
+        RB_TRY(part_phase(s->pd, s->c, s->d, s->dims, s->prec, s->f[s->cur], s->g[s->cur],
+                          s->fx, s->rho_wall, s->macro, s->stream));
+    }
+    s->step++;
+    return RBTP_OK;
+}
+
 static int one_step(rbtp_sim *s, bool macro_out) {
     const bool nostream = s->poststream;
     HIP_TRY(dispatch(s->prec, [&](auto *tag) -> hipError_t {
@@ -386,12 +417,7 @@ static int one_step(rbtp_sim *s, bool macro_out) {
         }
         hipError_t e = launch_lattice_step<S>(a, s->strict, nostream, s->stream);
         if (e != hipSuccess) return e;
-        if (s->kernel_timing) {
-            e = hipEventRecord(s->ev_k1, s->stream);
-            if (e != hipSuccess) return e;
-        }
-        if (s->particles && s->pd.special_pending)
-            e = launch_special_collide<S>(a, s->pd.sp, s->pd.sp.cap, s->strict, s->stream);
+        if (s->kernel_timing) e = hipEventRecord(s->ev_k1, s->stream);
         return e;
     }));
     if (s->kernel_timing) {
@@ -401,14 +427,7 @@ static int one_step(rbtp_sim *s, bool macro_out) {
         s->kernel_ms_sum += ms;
         s->kernel_launches++;
     }
-    s->cur = 1 - s->cur;
-    s->poststream = false;
-    if (s->particles) {
-        RB_TRY(part_phase(s->pd, s->c, s->d, s->dims, s->prec, s->f[s->cur], s->g[s->cur],
-                          s->fx, s->rho_wall, s->macro, s->stream));
-    }
-    s->step++;
-    return RBTP_OK;
+    return after_lattice(s, macro_out);
 }
 
 static int step_impl(rbtp_sim *s, int nsteps, unsigned flags, bool sync) {
@@ -863,6 +882,418 @@ extern "C" int rbtp_restart_load(rbtp_sim *s, const char *dir, int step) {
     s->poststream = true;
     s->macro_valid = true;
     s->step = step;
+    return RBTP_OK;
+}
+
+
+// ---------------------------------------------------------------------------------------------
+// Serving: a batch of independent cavities advanced together
+// ---------------------------------------------------------------------------------------------
+//
+// Each admitted request is an rbtp_sim (a slot) whose state stays resident in HBM.  All slots
+// share one stream; one batched, persistent lattice kernel advances every slot's lattice per
+// step (tiles numbered across slots), followed by each slot's special-node and particle work.
+// A step sequence of fixed length is captured once per batch composition into a hipGraph and
+// replayed; admissions and releases happen between replays and invalidate the graph.  The
+// lattice kernel's tile size and persistent grid come from rbtp_batch_tune, which times the
+// candidates on the current composition without changing any slot's state.
+
+struct rbtp_batch {
+    int prec;
+    bool strict;
+    int max_slots;
+    double hbm_fraction;
+    int device;
+    int num_cu;
+    hipStream_t stream;
+    hipEvent_t ev0, ev1;
+    std::vector<rbtp_sim *> slot;
+    std::vector<size_t> slot_bytes;
+    size_t used_bytes;
+    void *args_dev[4];  // StepArgs<S>[max_slots] for (parity, macro)
+    int *tile_start_dev;
+    std::vector<int> active;
+    std::vector<int> cur_at_build;  // each active slot's buffer parity when the tables were built
+    int total_tiles;
+    int block, grid;
+    bool tables_dirty;
+    int phase;
+    bool use_graph;
+    int graph_steps;
+    hipGraph_t graph;
+    hipGraphExec_t exec;
+    bool graph_valid;
+    double last_ms;
+    int64_t graph_launches;
+};
+
+static size_t args_size(int prec) {
+    switch (prec) {
+        case RBTP_PREC_FP64: return sizeof(StepArgs<double>);
+        case RBTP_PREC_FP32: return sizeof(StepArgs<float>);
+        case RBTP_PREC_FP16S: return sizeof(StepArgs<__half>);
+        default: return sizeof(StepArgs<__hip_bfloat16>);
+    }
+}
+
+// Device bytes one slot allocates (create_impl, part_alloc).
+static size_t sim_device_bytes(const rbtp_case &c, const rbtp_derived &d) {
+    const size_t n = (size_t)c.lx * c.ly * c.lz;
+    size_t b = 4 * 27 * n * storage_bytes(c.precision) + n * aux_bytes(c.precision) +
+               2 * (size_t)c.ly * c.lz * aux_bytes(c.precision) + 5 * n * sizeof(double);
+    if (c.activate_particles && d.npart > 0) {
+        const size_t np = (size_t)d.npart;
+        const long long w = (long long)std::ceil(2.0 * d.particle_radius) + 1;
+        long long cap = (long long)np * ((w + 2) * (w + 2) * (w + 2) + w * w * w);
+        if (cap > (long long)n) cap = (long long)n;
+        b += (RBTP_P_COUNT + OLD_COUNT) * np * sizeof(double) + 2 * n * sizeof(int32_t) +
+             12 * np * sizeof(int32_t) + (size_t)cap * (2 * 27 * sizeof(double) + sizeof(double) + 2 * sizeof(int32_t));
+    }
+    return b;
+}
+
+static void batch_invalidate_graph(rbtp_batch *b) {
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+    b->graph_valid = false;
+}
+
+extern "C" void rbtp_batch_destroy(rbtp_batch *b) {
+    if (!b) return;
+    (void)hipSetDevice(b->device);
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    batch_invalidate_graph(b);
+    for (rbtp_sim *s : b->slot)
+        if (s) rbtp_destroy(s);
+    for (void *p : b->args_dev) (void)hipFree(p);
+    (void)hipFree(b->tile_start_dev);
+    if (b->ev0) (void)hipEventDestroy(b->ev0);
+    if (b->ev1) (void)hipEventDestroy(b->ev1);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
+}
+
+extern "C" int rbtp_batch_create(int precision, int max_slots, double hbm_fraction, rbtp_batch **out) {
+    if (!out) return set_error(RBTP_E_ARG, "null argument");
+    *out = nullptr;
+    if (precision < RBTP_PREC_FP64 || precision > RBTP_PREC_BF16S) return set_error(RBTP_E_ARG, "unknown precision %d", precision);
+    if (max_slots < 1 || max_slots > 4096) return set_error(RBTP_E_ARG, "max_slots must be in [1, 4096]");
+    if (!(hbm_fraction > 0.0 && hbm_fraction <= 1.0)) return set_error(RBTP_E_ARG, "hbm_fraction must be in (0, 1]");
+    rbtp_batch *b = new rbtp_batch();
+    b->prec = precision;
+    b->strict = precision == RBTP_PREC_FP64;
+    b->max_slots = max_slots;
+    b->hbm_fraction = hbm_fraction;
+    b->slot.assign(max_slots, nullptr);
+    b->slot_bytes.assign(max_slots, 0);
+    b->block = 256;
+    b->use_graph = true;
+    b->graph_steps = 20;
+    b->tables_dirty = true;
+    auto fail = [&](hipError_t e, const char *what) {
+        const int r = set_error(RBTP_E_HIP, "%s: %s", what, hipGetErrorString(e));
+        std::string msg = g_last_error;
+        rbtp_batch_destroy(b);
+        g_last_error = msg;
+        return r;
+    };
+    hipError_t e = hipGetDevice(&b->device);
+    if (e != hipSuccess) return fail(e, "hipGetDevice");
+    e = hipDeviceGetAttribute(&b->num_cu, hipDeviceAttributeMultiprocessorCount, b->device);
+    if (e != hipSuccess) return fail(e, "hipDeviceGetAttribute");
+    b->grid = 4 * b->num_cu;
+    if ((e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking)) != hipSuccess) return fail(e, "hipStreamCreate");
+    if ((e = hipEventCreate(&b->ev0)) != hipSuccess) return fail(e, "hipEventCreate");
+    if ((e = hipEventCreate(&b->ev1)) != hipSuccess) return fail(e, "hipEventCreate");
+    for (void *&p : b->args_dev)
+        if ((e = hipMalloc(&p, args_size(precision) * max_slots)) != hipSuccess) return fail(e, "hipMalloc");
+    if ((e = hipMalloc(&b->tile_start_dev, sizeof(int) * (max_slots + 1))) != hipSuccess) return fail(e, "hipMalloc");
+    *out = b;
+    return RBTP_OK;
+}
+
+extern "C" int rbtp_batch_admit(rbtp_batch *b, const rbtp_case *c, const double *temperature,
+                                const double *ppos, const double *ptemp, int *slot_out) {
+    if (!b || !c || !slot_out) return set_error(RBTP_E_ARG, "null argument");
+    *slot_out = -1;
+    if (c->precision != b->prec)
+        return set_error(RBTP_E_ARG, "case precision %d differs from the batch's %d", c->precision, b->prec);
+    RB_TRY(validate_case(c));
+    int k = -1;
+    for (int i = 0; i < b->max_slots; i++)
+        if (!b->slot[i]) { k = i; break; }
+    if (k < 0) return set_error(RBTP_E_CAPACITY, "no free slot (%d in use)", b->max_slots);
+    rbtp_derived d;
+    rbtp_case_derive(c, &d);
+    const size_t need = sim_device_bytes(*c, d);
+    HIP_TRY(hipSetDevice(b->device));
+    size_t free_b = 0, total_b = 0;
+    HIP_TRY(hipMemGetInfo(&free_b, &total_b));
+    const size_t reserve = (size_t)1 << 30;
+    if (b->used_bytes + need > (size_t)(b->hbm_fraction * (double)total_b) || need + reserve > free_b)
+        return set_error(RBTP_E_CAPACITY, "not enough HBM for %zu bytes (batch uses %zu, %zu free)",
+                         need, b->used_bytes, free_b);
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    rbtp_sim *s = nullptr;
+    RB_TRY(create_on_stream(c, &s, b->stream));
+    int r = rbtp_init(s, temperature, ppos, ptemp);
+    if (r == RBTP_OK) r = step_impl(s, 1, 0, true);  // first step reads the post-stream start
+    if (r != RBTP_OK) {
+        std::string msg = g_last_error;
+        rbtp_destroy(s);
+        g_last_error = msg;
+        return r;
+    }
+    b->slot[k] = s;
+    b->slot_bytes[k] = need;
+    b->used_bytes += need;
+    b->tables_dirty = true;
+    batch_invalidate_graph(b);
+    *slot_out = k;
+    return RBTP_OK;
+}
+
+extern "C" int rbtp_batch_release(rbtp_batch *b, int k) {
+    if (!b) return set_error(RBTP_E_ARG, "null handle");
+    if (k < 0 || k >= b->max_slots || !b->slot[k]) return set_error(RBTP_E_ARG, "slot %d is not in use", k);
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    rbtp_destroy(b->slot[k]);
+    b->slot[k] = nullptr;
+    b->used_bytes -= b->slot_bytes[k];
+    b->slot_bytes[k] = 0;
+    b->tables_dirty = true;
+    batch_invalidate_graph(b);
+    return RBTP_OK;
+}
+
+extern "C" int rbtp_batch_slot(rbtp_batch *b, int k, rbtp_sim **out) {
+    if (!b || !out) return set_error(RBTP_E_ARG, "null argument");
+    if (k < 0 || k >= b->max_slots || !b->slot[k]) return set_error(RBTP_E_ARG, "slot %d is not in use", k);
+    *out = b->slot[k];
+    return RBTP_OK;
+}
+
+// Device tables of the batched kernel for the current composition and tile size.
+static int batch_build_tables(rbtp_batch *b) {
+    b->active.clear();
+    for (int i = 0; i < b->max_slots; i++)
+        if (b->slot[i]) b->active.push_back(i);
+    const int na = (int)b->active.size();
+    std::vector<int> tiles(na + 1, 0);
+    for (int j = 0; j < na; j++) {
+        const rbtp_sim *s = b->slot[b->active[j]];
+        tiles[j + 1] = tiles[j] + (s->dims.n + b->block - 1) / b->block;
+    }
+    b->total_tiles = tiles[na];
+    b->cur_at_build.clear();
+    for (int k : b->active) b->cur_at_build.push_back(b->slot[k]->cur);
+    if (na > 0) HIP_TRY(hipMemcpy(b->tile_start_dev, tiles.data(), sizeof(int) * na, hipMemcpyHostToDevice));
+    HIP_TRY(dispatch(b->prec, [&](auto *tag) -> hipError_t {
+        using S = std::remove_pointer_t<decltype(tag)>;
+        std::vector<StepArgs<S>> h(na > 0 ? na : 1);
+        for (int par = 0; par < 2; par++)
+            for (int mac = 0; mac < 2; mac++) {
+                for (int j = 0; j < na; j++) {
+                    rbtp_sim *s = b->slot[b->active[j]];
+                    if (par) s->cur = 1 - s->cur;
+                    h[j] = step_args<S>(s, mac != 0);
+                    if (par) s->cur = 1 - s->cur;
+                }
+                if (na > 0) {
+                    hipError_t e = hipMemcpy(b->args_dev[par * 2 + mac], h.data(), sizeof(StepArgs<S>) * na, hipMemcpyHostToDevice);
+                    if (e != hipSuccess) return e;
+                }
+            }
+        return hipSuccess;
+    }));
+    b->phase = 0;
+    b->tables_dirty = false;
+    batch_invalidate_graph(b);
+    return RBTP_OK;
+}
+
+static hipError_t batch_lattice(rbtp_batch *b, bool macro, int block, int grid) {
+    const int na = (int)b->active.size();
+    return dispatch(b->prec, [&](auto *tag) {
+        using S = std::remove_pointer_t<decltype(tag)>;
+        return launch_lattice_batched<S>((const StepArgs<S> *)b->args_dev[b->phase * 2 + (macro ? 1 : 0)],
+                                         b->tile_start_dev, na, b->total_tiles, block, grid,
+                                         b->strict, b->stream);
+    });
+}
+
+static int batch_step_eager(rbtp_batch *b, bool macro) {
+    HIP_TRY(batch_lattice(b, macro, b->block, b->grid));
+    for (int k : b->active) RB_TRY(after_lattice(b->slot[k], macro));
+    b->phase ^= 1;
+    return RBTP_OK;
+}
+
+static int batch_capture(rbtp_batch *b) {
+    batch_invalidate_graph(b);
+    std::vector<int> steps;
+    for (int k : b->active) steps.push_back(b->slot[k]->step);
+    HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    int r = RBTP_OK;
+    for (int j = 0; j < b->graph_steps && r == RBTP_OK; j++) r = batch_step_eager(b, false);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(b->stream, &g);
+    for (size_t j = 0; j < b->active.size(); j++) b->slot[b->active[j]]->step = steps[j];
+    if (r != RBTP_OK) {
+        if (g) (void)hipGraphDestroy(g);
+        return r;
+    }
+    if (e != hipSuccess) return set_error(RBTP_E_HIP, "hipStreamEndCapture: %s", hipGetErrorString(e));
+    b->graph = g;
+    HIP_TRY(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+    b->graph_valid = true;
+    return RBTP_OK;
+}
+
+extern "C" int rbtp_batch_step(rbtp_batch *b, int nsteps, unsigned flags) {
+    if (!b) return set_error(RBTP_E_ARG, "null handle");
+    if (nsteps < 0) return set_error(RBTP_E_ARG, "nsteps < 0");
+    HIP_TRY(hipSetDevice(b->device));
+    if (!b->tables_dirty)
+        for (size_t j = 0; j < b->active.size(); j++)
+            if (b->slot[b->active[j]]->cur != (b->cur_at_build[j] ^ b->phase)) b->tables_dirty = true;
+    if (b->tables_dirty) RB_TRY(batch_build_tables(b));
+    if (b->active.empty() || nsteps == 0) return RBTP_OK;
+    const bool macro_last = (flags & RBTP_STEP_MACRO_LAST) != 0;
+    int body = macro_last ? nsteps - 1 : nsteps;
+    HIP_TRY(hipEventRecord(b->ev0, b->stream));
+    if (b->use_graph && body >= b->graph_steps + b->phase) {
+        if (b->phase == 1) {
+            RB_TRY(batch_step_eager(b, false));
+            body--;
+        }
+        if (!b->graph_valid) RB_TRY(batch_capture(b));
+        while (body >= b->graph_steps) {
+            HIP_TRY(hipGraphLaunch(b->exec, b->stream));
+            b->graph_launches++;
+            for (int k : b->active) b->slot[k]->step += b->graph_steps;
+            body -= b->graph_steps;
+        }
+    }
+    while (body-- > 0) RB_TRY(batch_step_eager(b, false));
+    if (macro_last) RB_TRY(batch_step_eager(b, true));
+    HIP_TRY(hipEventRecord(b->ev1, b->stream));
+    HIP_TRY(hipEventSynchronize(b->ev1));
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+    b->last_ms = ms;
+    for (int k : b->active) {
+        rbtp_sim *s = b->slot[k];
+        s->macro_valid = macro_last;
+        if (s->particles) RB_TRY(part_check_errors(s->pd));
+    }
+    return RBTP_OK;
+}
+
+// Times the batched lattice kernel on the current composition for every tile size (64, 128,
+// 256 nodes) and persistent grid (1, 2, 4, 8, 16 workgroups per CU, or one per tile) and keeps
+// the fastest.  F_x and the wall rho (updated in place by the kernel) are saved and restored,
+// and the kernel's other outputs are rewritten by the next step, so no slot's state changes.
+extern "C" int rbtp_batch_tune(rbtp_batch *b, int *block_out, int *grid_out, double *ms_out) {
+    if (!b) return set_error(RBTP_E_ARG, "null handle");
+    HIP_TRY(hipSetDevice(b->device));
+    if (b->tables_dirty) RB_TRY(batch_build_tables(b));
+    if (b->active.empty()) return set_error(RBTP_E_STATE, "no active slot to tune on");
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    std::vector<void *> saved;
+    for (int k : b->active) {
+        rbtp_sim *s = b->slot[k];
+        const size_t fxb = (size_t)s->dims.n * aux_bytes(s->prec);
+        const size_t rwb = 2 * (size_t)s->c.ly * s->c.lz * aux_bytes(s->prec);
+        void *p = nullptr;
+        HIP_TRY(hipMalloc(&p, fxb + rwb));
+        saved.push_back(p);
+        HIP_TRY(hipMemcpyAsync(p, s->fx, fxb, hipMemcpyDeviceToDevice, b->stream));
+        HIP_TRY(hipMemcpyAsync((char *)p + fxb, s->rho_wall, rwb, hipMemcpyDeviceToDevice, b->stream));
+    }
+    const int phase0 = b->phase;
+    int best_block = b->block, best_grid = b->grid;
+    double best = 1e30;
+    int rc = RBTP_OK;
+    const int blocks[3] = {64, 128, 256};
+    for (int bi = 0; bi < 3 && rc == RBTP_OK; bi++) {
+        b->block = blocks[bi];
+        rc = batch_build_tables(b);
+        if (rc != RBTP_OK) break;
+        const int grids[6] = {b->num_cu, 2 * b->num_cu, 4 * b->num_cu, 8 * b->num_cu,
+                              16 * b->num_cu, b->total_tiles};
+        for (int gi = 0; gi < 6 && rc == RBTP_OK; gi++) {
+            const int g = std::min(grids[gi], b->total_tiles);
+            hipError_t e = batch_lattice(b, false, b->block, g);
+            if (e == hipSuccess) e = hipEventRecord(b->ev0, b->stream);
+            for (int rep = 0; rep < 3 && e == hipSuccess; rep++) e = batch_lattice(b, false, b->block, g);
+            if (e == hipSuccess) e = hipEventRecord(b->ev1, b->stream);
+            if (e == hipSuccess) e = hipEventSynchronize(b->ev1);
+            float ms = 0.f;
+            if (e == hipSuccess) e = hipEventElapsedTime(&ms, b->ev0, b->ev1);
+            if (e != hipSuccess) {
+                rc = set_error(RBTP_E_HIP, "tuning launch: %s", hipGetErrorString(e));
+                break;
+            }
+            if (ms / 3.0 < best) {
+                best = ms / 3.0;
+                best_block = b->block;
+                best_grid = g;
+            }
+        }
+    }
+    for (size_t j = 0; j < b->active.size(); j++) {
+        rbtp_sim *s = b->slot[b->active[j]];
+        const size_t fxb = (size_t)s->dims.n * aux_bytes(s->prec);
+        const size_t rwb = 2 * (size_t)s->c.ly * s->c.lz * aux_bytes(s->prec);
+        (void)hipMemcpyAsync(s->fx, saved[j], fxb, hipMemcpyDeviceToDevice, b->stream);
+        (void)hipMemcpyAsync(s->rho_wall, (char *)saved[j] + fxb, rwb, hipMemcpyDeviceToDevice, b->stream);
+        s->macro_valid = false;
+    }
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    for (void *p : saved) (void)hipFree(p);
+    if (rc != RBTP_OK) return rc;
+    b->block = best_block;
+    b->grid = best_grid;
+    RB_TRY(batch_build_tables(b));
+    (void)phase0;
+    if (block_out) *block_out = best_block;
+    if (grid_out) *grid_out = best_grid;
+    if (ms_out) *ms_out = best;
+    return RBTP_OK;
+}
+
+extern "C" int rbtp_batch_set_launch(rbtp_batch *b, int block, int grid, int graph_steps) {
+    if (!b) return set_error(RBTP_E_ARG, "null handle");
+    if (block != 64 && block != 128 && block != 256) return set_error(RBTP_E_ARG, "block must be 64, 128 or 256");
+    if (grid < 1) return set_error(RBTP_E_ARG, "grid must be >= 1");
+    if (graph_steps < 0 || graph_steps % 2) return set_error(RBTP_E_ARG, "graph_steps must be even and >= 0 (0 disables graphs)");
+    b->block = block;
+    b->grid = grid;
+    b->use_graph = graph_steps > 0;
+    b->graph_steps = graph_steps > 0 ? graph_steps : 2;
+    b->tables_dirty = true;
+    batch_invalidate_graph(b);
+    return RBTP_OK;
+}
+
+extern "C" int rbtp_batch_info(rbtp_batch *b, int *active, int64_t *used_bytes, int64_t *free_bytes,
+                               double *last_ms, int *block, int *grid) {
+    if (!b) return set_error(RBTP_E_ARG, "null handle");
+    int na = 0;
+    for (rbtp_sim *s : b->slot) na += s != nullptr;
+    if (active) *active = na;
+    if (used_bytes) *used_bytes = (int64_t)b->used_bytes;
+    if (free_bytes) {
+        size_t f = 0, t = 0;
+        HIP_TRY(hipMemGetInfo(&f, &t));
+        *free_bytes = (int64_t)f;
+    }
+    if (last_ms) *last_ms = b->last_ms;
+    if (block) *block = b->block;
+    if (grid) *grid = b->grid;
     return RBTP_OK;
 }
 

@@ -22,7 +22,8 @@ def load_rbtp():
     if "rbtp_amd" in sys.modules:
         return sys.modules["rbtp_amd"]
     spec = importlib.util.spec_from_file_location(
-        "rbtp_amd", os.path.join(ROOT, "rb-thermoparticles_amd", "__init__.py"))
+        "rbtp_amd", os.path.join(ROOT, "rb-thermoparticles_amd", "__init__.py"),
+        submodule_search_locations=[os.path.dirname(os.path.join(ROOT, "rb-thermoparticles_amd", "__init__.py"))])
     mod = importlib.util.module_from_spec(spec)
     sys.modules["rbtp_amd"] = mod
     try:

@@ -1,0 +1,128 @@
+"""GPU tests of the serving batch (rbtp_batch) and the request loop.
+
+Bar: a request's result is the same alone and in a mix.  Every node update, particle sum and
+diagnostic reduction runs in a fixed order that does not depend on the batch, so the expected
+agreement is bitwise; the assertions say so.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def case_kw(lx, ra, pr, npart, seed):
+    from rbtp_amd.serve import request_case_kw
+    kw = request_case_kw(lx, ra, pr, npart)
+    kw["particle_rand_seed"] = seed
+    return kw
+
+
+REQS = [case_kw(32, 3e4, 0.71, 0, 1), case_kw(48, 2e5, 4.3, 6, 2), case_kw(32, 1e6, 0.71, 3, 3),
+        case_kw(64, 5e4, 0.71, 0, 4)]
+
+
+def admit(rb, batch, kw, seed):
+    case = rb.default_case(**kw)
+    T = rb.linear_temperature(case, seed, 1e-3)
+    return batch.admit(case, temperature=T)
+
+
+def snapshot(sim):
+    return {n: sim.populations(n) for n in ("f", "g")} | {"phi": sim.field("phi")}
+
+
+@pytest.mark.parametrize("prec", [2, 0])
+def test_alone_equals_mixed(rb, prec):
+    alone = rb.Batch(precision=prec, max_slots=4)
+    s = admit(rb, alone, REQS[1], 11)
+    alone.step(60)
+    ref = snapshot(alone.sim(s))
+    nu_ref = alone.sim(s).nu()
+    alone.close()
+    mixed = rb.Batch(precision=prec, max_slots=8)
+    others = [admit(rb, mixed, REQS[0], 10)]
+    mixed.step(20)  # the mix is already running when the request arrives
+    s2 = admit(rb, mixed, REQS[1], 11)
+    others += [admit(rb, mixed, REQS[2], 12), admit(rb, mixed, REQS[3], 13)]
+    mixed.step(40)
+    mixed.release(others[0])
+    mixed.step(20)
+    got = snapshot(mixed.sim(s2))
+    for k in ref:
+        assert np.array_equal(got[k], ref[k]), k
+    assert np.array_equal(mixed.sim(s2).nu(), nu_ref)
+
+
+def test_batch_matches_single_sim(rb):
+    """A batched slot and a standalone rbtp_sim of the same case agree bitwise."""
+    kw = dict(REQS[2], precision=2)
+    case = rb.default_case(**kw)
+    T = rb.linear_temperature(case, 12, 1e-3)
+    sim = rb.Sim(case)
+    sim.init(temperature=T)
+    sim.step(41)
+    batch = rb.Batch(precision=2, max_slots=2)
+    s = batch.admit(case, temperature=T)  # admission runs the first step
+    batch.step(40)
+    b = batch.sim(s)
+    for k in ("f", "g"):
+        assert np.array_equal(b.populations(k), sim.populations(k)), k
+    for k in ("pos_x", "vel_x", "temp", "force_x"):
+        assert np.array_equal(b.particles(k), sim.particles(k)), k
+
+
+def test_graph_replay_equals_eager(rb):
+    res = []
+    for graph_steps in (0, 10):
+        batch = rb.Batch(precision=2, max_slots=4)
+        slots = [admit(rb, batch, REQS[i], 20 + i) for i in range(3)]
+        batch.set_launch(128, 512, graph_steps)
+        batch.step(47)
+        batch.step(33)
+        res.append([snapshot(batch.sim(s)) for s in slots])
+        batch.close()
+    for a, b in zip(*res):
+        for k in a:
+            assert np.array_equal(a[k], b[k]), k
+
+
+def test_tuning_leaves_state_unchanged(rb):
+    res = []
+    for tune in (False, True):
+        batch = rb.Batch(precision=2, max_slots=4)
+        slots = [admit(rb, batch, REQS[i], 30 + i) for i in range(3)]
+        batch.step(10)
+        if tune:
+            blk, grd, ms = batch.tune()
+            assert blk in (64, 128, 256) and grd >= 1 and ms > 0
+        batch.step(30)
+        res.append([snapshot(batch.sim(s)) for s in slots])
+        batch.close()
+    for a, b in zip(*res):
+        for k in a:
+            assert np.array_equal(a[k], b[k]), k
+
+
+def test_admission_limits(rb):
+    batch = rb.Batch(precision=2, max_slots=2)
+    admit(rb, batch, REQS[0], 1)
+    admit(rb, batch, REQS[0], 2)
+    with pytest.raises(rb.CapacityError):
+        admit(rb, batch, REQS[0], 3)
+    tiny = rb.Batch(precision=2, max_slots=4, hbm_fraction=1e-6)
+    with pytest.raises(rb.CapacityError):
+        admit(rb, tiny, REQS[0], 4)
+
+
+def test_request_loop_small(rb):
+    from rbtp_amd.serve import make_requests, serve
+    reqs = make_requests(6, mean_gap=0.05, seed=5, max_steps=600, max_particles=4,
+                         sizes=(32, 48))
+    for r in reqs:
+        r.window = 200
+    summary = serve(reqs, max_slots=4, chunk=100, tune=True)
+    assert summary["requests"] == 6
+    assert all(np.isfinite(r.latency) and r.latency > 0 for r in reqs)
+    assert all(r.steps >= 201 for r in reqs)
+    assert summary["p99_s"] >= summary["p50_s"] > 0
+    assert summary["aggregate_mlups"] > 0
