@@ -17,8 +17,17 @@ def case_kw(lx, ra, pr, npart, seed):
     return kw
 
 
-REQS = [case_kw(32, 3e4, 0.71, 0, 1), case_kw(48, 2e5, 4.3, 6, 2), case_kw(32, 1e6, 0.71, 3, 3),
-        case_kw(64, 5e4, 0.71, 0, 4)]
+class _Reqs:
+    """Four requests of different size, Ra, Pr and particle count (built on first use)."""
+
+    def __getitem__(self, i):
+        if not hasattr(self, "_v"):
+            self._v = [case_kw(32, 3e4, 0.71, 0, 1), case_kw(48, 2e5, 4.3, 6, 2),
+                       case_kw(32, 1e6, 0.71, 3, 3), case_kw(64, 5e4, 0.71, 0, 4)]
+        return self._v[i]
+
+
+REQS = _Reqs()
 
 
 def admit(rb, batch, kw, seed):
