@@ -152,6 +152,7 @@ def test_rb3d_driver_outputs_and_restart(tmp_path):
 #define T_REF 0.0
 #define ACTIVATE_PARTICLES 1
 #define PARTICLE_FRACTION 0.01
+#define NPART_OVERRIDE 2
 #define PARTICLE_DIAM_FRAC (6.0/24.0)
 #define RHO_PARTICLE_RATIO 2.0
 #define BETA 0.05
