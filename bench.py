@@ -91,6 +91,26 @@ def cpu_baseline(kw, log, budget_s=20.0):
                        f"{kw['lx']}x{kw['ly']}x{kw['lz']} Ra=1e8 workload")
 
 
+def run_serving(rb, nreq, log):
+    """Served batch: Poisson arrivals at 50% of the measured batch capacity."""
+    from rbtp_amd.serve import make_requests, serve
+    import numpy as np
+    # capacity: aggregate rate of a full batch of typical requests, then arrivals at half of it
+    probe = make_requests(16, mean_gap=0.0, seed=99, max_steps=1000)
+    cal = serve(probe, max_slots=64, chunk=100)
+    mean_updates = np.mean([r.nodes for r in probe]) * 5000
+    capacity = cal["aggregate_mlups"] * 1e6 / mean_updates  # requests per second
+    mean_gap = 1.0 / (0.5 * capacity)
+    log(f"serving calibration: {cal['aggregate_mlups']:.0f} MLUPS, capacity {capacity:.2f} req/s, "
+        f"mean gap {mean_gap:.3f} s")
+    reqs = make_requests(nreq, mean_gap=mean_gap, seed=2024)
+    out = serve(reqs, max_slots=64, chunk=100, log=log)
+    out.update(mean_gap_s=mean_gap, load=0.5, precision="fp16s")
+    log(f"serving: p50 {out['p50_s']:.2f} s, p99 {out['p99_s']:.2f} s, "
+        f"{out['aggregate_mlups']:.0f} MLUPS over {out['wall_s']:.1f} s")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -100,6 +120,11 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-forms", action="store_true", help="skip the FP16-storage form")
     ap.add_argument("--quiet", action="store_true")
+    ap.add_argument("--counter-run", action="store_true",
+                    help="minimal run for memory-counter passes: 3 FP64 lattice steps at "
+                         "320x320x160 and one 1 GiB copy-probe launch, no JSON")
+    ap.add_argument("--serve", type=int, default=0,
+                    help="also run the served-batch workload with this many requests")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -114,6 +139,17 @@ def main():
     from cases import config2
     rb = load_rbtp()
     kw64 = config2(precision=0)
+    if args.counter_run:
+        case = rb.default_case(**kw64)
+        sim = rb.Sim(case)
+        sim.init(temperature=rb.linear_temperature(case, 2024, 1e-3))
+        sim.step(3, macro_last=False)
+        sim.close()
+        rb.bandwidth_probe(1 << 30, 1)
+        log("counter run done")
+        return
+    copy_gbs = rb.bandwidth_probe(2 << 30, 20)
+    log(f"copy probe: {copy_gbs:.0f} GB/s")
     r64 = run_form(rb, kw64, args.steps, args.warmup, args.timing_steps, log)
     n = r64["n"]
     mlups = n * args.steps / (r64["wall_ms"] * 1e-3) / 1e6
@@ -135,6 +171,7 @@ def main():
         with open(tpath) as fh:
             traffic = json.load(fh).get("bytes_per_launch")
     cpu = None if args.no_cpu else cpu_baseline(kw64, log)
+    serving = run_serving(rb, args.serve, log) if args.serve else None
     out = {
         "metric": "MLUPS (D3Q27 f+g), FP64, 320x320x160 Ra=1e8",
         "value": mlups,
@@ -154,9 +191,11 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "k_lattice_step<double,double,strict>",
+                     "copy_probe_gbs": copy_gbs, "frac_of_copy_probe": achieved / copy_gbs,
                      "kernel_ms": r64["kernel_ms"], "bytes_per_node": BYTES_PER_NODE[0]},
         "cpu_baseline": cpu,
         "forms": forms,
+        "serving": serving,
         "nu_fp64": r64["nu"],
         "ke_fp64": r64["ke"],
     }

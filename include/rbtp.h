@@ -220,6 +220,10 @@ int rbtp_get_new_fluid(rbtp_sim *s, int32_t *nodes, int32_t *pids, int64_t capac
 int rbtp_nu(rbtp_sim *s, double out[7]);
 int rbtp_means(rbtp_sim *s, double out[4]);
 int rbtp_kinetic_energy(rbtp_sim *s, double *out);
+/* Achievable HBM bandwidth on the current device: a streaming copy of `bytes` (read + write,
+ * 16-byte vectors) timed over `reps` launches; *gbs = 2*bytes*reps/time in GB/s.  No reference
+ * counterpart; used to put the roofline fractions in context. */
+int rbtp_bandwidth_probe(size_t bytes, int reps, double *gbs);
 
 /*
  * Output-side fields and files (output/output_flow.cu, output/output_part.cu,

@@ -136,6 +136,7 @@ def _load():
         "rbtp_means": (I, [P, dp]),
         "rbtp_kinetic_energy": (I, [P, C.POINTER(D)]),
         "rbtp_case_from_parameters_h": (I, [C.c_char_p, C.POINTER(Case)]),
+        "rbtp_bandwidth_probe": (I, [C.c_size_t, I, C.POINTER(D)]),
         "rbtp_get_output_fields": (I, [P, dp]),
         "rbtp_profile": (I, [P, dp]),
         "rbtp_write_fg": (I, [P, C.c_char_p]),
@@ -182,6 +183,13 @@ def case_from_parameters_h(path: str) -> Case:
     c = Case()
     _check(lib.rbtp_case_from_parameters_h(os.fsencode(path), C.byref(c)))
     return c
+
+
+def bandwidth_probe(nbytes: int = 2 << 30, reps: int = 20) -> float:
+    """Achievable HBM bandwidth (GB/s) of a streaming copy on the current device."""
+    v = C.c_double()
+    _check(lib.rbtp_bandwidth_probe(int(nbytes), int(reps), C.byref(v)))
+    return v.value
 
 
 def derive(case: Case) -> Derived:

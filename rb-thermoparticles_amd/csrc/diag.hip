@@ -157,7 +157,8 @@ __global__ __launch_bounds__(DB) void k_output_fields(const S *fc, const S *gc,
         const double geq = tp * phi_local * (1.0 + eu + 0.5 * (eu * eu - uv));
         const double geqnew = (ip == 0) ? (geq - lambda0 * phi_local * rho_local)
                                         : (geq + tp * phi_local * rho_local);
-        const double diff = dec_g<S, double>(Enc<S>::ld(gc + (size_t)ip * n + idx), ip, k.t0) - geqnew;
+        const double diff = dec_g<S, double>(Enc<S>::ld(gc + (size_t)ip * n + idx), ip,
+                                             k.tl0 + k.tls * double(idx % d.lx)) - geqnew;
         gneqx += cx * diff;
         gneqy += cy * diff;
         gneqz += cz * diff;
@@ -216,6 +217,15 @@ __global__ __launch_bounds__(DB) void k_profile(const double *fields, Dims d, do
     block_tree<4>(red, t);
     if (t == 0)
         for (int c = 0; c < 4; c++) out[ix * 4 + c] = red[c][0];
+}
+
+
+// Streaming copy of 16-byte vectors: the reference kernel for the achievable HBM bandwidth and
+// for calibrating the memory counters (it reads and writes exactly `count` vectors).
+__global__ __launch_bounds__(256) void k_copy_probe(const double2 *__restrict__ src,
+                                                    double2 *__restrict__ dst, size_t count) {
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < count; i += (size_t)gridDim.x * 256)
+        dst[i] = src[i];
 }
 
 constexpr int RED_BLOCKS = 1024;
@@ -354,6 +364,46 @@ int diag_profile(const double *dev_fields, const Dims &dims, hipStream_t st, dou
     const double count = (double)dims.ly * (double)dims.lz;
     const double inv_count = (count > 0) ? (1.0 / count) : 0.0;
     for (size_t i = 0; i < h.size(); i++) out[i] = h[i] * inv_count;
+    return RBTP_OK;
+}
+
+
+int diag_copy_probe(size_t bytes, int reps, double *gbs) {
+    const size_t count = bytes / sizeof(double2);
+    if (count == 0 || reps < 1) return report(RBTP_E_ARG, "copy probe: bytes >= 16 and reps >= 1");
+    double2 *a = nullptr, *b = nullptr;
+    hipStream_t st = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipError_t e = hipMalloc(&a, count * sizeof(double2));
+    if (e == hipSuccess) e = hipMalloc(&b, count * sizeof(double2));
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    if (e == hipSuccess) e = hipMemsetAsync(a, 0, count * sizeof(double2), st);
+    int dev = 0, cus = 256;
+    if (e == hipSuccess) e = hipGetDevice(&dev);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    const int grid = cus * 16;
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_copy_probe, grid, 256, 0, st, a, b, count);  // warm-up
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipEventRecord(e0, st);
+    for (int r = 0; r < reps && e == hipSuccess; r++) {
+        hipLaunchKernelGGL(k_copy_probe, grid, 256, 0, st, a, b, count);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipEventRecord(e1, st);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    float ms = 0.f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    if (e == hipSuccess) *gbs = 2.0 * (double)(count * sizeof(double2)) * reps / (ms * 1e-3) / 1e9;
+    (void)hipFree(a);
+    (void)hipFree(b);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (st) (void)hipStreamDestroy(st);
+    if (e != hipSuccess) return report_hip(e, "copy probe");
     return RBTP_OK;
 }
 

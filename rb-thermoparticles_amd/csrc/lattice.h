@@ -19,17 +19,19 @@ hipError_t launch_special_collide(const StepArgs<S> &a, const SpecialBuf &sp, in
                                   bool strict, hipStream_t st);
 template <class S>
 hipError_t launch_stream_only(const S *fin, const S *gin, const typename Enc<S>::A *rho_wall,
-                              double *fout, double *gout, Dims d, double t0, double thot,
-                              double tcold, bool nostream, hipStream_t st);
+                              double *fout, double *gout, Dims d, double tl0, double tls,
+                              double thot, double tcold, bool nostream, hipStream_t st);
 template <class S>
-hipError_t launch_decode(const S *in, double *out, size_t n, int is_g, double t0, hipStream_t st);
+hipError_t launch_decode(const S *in, double *out, size_t n, int is_g, double tl0, double tls,
+                         int lx, hipStream_t st);
 template <class S>
-hipError_t launch_encode(const double *in, S *out, size_t n, int is_g, double t0, hipStream_t st);
+hipError_t launch_encode(const double *in, S *out, size_t n, int is_g, double tl0, double tls,
+                         int lx, hipStream_t st);
 template <class S>
 hipError_t launch_init(S *f, S *g, typename Enc<S>::A *fx, typename Enc<S>::A *rho_wall,
                        const double *temperature, double rho0, double t0, double grav0,
-                       double beta, Dims d, double *mr, double *mux, double *muy, double *muz,
-                       double *mphi, hipStream_t st);
+                       double beta, double tl0, double tls, Dims d, double *mr, double *mux,
+                       double *muy, double *muz, double *mphi, hipStream_t st);
 
 template <class S>
 hipError_t launch_forcing_field(typename Enc<S>::A *fx, typename Enc<S>::A *rho_wall,

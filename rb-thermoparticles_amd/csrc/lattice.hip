@@ -47,8 +47,20 @@ template <class S> struct NodeOut {
     size_t n;
 };
 
+// x / d, correctly rounded, from a double-word reciprocal yh + yl of d (yh = RN(1/d)).
+// q0 = RN(x*yh + RN(x*yl)) is within half an ulp plus 2^-100 relative of x/d, hence a faithful
+// rounding; by Markstein's theorem (yh within half an ulp of 1/d) the residual x - q0*d is exact
+// in one FMA and RN(q0 + r*yh) is the correctly rounded quotient.  Four FP64 operations replace
+// the ~10-instruction IEEE division sequence with an identical result.
+__device__ inline double div_rn(double x, double d, double yh, double yl) {
+    const double q0 = __builtin_fma(x, yh, x * yl);
+    const double r = __builtin_fma(-q0, d, x);
+    return __builtin_fma(r, yh, q0);
+}
+
 // Strict FP64 macrovar + forcing + collision_BGK + collision_BGK_scalar for one node.
-// Returns the new F_x; writes the post-collision populations.
+// Returns the new F_x; writes the post-collision populations.  Every rounding is the one of
+// the reference's expression (divisions via div_rn, which returns the IEEE quotient).
 template <class S>
 __device__ inline double collide_strict(double (&f)[NPOP], double (&g)[NPOP], double fx_old,
                                         int idx, const NodeOut<S> &o, const FluidK &k,
@@ -59,21 +71,21 @@ __device__ inline double collide_strict(double (&f)[NPOP], double (&g)[NPOP], do
     const double fx = k.gb * (phi - k.t0);  // forcing, macrovar.cu:44
     const double RT = 1.0 / 3.0;
     const double u2 = ux * ux + uy * uy + uz * uz;
-    const double uv = u2 / RT;
+    const double uv = div_rn(u2, RT, k.rt_h, k.rt_l);
     double X[NPOP];  // (1 + eu) + 0.5 (eu^2 - uv), shared by f's Guo term and g's equilibrium
     // collision_BGK (collision.cu:16-32)
 #pragma unroll
     for (int i = 0; i < NPOP; i++) {
         const double tp = weight(i);
-        const double eu = (i == 0) ? 0.0 : cdot<double>(i, ux, uy, uz) / RT;
+        const double eu = (i == 0) ? 0.0 : div_rn(cdot<double>(i, ux, uy, uz), RT, k.rt_h, k.rt_l);
         const double h = 0.5 * (eu * eu - uv);
         const double feq = tp * ((rho + eu) + h);
         const double fpop = f[i];
-        double fchange = -(fpop - feq) / k.tau;
+        double fchange = div_rn(-(fpop - feq), k.tau, k.tau_h, k.tau_l);
         X[i] = (1.0 + eu) + h;
         const double feq_rho0 = tp * X[i];
         const double cuF = ((double)D3Q27::cx[i] - ux) * fx;
-        fchange += k.force_coeff * cuF / RT * feq_rho0;
+        fchange += div_rn(k.force_coeff * cuF, RT, k.rt_h, k.rt_l) * feq_rho0;
         Enc<S>::st(o.fout + (size_t)i * o.n + idx, fpop + fchange);
     }
     // collision_BGK_scalar pass 1 (collision.cu:57-70)
@@ -93,28 +105,30 @@ __device__ inline double collide_strict(double (&f)[NPOP], double (&g)[NPOP], do
         if (D3Q27::cz[i] == -1) gz -= diff;
     }
     // pass 2 (collision.cu:72-87)
-    const double grad_denom = rho / 3.0 + k.c2t;
+    const double grad_denom = div_rn(rho, 3.0, k.three_h, k.three_l) + k.c2t;
+    const double gd_h = 1.0 / grad_denom;
+    const double gd_l = __builtin_fma(-grad_denom, gd_h, 1.0) * gd_h;
 #pragma unroll
     for (int i = 0; i < NPOP; i++) {
         const double tp = weight(i);
         const double gpop = g[i];
         double gchange;
         if (i == 0) {
-            gchange = -(gpop - X[i]) / k.tauc;
+            gchange = div_rn(-(gpop - X[i]), k.tauc, k.tauc_h, k.tauc_l);
         } else {
             const int cx = D3Q27::cx[i];
             const double gneq = cdot<double>(i, gx, gy, gz);
             double gradt, src;
             if (cx != 0) {
                 const double eb = cx > 0 ? fx : -fx;
-                gradt = -(2.0 * gneq + eb * phi) / grad_denom;
+                gradt = div_rn(-(2.0 * gneq + eb * phi), grad_denom, gd_h, gd_l);
                 src = gradt * rho + 3.0 * phi * eb;
             } else {
-                gradt = -(2.0 * gneq) / grad_denom;
+                gradt = div_rn(-(2.0 * gneq), grad_denom, gd_h, gd_l);
                 src = gradt * rho;
             }
             const double Ralph = tp * k.ralph_coeff * src;
-            gchange = -(gpop - X[i]) / k.tauc + Ralph;
+            gchange = div_rn(-(gpop - X[i]), k.tauc, k.tauc_h, k.tauc_l) + Ralph;
         }
         Enc<S>::st(o.gout + (size_t)i * o.n + idx, gpop + gchange);
     }
@@ -123,7 +137,7 @@ __device__ inline double collide_strict(double (&f)[NPOP], double (&g)[NPOP], do
 
 // Fast form of the same node update (reciprocals instead of divisions), in T = float or double.
 template <class S, class T>
-__device__ inline T collide_fast(T (&f)[NPOP], T (&g)[NPOP], T fx_old, int idx,
+__device__ inline T collide_fast(T (&f)[NPOP], T (&g)[NPOP], T fx_old, int idx, T gshift,
                                  const NodeOut<S> &o, const FluidK &k, T &rho, T &ux, T &uy,
                                  T &uz, T &phi) {
 #pragma clang fp contract(fast)
@@ -171,7 +185,7 @@ __device__ inline T collide_fast(T (&f)[NPOP], T (&g)[NPOP], T fx_old, int idx,
             const T gradt = -(T(2) * gneq + eb * phi) * inv_gd;
             gnew += tp * rc * (gradt * rho + T(3) * phi * eb);
         }
-        enc_g<S, T>(o.gout + (size_t)i * o.n + idx, gnew, i, t0);
+        enc_g<S, T>(o.gout + (size_t)i * o.n + idx, gnew, i, gshift);
     }
     return fx;
 }
@@ -181,14 +195,15 @@ template <> struct Arith<double> { static constexpr bool strict_ok = true; };
 template <> struct Arith<float> { static constexpr bool strict_ok = false; };
 
 template <class S, class T, bool STRICT>
-__device__ inline T collide_node(T (&f)[NPOP], T (&g)[NPOP], T fx_old, int idx,
+__device__ inline T collide_node(T (&f)[NPOP], T (&g)[NPOP], T fx_old, int idx, T gshift,
                                  const NodeOut<S> &o, const FluidK &k, T &rho, T &ux, T &uy,
                                  T &uz, T &phi) {
     if constexpr (STRICT) {
         static_assert(Arith<T>::strict_ok, "strict form needs FP64 arithmetic");
+        (void)gshift;
         return collide_strict<S>(f, g, fx_old, idx, o, k, rho, ux, uy, uz, phi);
     } else {
-        return collide_fast<S, T>(f, g, fx_old, idx, o, k, rho, ux, uy, uz, phi);
+        return collide_fast<S, T>(f, g, fx_old, idx, gshift, o, k, rho, ux, uy, uz, phi);
     }
 }
 
@@ -223,12 +238,14 @@ __device__ inline void node_step(const StepArgs<S> &a, int idx) {
         else if (ix == a.d.lx - 1) rho_w = T(a.rho_wall[a.d.ly * a.d.lz + iz * a.d.ly + iy]);
     }
     T f[NPOP], g[NPOP];
-    gather<S, T, NOSTREAM>(a.fin, a.gin, (size_t)a.d.n, idx, ix, iy, iz, a.d, rho_w,
-                           T(a.k.t0), T(a.k.thot), T(a.k.tcold), f, g);
+    const T tl0 = T(a.k.tl0), tls = T(a.k.tls);
+    gather<S, T, NOSTREAM>(a.fin, a.gin, (size_t)a.d.n, idx, ix, iy, iz, a.d, rho_w, tl0, tls,
+                           T(a.k.thot), T(a.k.tcold), f, g);
     const T fx_old = T(a.fx[idx]);
     NodeOut<S> o{a.fout, a.gout, (size_t)a.d.n};
     T rho, ux, uy, uz, phi;
-    const T fx = collide_node<S, T, STRICT>(f, g, fx_old, idx, o, a.k, rho, ux, uy, uz, phi);
+    const T fx = collide_node<S, T, STRICT>(f, g, fx_old, idx, tl0 + tls * T(ix), o, a.k, rho,
+                                            ux, uy, uz, phi);
     write_node_aux<S, T>(a, idx, ix, iy, iz, fx, rho, ux, uy, uz, phi);
 }
 
@@ -268,8 +285,8 @@ __global__ __launch_bounds__(BLOCK) void k_stream_only(const S *__restrict__ fin
                                                        const S *__restrict__ gin,
                                                        const typename Enc<S>::A *rho_wall,
                                                        double *fout, double *gout, Dims d,
-                                                       double t0, double thot, double tcold,
-                                                       int nostream) {
+                                                       double tl0, double tls, double thot,
+                                                       double tcold, int nostream) {
     const int idx = blockIdx.x * BLOCK + threadIdx.x;
     if (idx >= d.n) return;
     const int iz = idx / d.lxy;
@@ -282,11 +299,11 @@ __global__ __launch_bounds__(BLOCK) void k_stream_only(const S *__restrict__ fin
     else if (ix == d.lx - 1) rho_w = T(rho_wall[d.ly * d.lz + iz * d.ly + iy]);
     T f[NPOP], g[NPOP];
     if (nostream)
-        gather<S, T, true>(fin, gin, (size_t)d.n, idx, ix, iy, iz, d, rho_w, T(t0), T(thot),
-                           T(tcold), f, g);
+        gather<S, T, true>(fin, gin, (size_t)d.n, idx, ix, iy, iz, d, rho_w, T(tl0), T(tls),
+                           T(thot), T(tcold), f, g);
     else
-        gather<S, T, false>(fin, gin, (size_t)d.n, idx, ix, iy, iz, d, rho_w, T(t0), T(thot),
-                            T(tcold), f, g);
+        gather<S, T, false>(fin, gin, (size_t)d.n, idx, ix, iy, iz, d, rho_w, T(tl0), T(tls),
+                            T(thot), T(tcold), f, g);
 #pragma unroll
     for (int i = 0; i < NPOP; i++) {
         fout[(size_t)i * d.n + idx] = double(f[i]);
@@ -297,7 +314,7 @@ __global__ __launch_bounds__(BLOCK) void k_stream_only(const S *__restrict__ fin
 // Decode a population buffer to FP64 (post-collision accessor).
 template <class S>
 __global__ __launch_bounds__(BLOCK) void k_decode(const S *__restrict__ in, double *out, size_t n,
-                                                  int is_g, double t0) {
+                                                  int is_g, double tl0, double tls, int lx) {
     const size_t j = (size_t)blockIdx.x * BLOCK + threadIdx.x;
     if (j >= (size_t)NPOP * n) return;
     const int i = (int)(j / n);
@@ -306,7 +323,8 @@ __global__ __launch_bounds__(BLOCK) void k_decode(const S *__restrict__ in, doub
     const double w = i == 0 ? 8.0 / 27.0 : (i <= 6 ? 2.0 / 27.0 : (i <= 18 ? 1.0 / 54.0 : 1.0 / 216.0));
     const T s = T(Enc<S>::ld(in + j));
     if constexpr (Enc<S>::scaled) {
-        out[j] = is_g ? double((s + T(t0)) * T(w)) : double(s * T(w));
+        const T shift = T(tl0 + tls * double((j % n) % (size_t)lx));
+        out[j] = is_g ? double((s + shift) * T(w)) : double(s * T(w));
     } else {
         out[j] = double(s);
     }
@@ -315,14 +333,15 @@ __global__ __launch_bounds__(BLOCK) void k_decode(const S *__restrict__ in, doub
 // Encode FP64 populations into the storage form (restart load, set_populations).
 template <class S>
 __global__ __launch_bounds__(BLOCK) void k_encode(const double *__restrict__ in, S *out, size_t n,
-                                                  int is_g, double t0) {
+                                                  int is_g, double tl0, double tls, int lx) {
     const size_t j = (size_t)blockIdx.x * BLOCK + threadIdx.x;
     if (j >= (size_t)NPOP * n) return;
     const int i = (int)(j / n);
     const double w = i == 0 ? 8.0 / 27.0 : (i <= 6 ? 2.0 / 27.0 : (i <= 18 ? 1.0 / 54.0 : 1.0 / 216.0));
     const double v = in[j];
     if constexpr (Enc<S>::scaled) {
-        Enc<S>::st(out + j, is_g ? float(v * (1.0 / w) - t0) : float(v * (1.0 / w)));
+        const double shift = tl0 + tls * double((j % n) % (size_t)lx);
+        Enc<S>::st(out + j, is_g ? float(v * (1.0 / w) - shift) : float(v * (1.0 / w)));
     } else {
         Enc<S>::st(out + j, (S)v);
     }
@@ -335,7 +354,8 @@ template <class S>
 __global__ __launch_bounds__(BLOCK) void k_init(S *f, S *g, typename Enc<S>::A *fx_field,
                                                 typename Enc<S>::A *rho_wall,
                                                 const double *temperature, double rho0,
-                                                double t0, double grav0, double beta, Dims d,
+                                                double t0, double grav0, double beta,
+                                                double tl0, double tls, Dims d,
                                                 double *macro_rho, double *macro_ux,
                                                 double *macro_uy, double *macro_uz,
                                                 double *macro_phi) {
@@ -365,7 +385,7 @@ __global__ __launch_bounds__(BLOCK) void k_init(S *f, S *g, typename Enc<S>::A *
         double geq = weight(i) * tempT * (1.0 + eu + 0.5 * (eu * eu - uv));
         if (i == 0) geq -= lambda0 * tempT * r;
         else geq += weight(i) * tempT * r;
-        enc_g<S, double>(g + (size_t)i * n + idx, geq, i, t0);
+        enc_g<S, double>(g + (size_t)i * n + idx, geq, i, tl0 + tls * double(ix));
     }
     using A = typename Enc<S>::A;
     fx_field[idx] = A(grav0 * beta * (tempT - t0));
@@ -401,8 +421,9 @@ __global__ __launch_bounds__(BLOCK) void k_special_collide(const StepArgs<S> a, 
     }
     NodeOut<S> o{a.fout, a.gout, (size_t)a.d.n};
     T rho, ux, uy, uz, phi;
-    const T fx = collide_node<S, T, STRICT>(f, g, T(sp.fx_old[j]), idx, o, a.k, rho, ux, uy, uz,
-                                            phi);
+    const T fx = collide_node<S, T, STRICT>(f, g, T(sp.fx_old[j]), idx,
+                                            T(a.k.tl0) + T(a.k.tls) * T(ix), o, a.k, rho, ux,
+                                            uy, uz, phi);
     write_node_aux<S, T>(a, idx, ix, iy, iz, fx, rho, ux, uy, uz, phi);
 }
 
@@ -465,32 +486,36 @@ hipError_t launch_special_collide(const StepArgs<S> &a, const SpecialBuf &sp, in
 
 template <class S>
 hipError_t launch_stream_only(const S *fin, const S *gin, const typename Enc<S>::A *rho_wall,
-                              double *fout, double *gout, Dims d, double t0, double thot,
-                              double tcold, bool nostream, hipStream_t st) {
+                              double *fout, double *gout, Dims d, double tl0, double tls,
+                              double thot, double tcold, bool nostream, hipStream_t st) {
     hipLaunchKernelGGL((k_stream_only<S>), blocks_for((size_t)d.n), BLOCK, 0, st, fin, gin,
-                       rho_wall, fout, gout, d, t0, thot, tcold, nostream ? 1 : 0);
+                       rho_wall, fout, gout, d, tl0, tls, thot, tcold, nostream ? 1 : 0);
     return hipGetLastError();
 }
 
 template <class S>
-hipError_t launch_decode(const S *in, double *out, size_t n, int is_g, double t0, hipStream_t st) {
-    hipLaunchKernelGGL((k_decode<S>), blocks_for((size_t)NPOP * n), BLOCK, 0, st, in, out, n, is_g, t0);
+hipError_t launch_decode(const S *in, double *out, size_t n, int is_g, double tl0, double tls,
+                         int lx, hipStream_t st) {
+    hipLaunchKernelGGL((k_decode<S>), blocks_for((size_t)NPOP * n), BLOCK, 0, st, in, out, n, is_g,
+                       tl0, tls, lx);
     return hipGetLastError();
 }
 
 template <class S>
-hipError_t launch_encode(const double *in, S *out, size_t n, int is_g, double t0, hipStream_t st) {
-    hipLaunchKernelGGL((k_encode<S>), blocks_for((size_t)NPOP * n), BLOCK, 0, st, in, out, n, is_g, t0);
+hipError_t launch_encode(const double *in, S *out, size_t n, int is_g, double tl0, double tls,
+                         int lx, hipStream_t st) {
+    hipLaunchKernelGGL((k_encode<S>), blocks_for((size_t)NPOP * n), BLOCK, 0, st, in, out, n, is_g,
+                       tl0, tls, lx);
     return hipGetLastError();
 }
 
 template <class S>
 hipError_t launch_init(S *f, S *g, typename Enc<S>::A *fx, typename Enc<S>::A *rho_wall,
                        const double *temperature, double rho0, double t0, double grav0,
-                       double beta, Dims d, double *mr, double *mux, double *muy, double *muz,
-                       double *mphi, hipStream_t st) {
+                       double beta, double tl0, double tls, Dims d, double *mr, double *mux,
+                       double *muy, double *muz, double *mphi, hipStream_t st) {
     hipLaunchKernelGGL((k_init<S>), blocks_for((size_t)d.n), BLOCK, 0, st, f, g, fx, rho_wall,
-                       temperature, rho0, t0, grav0, beta, d, mr, mux, muy, muz, mphi);
+                       temperature, rho0, t0, grav0, beta, tl0, tls, d, mr, mux, muy, muz, mphi);
     return hipGetLastError();
 }
 
@@ -543,13 +568,16 @@ hipError_t launch_lattice_batched(const StepArgs<S> *slots_dev, const int *tile_
                                                   bool, hipStream_t);                           \
     template hipError_t launch_stream_only<S>(const S *, const S *,                             \
                                               const typename Enc<S>::A *, double *, double *,   \
-                                              Dims, double, double, double, bool, hipStream_t); \
-    template hipError_t launch_decode<S>(const S *, double *, size_t, int, double, hipStream_t); \
-    template hipError_t launch_encode<S>(const double *, S *, size_t, int, double, hipStream_t); \
+                                              Dims, double, double, double, double, bool,       \
+                                              hipStream_t);                                      \
+    template hipError_t launch_decode<S>(const S *, double *, size_t, int, double, double, int, \
+                                         hipStream_t);                                          \
+    template hipError_t launch_encode<S>(const double *, S *, size_t, int, double, double, int, \
+                                         hipStream_t);                                          \
     template hipError_t launch_init<S>(S *, S *, typename Enc<S>::A *, typename Enc<S>::A *,    \
-                                       const double *, double, double, double, double, Dims,    \
-                                       double *, double *, double *, double *, double *,        \
-                                       hipStream_t);                                            \
+                                       const double *, double, double, double, double, double,  \
+                                       double, Dims, double *, double *, double *, double *,    \
+                                       double *, hipStream_t);                                            \
     template hipError_t launch_forcing_field<S>(typename Enc<S>::A *, typename Enc<S>::A *,     \
                                                 const double *, const double *, double, double, \
                                                 Dims, hipStream_t);

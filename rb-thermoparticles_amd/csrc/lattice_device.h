@@ -14,8 +14,8 @@ __device__ inline T dec_f(S s, int i) {
     else return T(s);
 }
 template <class S, class T>
-__device__ inline T dec_g(S s, int i, T t0) {
-    if constexpr (Enc<S>::scaled) return (T(s) + t0) * T(weight(i));
+__device__ inline T dec_g(S s, int i, T shift) {
+    if constexpr (Enc<S>::scaled) return (T(s) + shift) * T(weight(i));
     else return T(s);
 }
 template <class S, class T>
@@ -24,8 +24,8 @@ __device__ inline void enc_f(S *p, T v, int i) {
     else Enc<S>::st(p, v);
 }
 template <class S, class T>
-__device__ inline void enc_g(S *p, T v, int i, T t0) {
-    if constexpr (Enc<S>::scaled) Enc<S>::st(p, float(v * T(1.0 / weight(i)) - t0));
+__device__ inline void enc_g(S *p, T v, int i, T shift) {
+    if constexpr (Enc<S>::scaled) Enc<S>::st(p, float(v * T(1.0 / weight(i)) - shift));
     else Enc<S>::st(p, v);
 }
 
@@ -35,15 +35,16 @@ __device__ inline void enc_g(S *p, T v, int i, T t0) {
 //      -g*_opp(x) + 2 w_i T_wall (1 + rho(x)) (streaming.cu:37-61).
 template <class S, class T, bool NOSTREAM>
 __device__ inline void gather(const S *__restrict__ fin, const S *__restrict__ gin, size_t n,
-                              int idx, int ix, int iy, int iz, const Dims &d, T rho_w, T t0,
-                              T thot, T tcold, T (&f)[NPOP], T (&g)[NPOP]) {
+                              int idx, int ix, int iy, int iz, const Dims &d, T rho_w, T tl0,
+                              T tls, T thot, T tcold, T (&f)[NPOP], T (&g)[NPOP]) {
+    const T own_shift = tl0 + tls * T(ix);
 #pragma unroll
     for (int i = 0; i < NPOP; i++) {
         const int cx = D3Q27::cx[i], cy = D3Q27::cy[i], cz = D3Q27::cz[i];
         const int op = D3Q27::opp[i];
         if (NOSTREAM) {
             f[i] = dec_f<S, T>(Enc<S>::ld(fin + (size_t)i * n + idx), i);
-            g[i] = dec_g<S, T>(Enc<S>::ld(gin + (size_t)i * n + idx), i, t0);
+            g[i] = dec_g<S, T>(Enc<S>::ld(gin + (size_t)i * n + idx), i, own_shift);
             continue;
         }
         const bool outx = (cx == 1 && ix == 0) || (cx == -1 && ix == d.lx - 1);
@@ -53,7 +54,8 @@ __device__ inline void gather(const S *__restrict__ fin, const S *__restrict__ g
         const int off = cx + cy * d.lx + cz * d.lxy;
         const size_t src = inside ? (size_t)i * n + (size_t)(idx - off) : (size_t)op * n + idx;
         const T fv = dec_f<S, T>(Enc<S>::ld(fin + src), i);
-        const T gv = dec_g<S, T>(Enc<S>::ld(gin + src), i, t0);
+        const T gv = dec_g<S, T>(Enc<S>::ld(gin + src), i,
+                                 inside ? tl0 + tls * T(ix - cx) : own_shift);
         f[i] = fv;
         if (inside || outz || outy) {
             g[i] = gv;

@@ -20,6 +20,7 @@ struct PartDev {
     // Case constants (globals.cu:126-148, pforces.cu:121-124).
     double radius, radius_sq, mass, inertia, heat_capacity, rho_ratio, grav0, coeff;
     double min_gap, min_gap_w, stf0, stf1, stf0_w, stf1_w, rho0, t0, thot, tcold;
+    double tl0, tls;  // shift of the 16-bit g encoding (conduction profile T = tl0 + tls*ix)
     double *base;            // one allocation for every P-length FP64 array
     double *p[RBTP_P_COUNT]; // particle state, indexed by RBTP_P_*
     double *old[OLD_COUNT];  // state before this step's motion
@@ -53,7 +54,7 @@ int part_collect_new_fluid(PartDev &pd, hipStream_t st,
 
 // Constants of the output-field kernels (compute_Ked / compute_ted, overwrite_particle_nodes).
 struct OutK {
-    double tau, tauc, visc, diff, t0, rho0;
+    double tau, tauc, visc, diff, t0, rho0, tl0, tls;
 };
 
 int diag_output_fields(int prec, const void *fc, const void *gc, const void *fx,
@@ -64,6 +65,7 @@ int diag_nu(const rbtp_case &c, const rbtp_derived &d, const Dims &dims, const d
             const double *phi, const int32_t *owner, int step, hipStream_t st, double out[7]);
 int diag_means(const Dims &dims, double *const *macro, const PartDev *pd, hipStream_t st,
                double out[4]);
+int diag_copy_probe(size_t bytes, int reps, double *gbs);
 int diag_kinetic_energy(const Dims &dims, double *const *macro, hipStream_t st, double *out);
 
 }  // namespace rbtp

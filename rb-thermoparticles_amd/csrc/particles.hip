@@ -240,12 +240,14 @@ template <class S>
 struct PopView {
     const S *f, *g;
     size_t n;
-    double t0;
+    double tl0, tls;  // shift of the 16-bit g encoding (conduction profile)
+    int lx;
     __device__ inline double F(int i, int idx) const {
         return dec_f<S, double>(Enc<S>::ld(f + (size_t)i * n + idx), i);
     }
     __device__ inline double G(int i, int idx) const {
-        return dec_g<S, double>(Enc<S>::ld(g + (size_t)i * n + idx), i, t0);
+        return dec_g<S, double>(Enc<S>::ld(g + (size_t)i * n + idx), i,
+                                tl0 + tls * double(idx % lx));
     }
 };
 
@@ -566,8 +568,8 @@ __global__ __launch_bounds__(128) void k_prefixup(PartDev pd, PopView<S> pop,
             if (ix == 0) rho_w = double(rho_wall[iz * d.ly + iy]);
             else if (ix == d.lx - 1) rho_w = double(rho_wall[d.ly * d.lz + iz * d.ly + iy]);
             double f[NPOP], g[NPOP];
-            gather<S, double, false>(pop.f, pop.g, pop.n, idx, ix, iy, iz, d, rho_w, pop.t0,
-                                     pd.thot, pd.tcold, f, g);
+            gather<S, double, false>(pop.f, pop.g, pop.n, idx, ix, iy, iz, d, rho_w, pop.tl0,
+                                     pop.tls, pd.thot, pd.tcold, f, g);
             const PView pv{pd.old[OLD_PX], pd.old[OLD_PY], pd.old[OLD_PZ], pd.old[OLD_VX],
                            pd.old[OLD_VY], pd.old[OLD_VZ], pd.old[OLD_WX], pd.old[OLD_WY],
                            pd.old[OLD_WZ], pd.old[OLD_T]};
@@ -767,6 +769,8 @@ int part_alloc(PartDev &pd, const rbtp_case &c, const rbtp_derived &d, const Dim
     pd.stf1_w = c.rep_stf1_w;
     pd.rho0 = c.rho0;
     pd.t0 = c.t_ref;
+    pd.tls = (c.t_cold - c.t_hot) / (double)c.lx;
+    pd.tl0 = c.t_hot + 0.5 * pd.tls;
     pd.thot = c.t_hot;
     pd.tcold = c.t_cold;
     const size_t np = (size_t)pd.np;
@@ -911,7 +915,7 @@ int part_phase(PartDev &pd, const rbtp_case &c, const rbtp_derived &d, const Dim
     PHIP(hipGetLastError());
     PHIP(dispatch_prec(prec, [&](auto *tag) {
         using S = std::remove_pointer_t<decltype(tag)>;
-        const PopView<S> pop{(const S *)fcur, (const S *)gcur, (size_t)dims.n, pd.t0};
+        const PopView<S> pop{(const S *)fcur, (const S *)gcur, (size_t)dims.n, pd.tl0, pd.tls, dims.lx};
         hipLaunchKernelGGL((k_link_sums<S>), np, PB, 0, st, pd, pop, pd.owner[prev],
                            pd.box[prev], macro[0], dims);
         return hipGetLastError();
@@ -933,7 +937,7 @@ int part_phase(PartDev &pd, const rbtp_case &c, const rbtp_derived &d, const Dim
     PHIP(dispatch_prec(prec, [&](auto *tag) {
         using S = std::remove_pointer_t<decltype(tag)>;
         using A = typename Enc<S>::A;
-        const PopView<S> pop{(const S *)fcur, (const S *)gcur, (size_t)dims.n, pd.t0};
+        const PopView<S> pop{(const S *)fcur, (const S *)gcur, (size_t)dims.n, pd.tl0, pd.tls, dims.lx};
         hipLaunchKernelGGL((k_prefixup<S>), grid_for(pd.sp.cap, 128, 2048), 128, 0, st, pd, pop,
                            pd.owner[prev], (A *)fx, (const A *)rho_wall, macro[0], macro[1],
                            macro[2], macro[3], macro[4], dims);

@@ -29,11 +29,18 @@ struct FluidK {
     double ralph_coeff;  // 1.0 - 1.0/(2.0*tauc)               (collision.cu:72)
     double c2t;          // (2.0*tauc)/3.0, second term of grad_denom (collision.cu:73)
     double inv_tau, inv_tauc;
+    // Double-word reciprocals (hi = RN(1/d), lo = RN((1 - d*hi)/d)) of the constant divisors of
+    // the strict form: RT = 1.0/3.0, 3.0, tau, tauc.
+    double rt_h, rt_l, three_h, three_l, tau_h, tau_l, tauc_h, tauc_l;
+    // Shift of the 16-bit g encoding: the conduction profile T(ix) = tl0 + tls*ix between the
+    // hot and cold walls at cell centres.
+    double tl0, tls;
 };
 
 // Storage encodings of the populations.
 //   double/float: the reference's value.
-//   half/bf16:    value/w_i for f; value/w_i - t0 for g (shifted and scaled).
+//   half/bf16:    value/w_i for f; value/w_i - T_cond(ix) for g (shifted and scaled), where
+//                 T_cond is the linear conduction profile at the node that holds the value.
 template <class S> struct Enc;
 template <> struct Enc<double> {
     using A = double;  // auxiliary per-node field type (F_x, wall rho)

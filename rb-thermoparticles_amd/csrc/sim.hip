@@ -230,6 +230,16 @@ static int create_impl(const rbtp_case *c, rbtp_sim *s, hipStream_t stream) {
     s->k.c2t = 2.0 * tauc / 3.0;
     s->k.inv_tau = 1.0 / tau;
     s->k.inv_tauc = 1.0 / tauc;
+    auto dw = [](double d, double &h, double &l) {
+        h = 1.0 / d;
+        l = std::fma(-d, h, 1.0) / d;
+    };
+    dw(1.0 / 3.0, s->k.rt_h, s->k.rt_l);
+    dw(3.0, s->k.three_h, s->k.three_l);
+    dw(tau, s->k.tau_h, s->k.tau_l);
+    dw(tauc, s->k.tauc_h, s->k.tauc_l);
+    s->k.tls = (c->t_cold - c->t_hot) / (double)c->lx;
+    s->k.tl0 = c->t_hot + 0.5 * s->k.tls;
     s->prec = c->precision;
     s->strict = (c->precision == RBTP_PREC_FP64);
     HIP_TRY(hipGetDevice(&s->device));
@@ -367,7 +377,7 @@ extern "C" int rbtp_init(rbtp_sim *s, const double *temperature, const double *p
         using S = std::remove_pointer_t<decltype(tag)>;
         return launch_init<S>((S *)s->f[0], (S *)s->g[0], (typename Enc<S>::A *)s->fx,
                               (typename Enc<S>::A *)s->rho_wall, tdev, s->c.rho0, s->c.t_ref,
-                              s->d.grav0, s->c.beta, s->dims, s->macro[0], s->macro[1],
+                              s->d.grav0, s->c.beta, s->k.tl0, s->k.tls, s->dims, s->macro[0], s->macro[1],
                               s->macro[2], s->macro[3], s->macro[4], s->stream);
     }));
     if (s->particles) {
@@ -543,14 +553,15 @@ extern "C" int rbtp_get_populations(rbtp_sim *s, int which, double *out) {
         e = dispatch(s->prec, [&](auto *tag) {
             using S = std::remove_pointer_t<decltype(tag)>;
             const void *src = want_g ? s->g[s->cur] : s->f[s->cur];
-            return launch_decode<S>((const S *)src, dev, n, want_g ? 1 : 0, s->c.t_ref, s->stream);
+            return launch_decode<S>((const S *)src, dev, n, want_g ? 1 : 0, s->k.tl0, s->k.tls,
+                                    s->dims.lx, s->stream);
         });
     } else {
         e = dispatch(s->prec, [&](auto *tag) {
             using S = std::remove_pointer_t<decltype(tag)>;
             return launch_stream_only<S>((const S *)s->f[s->cur], (const S *)s->g[s->cur],
                                          (const typename Enc<S>::A *)s->rho_wall, dev,
-                                         dev + 27 * n, s->dims, s->c.t_ref, s->c.t_hot,
+                                         dev + 27 * n, s->dims, s->k.tl0, s->k.tls, s->c.t_hot,
                                          s->c.t_cold, false, s->stream);
         });
         if (e == hipSuccess && s->particles)
@@ -580,7 +591,7 @@ extern "C" int rbtp_set_populations(rbtp_sim *s, const double *f, const double *
         e = dispatch(s->prec, [&](auto *tag) {
             using S = std::remove_pointer_t<decltype(tag)>;
             return launch_encode<S>(dev, (S *)(which ? s->g[s->cur] : s->f[s->cur]), n, which,
-                                    s->c.t_ref, s->stream);
+                                    s->k.tl0, s->k.tls, s->dims.lx, s->stream);
         });
     }
     if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
@@ -689,6 +700,11 @@ extern "C" int rbtp_means(rbtp_sim *s, double out[4]) {
     return RBTP_OK;
 }
 
+extern "C" int rbtp_bandwidth_probe(size_t bytes, int reps, double *gbs) {
+    if (!gbs) return set_error(RBTP_E_ARG, "null argument");
+    return diag_copy_probe(bytes, reps, gbs);
+}
+
 extern "C" int rbtp_kinetic_energy(rbtp_sim *s, double *out) {
     if (!s || !out) return set_error(RBTP_E_ARG, "null argument");
     if (!s->macro_valid) return set_error(RBTP_E_STATE, "run the last step with RBTP_STEP_MACRO_LAST first");
@@ -710,6 +726,8 @@ static OutK out_consts(const rbtp_sim *s) {
     k.diff = s->d.diff;
     k.t0 = s->c.t_ref;
     k.rho0 = s->c.rho0;
+    k.tl0 = s->k.tl0;
+    k.tls = s->k.tls;
     return k;
 }
 
