@@ -19,6 +19,8 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "tests"))
 
@@ -39,7 +41,7 @@ def load_rbtp():
     return mod
 
 
-def run_form(rb, kw, steps, warmup, timing_steps, log):
+def run_form(rb, kw, steps, warmup, timing_steps, log, nu_window=0):
     case = rb.default_case(**kw)
     sim = rb.Sim(case)
     T = rb.linear_temperature(case, 2024, 1e-3)
@@ -54,6 +56,12 @@ def run_form(rb, kw, steps, warmup, timing_steps, log):
     dev_ms = sim.timing().step_ms
     nu = sim.nu()
     ke = sim.kinetic_energy()
+    # untimed continuation: Nu sampled every 100 steps for a window average
+    nu_samples = []
+    for _ in range(nu_window // 100):
+        sim.step(100, macro_last=True)
+        nu_samples.append(sim.nu())
+    nu_avg = np.mean(nu_samples, axis=0) if nu_samples else nu
     # kernel-only timing (events around every lattice launch)
     sim.set_kernel_timing(True)
     sim.step(timing_steps, macro_last=False)
@@ -62,7 +70,8 @@ def run_form(rb, kw, steps, warmup, timing_steps, log):
     log(f"form precision={kw.get('precision', 0)}: {steps} steps in {wall_ms:.1f} ms wall "
         f"({dev_ms:.1f} ms device), kernel avg {tm.kernel_ms:.4f} ms over {tm.launches}")
     return dict(n=n, wall_ms=wall_ms, dev_ms=dev_ms, kernel_ms=tm.kernel_ms,
-                kernel_launches=tm.launches, nu=nu.tolist(), ke=ke)
+                kernel_launches=tm.launches, nu=nu.tolist(), ke=ke, nu_avg=np.asarray(nu_avg).tolist(),
+                nu_window=nu_window)
 
 
 def cpu_baseline(kw, log, budget_s=20.0):
@@ -91,10 +100,35 @@ def cpu_baseline(kw, log, budget_s=20.0):
                        f"{kw['lx']}x{kw['ly']}x{kw['lz']} Ra=1e8 workload")
 
 
+def run_particles(rb, steps, warmup, log):
+    """Particle-laden production case (320x320x160, 300 spheres): MLUPS of the full step."""
+    from cases import config3, local_particle_temperature
+    kw = config3(precision=0)
+    case = rb.default_case(**kw)
+    sim = rb.Sim(case)
+    T = rb.linear_temperature(case, 2024, 1e-3)
+    sim.init(temperature=T)  # reference rejection placement (glibc rand, seed 86143)
+    ppos = np.concatenate([sim.particles("pos_x"), sim.particles("pos_y"), sim.particles("pos_z")])
+    sim.set_particles("temp", local_particle_temperature(T, ppos, case.lx, case.ly, case.lz))
+    sim.step(warmup, macro_last=False)
+    sim.synchronize()
+    t0 = time.perf_counter()
+    sim.step(steps, macro_last=True)
+    dt = time.perf_counter() - t0
+    n = sim.n
+    nlinks = len(sim.links())
+    nu = sim.nu()
+    sim.close()
+    mlups = n * steps / dt / 1e6
+    log(f"particle-laden: {sim.np} spheres, {nlinks} links, {steps} steps in {dt*1e3:.1f} ms "
+        f"-> {mlups:.0f} MLUPS")
+    return dict(mlups=mlups, ms_per_step=dt * 1e3 / steps, npart=int(sim.np), links=nlinks,
+                steps=steps, nu=nu.tolist(), workload="320x320x160 Ra=1e8 + 300 spheres d=10, FP64")
+
+
 def run_serving(rb, nreq, log):
     """Served batch: Poisson arrivals at 50% of the measured batch capacity."""
     from rbtp_amd.serve import make_requests, serve
-    import numpy as np
     # capacity: aggregate rate of a full batch of typical requests, then arrivals at half of it
     probe = make_requests(16, mean_gap=0.0, seed=99, max_steps=1000)
     cal = serve(probe, max_slots=64, chunk=100)
@@ -117,8 +151,11 @@ def main():
     ap.add_argument("--steps", type=int, default=3000)
     ap.add_argument("--warmup", type=int, default=300)
     ap.add_argument("--timing-steps", type=int, default=100)
+    ap.add_argument("--nu-window", type=int, default=1000,
+                    help="untimed steps after the timed run over which Nu is averaged")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-forms", action="store_true", help="skip the FP16-storage form")
+    ap.add_argument("--no-particles", action="store_true", help="skip the particle-laden case")
     ap.add_argument("--quiet", action="store_true")
     ap.add_argument("--counter-run", action="store_true",
                     help="minimal run for memory-counter passes: 3 FP64 lattice steps at "
@@ -150,21 +187,28 @@ def main():
         return
     copy_gbs = rb.bandwidth_probe(2 << 30, 20)
     log(f"copy probe: {copy_gbs:.0f} GB/s")
-    r64 = run_form(rb, kw64, args.steps, args.warmup, args.timing_steps, log)
+    r64 = run_form(rb, kw64, args.steps, args.warmup, args.timing_steps, log, args.nu_window)
     n = r64["n"]
     mlups = n * args.steps / (r64["wall_ms"] * 1e-3) / 1e6
     achieved = BYTES_PER_NODE[0] * n / (r64["kernel_ms"] * 1e-3) / 1e9
     forms = {}
     if not args.no_forms:
-        r16 = run_form(rb, config2(precision=2), args.steps, args.warmup, args.timing_steps, log)
+        r16 = run_form(rb, config2(precision=2), args.steps, args.warmup, args.timing_steps, log,
+                       args.nu_window)
         m16 = n * args.steps / (r16["wall_ms"] * 1e-3) / 1e6
         forms["fp16s"] = dict(
             mlups=m16, ratio_vs_fp64=m16 / mlups, kernel_ms=r16["kernel_ms"],
             kernel_gbs=BYTES_PER_NODE[2] * n / (r16["kernel_ms"] * 1e-3) / 1e9,
             nu_bulk=r16["nu"][2], nu_hot=r16["nu"][4],
             nu_hot_rel_err_vs_fp64=abs(r16["nu"][4] - r64["nu"][4]) / abs(r64["nu"][4]),
+            nu_hot_window_avg=r16["nu_avg"][4], nu_hot_window_avg_fp64=r64["nu_avg"][4],
+            nu_hot_window_rel_err_vs_fp64=abs(r16["nu_avg"][4] - r64["nu_avg"][4]) / abs(r64["nu_avg"][4]),
+            nu_window_steps=args.nu_window,
             ke_rel_err_vs_fp64=abs(r16["ke"] - r64["ke"]) / abs(r64["ke"]),
             bytes_per_node=BYTES_PER_NODE[2])
+    if not args.no_particles:
+        forms["fp64_particle_laden"] = run_particles(rb, max(args.steps // 3, 10),
+                                                     max(args.warmup // 3, 5), log)
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "lattice_fp64_320x320x160_traffic.json")
     if os.path.exists(tpath):

@@ -9,10 +9,10 @@
 // unspecified (FP64 atomicAdd into per-particle accumulators: ibb.cu:182-187,204 and
 // pforces.cu:32-37) the oracle fixes one deterministic order and documents it at the function:
 //   * pair repulsion: partners j in ascending order, then the six walls, then the link sum;
-//   * link sums (force, torque, heat): per particle, links are dealt round-robin to 256 partial
-//     sums by their node's position k in the particle's link box (k mod 256, increasing k, then
-//     increasing direction), and the 256 partials are combined by a pairwise tree
-//     (stride 128, 64, ..., 1).  The link box is the particle's marking box
+//   * link sums (force, torque, heat): per particle, links are dealt round-robin to 1024 partial
+//     sums by their node's position k in the particle's link box (k mod 1024, increasing k, then
+//     increasing direction), and the 1024 partials are combined by a pairwise tree
+//     (stride 512, 256, ..., 1).  The link box is the particle's marking box
 //     floor(p -/+ R - 0.5) (links.cu:164-177) grown by one node per side, clamped to the domain.
 //
 // Parity status of each function (how it is pinned independently of this restatement) is listed
@@ -47,7 +47,7 @@ double weight(int i) {
     return 1.0 / 216.0;
 }
 
-constexpr int RED_LANES = 256;  // partial sums per particle for the link reductions
+constexpr int RED_LANES = 1024;  // partial sums per particle for the link reductions
 
 }  // namespace
 

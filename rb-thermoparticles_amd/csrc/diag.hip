@@ -221,11 +221,24 @@ __global__ __launch_bounds__(DB) void k_profile(const double *fields, Dims d, do
 
 
 // Streaming copy of 16-byte vectors: the reference kernel for the achievable HBM bandwidth and
-// for calibrating the memory counters (it reads and writes exactly `count` vectors).
+// for calibrating the memory counters (it reads and writes exactly `count` vectors).  Each
+// thread moves CP_UNROLL vectors 256 apart, all loads issued before the stores; one workgroup
+// per 256*CP_UNROLL vectors.
+constexpr int CP_UNROLL = 4;
 __global__ __launch_bounds__(256) void k_copy_probe(const double2 *__restrict__ src,
                                                     double2 *__restrict__ dst, size_t count) {
-    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < count; i += (size_t)gridDim.x * 256)
-        dst[i] = src[i];
+    const size_t base = (size_t)blockIdx.x * (256 * CP_UNROLL) + threadIdx.x;
+    double2 v[CP_UNROLL];
+#pragma unroll
+    for (int u = 0; u < CP_UNROLL; u++) {
+        const size_t i = base + (size_t)u * 256;
+        if (i < count) v[u] = src[i];
+    }
+#pragma unroll
+    for (int u = 0; u < CP_UNROLL; u++) {
+        const size_t i = base + (size_t)u * 256;
+        if (i < count) dst[i] = v[u];
+    }
 }
 
 constexpr int RED_BLOCKS = 1024;
@@ -380,10 +393,9 @@ int diag_copy_probe(size_t bytes, int reps, double *gbs) {
     if (e == hipSuccess) e = hipEventCreate(&e0);
     if (e == hipSuccess) e = hipEventCreate(&e1);
     if (e == hipSuccess) e = hipMemsetAsync(a, 0, count * sizeof(double2), st);
-    int dev = 0, cus = 256;
-    if (e == hipSuccess) e = hipGetDevice(&dev);
-    if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-    const int grid = cus * 16;
+    const size_t grid_sz = (count + 256 * CP_UNROLL - 1) / (256 * CP_UNROLL);
+    const unsigned grid = (unsigned)grid_sz;
+    if (e == hipSuccess && (grid_sz == 0 || grid_sz > 0x7fffffffu)) e = hipErrorInvalidValue;
     if (e == hipSuccess) {
         hipLaunchKernelGGL(k_copy_probe, grid, 256, 0, st, a, b, count);  // warm-up
         e = hipGetLastError();

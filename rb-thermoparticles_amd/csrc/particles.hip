@@ -34,7 +34,9 @@ namespace rbtp {
 
 namespace {
 
-constexpr int PB = 256;  // threads per particle workgroup (also the reduction width)
+constexpr int PB = 256;   // threads per particle workgroup of the box clear/mark/collect kernels
+constexpr int LB = 1024;  // threads per particle workgroup of the link sums (also their reduction
+                          // width) and of the special-node search
 
 __constant__ int c_cx[NPOP] = {0, 1, 0, -1, 0, 0, 0, 1, -1, -1, 1, 1, 0, -1,
                                0, 1, 0, -1, 0, 1, -1, -1, 1, 1, -1, -1, 1};
@@ -333,15 +335,15 @@ __device__ inline LinkOut eval_link(int ip, int ix, int iy, int iz, double q, in
 
 // Hydrodynamic force, torque and heat of every particle (compute_particle_forces_kernel,
 // ibb.cu:138-188; compute_particle_heat_kernel, ibb.cu:190-205).  Lane t of particle p's
-// workgroup takes the nodes k = t, t+256, ... of p's link box, each node's links in direction
-// order; the 256 partials are combined pairwise (stride 128 ... 1) and added to the sums left
+// workgroup takes the nodes k = t, t+1024, ... of p's link box, each node's links in direction
+// order; the 1024 partials are combined pairwise (stride 512 ... 1) and added to the sums left
 // by k_contacts.
 template <class S>
-__global__ __launch_bounds__(PB) void k_link_sums(PartDev pd, PopView<S> pop,
+__global__ __launch_bounds__(LB) void k_link_sums(PartDev pd, PopView<S> pop,
                                                   const int32_t *owner_prev,
                                                   const int32_t *box_prev, const double *rho,
                                                   Dims d) {
-    __shared__ double red[7][PB];
+    __shared__ double red[7][LB];
     const int p = blockIdx.x;
     const int t = threadIdx.x;
     double s[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
@@ -355,7 +357,7 @@ __global__ __launch_bounds__(PB) void k_link_sums(PartDev pd, PopView<S> pop,
                        pd.p[RBTP_P_TEMP]};
         const double X = pv.px[p], Y = pv.py[p], Z = pv.pz[p];
         const int cnt = box_count(lo, hi);
-        for (int k = t; k < cnt; k += PB) {
+        for (int k = t; k < cnt; k += LB) {
             int ix, iy, iz;
             const int idx = box_node(d, lo, hi, k, ix, iy, iz);
             if (owner_prev[idx] != -1) continue;
@@ -397,7 +399,7 @@ __global__ __launch_bounds__(PB) void k_link_sums(PartDev pd, PopView<S> pop,
     if (nunh_g) atomicAdd(pd.err + ERR_UNHANDLED_G, nunh_g);
     for (int c = 0; c < 7; c++) red[c][t] = s[c];
     __syncthreads();
-    for (int stride = PB / 2; stride >= 1; stride >>= 1) {
+    for (int stride = LB / 2; stride >= 1; stride >>= 1) {
         if (t < stride)
             for (int c = 0; c < 7; c++) red[c][t] = red[c][t] + red[c][t + stride];
         __syncthreads();
@@ -485,7 +487,7 @@ __device__ inline void push_special(const PartDev &pd, int idx, int kind) {
 
 // Link nodes of the previous geometry (the nodes bounce_back_particles writes).  A node with
 // links to several particles is listed once, by the lowest linked pid.
-__global__ __launch_bounds__(PB) void k_special_links(PartDev pd, const int32_t *owner_prev,
+__global__ __launch_bounds__(LB) void k_special_links(PartDev pd, const int32_t *owner_prev,
                                                       const int32_t *box_prev, Dims d) {
     const int p = blockIdx.x;
     int lo[3], hi[3];
@@ -493,7 +495,7 @@ __global__ __launch_bounds__(PB) void k_special_links(PartDev pd, const int32_t 
     grow_box(d, lo, hi);
     const double *OX = pd.old[OLD_PX], *OY = pd.old[OLD_PY], *OZ = pd.old[OLD_PZ];
     const int cnt = box_count(lo, hi);
-    for (int k = threadIdx.x; k < cnt; k += PB) {
+    for (int k = threadIdx.x; k < cnt; k += LB) {
         int ix, iy, iz;
         const int idx = box_node(d, lo, hi, k, ix, iy, iz);
         if (owner_prev[idx] != -1) continue;
@@ -516,7 +518,7 @@ __global__ __launch_bounds__(PB) void k_special_links(PartDev pd, const int32_t 
 
 // New-fluid nodes (new_fluid_count_kernel, refill.cu:8-41): solid in the previous geometry,
 // fluid now, previous owner valid, and within close_thresh of that owner's new surface.
-__global__ __launch_bounds__(PB) void k_special_refill(PartDev pd, const int32_t *owner_prev,
+__global__ __launch_bounds__(LB) void k_special_refill(PartDev pd, const int32_t *owner_prev,
                                                        const int32_t *box_prev,
                                                        const int32_t *owner_curr, Dims d) {
     const int p = blockIdx.x;
@@ -524,7 +526,7 @@ __global__ __launch_bounds__(PB) void k_special_refill(PartDev pd, const int32_t
     if (!load_box(box_prev, p, lo, hi)) return;
     const double PXv = pd.p[RBTP_P_POS_X][p], PYv = pd.p[RBTP_P_POS_Y][p], PZv = pd.p[RBTP_P_POS_Z][p];
     const int cnt = box_count(lo, hi);
-    for (int k = threadIdx.x; k < cnt; k += PB) {
+    for (int k = threadIdx.x; k < cnt; k += LB) {
         int ix, iy, iz;
         const int idx = box_node(d, lo, hi, k, ix, iy, iz);
         if (owner_prev[idx] != p || owner_curr[idx] != -1) continue;
@@ -538,13 +540,44 @@ __global__ __launch_bounds__(PB) void k_special_refill(PartDev pd, const int32_t
     }
 }
 
-// Corrected post-stream populations of the special nodes.
-//   link node: pull from the post-collision state (streaming rules), then replace f_opp and
-//              g_opp of every link by the IBB values (bounce_back_particles /
-//              bounce_back_thermal_particles, with the state before this step's motion);
-//   new fluid: refill_new_fluid_kernel (refill.cu:78-240) from the post-collision state at
-//              neighbours that were fluid in the previous geometry, with the moved state for
-//              the fallback equilibrium; it also writes rho, u, phi and F = 0 at the node.
+// Post-stream value of population j at node (ix,iy,iz) under the wall rules of streaming and
+// streaming_scalar (the single-direction form of gather()).
+template <class S>
+__device__ inline void pull_one(const PopView<S> &pop, int j, int idx, int ix, int iy, int iz,
+                                const Dims &d, double rho_w, double thot, double tcold,
+                                double &fv, double &gv) {
+    const int cx = c_cx[j], cy = c_cy[j], cz = c_cz[j];
+    const bool outx = (cx == 1 && ix == 0) || (cx == -1 && ix == d.lx - 1);
+    const bool outy = (cy == 1 && iy == 0) || (cy == -1 && iy == d.ly - 1);
+    const bool outz = (cz == 1 && iz == 0) || (cz == -1 && iz == d.lz - 1);
+    if (!(outx || outy || outz)) {
+        const int src = idx - (cx + cy * d.lx + cz * d.lxy);
+        fv = pop.F(j, src);
+        gv = pop.G(j, src);
+        return;
+    }
+    const int op = c_opp[j];
+    fv = pop.F(op, idx);
+    const double g_opp = pop.G(op, idx);
+    if (outz || outy) {
+        gv = g_opp;
+    } else {
+        const double twall = (cx == 1) ? thot : tcold;
+        gv = -g_opp + 2.0 * wt(j) * twall * (1.0 + rho_w);
+    }
+}
+
+// Corrected post-stream populations of the special nodes, one 32-lane group per node and one
+// lane per population j.
+//   link node: lane j pulls population j (streaming rules); when the pull source x - c_j is a
+//              solid node linked to x (the link x -> x + c_i with i = opp(j)), the value is the
+//              bounce-back of that link instead (bounce_back_particles /
+//              bounce_back_thermal_particles, ibb.cu:5-136, with the state before this step's
+//              motion);
+//   new fluid: lane j is population j of refill_new_fluid_kernel (refill.cu:121-215) from the
+//              post-collision state at neighbours that were fluid in the previous geometry, with
+//              the moved state for the fallback equilibrium; lane 0 then forms rho, u, phi
+//              (refill.cu:217-235) in population order and sets F = 0.
 template <class S>
 __global__ __launch_bounds__(128) void k_prefixup(PartDev pd, PopView<S> pop,
                                                   const int32_t *owner_prev,
@@ -553,56 +586,50 @@ __global__ __launch_bounds__(128) void k_prefixup(PartDev pd, PopView<S> pop,
                                                   double *macro_rho, double *macro_ux,
                                                   double *macro_uy, double *macro_uz,
                                                   double *macro_phi, Dims d) {
+    __shared__ double sfv[4][32], sgv[4][32];
+    const int grp = threadIdx.x >> 5, lane = threadIdx.x & 31;
     const int count = min(*pd.sp.count, pd.sp.cap);
     const size_t cap = (size_t)pd.sp.cap;
-    for (int j = blockIdx.x * 128 + threadIdx.x; j < count; j += gridDim.x * 128) {
-        const int idx = pd.sp.node[j];
-        const int kind = pd.sp_kind[j];
-        const int iz = idx / d.lxy;
-        const int rem = idx - iz * d.lxy;
-        const int iy = rem / d.lx;
-        const int ix = rem - iy * d.lx;
-        double *out = pd.sp.pop;
-        if (kind < 0) {
-            double rho_w = 0.0;
-            if (ix == 0) rho_w = double(rho_wall[iz * d.ly + iy]);
-            else if (ix == d.lx - 1) rho_w = double(rho_wall[d.ly * d.lz + iz * d.ly + iy]);
-            double f[NPOP], g[NPOP];
-            gather<S, double, false>(pop.f, pop.g, pop.n, idx, ix, iy, iz, d, rho_w, pop.tl0,
-                                     pop.tls, pd.thot, pd.tcold, f, g);
-            const PView pv{pd.old[OLD_PX], pd.old[OLD_PY], pd.old[OLD_PZ], pd.old[OLD_VX],
-                           pd.old[OLD_VY], pd.old[OLD_VZ], pd.old[OLD_WX], pd.old[OLD_WY],
-                           pd.old[OLD_WZ], pd.old[OLD_T]};
-            const double x0 = static_cast<double>(ix) + 0.5;
-            const double y0 = static_cast<double>(iy) + 0.5;
-            const double z0 = static_cast<double>(iz) + 0.5;
-            const double density = macro_rho[idx];
-            for (int ip = 1; ip < NPOP; ip++) {
+    double *out = pd.sp.pop;
+    for (int base = blockIdx.x * 4; base < count; base += gridDim.x * 4) {
+        const int j = base + grp;
+        const bool live = j < count;
+        int idx = 0, kind = -1, ix = 0, iy = 0, iz = 0;
+        if (live) {
+            idx = pd.sp.node[j];
+            kind = pd.sp_kind[j];
+            iz = idx / d.lxy;
+            const int rem = idx - iz * d.lxy;
+            iy = rem / d.lx;
+            ix = rem - iy * d.lx;
+        }
+        double fv = 0.0, gv = 0.0;
+        if (live && lane < NPOP) {
+            if (kind < 0) {
+                double rho_w = 0.0;
+                if (ix == 0) rho_w = double(rho_wall[iz * d.ly + iy]);
+                else if (ix == d.lx - 1) rho_w = double(rho_wall[d.ly * d.lz + iz * d.ly + iy]);
+                pull_one<S>(pop, lane, idx, ix, iy, iz, d, rho_w, pd.thot, pd.tcold, fv, gv);
+                const int ip = c_opp[lane];
                 const int nx = ix + c_cx[ip], ny = iy + c_cy[ip], nz = iz + c_cz[ip];
-                if (!in_domain(d, nx, ny, nz)) continue;
-                const int q_owner = owner_prev[nz * d.lxy + ny * d.lx + nx];
-                if (q_owner < 0) continue;
-                double q = 0.0;
-                if (!link_q(ip, x0, y0, z0, pv.px[q_owner], pv.py[q_owner], pv.pz[q_owner],
-                            pd.radius_sq, q))
-                    continue;
-                const LinkOut L = eval_link<S>(ip, ix, iy, iz, q, q_owner, pv, pop, owner_prev,
-                                               density, pd.rho0, d);
-                const int opp = c_opp[ip];
-                f[opp] = L.f_out;
-                g[opp] = L.g_out;
-            }
-            for (int i = 0; i < NPOP; i++) {
-                out[(size_t)i * cap + j] = f[i];
-                out[(size_t)(NPOP + i) * cap + j] = g[i];
-            }
-            pd.sp.fx_old[j] = double(fx_field[idx]);
-        } else {
-            const int pid = kind;
-            const double lambda0 = 1.0 - wt(0);
-            const double RT = 1.0 / 3.0;
-            double rho_loc = 0.0, mom_x = 0.0, mom_y = 0.0, mom_z = 0.0, phi_loc = 0.0;
-            for (int ip = 0; ip < NPOP; ++ip) {
+                if (lane != 0 && in_domain(d, nx, ny, nz)) {
+                    const int q_owner = owner_prev[nz * d.lxy + ny * d.lx + nx];
+                    double q = 0.0;
+                    if (q_owner >= 0 &&
+                        link_q(ip, static_cast<double>(ix) + 0.5, static_cast<double>(iy) + 0.5,
+                               static_cast<double>(iz) + 0.5, pd.old[OLD_PX][q_owner],
+                               pd.old[OLD_PY][q_owner], pd.old[OLD_PZ][q_owner], pd.radius_sq, q)) {
+                        const PView pv{pd.old[OLD_PX], pd.old[OLD_PY], pd.old[OLD_PZ], pd.old[OLD_VX],
+                                       pd.old[OLD_VY], pd.old[OLD_VZ], pd.old[OLD_WX], pd.old[OLD_WY],
+                                       pd.old[OLD_WZ], pd.old[OLD_T]};
+                        const LinkOut L = eval_link<S>(ip, ix, iy, iz, q, q_owner, pv, pop,
+                                                       owner_prev, macro_rho[idx], pd.rho0, d);
+                        fv = L.f_out;
+                        gv = L.g_out;
+                    }
+                }
+            } else {
+                const int ipop = lane, pid = kind;
                 double temp_f = 0.0, temp_g = 0.0;
                 int sample_count = 0;
                 for (int ip2 = 1; ip2 < NPOP; ++ip2) {
@@ -614,17 +641,19 @@ __global__ __launch_bounds__(128) void k_prefixup(PartDev pd, PopView<S> pop,
                     if (in_domain(d, ix2, iy2, iz2)) {
                         const int idx2 = iz2 * d.lxy + iy2 * d.lx + ix2;
                         if (owner_prev[idx2] == -1) {
-                            temp_f += 2.0 * pop.F(ip, idx1) - pop.F(ip, idx2);
-                            temp_g += 2.0 * pop.G(ip, idx1) - pop.G(ip, idx2);
+                            temp_f += 2.0 * pop.F(ipop, idx1) - pop.F(ipop, idx2);
+                            temp_g += 2.0 * pop.G(ipop, idx1) - pop.G(ipop, idx2);
                             ++sample_count;
                             continue;
                         }
                     }
-                    temp_f += pop.F(ip, idx1);
-                    temp_g += pop.G(ip, idx1);
+                    temp_f += pop.F(ipop, idx1);
+                    temp_g += pop.G(ipop, idx1);
                     ++sample_count;
                 }
                 if (sample_count == 0) {
+                    const double lambda0 = 1.0 - wt(0);
+                    const double RT = 1.0 / 3.0;
                     const double px = pd.p[RBTP_P_POS_X][pid], py = pd.p[RBTP_P_POS_Y][pid], pz = pd.p[RBTP_P_POS_Z][pid];
                     const double vx = pd.p[RBTP_P_VEL_X][pid], vy = pd.p[RBTP_P_VEL_Y][pid], vz = pd.p[RBTP_P_VEL_Z][pid];
                     const double omega_x = pd.p[RBTP_P_OMEGA_X][pid], omega_y = pd.p[RBTP_P_OMEGA_Y][pid], omega_z = pd.p[RBTP_P_OMEGA_Z][pid];
@@ -637,40 +666,54 @@ __global__ __launch_bounds__(128) void k_prefixup(PartDev pd, PopView<S> pop,
                     const double v = vy + (relx * omega_z - relz * omega_x);
                     const double w = vz + (rely * omega_x - relx * omega_y);
                     const double r = pd.rho0;
-                    const double eu = c_cx[ip] * u + c_cy[ip] * v + c_cz[ip] * w;
+                    const double eu = c_cx[ipop] * u + c_cy[ipop] * v + c_cz[ipop] * w;
                     const double uv = (u * u + v * v + w * w) / RT;
-                    temp_f = wt(ip) * (r + eu + 0.5 * (eu * eu - uv));
-                    temp_g = wt(ip) * tempT * (1.0 + eu + 0.5 * (eu * eu - uv));
-                    if (ip == 0) temp_g -= lambda0 * tempT * r;
-                    else temp_g += wt(ip) * tempT * r;
+                    temp_f = wt(ipop) * (r + eu + 0.5 * (eu * eu - uv));
+                    temp_g = wt(ipop) * tempT * (1.0 + eu + 0.5 * (eu * eu - uv));
+                    if (ipop == 0) temp_g -= lambda0 * tempT * r;
+                    else temp_g += wt(ipop) * tempT * r;
                     sample_count = 1;
                 }
                 const double inv_count = 1.0 / static_cast<double>(sample_count);
-                const double f_val = temp_f * inv_count;
-                const double g_val = temp_g * inv_count;
-                out[(size_t)ip * cap + j] = f_val;
-                out[(size_t)(NPOP + ip) * cap + j] = g_val;
-                rho_loc += f_val;
-                mom_x += c_cx[ip] * f_val;
-                mom_y += c_cy[ip] * f_val;
-                mom_z += c_cz[ip] * f_val;
-                phi_loc += g_val;
+                fv = temp_f * inv_count;
+                gv = temp_g * inv_count;
             }
-            macro_rho[idx] = rho_loc;
-            if (rho_loc > 1e-12) {
-                const double inv_rho = 1.0 / rho_loc;
-                macro_ux[idx] = mom_x * inv_rho;
-                macro_uy[idx] = mom_y * inv_rho;
-                macro_uz[idx] = mom_z * inv_rho;
-            } else {
-                macro_ux[idx] = 0.0;
-                macro_uy[idx] = 0.0;
-                macro_uz[idx] = 0.0;
-            }
-            macro_phi[idx] = phi_loc;
-            fx_field[idx] = typename Enc<S>::A(0.0);
-            pd.sp.fx_old[j] = 0.0;
+            out[(size_t)lane * cap + j] = fv;
+            out[(size_t)(NPOP + lane) * cap + j] = gv;
         }
+        sfv[grp][lane] = fv;
+        sgv[grp][lane] = gv;
+        __syncthreads();
+        if (live && lane == 0) {
+            if (kind < 0) {
+                pd.sp.fx_old[j] = double(fx_field[idx]);
+            } else {
+                double rho_loc = 0.0, mom_x = 0.0, mom_y = 0.0, mom_z = 0.0, phi_loc = 0.0;
+                for (int ip = 0; ip < NPOP; ip++) {
+                    const double f_val = sfv[grp][ip];
+                    rho_loc += f_val;
+                    mom_x += c_cx[ip] * f_val;
+                    mom_y += c_cy[ip] * f_val;
+                    mom_z += c_cz[ip] * f_val;
+                    phi_loc += sgv[grp][ip];
+                }
+                macro_rho[idx] = rho_loc;
+                if (rho_loc > 1e-12) {
+                    const double inv_rho = 1.0 / rho_loc;
+                    macro_ux[idx] = mom_x * inv_rho;
+                    macro_uy[idx] = mom_y * inv_rho;
+                    macro_uz[idx] = mom_z * inv_rho;
+                } else {
+                    macro_ux[idx] = 0.0;
+                    macro_uy[idx] = 0.0;
+                    macro_uz[idx] = 0.0;
+                }
+                macro_phi[idx] = phi_loc;
+                fx_field[idx] = typename Enc<S>::A(0.0);
+                pd.sp.fx_old[j] = 0.0;
+            }
+        }
+        __syncthreads();
     }
 }
 
@@ -916,7 +959,7 @@ int part_phase(PartDev &pd, const rbtp_case &c, const rbtp_derived &d, const Dim
     PHIP(dispatch_prec(prec, [&](auto *tag) {
         using S = std::remove_pointer_t<decltype(tag)>;
         const PopView<S> pop{(const S *)fcur, (const S *)gcur, (size_t)dims.n, pd.tl0, pd.tls, dims.lx};
-        hipLaunchKernelGGL((k_link_sums<S>), np, PB, 0, st, pd, pop, pd.owner[prev],
+        hipLaunchKernelGGL((k_link_sums<S>), np, LB, 0, st, pd, pop, pd.owner[prev],
                            pd.box[prev], macro[0], dims);
         return hipGetLastError();
     }));
@@ -929,16 +972,16 @@ int part_phase(PartDev &pd, const rbtp_case &c, const rbtp_derived &d, const Dim
     PHIP(hipGetLastError());
     hipLaunchKernelGGL(k_zero_count, 1, 1, 0, st, pd.sp.count);
     PHIP(hipGetLastError());
-    hipLaunchKernelGGL(k_special_links, np, PB, 0, st, pd, pd.owner[prev], pd.box[prev], dims);
+    hipLaunchKernelGGL(k_special_links, np, LB, 0, st, pd, pd.owner[prev], pd.box[prev], dims);
     PHIP(hipGetLastError());
-    hipLaunchKernelGGL(k_special_refill, np, PB, 0, st, pd, pd.owner[prev], pd.box[prev],
+    hipLaunchKernelGGL(k_special_refill, np, LB, 0, st, pd, pd.owner[prev], pd.box[prev],
                        pd.owner[curr], dims);
     PHIP(hipGetLastError());
     PHIP(dispatch_prec(prec, [&](auto *tag) {
         using S = std::remove_pointer_t<decltype(tag)>;
         using A = typename Enc<S>::A;
         const PopView<S> pop{(const S *)fcur, (const S *)gcur, (size_t)dims.n, pd.tl0, pd.tls, dims.lx};
-        hipLaunchKernelGGL((k_prefixup<S>), grid_for(pd.sp.cap, 128, 2048), 128, 0, st, pd, pop,
+        hipLaunchKernelGGL((k_prefixup<S>), grid_for(pd.sp.cap, 4, 16384), 128, 0, st, pd, pop,
                            pd.owner[prev], (A *)fx, (const A *)rho_wall, macro[0], macro[1],
                            macro[2], macro[3], macro[4], dims);
         return hipGetLastError();

@@ -39,3 +39,26 @@ def particle_case(lx, ly, lz, npart, diam, ra=1e5, pr=0.71, beta=None, **kw):
     if beta is not None:
         d["beta"] = beta
     return d
+
+
+def config3(precision=0):
+    """320x320x160, Ra=1e8, Pr=0.71, 300 spheres of diameter 10 (PARTICLE_DIAM_FRAC = 1/32,
+    fraction 0.0096 -> floor(N*0.0096/V) = 300), density ratio 2, heat-capacity ratio 1.
+    BETA is raised to 0.625 so that g0 = 1e-4 (fluid buoyancy g0*BETA is unchanged) and the
+    spheres' buoyancy (1 - 1/rho_r) g0 stays a small acceleration."""
+    d = single_phase(320, 320, 160, ra=1e8, visc=visc_for(320, 1e8, 0.71), precision=precision,
+                     beta=0.625)
+    d.update(activate_particles=1, particle_diam_frac=1.0 / 32.0, particle_fraction=0.0096,
+             rho_particle_ratio=2.0, cp_particle_ratio=1.0, npart_override=-1)
+    return d
+
+
+def local_particle_temperature(temperature, ppos, lx, ly, lz):
+    """Initial sphere temperature = fluid temperature at the cell holding the centre."""
+    import numpy as np
+    np_ = len(ppos) // 3
+    x, y, z = ppos[:np_], ppos[np_:2 * np_], ppos[2 * np_:]
+    ix = np.clip(np.floor(x).astype(int), 0, lx - 1)
+    iy = np.clip(np.floor(y).astype(int), 0, ly - 1)
+    iz = np.clip(np.floor(z).astype(int), 0, lz - 1)
+    return temperature[(iz * ly + iy) * lx + ix]
