@@ -32,6 +32,13 @@ struct PartDev {
     bool special_pending;    // sp holds corrected populations for the next lattice step
     int32_t *err;            // ERR_COUNT device flags
     int box_w;               // upper bound of marking-box nodes per axis
+    // Uniform cell grid of the pair search: cells at least as wide as the pair range
+    // 2R + REP_MIN_GAP, each holding up to CELL_CAP particle ids.
+    int ncell[3];
+    double cell_scale[3];    // cells per lattice unit along each axis
+    int32_t *cell_cnt;       // [ncells] particles binned into each cell (may exceed CELL_CAP)
+    int32_t *cell_pid;       // [ncells][CELL_CAP]
+    bool use_cells;          // pair search through the cell grid (else every pair)
 };
 
 int report(int code, const char *msg);

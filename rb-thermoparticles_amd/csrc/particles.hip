@@ -35,6 +35,9 @@ namespace rbtp {
 namespace {
 
 constexpr int PB = 256;   // threads per particle workgroup of the box clear/mark/collect kernels
+constexpr int CELL_CAP = 16;  // particle ids per cell of the pair-search grid
+constexpr int CAND_MAX = 48;  // pair candidates per particle held in LDS
+constexpr int CELLS_MIN_NP = 32;  // particle count from which the pair search uses the grid
 constexpr int LB = 1024;  // threads per particle workgroup of the link sums (also their reduction
                           // width) and of the special-node search
 
@@ -169,11 +172,62 @@ __global__ __launch_bounds__(PB) void k_geo_mark(int32_t *owner, int32_t *box, c
 
 // ---- contacts and force history ----------------------------------------------------------------
 
+// ---- pair search grid -----------------------------------------------------------------------
+
+__device__ inline int cell_coord(double x, double scale, int n) {
+    const double t = fmin(fmax(x * scale, 0.0), static_cast<double>(n - 1));  // NaN -> 0
+    return static_cast<int>(t);
+}
+
+__global__ __launch_bounds__(256) void k_cell_clear(int32_t *cnt, int ncells) {
+    for (int c = blockIdx.x * 256 + threadIdx.x; c < ncells; c += gridDim.x * 256) cnt[c] = 0;
+}
+
+// Bins every particle by its centre.  The order inside a cell is arbitrary: the pair sum sorts
+// its candidates by pid.
+__global__ __launch_bounds__(256) void k_cell_bin(PartDev pd) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= pd.np) return;
+    const int cx = cell_coord(pd.p[RBTP_P_POS_X][i], pd.cell_scale[0], pd.ncell[0]);
+    const int cy = cell_coord(pd.p[RBTP_P_POS_Y][i], pd.cell_scale[1], pd.ncell[1]);
+    const int cz = cell_coord(pd.p[RBTP_P_POS_Z][i], pd.cell_scale[2], pd.ncell[2]);
+    const int c = (cz * pd.ncell[1] + cy) * pd.ncell[0] + cx;
+    const int slot = atomicAdd(pd.cell_cnt + c, 1);
+    if (slot < CELL_CAP) pd.cell_pid[(size_t)c * CELL_CAP + slot] = i;
+}
+
+// Lubrication/overlap force of partner j on particle i (repulsive_pair_kernel, pforces.cu:6-38);
+// false where the reference skips the pair.
+__device__ inline bool pair_term(const PartDev &pd, const double *X, const double *Y,
+                                 const double *Z, double xi, double yi, double zi, int j,
+                                 double &fx, double &fy, double &fz) {
+    const double dx = xi - X[j], dy = yi - Y[j], dz = zi - Z[j];
+    const double dist_sq = dx * dx + dy * dy + dz * dz;
+    if (dist_sq <= 1.0e-12) return false;
+    const double dist = sqrt(dist_sq);
+    const double min_gap = pd.min_gap;
+    const double hgap = dist - 2.0 * pd.radius;
+    if (hgap >= min_gap) return false;
+    const double overlap = (hgap < 0.0) ? -hgap : 0.0;
+    const double factor = (min_gap - hgap) / min_gap;
+    const double lubforce = pd.coeff * (factor * factor / pd.stf0 + overlap / (min_gap * pd.stf1));
+    fx = lubforce * (dx / dist);
+    fy = lubforce * (dy / dist);
+    fz = lubforce * (dz / dist);
+    return true;
+}
+
 // prepare_particle_forces_step (pforces.cu:80-116), the heat history of compute_particle_heat
 // (ibb.cu:260-261), and apply_repulsive_forces (pforces.cu:118-148).  Pair terms are summed per
 // particle in ascending partner order; the term on the higher pid of a pair is evaluated from its
 // own side, which is bit-for-bit the negation the reference applies (pforces.cu:35-37).
+// With the cell grid, the partners are the particles binned in the 27 cells around i's cell,
+// sorted by pid; every pair within range is among them, and the pairs left out are ones the
+// reference skips, so the sum is the same.  When a cell overflowed or i has more than
+// CAND_MAX candidates, i falls back to every partner.
 __global__ __launch_bounds__(128) void k_contacts(PartDev pd, Dims d) {
+    __shared__ int32_t cand_all[128][CAND_MAX];
+    int32_t *cand = cand_all[threadIdx.x];
     const int i = blockIdx.x * 128 + threadIdx.x;
     if (i >= pd.np) return;
     double **P = pd.p;
@@ -189,23 +243,45 @@ __global__ __launch_bounds__(128) void k_contacts(PartDev pd, Dims d) {
     const double *X = P[RBTP_P_POS_X], *Y = P[RBTP_P_POS_Y], *Z = P[RBTP_P_POS_Z];
     const double xi = X[i], yi = Y[i], zi = Z[i];
     double FX = 0.0, FY = 0.0, FZ = 0.0;
-    const double radius = pd.radius, coeff = pd.coeff, min_gap = pd.min_gap;
+    const double radius = pd.radius, coeff = pd.coeff;
     if (pd.np > 1) {
-        for (int j = 0; j < pd.np; j++) {
+        int ncand = -1;  // -1: every partner
+        if (pd.use_cells) {
+            const int cx = cell_coord(xi, pd.cell_scale[0], pd.ncell[0]);
+            const int cy = cell_coord(yi, pd.cell_scale[1], pd.ncell[1]);
+            const int cz = cell_coord(zi, pd.cell_scale[2], pd.ncell[2]);
+            ncand = 0;
+            for (int z = max(cz - 1, 0); z <= min(cz + 1, pd.ncell[2] - 1) && ncand >= 0; z++)
+                for (int y = max(cy - 1, 0); y <= min(cy + 1, pd.ncell[1] - 1) && ncand >= 0; y++)
+                    for (int x = max(cx - 1, 0); x <= min(cx + 1, pd.ncell[0] - 1) && ncand >= 0; x++) {
+                        const int c = (z * pd.ncell[1] + y) * pd.ncell[0] + x;
+                        const int m = pd.cell_cnt[c];
+                        if (m > CELL_CAP || ncand + m > CAND_MAX) {
+                            ncand = -1;
+                            break;
+                        }
+                        for (int k = 0; k < m; k++) {
+                            // insertion in ascending pid order
+                            const int32_t v = pd.cell_pid[(size_t)c * CELL_CAP + k];
+                            int pos = ncand;
+                            while (pos > 0 && cand[pos - 1] > v) {
+                                cand[pos] = cand[pos - 1];
+                                --pos;
+                            }
+                            cand[pos] = v;
+                            ++ncand;
+                        }
+                    }
+        }
+        const int count = ncand < 0 ? pd.np : ncand;
+        for (int k = 0; k < count; k++) {
+            const int j = ncand < 0 ? k : cand[k];
             if (j == i) continue;
-            const double dx = xi - X[j], dy = yi - Y[j], dz = zi - Z[j];
-            const double dist_sq = dx * dx + dy * dy + dz * dz;
-            if (dist_sq <= 1.0e-12) continue;
-            const double dist = sqrt(dist_sq);
-            const double hgap = dist - 2.0 * radius;
-            if (hgap >= min_gap) continue;
-            const double overlap = (hgap < 0.0) ? -hgap : 0.0;
-            const double factor = (min_gap - hgap) / min_gap;
-            const double lubforce = coeff * (factor * factor / pd.stf0 + overlap / (min_gap * pd.stf1));
-            const double nx = dx / dist, ny = dy / dist, nz = dz / dist;
-            FX += lubforce * nx;
-            FY += lubforce * ny;
-            FZ += lubforce * nz;
+            double fx, fy, fz;
+            if (!pair_term(pd, X, Y, Z, xi, yi, zi, j, fx, fy, fz)) continue;
+            FX += fx;
+            FY += fy;
+            FZ += fz;
         }
     }
     // repulsive_wall_kernel (pforces.cu:40-76)
@@ -841,6 +917,31 @@ int part_alloc(PartDev &pd, const rbtp_case &c, const rbtp_derived &d, const Dim
     PHIP(hipMemset(pd.sp.count, 0, sizeof(int)));
     PHIP(hipMalloc(&pd.err, ERR_COUNT * sizeof(int32_t)));
     PHIP(hipMemset(pd.err, 0, ERR_COUNT * sizeof(int32_t)));
+    // Pair-search grid: cells 1% wider than the pair range so that rounding in the binning
+    // never separates a pair in range by more than one cell; coarsened when it would hold more
+    // than 8 cells per particle.
+    pd.use_cells = pd.np >= CELLS_MIN_NP;
+    if (pd.use_cells) {
+        const int L[3] = {dims.lx, dims.ly, dims.lz};
+        double w = 1.01 * (2.0 * pd.radius + std::fmax(pd.min_gap, 0.0)) + 1e-9;
+        const double limit = 8.0 * (double)np + 4096.0;
+        for (int attempt = 0; attempt < 64; attempt++) {
+            double cells = 1.0;
+            for (int a = 0; a < 3; a++) {
+                pd.ncell[a] = std::max(1, (int)std::floor((double)L[a] / w));
+                cells *= pd.ncell[a];
+            }
+            if (cells <= limit) break;
+            w *= 1.1;
+        }
+        size_t ncells = 1;
+        for (int a = 0; a < 3; a++) {
+            pd.cell_scale[a] = (double)pd.ncell[a] / (double)L[a];
+            ncells *= (size_t)pd.ncell[a];
+        }
+        PHIP(hipMalloc(&pd.cell_cnt, ncells * sizeof(int32_t)));
+        PHIP(hipMalloc(&pd.cell_pid, ncells * CELL_CAP * sizeof(int32_t)));
+    }
     return RBTP_OK;
 }
 
@@ -850,6 +951,8 @@ void part_free(PartDev &pd) {
         (void)hipFree(pd.owner[b]);
         (void)hipFree(pd.box[b]);
     }
+    (void)hipFree(pd.cell_cnt);
+    (void)hipFree(pd.cell_pid);
     (void)hipFree(pd.sp.node);
     (void)hipFree(pd.sp_kind);
     (void)hipFree(pd.sp.pop);
@@ -954,6 +1057,12 @@ int part_phase(PartDev &pd, const rbtp_case &c, const rbtp_derived &d, const Dim
     (void)d;
     const int np = pd.np;
     const int prev = pd.geo_cur, curr = 1 - pd.geo_cur;
+    if (pd.use_cells) {
+        const int ncells = pd.ncell[0] * pd.ncell[1] * pd.ncell[2];
+        hipLaunchKernelGGL(k_cell_clear, std::min((ncells + 255) / 256, 1024), 256, 0, st,
+                           pd.cell_cnt, ncells);
+        hipLaunchKernelGGL(k_cell_bin, (np + 255) / 256, 256, 0, st, pd);
+    }
     hipLaunchKernelGGL(k_contacts, (np + 127) / 128, 128, 0, st, pd, dims);
     PHIP(hipGetLastError());
     PHIP(dispatch_prec(prec, [&](auto *tag) {

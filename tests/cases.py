@@ -62,3 +62,14 @@ def local_particle_temperature(temperature, ppos, lx, ly, lz):
     iy = np.clip(np.floor(y).astype(int), 0, ly - 1)
     iz = np.clip(np.floor(z).astype(int), 0, lz - 1)
     return temperature[(iz * ly + iy) * lx + ix]
+
+
+def config5(precision=0):
+    """512x512x256, Ra=1e9, Pr=0.71, 5000 spheres of diameter 8 (PARTICLE_DIAM_FRAC = 1/64,
+    fraction 0.019975 -> floor(N*0.019975/V) = 5000), density ratio 2, heat-capacity ratio 1.
+    BETA = 0.39 puts g0 at 1.0e-4, as for the 320x320x160 particle case."""
+    d = single_phase(512, 512, 256, ra=1e9, visc=visc_for(512, 1e9, 0.71), precision=precision,
+                     beta=0.39)
+    d.update(activate_particles=1, particle_diam_frac=1.0 / 64.0, particle_fraction=0.019975,
+             rho_particle_ratio=2.0, cp_particle_ratio=1.0, npart_override=-1)
+    return d

@@ -142,3 +142,47 @@ def test_reference_placement(rb, O):
     for name in ("pos_x", "pos_y", "pos_z", "temp"):
         assert np.array_equal(sim.particles(name), ora.particles(name))
     check_geometry(sim, ora, "placed")
+
+
+def _pairs_layout(rng, nx, ny, nz, spacing, split, origin=5.0):
+    pos = []
+    for k in range(nz):
+        for j in range(ny):
+            for i in range(nx):
+                c = np.array([origin + i * spacing, origin + j * spacing, origin + k * spacing])
+                c = c + rng.uniform(-0.3, 0.3, 3)
+                d = rng.normal(size=3)
+                d *= split / (2.0 * np.linalg.norm(d))
+                pos.append(tuple(c + d))
+                pos.append(tuple(c - d))
+    return pos
+
+
+def test_many_spheres_cell_grid(rb, O):
+    """48 spheres in close pairs: the pair search runs through the cell grid."""
+    rng = np.random.default_rng(7)
+    pos = _pairs_layout(rng, 4, 3, 2, 8.0, 3.6)
+    kw = particle_case(40, 32, 24, npart=len(pos), diam=3.0, ra=1e4, beta=0.05)
+    sim, ora = make_pair(rb, O, kw, pos)
+    sim.step(4)
+    ora.step(4)
+    check_particles(sim, ora, "cell grid")
+    check_geometry(sim, ora, "cell grid")
+    check_fields(sim, ora, "cell grid")
+    assert np.count_nonzero(ora.particles("force_x")) == len(pos)
+
+
+def test_crowded_cell_falls_back_to_all_pairs(rb, O):
+    """A cluster of 20 overlapping small spheres overflows one cell of the grid."""
+    rng = np.random.default_rng(11)
+    # one grid cell (width >= 1.01 * (2R + REP_MIN_GAP) = 3.23) holds all 20 centres
+    cluster = [tuple(np.array([11.5, 8.9, 8.6]) + rng.uniform(-0.5, 0.5, 3)) for _ in range(20)]
+    spread = [tuple(rng.uniform([4, 4, 4], [36, 28, 20])) for _ in range(20)]
+    pos = cluster + spread
+    kw = particle_case(40, 32, 24, npart=len(pos), diam=1.2, ra=1e4, beta=0.05)
+    sim, ora = make_pair(rb, O, kw, pos)
+    sim.step(2)
+    ora.step(2)
+    check_particles(sim, ora, "crowded")
+    check_geometry(sim, ora, "crowded")
+    check_fields(sim, ora, "crowded")
