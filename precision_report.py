@@ -55,12 +55,15 @@ def run(args):
     rb = load_rbtp()
     prec = PRECISIONS[args.precision]
     kw = workload(args.workload, prec)
+    if args.no_particles:
+        kw.update(activate_particles=0, particle_fraction=0.0)
     case = rb.default_case(**kw)
     sim = rb.Sim(case)
     T = rb.linear_temperature(case, 2024, 1e-3)
     sim.init(temperature=T)
-    ppos = np.concatenate([sim.particles("pos_x"), sim.particles("pos_y"), sim.particles("pos_z")])
-    sim.set_particles("temp", local_particle_temperature(T, ppos, case.lx, case.ly, case.lz))
+    if sim.np:
+        ppos = np.concatenate([sim.particles("pos_x"), sim.particles("pos_y"), sim.particles("pos_z")])
+        sim.set_particles("temp", local_particle_temperature(T, ppos, case.lx, case.ly, case.lz))
     sim.synchronize()
     samples, ke, steps_at = [], [], []
     dev_ms = 0.0
@@ -83,7 +86,7 @@ def run(args):
     wall = time.perf_counter() - wall0
     parts = {name: sim.particles(name) for name in ("pos_x", "pos_y", "pos_z", "vel_x", "vel_y",
                                                      "vel_z", "temp")}
-    nlinks = len(sim.links())
+    nlinks = len(sim.links()) if sim.np else 0
     np.savez(args.out, nu=np.array(samples), ke=np.array(ke), steps_at=np.array(steps_at),
              dev_ms=dev_ms, wall_s=wall, n=sim.n, npart=sim.np, links=nlinks, precision=prec,
              workload=args.workload, **parts)
@@ -137,6 +140,7 @@ def main():
     r.add_argument("--steps", type=int, default=20000)
     r.add_argument("--sample", type=int, default=500)
     r.add_argument("--out", required=True)
+    r.add_argument("--no-particles", action="store_true", help="the lattice alone")
     c = sub.add_parser("compare")
     c.add_argument("files", nargs="+")
     args = ap.parse_args()

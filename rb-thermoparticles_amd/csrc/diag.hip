@@ -100,16 +100,25 @@ __global__ __launch_bounds__(DB) void k_means(const double *ux, const double *uy
         for (int c = 0; c < 4; c++) out[blockIdx.x * 4 + c] = red[c][0];
 }
 
+// Kinetic energy over the fluid nodes (owner -1, or every node without particles): per-block
+// partial sums of 0.5|u|^2 and of the node count.
 __global__ __launch_bounds__(DB) void k_ke(const double *ux, const double *uy, const double *uz,
-                                           Dims d, double *out) {
-    __shared__ double red[1][DB];
+                                           const int32_t *owner, Dims d, double *out) {
+    __shared__ double red[2][DB];
     const int t = threadIdx.x;
-    double s = 0.0;
-    for (int idx = blockIdx.x * DB + t; idx < d.n; idx += gridDim.x * DB)
+    double s = 0.0, c = 0.0;
+    for (int idx = blockIdx.x * DB + t; idx < d.n; idx += gridDim.x * DB) {
+        if (owner && owner[idx] != -1) continue;
         s += 0.5 * (ux[idx] * ux[idx] + uy[idx] * uy[idx] + uz[idx] * uz[idx]);
+        c += 1.0;
+    }
     red[0][t] = s;
-    block_tree<1>(red, t);
-    if (t == 0) out[blockIdx.x] = red[0][0];
+    red[1][t] = c;
+    block_tree<2>(red, t);
+    if (t == 0) {
+        out[blockIdx.x * 2] = red[0][0];
+        out[blockIdx.x * 2 + 1] = red[1][0];
+    }
 }
 
 
@@ -325,17 +334,22 @@ int diag_means(const Dims &dims, double *const *macro, const PartDev *pd, hipStr
     return RBTP_OK;
 }
 
-int diag_kinetic_energy(const Dims &dims, double *const *macro, hipStream_t st, double *out) {
+int diag_kinetic_energy(const Dims &dims, double *const *macro, const int32_t *owner,
+                        hipStream_t st, double *out) {
     DevBuf buf;
-    DHIP(hipMalloc(&buf.p, (size_t)RED_BLOCKS * sizeof(double)));
-    hipLaunchKernelGGL(k_ke, RED_BLOCKS, DB, 0, st, macro[1], macro[2], macro[3], dims, buf.p);
+    DHIP(hipMalloc(&buf.p, (size_t)2 * RED_BLOCKS * sizeof(double)));
+    hipLaunchKernelGGL(k_ke, RED_BLOCKS, DB, 0, st, macro[1], macro[2], macro[3], owner, dims,
+                       buf.p);
     DHIP(hipGetLastError());
-    std::vector<double> h(RED_BLOCKS);
+    std::vector<double> h(2 * RED_BLOCKS);
     DHIP(hipMemcpyAsync(h.data(), buf.p, h.size() * sizeof(double), hipMemcpyDeviceToHost, st));
     DHIP(hipStreamSynchronize(st));
-    double s = 0.0;
-    for (int b = 0; b < RED_BLOCKS; b++) s += h[b];
-    *out = s / (double)dims.n;
+    double s = 0.0, c = 0.0;
+    for (int b = 0; b < RED_BLOCKS; b++) {
+        s += h[2 * b];
+        c += h[2 * b + 1];
+    }
+    *out = c > 0.0 ? s / c : 0.0;
     return RBTP_OK;
 }
 

@@ -73,6 +73,7 @@ int diag_nu(const rbtp_case &c, const rbtp_derived &d, const Dims &dims, const d
 int diag_means(const Dims &dims, double *const *macro, const PartDev *pd, hipStream_t st,
                double out[4]);
 int diag_copy_probe(size_t bytes, int reps, double *gbs);
-int diag_kinetic_energy(const Dims &dims, double *const *macro, hipStream_t st, double *out);
+int diag_kinetic_energy(const Dims &dims, double *const *macro, const int32_t *owner,
+                        hipStream_t st, double *out);
 
 }  // namespace rbtp

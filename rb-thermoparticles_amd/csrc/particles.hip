@@ -1009,7 +1009,7 @@ static int place_particles(const rbtp_case &c, const rbtp_derived &d, std::vecto
 int part_load_state(PartDev &pd, const double *host, const Dims &dims, hipStream_t st) {
     const int np = pd.np;
     PHIP(hipMemsetAsync(pd.base, 0, (size_t)(RBTP_P_COUNT + OLD_COUNT) * np * sizeof(double), st));
-    PHIP(hipMemcpyAsync(pd.base, host, (size_t)RBTP_P_COUNT * np * sizeof(double), hipMemcpyHostToDevice, st));
+    PHIP(copy_to_device(pd.base, host, (size_t)RBTP_P_COUNT * np * sizeof(double), st));
     PHIP(hipMemsetAsync(pd.owner[0], 0xFF, (size_t)dims.n * sizeof(int32_t), st));
     hipLaunchKernelGGL(k_geo_mark, np, PB, 0, st, pd.owner[0], pd.box[0], pd.p[RBTP_P_POS_X],
                        pd.p[RBTP_P_POS_Y], pd.p[RBTP_P_POS_Z], pd.radius, pd.radius_sq, dims);
@@ -1140,7 +1140,7 @@ int part_collect_links(PartDev &pd, const Dims &dims, hipStream_t st, std::vecto
         }
         out.resize(total);
         if (e == hipSuccess)
-            e = hipMemcpyAsync(out.data(), buf, (size_t)total * sizeof(rbtp_link), hipMemcpyDeviceToHost, st);
+            e = copy_to_host(out.data(), buf, (size_t)total * sizeof(rbtp_link), st);
         if (e == hipSuccess) e = hipStreamSynchronize(st);
     }
     (void)hipFree(buf);
@@ -1158,8 +1158,8 @@ int part_collect_new_fluid(PartDev &pd, hipStream_t st, std::vector<std::pair<in
     count = std::min(count, pd.sp.cap);
     std::vector<int32_t> nodes(count), kinds(count);
     if (count > 0) {
-        PHIP(hipMemcpy(nodes.data(), pd.sp.node, count * sizeof(int32_t), hipMemcpyDeviceToHost));
-        PHIP(hipMemcpy(kinds.data(), pd.sp_kind, count * sizeof(int32_t), hipMemcpyDeviceToHost));
+        PHIP(copy_to_host(nodes.data(), pd.sp.node, count * sizeof(int32_t), st));
+        PHIP(copy_to_host(kinds.data(), pd.sp_kind, count * sizeof(int32_t), st));
     }
     for (int j = 0; j < count; j++)
         if (kinds[j] >= 0) out.emplace_back(nodes[j], kinds[j]);

@@ -91,4 +91,9 @@ struct SpecialBuf {
     int cap;
 };
 
+// Synchronous copies between caller-owned host memory and the device on stream st, staged
+// through library-owned pinned buffers (xfer.hip).
+hipError_t copy_to_device(void *dst, const void *src, size_t bytes, hipStream_t st);
+hipError_t copy_to_host(void *dst, const void *src, size_t bytes, hipStream_t st);
+
 }  // namespace rbtp

@@ -80,6 +80,7 @@ struct rbtp_sim {
     hipStream_t stream;
     bool own_stream;
     hipEvent_t ev_start, ev_stop, ev_k0, ev_k1;
+    bool step_events;  // ev_start/ev_stop bracket an asynchronous run not yet timed
     bool kernel_timing;
     double kernel_ms_sum;
     int64_t kernel_launches;
@@ -368,8 +369,7 @@ extern "C" int rbtp_init(rbtp_sim *s, const double *temperature, const double *p
     const double *tdev = nullptr;
     if (temperature) {
         if (!s->temp_dev) HIP_TRY(hipMalloc(&s->temp_dev, n * sizeof(double)));
-        HIP_TRY(hipMemcpyAsync(s->temp_dev, temperature, n * sizeof(double), hipMemcpyHostToDevice,
-                               s->stream));
+        HIP_TRY(copy_to_device(s->temp_dev, temperature, n * sizeof(double), s->stream));
         tdev = s->temp_dev;
     }
     s->cur = 0;
@@ -418,6 +418,10 @@ static int after_lattice(rbtp_sim *s, bool macro_out) {
 
 static int one_step(rbtp_sim *s, bool macro_out) {
     const bool nostream = s->poststream;
+    const hipError_t pending = hipGetLastError();
+    if (pending != hipSuccess)
+        return set_error(RBTP_E_HIP, "step %d: HIP error left by earlier work: %s", s->step,
+                         hipGetErrorString(pending));
     HIP_TRY(dispatch(s->prec, [&](auto *tag) -> hipError_t {
         using S = std::remove_pointer_t<decltype(tag)>;
         const StepArgs<S> a = step_args<S>(s, macro_out);
@@ -456,6 +460,7 @@ static int step_impl(rbtp_sim *s, int nsteps, unsigned flags, bool sync) {
         if (nsteps > 0) s->macro_valid = macro_out;
     }
     HIP_TRY(hipEventRecord(s->ev_stop, s->stream));
+    s->step_events = !sync;
     if (sync) {
         HIP_TRY(hipEventSynchronize(s->ev_stop));
         float ms = 0.f;
@@ -475,8 +480,12 @@ extern "C" int rbtp_step_async(rbtp_sim *s, int nsteps, unsigned flags) {
 extern "C" int rbtp_synchronize(rbtp_sim *s) {
     if (!s) return set_error(RBTP_E_ARG, "null handle");
     HIP_TRY(hipStreamSynchronize(s->stream));
-    float ms = 0.f;
-    if (hipEventElapsedTime(&ms, s->ev_start, s->ev_stop) == hipSuccess) s->last_step_ms = ms;
+    if (s->step_events) {
+        float ms = 0.f;
+        HIP_TRY(hipEventElapsedTime(&ms, s->ev_start, s->ev_stop));
+        s->last_step_ms = ms;
+        s->step_events = false;
+    }
     if (s->particles) RB_TRY(part_check_errors(s->pd));
     return RBTP_OK;
 }
@@ -517,15 +526,15 @@ extern "C" int rbtp_get_field(rbtp_sim *s, int which, double *out) {
             return set_error(RBTP_E_STATE,
                              "macroscopic fields are only kept when the last step ran with "
                              "RBTP_STEP_MACRO_LAST");
-        HIP_TRY(hipMemcpy(out, s->macro[which], n * sizeof(double), hipMemcpyDeviceToHost));
+        HIP_TRY(copy_to_host(out, s->macro[which], n * sizeof(double), s->stream));
         return RBTP_OK;
     }
     if (which == RBTP_FIELD_FX) {
         if (s->prec == RBTP_PREC_FP64) {
-            HIP_TRY(hipMemcpy(out, s->fx, n * sizeof(double), hipMemcpyDeviceToHost));
+            HIP_TRY(copy_to_host(out, s->fx, n * sizeof(double), s->stream));
         } else {
             std::vector<float> tmp(n);
-            HIP_TRY(hipMemcpy(tmp.data(), s->fx, n * sizeof(float), hipMemcpyDeviceToHost));
+            HIP_TRY(copy_to_host(tmp.data(), s->fx, n * sizeof(float), s->stream));
             for (size_t i = 0; i < n; i++) out[i] = tmp[i];
         }
         return RBTP_OK;
@@ -570,8 +579,7 @@ extern "C" int rbtp_get_populations(rbtp_sim *s, int which, double *out) {
             e = hipMemcpyAsync(dev, dev + 27 * n, 27 * n * sizeof(double), hipMemcpyDeviceToDevice, s->stream);
         }
     }
-    if (e == hipSuccess) e = hipMemcpyAsync(out, dev, 27 * n * sizeof(double), hipMemcpyDeviceToHost, s->stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+    if (e == hipSuccess) e = copy_to_host(out, dev, 27 * n * sizeof(double), s->stream);
     if (e != hipSuccess) rc = set_error(RBTP_E_HIP, "rbtp_get_populations: %s", hipGetErrorString(e));
     (void)hipFree(dev);
     return rc;
@@ -586,7 +594,7 @@ extern "C" int rbtp_set_populations(rbtp_sim *s, const double *f, const double *
     HIP_TRY(hipMalloc(&dev, 27 * n * sizeof(double)));
     hipError_t e = hipSuccess;
     for (int which = 0; which < 2 && e == hipSuccess; which++) {
-        e = hipMemcpyAsync(dev, which ? g : f, 27 * n * sizeof(double), hipMemcpyHostToDevice, s->stream);
+        e = copy_to_device(dev, which ? g : f, 27 * n * sizeof(double), s->stream);
         if (e != hipSuccess) break;
         e = dispatch(s->prec, [&](auto *tag) {
             using S = std::remove_pointer_t<decltype(tag)>;
@@ -610,7 +618,7 @@ extern "C" int rbtp_get_flags(rbtp_sim *s, int32_t *ibnode, int32_t *owner) {
     std::vector<int32_t> own(n, -1);
     if (s->particles) {
         HIP_TRY(hipStreamSynchronize(s->stream));
-        HIP_TRY(hipMemcpy(own.data(), s->pd.owner[s->pd.geo_cur], n * sizeof(int32_t), hipMemcpyDeviceToHost));
+        HIP_TRY(copy_to_host(own.data(), s->pd.owner[s->pd.geo_cur], n * sizeof(int32_t), s->stream));
     }
     if (owner) std::copy(own.begin(), own.end(), owner);
     if (ibnode)
@@ -624,7 +632,7 @@ extern "C" int rbtp_get_particles(rbtp_sim *s, int which, double *out) {
     if (!s->particles) return RBTP_OK;
     HIP_TRY(hipSetDevice(s->device));
     HIP_TRY(hipStreamSynchronize(s->stream));
-    HIP_TRY(hipMemcpy(out, s->pd.p[which], s->pd.np * sizeof(double), hipMemcpyDeviceToHost));
+    HIP_TRY(copy_to_host(out, s->pd.p[which], s->pd.np * sizeof(double), s->stream));
     return RBTP_OK;
 }
 
@@ -634,7 +642,7 @@ extern "C" int rbtp_set_particles(rbtp_sim *s, int which, const double *in) {
     if (!s->particles) return RBTP_OK;
     HIP_TRY(hipSetDevice(s->device));
     HIP_TRY(hipStreamSynchronize(s->stream));
-    HIP_TRY(hipMemcpy(s->pd.p[which], in, s->pd.np * sizeof(double), hipMemcpyHostToDevice));
+    HIP_TRY(copy_to_device(s->pd.p[which], in, s->pd.np * sizeof(double), s->stream));
     return RBTP_OK;
 }
 
@@ -709,7 +717,9 @@ extern "C" int rbtp_kinetic_energy(rbtp_sim *s, double *out) {
     if (!s || !out) return set_error(RBTP_E_ARG, "null argument");
     if (!s->macro_valid) return set_error(RBTP_E_STATE, "run the last step with RBTP_STEP_MACRO_LAST first");
     HIP_TRY(hipSetDevice(s->device));
-    RB_TRY(diag_kinetic_energy(s->dims, s->macro, s->stream, out));
+    RB_TRY(diag_kinetic_energy(s->dims, s->macro,
+                               s->particles ? s->pd.owner[s->pd.geo_cur] : nullptr, s->stream,
+                               out));
     return RBTP_OK;
 }
 
@@ -752,9 +762,7 @@ extern "C" int rbtp_get_output_fields(rbtp_sim *s, double *out) {
     HIP_TRY(hipSetDevice(s->device));
     double *dev = nullptr;
     RB_TRY(output_fields_dev(s, &dev));
-    hipError_t e = hipMemcpyAsync(out, dev, 7 * (size_t)s->dims.n * sizeof(double),
-                                  hipMemcpyDeviceToHost, s->stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+    const hipError_t e = copy_to_host(out, dev, 7 * (size_t)s->dims.n * sizeof(double), s->stream);
     (void)hipFree(dev);
     if (e != hipSuccess) return set_error(RBTP_E_HIP, "rbtp_get_output_fields: %s", hipGetErrorString(e));
     return RBTP_OK;
@@ -819,7 +827,7 @@ extern "C" int rbtp_write_particles(rbtp_sim *s, const char *path) {
     HIP_TRY(hipSetDevice(s->device));
     HIP_TRY(hipStreamSynchronize(s->stream));
     std::vector<double> h((size_t)RBTP_P_COUNT * np);
-    HIP_TRY(hipMemcpy(h.data(), s->pd.base, h.size() * sizeof(double), hipMemcpyDeviceToHost));
+    HIP_TRY(copy_to_host(h.data(), s->pd.base, h.size() * sizeof(double), s->stream));
     FILE *fp = fopen(path, "w");
     if (!fp) return set_error(RBTP_E_IO, "cannot open %s", path);
     for (int pid = 0; pid < np; ++pid) {
@@ -888,7 +896,7 @@ extern "C" int rbtp_restart_load(rbtp_sim *s, const char *dir, int step) {
         if (rd != fields.size())
             return set_error(RBTP_E_IO, "flow restart file %s is short (%zu of %zu values)", path, rd, fields.size());
         for (int k = 0; k < 5; k++)
-            HIP_TRY(hipMemcpy(s->macro[k], fields.data() + k * n, n * sizeof(double), hipMemcpyHostToDevice));
+            HIP_TRY(copy_to_device(s->macro[k], fields.data() + k * n, n * sizeof(double), s->stream));
         HIP_TRY(dispatch(s->prec, [&](auto *tag) {
             using S = std::remove_pointer_t<decltype(tag)>;
             return launch_forcing_field<S>((typename Enc<S>::A *)s->fx,

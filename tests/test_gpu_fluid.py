@@ -115,3 +115,18 @@ def test_low_precision_forms_track_fp64(rb, prec):
     assert rel_err(low.field("phi"), ref.field("phi")) <= tol
     nu_r, nu_l = ref.nu(), low.nu()
     assert abs(nu_l[4] - nu_r[4]) <= 10 * tol * abs(nu_r[4])
+
+
+def test_synchronize_before_and_after_async_steps(rb, O):
+    """rbtp_synchronize with no step in flight leaves no error behind; an asynchronous run is
+    timed at the synchronisation and gives the same state as a synchronous one."""
+    sim, ora = make_pair(rb, O, single_phase(16, 12, 10, ra=1e5))
+    sim.synchronize()
+    sim.step(2)
+    sim.step_async(3)
+    sim.synchronize()
+    assert sim.timing().step_ms > 0.0
+    sim.synchronize()
+    ora.step(5)
+    assert np.array_equal(sim.populations("f"), ora.populations("f"))
+    assert np.array_equal(sim.populations("g"), ora.populations("g"))

@@ -177,3 +177,15 @@ def test_rb3d_driver_outputs_and_restart(tmp_path):
     assert r.returncode == 0, r.stderr
     assert "Restarted from step: 20" in r.stdout
     assert (d / "fg000000030.dat").exists()
+
+
+def test_kinetic_energy_over_fluid_nodes(rb, O):
+    """rbtp_kinetic_energy is the mean of 0.5|u|^2 over the nodes that are fluid now."""
+    kw = particle_case(24, 22, 20, npart=2, diam=6.0, ra=1e5, beta=0.05)
+    sim, ora = pair(rb, O, kw, ppos=[(8.2, 10.1, 9.7), (15.9, 12.4, 10.3)])
+    ib, _ = sim.flags()
+    fluid = ib.reshape(-1) == 0
+    u = [sim.field(k).reshape(-1)[fluid] for k in ("ux", "uy", "uz")]
+    expect = np.mean(0.5 * (u[0] ** 2 + u[1] ** 2 + u[2] ** 2))
+    assert sim.kinetic_energy() == pytest.approx(expect, rel=1e-12)
+    assert fluid.sum() < fluid.size
