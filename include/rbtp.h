@@ -215,8 +215,9 @@ int rbtp_get_new_fluid(rbtp_sim *s, int32_t *nodes, int32_t *pids, int64_t capac
  *               Nu_top, Nu_bottom, Nu_bulk_fluid, Nu_bulk_part}.
  *   rbtp_means: diag_flow (output_statis.cu:64-109) after overwrite_particle_nodes:
  *               out[4] = {umean, vmean, wmean, tmean}.
- *   rbtp_kinetic_energy: mean over the fluid nodes (all nodes without particles) of
- *                        0.5*(ux^2+uy^2+uz^2) from the last macro fields (no reference counterpart).
+ *   rbtp_kinetic_energy: mean of 0.5*(ux^2+uy^2+uz^2) from the last macro fields over the nodes
+ *                        that are fluid in the current and the previous geometry (all nodes
+ *                        without particles); no reference counterpart.
  */
 int rbtp_nu(rbtp_sim *s, double out[7]);
 int rbtp_means(rbtp_sim *s, double out[4]);

@@ -100,15 +100,18 @@ __global__ __launch_bounds__(DB) void k_means(const double *ux, const double *uy
         for (int c = 0; c < 4; c++) out[blockIdx.x * 4 + c] = red[c][0];
 }
 
-// Kinetic energy over the fluid nodes (owner -1, or every node without particles): per-block
-// partial sums of 0.5|u|^2 and of the node count.
+// Kinetic energy over the nodes that are fluid in both the current and the previous geometry
+// (every node without particles): per-block partial sums of 0.5|u|^2 and of the node count.
+// Nodes refilled this step are left out: refill_new_fluid_kernel sets their u to momentum over
+// the density deviation sum(f) (refill.cu:224-229), which is not a velocity.
 __global__ __launch_bounds__(DB) void k_ke(const double *ux, const double *uy, const double *uz,
-                                           const int32_t *owner, Dims d, double *out) {
+                                           const int32_t *owner, const int32_t *owner_prev,
+                                           Dims d, double *out) {
     __shared__ double red[2][DB];
     const int t = threadIdx.x;
     double s = 0.0, c = 0.0;
     for (int idx = blockIdx.x * DB + t; idx < d.n; idx += gridDim.x * DB) {
-        if (owner && owner[idx] != -1) continue;
+        if (owner && (owner[idx] != -1 || owner_prev[idx] != -1)) continue;
         s += 0.5 * (ux[idx] * ux[idx] + uy[idx] * uy[idx] + uz[idx] * uz[idx]);
         c += 1.0;
     }
@@ -335,11 +338,11 @@ int diag_means(const Dims &dims, double *const *macro, const PartDev *pd, hipStr
 }
 
 int diag_kinetic_energy(const Dims &dims, double *const *macro, const int32_t *owner,
-                        hipStream_t st, double *out) {
+                        const int32_t *owner_prev, hipStream_t st, double *out) {
     DevBuf buf;
     DHIP(hipMalloc(&buf.p, (size_t)2 * RED_BLOCKS * sizeof(double)));
-    hipLaunchKernelGGL(k_ke, RED_BLOCKS, DB, 0, st, macro[1], macro[2], macro[3], owner, dims,
-                       buf.p);
+    hipLaunchKernelGGL(k_ke, RED_BLOCKS, DB, 0, st, macro[1], macro[2], macro[3], owner,
+                       owner_prev, dims, buf.p);
     DHIP(hipGetLastError());
     std::vector<double> h(2 * RED_BLOCKS);
     DHIP(hipMemcpyAsync(h.data(), buf.p, h.size() * sizeof(double), hipMemcpyDeviceToHost, st));

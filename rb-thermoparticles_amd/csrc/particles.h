@@ -49,9 +49,49 @@ void part_free(PartDev &pd);
 int part_init(PartDev &pd, const rbtp_case &c, const rbtp_derived &d, const Dims &dims,
               const double *ppos, const double *ptemp, hipStream_t st);
 int part_load_state(PartDev &pd, const double *host, const Dims &dims, hipStream_t st);
-int part_phase(PartDev &pd, const rbtp_case &c, const rbtp_derived &d, const Dims &dims,
-               int prec, const void *fcur, const void *gcur, void *fx, void *rho_wall,
-               double *const *macro, hipStream_t st);
+// One lattice's share of a particle phase: the particle state and the lattice arrays the
+// phase reads and writes at one time step.  Kept in device memory, so that one launch of each
+// particle kernel serves every lattice of a serving batch (or the one lattice of an rbtp_sim).
+struct PartSlot {
+    PartDev pd;
+    Dims d;
+    const int32_t *owner_prev, *box_prev;  // geometry of this step's IBB and refill
+    int32_t *owner_curr, *box_curr;        // geometry built from the moved particles
+    const void *fcur, *gcur;               // this step's post-collision populations
+    void *fx;                              // F_x (Enc<S>::A)
+    const void *rho_wall;                  // wall rho (Enc<S>::A)
+    double *macro[5];                      // rho (also the full rho field), ux, uy, uz, phi
+};
+
+// Launch shape of a particle phase over a table of slots.  The prefix arrays map a flat index
+// (a particle, a pair-search cell, a fix-up workgroup) to its slot.
+struct PartGrid {
+    const PartSlot *tab;
+    const int *pstart;  // [nslots + 1] particles
+    const int *cstart;  // [nslots + 1] pair-search cells
+    const int *sstart;  // [nslots + 1] workgroups of the special-node fix-up
+    const int *kstart;  // [nslots + 1] workgroups of the special-node re-collision
+    int nslots, total_p, total_c, total_s, total_k;
+};
+
+// Device tables for the two phases of a time step (the geometry generations and the
+// population buffers alternate every step).
+struct PartTables {
+    char *mem = nullptr;
+    size_t bytes = 0;
+    PartGrid g[2] = {};
+};
+
+int part_tables_build(PartTables &t, const std::vector<PartSlot> (&slots)[2], hipStream_t st);
+void part_tables_free(PartTables &t);
+// Workgroups the fix-up (128 threads, 4 nodes each) and the re-collision (256 threads) give to
+// one lattice: from an estimate of its special-node count, which the grid-stride loops only use
+// for parallelism.
+int part_fixup_blocks(const PartDev &pd);
+int part_collide_blocks(const PartDev &pd);
+// The particle phase (main.cu:52-84 minus the lattice passes) of every slot of g, in precision
+// form prec.  The caller then advances each slot's geo_cur and sets special_pending.
+int part_phase(const PartGrid &g, int prec, hipStream_t st);
 int part_check_errors(PartDev &pd);
 hipError_t part_overwrite_special(PartDev &pd, double *f, double *g, const Dims &dims,
                                   hipStream_t st);
@@ -74,6 +114,6 @@ int diag_means(const Dims &dims, double *const *macro, const PartDev *pd, hipStr
                double out[4]);
 int diag_copy_probe(size_t bytes, int reps, double *gbs);
 int diag_kinetic_energy(const Dims &dims, double *const *macro, const int32_t *owner,
-                        hipStream_t st, double *out);
+                        const int32_t *owner_prev, hipStream_t st, double *out);
 
 }  // namespace rbtp

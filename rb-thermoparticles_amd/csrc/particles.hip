@@ -6,16 +6,20 @@
 // (link_count_kernel, links.cu:50-87).  Each consumer re-derives the links it needs from the two
 // owner maps with the same bit-exact test:
 //   * k_link_sums: one workgroup per particle walks the particle's link box and sums
-//     force, torque and heat in a fixed order (256 lane partials + pairwise tree), replacing the
-//     reference's FP64 atomics (ibb.cu:182-187,204) with a reproducible reduction;
-//   * k_special_links / k_special_refill: compact list of the nodes whose post-stream
-//     populations the coupling replaces (IBB link nodes, refilled nodes);
+//     force, torque and heat in a fixed order (1024 lane partials + pairwise tree), replacing
+//     the reference's FP64 atomics (ibb.cu:182-187,204) with a reproducible reduction;
+//   * k_special_find: compact list of the nodes whose post-stream populations the coupling
+//     replaces (IBB link nodes, refilled nodes);
 //   * k_prefixup: those nodes' corrected post-stream populations (pull + IBB, or refill),
 //     buffered for the next lattice step.
 // Geometry is two owner maps (previous and current generation); each generation is cleared only
 // inside the marking boxes it was built from, so a step touches O(P R^3) nodes instead of the
 // reference's full-lattice memsets, scans and host round trips (links.cu:147-256,
 // refill.cu:244-329).  Nothing is read back to the host inside a step.
+//
+// The step kernels take a PartGrid: a table of slots (one lattice each) in device memory plus
+// prefix arrays that map a flat particle, cell or workgroup index to its slot, so one launch
+// serves one lattice or every lattice of a serving batch.
 //
 // The translation unit is compiled with -ffp-contract=off: every expression is evaluated in the
 // reference's operation order.
@@ -127,10 +131,39 @@ __device__ inline bool in_domain(const Dims &d, int x, int y, int z) {
     return x >= 0 && x < d.lx && y >= 0 && y < d.ly && z >= 0 && z < d.lz;
 }
 
+// Owners of the 26 neighbours of (ix, iy, iz), all loads issued together (addresses clamped
+// into the domain); -2 where the neighbour is outside the domain.
+__device__ inline void neighbour_owners(const int32_t *owner, const Dims &d, int ix, int iy,
+                                        int iz, int (&own)[NPOP]) {
+    own[0] = -2;
+#pragma unroll
+    for (int ip = 1; ip < NPOP; ip++) {
+        const int nx = ix + c_cx[ip], ny = iy + c_cy[ip], nz = iz + c_cz[ip];
+        const int cx = min(max(nx, 0), d.lx - 1);
+        const int cy = min(max(ny, 0), d.ly - 1);
+        const int cz = min(max(nz, 0), d.lz - 1);
+        const int v = owner[cz * d.lxy + cy * d.lx + cx];
+        own[ip] = in_domain(d, nx, ny, nz) ? v : -2;
+    }
+}
+
+// ---- slot lookup -------------------------------------------------------------------------------
+
+// Slot that owns flat index u under the prefix array start[0..n] (start[0] = 0, non-decreasing;
+// slots with an empty range own nothing).
+__device__ inline int slot_of(const int *start, int n, int u) {
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (start[mid] <= u) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
 // ---- geometry --------------------------------------------------------------------------------
 
-__global__ __launch_bounds__(PB) void k_geo_clear(int32_t *owner, const int32_t *box, Dims d) {
-    const int p = blockIdx.x;
+__device__ inline void geo_clear_body(int32_t *owner, const int32_t *box, const Dims &d, int p) {
     int lo[3], hi[3];
     if (!load_box(box, p, lo, hi)) return;
     const int cnt = box_count(lo, hi);
@@ -140,13 +173,18 @@ __global__ __launch_bounds__(PB) void k_geo_clear(int32_t *owner, const int32_t 
     }
 }
 
+__global__ __launch_bounds__(PB) void k_geo_clear(PartGrid g) {
+    const int s = slot_of(g.pstart, g.nslots, blockIdx.x);
+    const PartSlot &sl = g.tab[s];
+    geo_clear_body(sl.owner_curr, sl.box_curr, sl.d, blockIdx.x - g.pstart[s]);
+}
+
 // mark_ibnode_kernel (links.cu:25-48) for every particle at once.  The reference launches the
 // particles one after another so the lowest pid claims a shared node; atomicMin on the owner
 // (-1 read as the largest unsigned value) gives the same result in one launch.
-__global__ __launch_bounds__(PB) void k_geo_mark(int32_t *owner, int32_t *box, const double *px,
-                                                 const double *py, const double *pz, double R,
-                                                 double radius_sq, Dims d) {
-    const int p = blockIdx.x;
+__device__ inline void geo_mark_body(int32_t *owner, int32_t *box, const double *px,
+                                     const double *py, const double *pz, double R,
+                                     double radius_sq, const Dims &d, int p) {
     int lo[3], hi[3];
     const double X = px[p], Y = py[p], Z = pz[p];
     const bool ok = marking_box(X, Y, Z, R, d, lo, hi);
@@ -170,6 +208,22 @@ __global__ __launch_bounds__(PB) void k_geo_mark(int32_t *owner, int32_t *box, c
     }
 }
 
+// Initial geometry of one lattice.
+__global__ __launch_bounds__(PB) void k_geo_mark_init(int32_t *owner, int32_t *box,
+                                                      const double *px, const double *py,
+                                                      const double *pz, double R, double radius_sq,
+                                                      Dims d) {
+    geo_mark_body(owner, box, px, py, pz, R, radius_sq, d, blockIdx.x);
+}
+
+__global__ __launch_bounds__(PB) void k_geo_mark(PartGrid g) {
+    const int s = slot_of(g.pstart, g.nslots, blockIdx.x);
+    const PartSlot &sl = g.tab[s];
+    const PartDev &pd = sl.pd;
+    geo_mark_body(sl.owner_curr, sl.box_curr, pd.p[RBTP_P_POS_X], pd.p[RBTP_P_POS_Y],
+                  pd.p[RBTP_P_POS_Z], pd.radius, pd.radius_sq, sl.d, blockIdx.x - g.pstart[s]);
+}
+
 // ---- contacts and force history ----------------------------------------------------------------
 
 // ---- pair search grid -----------------------------------------------------------------------
@@ -179,15 +233,22 @@ __device__ inline int cell_coord(double x, double scale, int n) {
     return static_cast<int>(t);
 }
 
-__global__ __launch_bounds__(256) void k_cell_clear(int32_t *cnt, int ncells) {
-    for (int c = blockIdx.x * 256 + threadIdx.x; c < ncells; c += gridDim.x * 256) cnt[c] = 0;
+__global__ __launch_bounds__(256) void k_cell_clear(PartGrid g) {
+    for (int u = blockIdx.x * 256 + threadIdx.x; u < g.total_c; u += gridDim.x * 256) {
+        const int s = slot_of(g.cstart, g.nslots, u);
+        g.tab[s].pd.cell_cnt[u - g.cstart[s]] = 0;
+    }
 }
 
 // Bins every particle by its centre.  The order inside a cell is arbitrary: the pair sum sorts
 // its candidates by pid.
-__global__ __launch_bounds__(256) void k_cell_bin(PartDev pd) {
-    const int i = blockIdx.x * 256 + threadIdx.x;
-    if (i >= pd.np) return;
+__global__ __launch_bounds__(256) void k_cell_bin(PartGrid g) {
+    const int u = blockIdx.x * 256 + threadIdx.x;
+    if (u >= g.total_p) return;
+    const int s = slot_of(g.pstart, g.nslots, u);
+    const PartDev &pd = g.tab[s].pd;
+    if (!pd.use_cells) return;
+    const int i = u - g.pstart[s];
     const int cx = cell_coord(pd.p[RBTP_P_POS_X][i], pd.cell_scale[0], pd.ncell[0]);
     const int cy = cell_coord(pd.p[RBTP_P_POS_Y][i], pd.cell_scale[1], pd.ncell[1]);
     const int cz = cell_coord(pd.p[RBTP_P_POS_Z][i], pd.cell_scale[2], pd.ncell[2]);
@@ -225,12 +286,16 @@ __device__ inline bool pair_term(const PartDev &pd, const double *X, const doubl
 // sorted by pid; every pair within range is among them, and the pairs left out are ones the
 // reference skips, so the sum is the same.  When a cell overflowed or i has more than
 // CAND_MAX candidates, i falls back to every partner.
-__global__ __launch_bounds__(128) void k_contacts(PartDev pd, Dims d) {
+__global__ __launch_bounds__(128) void k_contacts(PartGrid g) {
     __shared__ int32_t cand_all[128][CAND_MAX];
     int32_t *cand = cand_all[threadIdx.x];
-    const int i = blockIdx.x * 128 + threadIdx.x;
-    if (i >= pd.np) return;
-    double **P = pd.p;
+    const int u = blockIdx.x * 128 + threadIdx.x;
+    if (u >= g.total_p) return;
+    const int s = slot_of(g.pstart, g.nslots, u);
+    const PartDev &pd = g.tab[s].pd;
+    const Dims d = g.tab[s].d;
+    const int i = u - g.pstart[s];
+    double *const *P = pd.p;
     for (int a = 0; a < 3; a++) {
         P[RBTP_P_FORCE_PREV2_X + a][i] = P[RBTP_P_FORCE_PREV_X + a][i];
         P[RBTP_P_TORQUE_PREV2_X + a][i] = P[RBTP_P_TORQUE_PREV_X + a][i];
@@ -415,12 +480,17 @@ __device__ inline LinkOut eval_link(int ip, int ix, int iy, int iz, double q, in
 // order; the 1024 partials are combined pairwise (stride 512 ... 1) and added to the sums left
 // by k_contacts.
 template <class S>
-__global__ __launch_bounds__(LB) void k_link_sums(PartDev pd, PopView<S> pop,
-                                                  const int32_t *owner_prev,
-                                                  const int32_t *box_prev, const double *rho,
-                                                  Dims d) {
+__global__ __launch_bounds__(LB) void k_link_sums(PartGrid g) {
     __shared__ double red[7][LB];
-    const int p = blockIdx.x;
+    const int slot = slot_of(g.pstart, g.nslots, blockIdx.x);
+    const PartSlot &sl = g.tab[slot];
+    const PartDev &pd = sl.pd;
+    const Dims d = sl.d;
+    const PopView<S> pop{(const S *)sl.fcur, (const S *)sl.gcur, (size_t)d.n, pd.tl0, pd.tls, d.lx};
+    const int32_t *owner_prev = sl.owner_prev;
+    const int32_t *box_prev = sl.box_prev;
+    const double *rho = sl.macro[0];
+    const int p = blockIdx.x - g.pstart[slot];
     const int t = threadIdx.x;
     double s[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     int lo[3], hi[3];
@@ -440,10 +510,10 @@ __global__ __launch_bounds__(LB) void k_link_sums(PartDev pd, PopView<S> pop,
             const double x0 = static_cast<double>(ix) + 0.5;
             const double y0 = static_cast<double>(iy) + 0.5;
             const double z0 = static_cast<double>(iz) + 0.5;
+            int own[NPOP];
+            neighbour_owners(owner_prev, d, ix, iy, iz, own);
             for (int ip = 1; ip < NPOP; ip++) {
-                const int nx = ix + c_cx[ip], ny = iy + c_cy[ip], nz = iz + c_cz[ip];
-                if (!in_domain(d, nx, ny, nz)) continue;
-                if (owner_prev[nz * d.lxy + ny * d.lx + nx] != p) continue;
+                if (own[ip] != p) continue;
                 double q = 0.0;
                 if (!link_q(ip, x0, y0, z0, X, Y, Z, pd.radius_sq, q)) continue;
                 const LinkOut L = eval_link<S>(ip, ix, iy, iz, q, p, pv, pop, owner_prev,
@@ -494,10 +564,16 @@ __global__ __launch_bounds__(LB) void k_link_sums(PartDev pd, PopView<S> pop,
 // ---- motion -------------------------------------------------------------------------------------
 
 // update_particles_kernel (move.cu:5-103), after saving the state the IBB of this step used.
-__global__ __launch_bounds__(128) void k_move(PartDev pd) {
-    const int pid = blockIdx.x * 128 + threadIdx.x;
-    if (pid >= pd.np) return;
-    double **P = pd.p;
+// The special-node list of the step is rebuilt after the motion: particle 0 clears its count
+// (the re-collision that read it ran before this kernel).
+__global__ __launch_bounds__(128) void k_move(PartGrid g) {
+    const int u = blockIdx.x * 128 + threadIdx.x;
+    if (u >= g.total_p) return;
+    const int s = slot_of(g.pstart, g.nslots, u);
+    const PartDev &pd = g.tab[s].pd;
+    const int pid = u - g.pstart[s];
+    if (pid == 0) *pd.sp.count = 0;
+    double *const *P = pd.p;
     const int src[OLD_COUNT] = {RBTP_P_POS_X, RBTP_P_POS_Y, RBTP_P_POS_Z, RBTP_P_VEL_X,
                                 RBTP_P_VEL_Y, RBTP_P_VEL_Z, RBTP_P_OMEGA_X, RBTP_P_OMEGA_Y,
                                 RBTP_P_OMEGA_Z, RBTP_P_TEMP};
@@ -549,10 +625,19 @@ __global__ __launch_bounds__(128) void k_move(PartDev pd) {
 
 // ---- special nodes ------------------------------------------------------------------------------
 
-__global__ void k_zero_count(int *count) { *count = 0; }
-
-__device__ inline void push_special(const PartDev &pd, int idx, int kind) {
-    const int j = atomicAdd(pd.sp.count, 1);
+// Appends the nodes of the calling wave's lanes that have `want` set, in lane order, with one
+// atomic per wave: consecutive box nodes stay consecutive in the list, which keeps the later
+// passes over the list coalesced.  Every active lane of the wave must call it.
+__device__ inline void push_special(const PartDev &pd, bool want, int idx, int kind) {
+    const unsigned long long mask = __ballot(want);
+    if (mask == 0) return;
+    const int lane = __lane_id();
+    const int leader = __ffsll((long long)mask) - 1;
+    int base = 0;
+    if (lane == leader) base = atomicAdd(pd.sp.count, __popcll(mask));
+    base = __shfl(base, leader);
+    if (!want) return;
+    const int j = base + __popcll(mask & ((1ull << lane) - 1ull));
     if (j < pd.sp.cap) {
         pd.sp.node[j] = idx;
         pd.sp_kind[j] = kind;
@@ -561,58 +646,64 @@ __device__ inline void push_special(const PartDev &pd, int idx, int kind) {
     }
 }
 
-// Link nodes of the previous geometry (the nodes bounce_back_particles writes).  A node with
-// links to several particles is listed once, by the lowest linked pid.
-__global__ __launch_bounds__(LB) void k_special_links(PartDev pd, const int32_t *owner_prev,
-                                                      const int32_t *box_prev, Dims d) {
-    const int p = blockIdx.x;
+// The special-node list of one particle's boxes.
+//   Link nodes of the previous geometry (the nodes bounce_back_particles writes), in the link
+//   box; a node with links to several particles is listed once, by the lowest linked pid.
+//   New-fluid nodes (new_fluid_count_kernel, refill.cu:8-41) in the marking box: solid in the
+//   previous geometry, fluid now, previous owner valid, and within close_thresh of that owner's
+//   new surface.
+__global__ __launch_bounds__(LB) void k_special_find(PartGrid g) {
+    const int slot = slot_of(g.pstart, g.nslots, blockIdx.x);
+    const PartSlot &sl = g.tab[slot];
+    const PartDev &pd = sl.pd;
+    const Dims d = sl.d;
+    const int32_t *owner_prev = sl.owner_prev;
+    const int32_t *owner_curr = sl.owner_curr;
+    const int p = blockIdx.x - g.pstart[slot];
     int lo[3], hi[3];
-    if (!load_box(box_prev, p, lo, hi)) return;
-    grow_box(d, lo, hi);
-    const double *OX = pd.old[OLD_PX], *OY = pd.old[OLD_PY], *OZ = pd.old[OLD_PZ];
-    const int cnt = box_count(lo, hi);
-    for (int k = threadIdx.x; k < cnt; k += LB) {
-        int ix, iy, iz;
-        const int idx = box_node(d, lo, hi, k, ix, iy, iz);
-        if (owner_prev[idx] != -1) continue;
-        const double x0 = static_cast<double>(ix) + 0.5;
-        const double y0 = static_cast<double>(iy) + 0.5;
-        const double z0 = static_cast<double>(iz) + 0.5;
-        int minpid = INT_MAX;
-        for (int ip = 1; ip < NPOP; ip++) {
-            const int nx = ix + c_cx[ip], ny = iy + c_cy[ip], nz = iz + c_cz[ip];
-            if (!in_domain(d, nx, ny, nz)) continue;
-            const int q_owner = owner_prev[nz * d.lxy + ny * d.lx + nx];
-            if (q_owner < 0 || q_owner >= minpid) continue;
-            double q = 0.0;
-            if (link_q(ip, x0, y0, z0, OX[q_owner], OY[q_owner], OZ[q_owner], pd.radius_sq, q))
-                minpid = q_owner;
+    if (!load_box(sl.box_prev, p, lo, hi)) return;
+    {
+        int llo[3] = {lo[0], lo[1], lo[2]}, lhi[3] = {hi[0], hi[1], hi[2]};
+        grow_box(d, llo, lhi);
+        const double *OX = pd.old[OLD_PX], *OY = pd.old[OLD_PY], *OZ = pd.old[OLD_PZ];
+        const int cnt = box_count(llo, lhi);
+        for (int k = threadIdx.x; k < cnt; k += LB) {
+            int ix, iy, iz;
+            const int idx = box_node(d, llo, lhi, k, ix, iy, iz);
+            int minpid = INT_MAX;
+            if (owner_prev[idx] == -1) {
+                const double x0 = static_cast<double>(ix) + 0.5;
+                const double y0 = static_cast<double>(iy) + 0.5;
+                const double z0 = static_cast<double>(iz) + 0.5;
+                int own[NPOP];
+                neighbour_owners(owner_prev, d, ix, iy, iz, own);
+                for (int ip = 1; ip < NPOP; ip++) {
+                    const int q_owner = own[ip];
+                    if (q_owner < 0 || q_owner >= minpid) continue;
+                    double q = 0.0;
+                    if (link_q(ip, x0, y0, z0, OX[q_owner], OY[q_owner], OZ[q_owner], pd.radius_sq, q))
+                        minpid = q_owner;
+                }
+            }
+            push_special(pd, minpid == p, idx, -1);
         }
-        if (minpid == p) push_special(pd, idx, -1);
     }
-}
-
-// New-fluid nodes (new_fluid_count_kernel, refill.cu:8-41): solid in the previous geometry,
-// fluid now, previous owner valid, and within close_thresh of that owner's new surface.
-__global__ __launch_bounds__(LB) void k_special_refill(PartDev pd, const int32_t *owner_prev,
-                                                       const int32_t *box_prev,
-                                                       const int32_t *owner_curr, Dims d) {
-    const int p = blockIdx.x;
-    int lo[3], hi[3];
-    if (!load_box(box_prev, p, lo, hi)) return;
     const double PXv = pd.p[RBTP_P_POS_X][p], PYv = pd.p[RBTP_P_POS_Y][p], PZv = pd.p[RBTP_P_POS_Z][p];
     const int cnt = box_count(lo, hi);
     for (int k = threadIdx.x; k < cnt; k += LB) {
         int ix, iy, iz;
         const int idx = box_node(d, lo, hi, k, ix, iy, iz);
-        if (owner_prev[idx] != p || owner_curr[idx] != -1) continue;
-        const double x0 = static_cast<double>(ix) + 0.5;
-        const double y0 = static_cast<double>(iy) + 0.5;
-        const double z0 = static_cast<double>(iz) + 0.5;
-        const double dx = x0 - PXv, dy = y0 - PYv, dz = z0 - PZv;
-        const double dist = sqrt(dx * dx + dy * dy + dz * dz);
-        const double close_thresh = 1.7606816861659007;
-        if (dist - pd.radius <= close_thresh) push_special(pd, idx, p);
+        bool want = false;
+        if (owner_prev[idx] == p && owner_curr[idx] == -1) {
+            const double x0 = static_cast<double>(ix) + 0.5;
+            const double y0 = static_cast<double>(iy) + 0.5;
+            const double z0 = static_cast<double>(iz) + 0.5;
+            const double dx = x0 - PXv, dy = y0 - PYv, dz = z0 - PZv;
+            const double dist = sqrt(dx * dx + dy * dy + dz * dz);
+            const double close_thresh = 1.7606816861659007;
+            want = dist - pd.radius <= close_thresh;
+        }
+        push_special(pd, want, idx, p);
     }
 }
 
@@ -655,19 +746,26 @@ __device__ inline void pull_one(const PopView<S> &pop, int j, int idx, int ix, i
 //              the moved state for the fallback equilibrium; lane 0 then forms rho, u, phi
 //              (refill.cu:217-235) in population order and sets F = 0.
 template <class S>
-__global__ __launch_bounds__(128) void k_prefixup(PartDev pd, PopView<S> pop,
-                                                  const int32_t *owner_prev,
-                                                  typename Enc<S>::A *fx_field,
-                                                  const typename Enc<S>::A *rho_wall,
-                                                  double *macro_rho, double *macro_ux,
-                                                  double *macro_uy, double *macro_uz,
-                                                  double *macro_phi, Dims d) {
+__global__ __launch_bounds__(128) void k_prefixup(PartGrid g) {
     __shared__ double sfv[4][32], sgv[4][32];
+    const int slot = slot_of(g.sstart, g.nslots, blockIdx.x);
+    const PartSlot &sl = g.tab[slot];
+    const PartDev &pd = sl.pd;
+    const Dims d = sl.d;
+    using A = typename Enc<S>::A;
+    const PopView<S> pop{(const S *)sl.fcur, (const S *)sl.gcur, (size_t)d.n, pd.tl0, pd.tls, d.lx};
+    const int32_t *owner_prev = sl.owner_prev;
+    A *fx_field = (A *)sl.fx;
+    const A *rho_wall = (const A *)sl.rho_wall;
+    double *macro_rho = sl.macro[0], *macro_ux = sl.macro[1], *macro_uy = sl.macro[2];
+    double *macro_uz = sl.macro[3], *macro_phi = sl.macro[4];
+    const int blk = blockIdx.x - g.sstart[slot];
+    const int nblk = g.sstart[slot + 1] - g.sstart[slot];
     const int grp = threadIdx.x >> 5, lane = threadIdx.x & 31;
     const int count = min(*pd.sp.count, pd.sp.cap);
     const size_t cap = (size_t)pd.sp.cap;
     double *out = pd.sp.pop;
-    for (int base = blockIdx.x * 4; base < count; base += gridDim.x * 4) {
+    for (int base = blk * 4; base < count; base += nblk * 4) {
         const int j = base + grp;
         const bool live = j < count;
         int idx = 0, kind = -1, ix = 0, iy = 0, iz = 0;
@@ -785,7 +883,7 @@ __global__ __launch_bounds__(128) void k_prefixup(PartDev pd, PopView<S> pop,
                     macro_uz[idx] = 0.0;
                 }
                 macro_phi[idx] = phi_loc;
-                fx_field[idx] = typename Enc<S>::A(0.0);
+                fx_field[idx] = A(0.0);
                 pd.sp.fx_old[j] = 0.0;
             }
         }
@@ -1011,7 +1109,7 @@ int part_load_state(PartDev &pd, const double *host, const Dims &dims, hipStream
     PHIP(hipMemsetAsync(pd.base, 0, (size_t)(RBTP_P_COUNT + OLD_COUNT) * np * sizeof(double), st));
     PHIP(copy_to_device(pd.base, host, (size_t)RBTP_P_COUNT * np * sizeof(double), st));
     PHIP(hipMemsetAsync(pd.owner[0], 0xFF, (size_t)dims.n * sizeof(int32_t), st));
-    hipLaunchKernelGGL(k_geo_mark, np, PB, 0, st, pd.owner[0], pd.box[0], pd.p[RBTP_P_POS_X],
+    hipLaunchKernelGGL(k_geo_mark_init, np, PB, 0, st, pd.owner[0], pd.box[0], pd.p[RBTP_P_POS_X],
                        pd.p[RBTP_P_POS_Y], pd.p[RBTP_P_POS_Z], pd.radius, pd.radius_sq, dims);
     PHIP(hipGetLastError());
     PHIP(hipMemcpyAsync(pd.owner[1], pd.owner[0], (size_t)dims.n * sizeof(int32_t), hipMemcpyDeviceToDevice, st));
@@ -1049,54 +1147,106 @@ int part_init(PartDev &pd, const rbtp_case &c, const rbtp_derived &d, const Dims
     return part_load_state(pd, host.data(), dims, st);
 }
 
-// One particle phase (main.cu:52-84 minus the lattice passes), enqueued on st.
-int part_phase(PartDev &pd, const rbtp_case &c, const rbtp_derived &d, const Dims &dims,
-               int prec, const void *fcur, const void *gcur, void *fx, void *rho_wall,
-               double *const *macro, hipStream_t st) {
-    (void)c;
-    (void)d;
-    const int np = pd.np;
-    const int prev = pd.geo_cur, curr = 1 - pd.geo_cur;
-    if (pd.use_cells) {
-        const int ncells = pd.ncell[0] * pd.ncell[1] * pd.ncell[2];
-        hipLaunchKernelGGL(k_cell_clear, std::min((ncells + 255) / 256, 1024), 256, 0, st,
-                           pd.cell_cnt, ncells);
-        hipLaunchKernelGGL(k_cell_bin, (np + 255) / 256, 256, 0, st, pd);
+int part_fixup_blocks(const PartDev &pd) {
+    // link nodes lie in a shell about one lattice spacing thick around each sphere
+    const double r = pd.radius + 1.0;
+    const double est = (double)pd.np * (4.0 * 3.141592653589793 * r * r * 2.0) + 64.0;
+    const double capped = std::min(est, (double)pd.sp.cap);
+    return (int)std::min(std::max(std::ceil(capped / 4.0), 1.0), 16384.0);
+}
+
+int part_collide_blocks(const PartDev &pd) {
+    const double r = pd.radius + 1.0;
+    const double est = (double)pd.np * (4.0 * 3.141592653589793 * r * r * 2.0) + 64.0;
+    const double capped = std::min(est, (double)pd.sp.cap);
+    return (int)std::min(std::max(std::ceil(capped / 256.0), 1.0), 4096.0);
+}
+
+void part_tables_free(PartTables &t) {
+    (void)hipFree(t.mem);
+    t.mem = nullptr;
+    t.bytes = 0;
+    t.g[0] = t.g[1] = PartGrid{};
+}
+
+int part_tables_build(PartTables &t, const std::vector<PartSlot> (&slots)[2], hipStream_t st) {
+    // kernels already queued on st may still read the old tables
+    PHIP(hipStreamSynchronize(st));
+    const int n = (int)slots[0].size();
+    const size_t tab_bytes = ((sizeof(PartSlot) * (size_t)(n > 0 ? n : 1) + 255) / 256) * 256;
+    const size_t pre_bytes = ((sizeof(int) * (size_t)(n + 1) + 255) / 256) * 256;
+    const size_t need = 2 * tab_bytes + 4 * pre_bytes;
+    if (t.bytes < need) {
+        (void)hipFree(t.mem);
+        t.mem = nullptr;
+        t.bytes = 0;
+        PHIP(hipMalloc(&t.mem, need));
+        t.bytes = need;
     }
-    hipLaunchKernelGGL(k_contacts, (np + 127) / 128, 128, 0, st, pd, dims);
+    std::vector<int> pre[4];
+    for (auto &v : pre) v.assign(n + 1, 0);
+    for (int j = 0; j < n; j++) {
+        const PartDev &pd = slots[0][j].pd;
+        pre[0][j + 1] = pre[0][j] + pd.np;
+        pre[1][j + 1] = pre[1][j] + (pd.use_cells ? pd.ncell[0] * pd.ncell[1] * pd.ncell[2] : 0);
+        pre[2][j + 1] = pre[2][j] + part_fixup_blocks(pd);
+        pre[3][j + 1] = pre[3][j] + part_collide_blocks(pd);
+    }
+    std::vector<char> host(need, 0);
+    for (int ph = 0; ph < 2; ph++)
+        if (n > 0) std::memcpy(host.data() + ph * tab_bytes, slots[ph].data(), sizeof(PartSlot) * n);
+    for (int k = 0; k < 4; k++)
+        std::memcpy(host.data() + 2 * tab_bytes + k * pre_bytes, pre[k].data(), sizeof(int) * (n + 1));
+    PHIP(hipMemcpyAsync(t.mem, host.data(), need, hipMemcpyHostToDevice, st));
+    PHIP(hipStreamSynchronize(st));
+    for (int ph = 0; ph < 2; ph++) {
+        PartGrid &g = t.g[ph];
+        g.tab = (const PartSlot *)(t.mem + ph * tab_bytes);
+        g.pstart = (const int *)(t.mem + 2 * tab_bytes);
+        g.cstart = (const int *)(t.mem + 2 * tab_bytes + pre_bytes);
+        g.sstart = (const int *)(t.mem + 2 * tab_bytes + 2 * pre_bytes);
+        g.kstart = (const int *)(t.mem + 2 * tab_bytes + 3 * pre_bytes);
+        g.nslots = n;
+        g.total_p = pre[0][n];
+        g.total_c = pre[1][n];
+        g.total_s = pre[2][n];
+        g.total_k = pre[3][n];
+    }
+    return RBTP_OK;
+}
+
+// One particle phase (main.cu:52-84 minus the lattice passes) of every slot of g, enqueued on
+// st: history shift and repulsion, hydrodynamic sums, motion, the new geometry, the special-node
+// list and the special nodes' corrected post-stream populations.
+int part_phase(const PartGrid &g, int prec, hipStream_t st) {
+    if (g.nslots == 0 || g.total_p == 0) return RBTP_OK;
+    const int tp = g.total_p;
+    if (g.total_c > 0) {
+        hipLaunchKernelGGL(k_cell_clear, std::min((g.total_c + 255) / 256, 1024), 256, 0, st, g);
+        PHIP(hipGetLastError());
+    }
+    hipLaunchKernelGGL(k_cell_bin, (tp + 255) / 256, 256, 0, st, g);
+    PHIP(hipGetLastError());
+    hipLaunchKernelGGL(k_contacts, (tp + 127) / 128, 128, 0, st, g);
     PHIP(hipGetLastError());
     PHIP(dispatch_prec(prec, [&](auto *tag) {
         using S = std::remove_pointer_t<decltype(tag)>;
-        const PopView<S> pop{(const S *)fcur, (const S *)gcur, (size_t)dims.n, pd.tl0, pd.tls, dims.lx};
-        hipLaunchKernelGGL((k_link_sums<S>), np, LB, 0, st, pd, pop, pd.owner[prev],
-                           pd.box[prev], macro[0], dims);
+        hipLaunchKernelGGL((k_link_sums<S>), tp, LB, 0, st, g);
         return hipGetLastError();
     }));
-    hipLaunchKernelGGL(k_move, (np + 127) / 128, 128, 0, st, pd);
+    hipLaunchKernelGGL(k_move, (tp + 127) / 128, 128, 0, st, g);
     PHIP(hipGetLastError());
-    hipLaunchKernelGGL(k_geo_clear, np, PB, 0, st, pd.owner[curr], pd.box[curr], dims);
+    hipLaunchKernelGGL(k_geo_clear, tp, PB, 0, st, g);
     PHIP(hipGetLastError());
-    hipLaunchKernelGGL(k_geo_mark, np, PB, 0, st, pd.owner[curr], pd.box[curr], pd.p[RBTP_P_POS_X],
-                       pd.p[RBTP_P_POS_Y], pd.p[RBTP_P_POS_Z], pd.radius, pd.radius_sq, dims);
+    hipLaunchKernelGGL(k_geo_mark, tp, PB, 0, st, g);
     PHIP(hipGetLastError());
-    hipLaunchKernelGGL(k_zero_count, 1, 1, 0, st, pd.sp.count);
-    PHIP(hipGetLastError());
-    hipLaunchKernelGGL(k_special_links, np, LB, 0, st, pd, pd.owner[prev], pd.box[prev], dims);
-    PHIP(hipGetLastError());
-    hipLaunchKernelGGL(k_special_refill, np, LB, 0, st, pd, pd.owner[prev], pd.box[prev],
-                       pd.owner[curr], dims);
+    hipLaunchKernelGGL(k_special_find, tp, LB, 0, st, g);
     PHIP(hipGetLastError());
     PHIP(dispatch_prec(prec, [&](auto *tag) {
         using S = std::remove_pointer_t<decltype(tag)>;
-        using A = typename Enc<S>::A;
-        const PopView<S> pop{(const S *)fcur, (const S *)gcur, (size_t)dims.n, pd.tl0, pd.tls, dims.lx};
-        hipLaunchKernelGGL((k_prefixup<S>), grid_for(pd.sp.cap, 4, 16384), 128, 0, st, pd, pop,
-                           pd.owner[prev], (A *)fx, (const A *)rho_wall, macro[0], macro[1],
-                           macro[2], macro[3], macro[4], dims);
+        hipLaunchKernelGGL((k_prefixup<S>), g.total_s, 128, 0, st, g);
         return hipGetLastError();
     }));
-    pd.geo_cur = curr;
-    pd.special_pending = true;
     return RBTP_OK;
 }
 

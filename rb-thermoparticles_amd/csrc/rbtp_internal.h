@@ -91,6 +91,12 @@ struct SpecialBuf {
     int cap;
 };
 
+// One lattice's re-collision of its special nodes inside a batched launch.
+template <class S> struct SpecialSlot {
+    StepArgs<S> a;
+    SpecialBuf sp;
+};
+
 // Synchronous copies between caller-owned host memory and the device on stream st, staged
 // through library-owned pinned buffers (xfer.hip).
 hipError_t copy_to_device(void *dst, const void *src, size_t bytes, hipStream_t st);

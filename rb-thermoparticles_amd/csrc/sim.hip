@@ -99,6 +99,9 @@ struct rbtp_sim {
     // particles
     PartDev pd;
     bool particles;
+    PartTables ptab;  // this lattice's particle-phase tables (one slot)
+    bool ptab_valid;
+    int ptab_rel;     // cur ^ pd.geo_cur at the particle phase the tables describe
 };
 
 // ---------------------------------------------------------------------------------------------
@@ -206,6 +209,7 @@ extern "C" void rbtp_destroy(rbtp_sim *s) {
     for (double *m : s->macro) (void)hipFree(m);
     (void)hipFree(s->temp_dev);
     part_free(s->pd);
+    part_tables_free(s->ptab);
     if (s->ev_start) (void)hipEventDestroy(s->ev_start);
     if (s->ev_stop) (void)hipEventDestroy(s->ev_stop);
     if (s->ev_k0) (void)hipEventDestroy(s->ev_k0);
@@ -398,19 +402,49 @@ extern "C" int rbtp_init(rbtp_sim *s, const double *temperature, const double *p
 
 // Everything of a time step after the lattice kernel: the special-node re-collision (reads the
 // same StepArgs as the lattice kernel), the buffer flip and the particle phase.
+// The particle phase's view of this lattice when its post-collision populations are in buffer
+// `cur` and the geometry of the step's IBB is generation `geo`.
+static PartSlot part_slot(const rbtp_sim *s, int cur, int geo) {
+    PartSlot sl;
+    sl.pd = s->pd;
+    sl.d = s->dims;
+    sl.owner_prev = s->pd.owner[geo];
+    sl.box_prev = s->pd.box[geo];
+    sl.owner_curr = s->pd.owner[1 - geo];
+    sl.box_curr = s->pd.box[1 - geo];
+    sl.fcur = s->f[cur];
+    sl.gcur = s->g[cur];
+    sl.fx = s->fx;
+    sl.rho_wall = s->rho_wall;
+    for (int k = 0; k < 5; k++) sl.macro[k] = s->macro[k];
+    return sl;
+}
+
 static int after_lattice(rbtp_sim *s, bool macro_out) {
     if (s->particles && s->pd.special_pending) {
         HIP_TRY(dispatch(s->prec, [&](auto *tag) -> hipError_t {
             using S = std::remove_pointer_t<decltype(tag)>;
             const StepArgs<S> a = step_args<S>(s, macro_out);
-            return launch_special_collide<S>(a, s->pd.sp, s->pd.sp.cap, s->strict, s->stream);
+            return launch_special_collide<S>(a, s->pd.sp, part_collide_blocks(s->pd), s->strict,
+                                             s->stream);
         }));
     }
     s->cur = 1 - s->cur;
     s->poststream = false;
     if (s->particles) {
-        RB_TRY(part_phase(s->pd, s->c, s->d, s->dims, s->prec, s->f[s->cur], s->g[s->cur],
-                          s->fx, s->rho_wall, s->macro, s->stream));
+        // cur and geo_cur alternate together, so two tables (one per geometry generation) cover
+        // every step until a reload changes their relation
+        const int rel = s->cur ^ s->pd.geo_cur;
+        if (!s->ptab_valid || rel != s->ptab_rel) {
+            std::vector<PartSlot> v[2];
+            for (int g = 0; g < 2; g++) v[g].push_back(part_slot(s, g ^ rel, g));
+            RB_TRY(part_tables_build(s->ptab, v, s->stream));
+            s->ptab_valid = true;
+            s->ptab_rel = rel;
+        }
+        RB_TRY(part_phase(s->ptab.g[s->pd.geo_cur], s->prec, s->stream));
+        s->pd.geo_cur ^= 1;
+        s->pd.special_pending = true;
     }
     s->step++;
     return RBTP_OK;
@@ -606,7 +640,11 @@ extern "C" int rbtp_set_populations(rbtp_sim *s, const double *f, const double *
     (void)hipFree(dev);
     if (e != hipSuccess) return set_error(RBTP_E_HIP, "rbtp_set_populations: %s", hipGetErrorString(e));
     s->poststream = true;
-    if (s->particles) s->pd.special_pending = false;
+    if (s->particles) {
+        s->pd.special_pending = false;
+        HIP_TRY(hipMemsetAsync(s->pd.sp.count, 0, sizeof(int), s->stream));
+        HIP_TRY(hipStreamSynchronize(s->stream));
+    }
     return RBTP_OK;
 }
 
@@ -718,7 +756,8 @@ extern "C" int rbtp_kinetic_energy(rbtp_sim *s, double *out) {
     if (!s->macro_valid) return set_error(RBTP_E_STATE, "run the last step with RBTP_STEP_MACRO_LAST first");
     HIP_TRY(hipSetDevice(s->device));
     RB_TRY(diag_kinetic_energy(s->dims, s->macro,
-                               s->particles ? s->pd.owner[s->pd.geo_cur] : nullptr, s->stream,
+                               s->particles ? s->pd.owner[s->pd.geo_cur] : nullptr,
+                               s->particles ? s->pd.owner[1 - s->pd.geo_cur] : nullptr, s->stream,
                                out));
     return RBTP_OK;
 }
@@ -940,6 +979,10 @@ struct rbtp_batch {
     int *tile_start_dev;
     std::vector<int> active;
     std::vector<int> cur_at_build;  // each active slot's buffer parity when the tables were built
+    std::vector<int> geo_at_build;  // each active slot's geometry generation then (0 without particles)
+    std::vector<int> part_slots;    // positions in `active` of the slots with particles
+    PartTables ptab;                // particle-phase tables over part_slots, per phase
+    void *spec_dev[4];              // SpecialSlot<S>[max_slots] for (parity, macro)
     int total_tiles;
     int block, grid;
     bool tables_dirty;
@@ -952,6 +995,15 @@ struct rbtp_batch {
     double last_ms;
     int64_t graph_launches;
 };
+
+static size_t special_slot_size(int prec) {
+    switch (prec) {
+        case RBTP_PREC_FP64: return sizeof(SpecialSlot<double>);
+        case RBTP_PREC_FP32: return sizeof(SpecialSlot<float>);
+        case RBTP_PREC_FP16S: return sizeof(SpecialSlot<__half>);
+        default: return sizeof(SpecialSlot<__hip_bfloat16>);
+    }
+}
 
 static size_t args_size(int prec) {
     switch (prec) {
@@ -994,6 +1046,8 @@ extern "C" void rbtp_batch_destroy(rbtp_batch *b) {
     for (rbtp_sim *s : b->slot)
         if (s) rbtp_destroy(s);
     for (void *p : b->args_dev) (void)hipFree(p);
+    for (void *p : b->spec_dev) (void)hipFree(p);
+    part_tables_free(b->ptab);
     (void)hipFree(b->tile_start_dev);
     if (b->ev0) (void)hipEventDestroy(b->ev0);
     if (b->ev1) (void)hipEventDestroy(b->ev1);
@@ -1035,6 +1089,8 @@ extern "C" int rbtp_batch_create(int precision, int max_slots, double hbm_fracti
     if ((e = hipEventCreate(&b->ev1)) != hipSuccess) return fail(e, "hipEventCreate");
     for (void *&p : b->args_dev)
         if ((e = hipMalloc(&p, args_size(precision) * max_slots)) != hipSuccess) return fail(e, "hipMalloc");
+    for (void *&p : b->spec_dev)
+        if ((e = hipMalloc(&p, special_slot_size(precision) * max_slots)) != hipSuccess) return fail(e, "hipMalloc");
     if ((e = hipMalloc(&b->tile_start_dev, sizeof(int) * (max_slots + 1))) != hipSuccess) return fail(e, "hipMalloc");
     *out = b;
     return RBTP_OK;
@@ -1114,11 +1170,22 @@ static int batch_build_tables(rbtp_batch *b) {
     }
     b->total_tiles = tiles[na];
     b->cur_at_build.clear();
-    for (int k : b->active) b->cur_at_build.push_back(b->slot[k]->cur);
+    b->geo_at_build.clear();
+    b->part_slots.clear();
+    for (int j = 0; j < na; j++) {
+        const rbtp_sim *s = b->slot[b->active[j]];
+        b->cur_at_build.push_back(s->cur);
+        b->geo_at_build.push_back(s->particles ? s->pd.geo_cur : 0);
+        if (s->particles) b->part_slots.push_back(j);
+    }
+    // kernels already queued may still read the old tables
+    HIP_TRY(hipStreamSynchronize(b->stream));
     if (na > 0) HIP_TRY(hipMemcpy(b->tile_start_dev, tiles.data(), sizeof(int) * na, hipMemcpyHostToDevice));
+    const int nps = (int)b->part_slots.size();
     HIP_TRY(dispatch(b->prec, [&](auto *tag) -> hipError_t {
         using S = std::remove_pointer_t<decltype(tag)>;
         std::vector<StepArgs<S>> h(na > 0 ? na : 1);
+        std::vector<SpecialSlot<S>> sp(nps > 0 ? nps : 1);
         for (int par = 0; par < 2; par++)
             for (int mac = 0; mac < 2; mac++) {
                 for (int j = 0; j < na; j++) {
@@ -1127,13 +1194,30 @@ static int batch_build_tables(rbtp_batch *b) {
                     h[j] = step_args<S>(s, mac != 0);
                     if (par) s->cur = 1 - s->cur;
                 }
+                for (int q = 0; q < nps; q++) {
+                    sp[q].a = h[b->part_slots[q]];
+                    sp[q].sp = b->slot[b->active[b->part_slots[q]]]->pd.sp;
+                }
                 if (na > 0) {
                     hipError_t e = hipMemcpy(b->args_dev[par * 2 + mac], h.data(), sizeof(StepArgs<S>) * na, hipMemcpyHostToDevice);
+                    if (e != hipSuccess) return e;
+                }
+                if (nps > 0) {
+                    hipError_t e = hipMemcpy(b->spec_dev[par * 2 + mac], sp.data(), sizeof(SpecialSlot<S>) * nps, hipMemcpyHostToDevice);
                     if (e != hipSuccess) return e;
                 }
             }
         return hipSuccess;
     }));
+    // Particle phase of a step taken at batch phase ph: populations in buffer cur ^ ph ^ 1
+    // (after the lattice kernel's flip), IBB geometry geo_cur ^ ph.
+    std::vector<PartSlot> pv[2];
+    for (int ph = 0; ph < 2; ph++)
+        for (int q = 0; q < nps; q++) {
+            const rbtp_sim *s = b->slot[b->active[b->part_slots[q]]];
+            pv[ph].push_back(part_slot(s, s->cur ^ ph ^ 1, s->pd.geo_cur ^ ph));
+        }
+    RB_TRY(part_tables_build(b->ptab, pv, b->stream));
     b->phase = 0;
     b->tables_dirty = false;
     batch_invalidate_graph(b);
@@ -1150,9 +1234,35 @@ static hipError_t batch_lattice(rbtp_batch *b, bool macro, int block, int grid) 
     });
 }
 
+// One step of every slot: the batched lattice kernel, then one launch of each special-node and
+// particle kernel for all slots with particles.  Special nodes are re-collided from the
+// device-side list count, which is zero until a slot's first particle phase.
 static int batch_step_eager(rbtp_batch *b, bool macro) {
     HIP_TRY(batch_lattice(b, macro, b->block, b->grid));
-    for (int k : b->active) RB_TRY(after_lattice(b->slot[k], macro));
+    const int nps = (int)b->part_slots.size();
+    const PartGrid &pg = b->ptab.g[b->phase];
+    if (nps > 0) {
+        HIP_TRY(dispatch(b->prec, [&](auto *tag) {
+            using S = std::remove_pointer_t<decltype(tag)>;
+            return launch_special_collide_batched<S>(
+                (const SpecialSlot<S> *)b->spec_dev[b->phase * 2 + (macro ? 1 : 0)], pg.kstart,
+                nps, pg.total_k, b->strict, b->stream);
+        }));
+    }
+    for (int k : b->active) {
+        rbtp_sim *s = b->slot[k];
+        s->cur = 1 - s->cur;
+        s->poststream = false;
+    }
+    if (nps > 0) {
+        RB_TRY(part_phase(pg, b->prec, b->stream));
+        for (int q : b->part_slots) {
+            rbtp_sim *s = b->slot[b->active[q]];
+            s->pd.geo_cur ^= 1;
+            s->pd.special_pending = true;
+        }
+    }
+    for (int k : b->active) b->slot[k]->step++;
     b->phase ^= 1;
     return RBTP_OK;
 }
@@ -1183,8 +1293,11 @@ extern "C" int rbtp_batch_step(rbtp_batch *b, int nsteps, unsigned flags) {
     if (nsteps < 0) return set_error(RBTP_E_ARG, "nsteps < 0");
     HIP_TRY(hipSetDevice(b->device));
     if (!b->tables_dirty)
-        for (size_t j = 0; j < b->active.size(); j++)
-            if (b->slot[b->active[j]]->cur != (b->cur_at_build[j] ^ b->phase)) b->tables_dirty = true;
+        for (size_t j = 0; j < b->active.size(); j++) {
+            const rbtp_sim *s = b->slot[b->active[j]];
+            if (s->cur != (b->cur_at_build[j] ^ b->phase)) b->tables_dirty = true;
+            if (s->particles && s->pd.geo_cur != (b->geo_at_build[j] ^ b->phase)) b->tables_dirty = true;
+        }
     if (b->tables_dirty) RB_TRY(batch_build_tables(b));
     if (b->active.empty() || nsteps == 0) return RBTP_OK;
     const bool macro_last = (flags & RBTP_STEP_MACRO_LAST) != 0;

@@ -180,11 +180,16 @@ def test_rb3d_driver_outputs_and_restart(tmp_path):
 
 
 def test_kinetic_energy_over_fluid_nodes(rb, O):
-    """rbtp_kinetic_energy is the mean of 0.5|u|^2 over the nodes that are fluid now."""
-    kw = particle_case(24, 22, 20, npart=2, diam=6.0, ra=1e5, beta=0.05)
-    sim, ora = pair(rb, O, kw, ppos=[(8.2, 10.1, 9.7), (15.9, 12.4, 10.3)])
+    """rbtp_kinetic_energy is the mean of 0.5|u|^2 over the nodes that are fluid in the current
+    and in the previous geometry."""
+    kw = particle_case(24, 22, 20, npart=2, diam=6.0, ra=1e5, beta=0.5)
+    sim, ora = pair(rb, O, kw, ppos=[(8.2, 10.1, 9.7), (15.9, 12.4, 10.3)], steps=11)
+    ib_prev, _ = ora.flags()  # geometry at the end of step 11: the previous one after step 12
+    sim.step(1)
+    ora.step(1)
     ib, _ = sim.flags()
-    fluid = ib.reshape(-1) == 0
+    assert np.array_equal(ib, ora.flags()[0])
+    fluid = (ib.reshape(-1) == 0) & (ib_prev.reshape(-1) == 0)
     u = [sim.field(k).reshape(-1)[fluid] for k in ("ux", "uy", "uz")]
     expect = np.mean(0.5 * (u[0] ** 2 + u[1] ** 2 + u[2] ** 2))
     assert sim.kinetic_energy() == pytest.approx(expect, rel=1e-12)
