@@ -157,6 +157,9 @@ def main():
     ap.add_argument("--no-forms", action="store_true", help="skip the FP16-storage form")
     ap.add_argument("--no-particles", action="store_true", help="skip the particle-laden case")
     ap.add_argument("--quiet", action="store_true")
+    ap.add_argument("--batch-probe", type=int, default=0, metavar="K",
+                    help="K eager (not graph-replayed) steps of a 16-request serving batch, "
+                         "for kernel traces; no JSON")
     ap.add_argument("--counter-run", action="store_true",
                     help="minimal run for memory-counter passes: 3 FP64 lattice steps at "
                          "320x320x160 and one 1 GiB copy-probe launch, no JSON")
@@ -176,6 +179,21 @@ def main():
     from cases import config2
     rb = load_rbtp()
     kw64 = config2(precision=0)
+    if args.batch_probe:
+        # eager steps of a 16-request serving batch (FP16 storage, 0-64 spheres each)
+        from rbtp_amd.serve import admit_request, make_requests
+        batch = rb.Batch(precision=rb.PREC_FP16S, max_slots=64)
+        reqs = make_requests(16, mean_gap=0.0, seed=99, max_steps=1000)
+        for r in reqs:
+            admit_request(batch, r)
+        batch.set_launch(256, 16384, 0)
+        batch.step(10)
+        batch.step(args.batch_probe)
+        ms = batch.info()["last_ms"]
+        log(f"batch probe: {len(reqs)} requests, {sum(r.nodes for r in reqs)} nodes, "
+            f"{ms / args.batch_probe:.3f} ms per batched step")
+        batch.close()
+        return
     if args.counter_run:
         case = rb.default_case(**kw64)
         sim = rb.Sim(case)

@@ -22,6 +22,22 @@ struct D3Q27 {
                                       16, 13, 14, 11, 12, 25, 26, 23, 24, 21, 22, 19, 20};
 };
 
+// Storage order of a buffer of 27 populations.  Nodes are taken in runs of POP_RUN consecutive
+// indices; a run keeps population 0 of its nodes, then population 1, ..., then population 26.
+// A workgroup's access to one population is one contiguous piece and the 27 pieces of a run
+// sit together in HBM, instead of 27 streams N values apart; the reference's order (pop[i*N +
+// idx], initial.cu:276-279) is what the accessors and files see.
+constexpr int POP_RUN_LOG2 = 10;
+constexpr int POP_RUN = 1 << POP_RUN_LOG2;
+__host__ __device__ inline size_t pop_at(int i, int idx) {
+    return ((size_t)(idx >> POP_RUN_LOG2) * NPOP + (size_t)i) * POP_RUN +
+           (size_t)(idx & (POP_RUN - 1));
+}
+// Values in one buffer of 27 populations over n nodes (the last run padded).
+__host__ __device__ inline size_t pop_buffer_len(size_t n) {
+    return (size_t)NPOP * ((n + POP_RUN - 1) / POP_RUN) * POP_RUN;
+}
+
 // Weights exactly as the reference evaluates them (8.0/27.0 etc. in double).
 __host__ __device__ constexpr double weight(int i) {
     return i == 0 ? 8.0 / 27.0 : (i <= 6 ? 2.0 / 27.0 : (i <= 18 ? 1.0 / 54.0 : 1.0 / 216.0));

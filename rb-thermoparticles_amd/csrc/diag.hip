@@ -157,7 +157,7 @@ __global__ __launch_bounds__(DB) void k_output_fields(const S *fc, const S *gc,
         const double tp = weight(ip);
         const double eu = (cx * ux_local + cy * uy_local + cz * uz_local) / RT;
         const double feq = tp * (rho_local + eu + 0.5 * (eu * eu - uv));
-        const double fneq = dec_f<S, double>(Enc<S>::ld(fc + (size_t)ip * n + idx), ip) - feq;
+        const double fneq = dec_f<S, double>(Enc<S>::ld(fc + pop_at(ip, idx)), ip) - feq;
         const double cidotF = cx * fx + cy * fy + cz * fz;
         const double feq_term = 0.5 * (cidotF - bu) * feq / RT;
         txx += cx * cx * (fneq + feq_term);
@@ -169,7 +169,7 @@ __global__ __launch_bounds__(DB) void k_output_fields(const S *fc, const S *gc,
         const double geq = tp * phi_local * (1.0 + eu + 0.5 * (eu * eu - uv));
         const double geqnew = (ip == 0) ? (geq - lambda0 * phi_local * rho_local)
                                         : (geq + tp * phi_local * rho_local);
-        const double diff = dec_g<S, double>(Enc<S>::ld(gc + (size_t)ip * n + idx), ip,
+        const double diff = dec_g<S, double>(Enc<S>::ld(gc + pop_at(ip, idx)), ip,
                                              k.tl0 + k.tls * double(idx % d.lx)) - geqnew;
         gneqx += cx * diff;
         gneqy += cy * diff;

@@ -44,7 +44,6 @@ __device__ inline int xcd_block(int nblocks) {
 template <class S> struct NodeOut {
     S *fout;
     S *gout;
-    size_t n;
 };
 
 // x / d, correctly rounded, from a double-word reciprocal yh + yl of d (yh = RN(1/d)).
@@ -86,7 +85,7 @@ __device__ inline double collide_strict(double (&f)[NPOP], double (&g)[NPOP], do
         const double feq_rho0 = tp * X[i];
         const double cuF = ((double)D3Q27::cx[i] - ux) * fx;
         fchange += div_rn(k.force_coeff * cuF, RT, k.rt_h, k.rt_l) * feq_rho0;
-        Enc<S>::st(o.fout + (size_t)i * o.n + idx, fpop + fchange);
+        Enc<S>::st(o.fout + pop_at(i, idx), fpop + fchange);
     }
     // collision_BGK_scalar pass 1 (collision.cu:57-70)
     double gx = 0.0, gy = 0.0, gz = 0.0;
@@ -130,7 +129,7 @@ __device__ inline double collide_strict(double (&f)[NPOP], double (&g)[NPOP], do
             const double Ralph = tp * k.ralph_coeff * src;
             gchange = div_rn(-(gpop - X[i]), k.tauc, k.tauc_h, k.tauc_l) + Ralph;
         }
-        Enc<S>::st(o.gout + (size_t)i * o.n + idx, gpop + gchange);
+        Enc<S>::st(o.gout + pop_at(i, idx), gpop + gchange);
     }
     return fx;
 }
@@ -159,7 +158,7 @@ __device__ inline T collide_fast(T (&f)[NPOP], T (&g)[NPOP], T fx_old, int idx, 
         const T feq = tp * ((rho + eu) + h);
         const T fpop = f[i];
         const T fnew = fpop + (feq - fpop) * inv_tau + (T(D3Q27::cx[i]) - ux) * fc3 * tp * Xi;
-        enc_f<S, T>(o.fout + (size_t)i * o.n + idx, fnew, i);
+        enc_f<S, T>(o.fout + pop_at(i, idx), fnew, i);
         const T geq = tp * phi * Xi;
         const T geqnew = (i == 0) ? (geq - lam) : (geq + tp * phi * rho);
         f[i] = geqnew;  // reuse the register for g's equilibrium
@@ -185,21 +184,17 @@ __device__ inline T collide_fast(T (&f)[NPOP], T (&g)[NPOP], T fx_old, int idx, 
             const T gradt = -(T(2) * gneq + eb * phi) * inv_gd;
             gnew += tp * rc * (gradt * rho + T(3) * phi * eb);
         }
-        enc_g<S, T>(o.gout + (size_t)i * o.n + idx, gnew, i, gshift);
+        enc_g<S, T>(o.gout + pop_at(i, idx), gnew, i, gshift);
     }
     return fx;
 }
-
-template <class T> struct Arith;
-template <> struct Arith<double> { static constexpr bool strict_ok = true; };
-template <> struct Arith<float> { static constexpr bool strict_ok = false; };
 
 template <class S, class T, bool STRICT>
 __device__ inline T collide_node(T (&f)[NPOP], T (&g)[NPOP], T fx_old, int idx, T gshift,
                                  const NodeOut<S> &o, const FluidK &k, T &rho, T &ux, T &uy,
                                  T &uz, T &phi) {
     if constexpr (STRICT) {
-        static_assert(Arith<T>::strict_ok, "strict form needs FP64 arithmetic");
+        static_assert(std::is_same<T, double>::value, "strict form needs FP64 arithmetic");
         (void)gshift;
         return collide_strict<S>(f, g, fx_old, idx, o, k, rho, ux, uy, uz, phi);
     } else {
@@ -239,10 +234,10 @@ __device__ inline void node_step(const StepArgs<S> &a, int idx) {
     }
     T f[NPOP], g[NPOP];
     const T tl0 = T(a.k.tl0), tls = T(a.k.tls);
-    gather<S, T, NOSTREAM>(a.fin, a.gin, (size_t)a.d.n, idx, ix, iy, iz, a.d, rho_w, tl0, tls,
+    gather<S, T, NOSTREAM>(a.fin, a.gin, idx, ix, iy, iz, a.d, rho_w, tl0, tls,
                            T(a.k.thot), T(a.k.tcold), f, g);
     const T fx_old = T(a.fx[idx]);
-    NodeOut<S> o{a.fout, a.gout, (size_t)a.d.n};
+    NodeOut<S> o{a.fout, a.gout};
     T rho, ux, uy, uz, phi;
     const T fx = collide_node<S, T, STRICT>(f, g, fx_old, idx, tl0 + tls * T(ix), o, a.k, rho,
                                             ux, uy, uz, phi);
@@ -299,10 +294,10 @@ __global__ __launch_bounds__(BLOCK) void k_stream_only(const S *__restrict__ fin
     else if (ix == d.lx - 1) rho_w = T(rho_wall[d.ly * d.lz + iz * d.ly + iy]);
     T f[NPOP], g[NPOP];
     if (nostream)
-        gather<S, T, true>(fin, gin, (size_t)d.n, idx, ix, iy, iz, d, rho_w, T(tl0), T(tls),
+        gather<S, T, true>(fin, gin, idx, ix, iy, iz, d, rho_w, T(tl0), T(tls),
                            T(thot), T(tcold), f, g);
     else
-        gather<S, T, false>(fin, gin, (size_t)d.n, idx, ix, iy, iz, d, rho_w, T(tl0), T(tls),
+        gather<S, T, false>(fin, gin, idx, ix, iy, iz, d, rho_w, T(tl0), T(tls),
                             T(thot), T(tcold), f, g);
 #pragma unroll
     for (int i = 0; i < NPOP; i++) {
@@ -321,7 +316,7 @@ __global__ __launch_bounds__(BLOCK) void k_decode(const S *__restrict__ in, doub
     using T = typename std::conditional<std::is_same<S, double>::value, double, float>::type;
     // weight(i) with a runtime index: evaluate the same constant expression.
     const double w = i == 0 ? 8.0 / 27.0 : (i <= 6 ? 2.0 / 27.0 : (i <= 18 ? 1.0 / 54.0 : 1.0 / 216.0));
-    const T s = T(Enc<S>::ld(in + j));
+    const T s = T(Enc<S>::ld(in + pop_at(i, (int)(j % n))));
     if constexpr (Enc<S>::scaled) {
         const T shift = T(tl0 + tls * double((j % n) % (size_t)lx));
         out[j] = is_g ? double((s + shift) * T(w)) : double(s * T(w));
@@ -341,9 +336,9 @@ __global__ __launch_bounds__(BLOCK) void k_encode(const double *__restrict__ in,
     const double v = in[j];
     if constexpr (Enc<S>::scaled) {
         const double shift = tl0 + tls * double((j % n) % (size_t)lx);
-        Enc<S>::st(out + j, is_g ? float(v * (1.0 / w) - shift) : float(v * (1.0 / w)));
+        Enc<S>::st(out + pop_at(i, (int)(j % n)), is_g ? float(v * (1.0 / w) - shift) : float(v * (1.0 / w)));
     } else {
-        Enc<S>::st(out + j, (S)v);
+        Enc<S>::st(out + pop_at(i, (int)(j % n)), (S)v);
     }
 }
 
@@ -365,7 +360,6 @@ __global__ __launch_bounds__(BLOCK) void k_init(S *f, S *g, typename Enc<S>::A *
     const int rem = idx - iz * d.lxy;
     const int iy = rem / d.lx;
     const int ix = rem - iy * d.lx;
-    const size_t n = (size_t)d.n;
     // init_f: r = u = v = w = 0, so eu = 0 and feq = w_i (0 + 0 + 0.5 (0 - 0)) = 0.
     const double RT = 1.0 / 3.0;
     const double uv = (0.0 * 0.0 + 0.0 * 0.0 + 0.0 * 0.0) / RT;
@@ -373,7 +367,7 @@ __global__ __launch_bounds__(BLOCK) void k_init(S *f, S *g, typename Enc<S>::A *
     for (int i = 0; i < NPOP; i++) {
         const double eu = 0.0 / RT;
         const double feq = weight(i) * (0.0 + eu + 0.5 * (eu * eu - uv));
-        enc_f<S, double>(f + (size_t)i * n + idx, feq, i);
+        enc_f<S, double>(f + pop_at(i, idx), feq, i);
     }
     // init_g with r = rho = RHO0 (written by init_f, initial.cu:350) and u = 0.
     const double r = rho0;
@@ -385,7 +379,7 @@ __global__ __launch_bounds__(BLOCK) void k_init(S *f, S *g, typename Enc<S>::A *
         double geq = weight(i) * tempT * (1.0 + eu + 0.5 * (eu * eu - uv));
         if (i == 0) geq -= lambda0 * tempT * r;
         else geq += weight(i) * tempT * r;
-        enc_g<S, double>(g + (size_t)i * n + idx, geq, i, tl0 + tls * double(ix));
+        enc_g<S, double>(g + pop_at(i, idx), geq, i, tl0 + tls * double(ix));
     }
     using A = typename Enc<S>::A;
     fx_field[idx] = A(grav0 * beta * (tempT - t0));
@@ -416,7 +410,7 @@ __device__ inline void special_collide_node(const StepArgs<S> &a, const SpecialB
         f[i] = T(sp.pop[(size_t)i * cap + j]);
         g[i] = T(sp.pop[(size_t)(NPOP + i) * cap + j]);
     }
-    NodeOut<S> o{a.fout, a.gout, (size_t)a.d.n};
+    NodeOut<S> o{a.fout, a.gout};
     T rho, ux, uy, uz, phi;
     const T fx = collide_node<S, T, STRICT>(f, g, T(sp.fx_old[j]), idx,
                                             T(a.k.tl0) + T(a.k.tls) * T(ix), o, a.k, rho, ux,

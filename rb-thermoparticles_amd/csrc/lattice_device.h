@@ -34,17 +34,17 @@ __device__ inline void enc_g(S *p, T v, int i, T shift) {
 //   g: as f inside; outside, the z then y walls reflect (adiabatic) and the x walls apply
 //      -g*_opp(x) + 2 w_i T_wall (1 + rho(x)) (streaming.cu:37-61).
 template <class S, class T, bool NOSTREAM>
-__device__ inline void gather(const S *__restrict__ fin, const S *__restrict__ gin, size_t n,
-                              int idx, int ix, int iy, int iz, const Dims &d, T rho_w, T tl0,
-                              T tls, T thot, T tcold, T (&f)[NPOP], T (&g)[NPOP]) {
+__device__ inline void gather(const S *__restrict__ fin, const S *__restrict__ gin, int idx,
+                              int ix, int iy, int iz, const Dims &d, T rho_w, T tl0, T tls,
+                              T thot, T tcold, T (&f)[NPOP], T (&g)[NPOP]) {
     const T own_shift = tl0 + tls * T(ix);
 #pragma unroll
     for (int i = 0; i < NPOP; i++) {
         const int cx = D3Q27::cx[i], cy = D3Q27::cy[i], cz = D3Q27::cz[i];
         const int op = D3Q27::opp[i];
         if (NOSTREAM) {
-            f[i] = dec_f<S, T>(Enc<S>::ld(fin + (size_t)i * n + idx), i);
-            g[i] = dec_g<S, T>(Enc<S>::ld(gin + (size_t)i * n + idx), i, own_shift);
+            f[i] = dec_f<S, T>(Enc<S>::ld(fin + pop_at(i, idx)), i);
+            g[i] = dec_g<S, T>(Enc<S>::ld(gin + pop_at(i, idx)), i, own_shift);
             continue;
         }
         const bool outx = (cx == 1 && ix == 0) || (cx == -1 && ix == d.lx - 1);
@@ -52,7 +52,7 @@ __device__ inline void gather(const S *__restrict__ fin, const S *__restrict__ g
         const bool outz = (cz == 1 && iz == 0) || (cz == -1 && iz == d.lz - 1);
         const bool inside = !(outx || outy || outz);
         const int off = cx + cy * d.lx + cz * d.lxy;
-        const size_t src = inside ? (size_t)i * n + (size_t)(idx - off) : (size_t)op * n + idx;
+        const size_t src = inside ? pop_at(i, idx - off) : pop_at(op, idx);
         const T fv = dec_f<S, T>(Enc<S>::ld(fin + src), i);
         const T gv = dec_g<S, T>(Enc<S>::ld(gin + src), i,
                                  inside ? tl0 + tls * T(ix - cx) : own_shift);

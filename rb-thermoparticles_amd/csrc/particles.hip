@@ -386,10 +386,10 @@ struct PopView {
     double tl0, tls;  // shift of the 16-bit g encoding (conduction profile)
     int lx;
     __device__ inline double F(int i, int idx) const {
-        return dec_f<S, double>(Enc<S>::ld(f + (size_t)i * n + idx), i);
+        return dec_f<S, double>(Enc<S>::ld(f + pop_at(i, idx)), i);
     }
     __device__ inline double G(int i, int idx) const {
-        return dec_g<S, double>(Enc<S>::ld(g + (size_t)i * n + idx), i,
+        return dec_g<S, double>(Enc<S>::ld(g + pop_at(i, idx)), i,
                                 tl0 + tls * double(idx % lx));
     }
 };
@@ -400,22 +400,49 @@ struct LinkOut {
     bool unhandled_f, unhandled_g;
 };
 
-// bounce_back_particles_kernel (ibb.cu:19-81) and bounce_back_thermal_kernel (ibb.cu:94-135)
-// for one link, reading the post-collision populations; returns the values they write into
-// f[opp][idx] and g[opp][idx] plus what the force and heat sums read (ibb.cu:160-180,202-203).
+// Everything the bounce-back of link (x, ip) reads from the lattice and the owner map, loaded
+// together so that the loads of several links can be in flight at once.  Every address is valid
+// whichever branch eval_link_at takes: the upstream node x - c_ip stands in as x when it is
+// outside the domain, and the solid node x + c_ip is clamped into it.
+struct LinkLoads {
+    int up_owner, here_owner;
+    double f_here, f_up, f_opp, g_here, g_up, g_opp, g_solid;
+};
+
 template <class S>
-__device__ inline LinkOut eval_link(int ip, int ix, int iy, int iz, double q, int pid,
-                                    const PView &pv, const PopView<S> &pop,
-                                    const int32_t *owner_prev, double density, double rho0,
-                                    const Dims &d) {
-    LinkOut o;
+__device__ inline LinkLoads link_loads(int ip, int ix, int iy, int iz, const PopView<S> &pop,
+                                       const int32_t *owner_prev, const Dims &d) {
+    LinkLoads L;
     const int opp = c_opp[ip];
     const int idx = iz * d.lxy + iy * d.lx + ix;
     const int ix1 = ix - c_cx[ip], iy1 = iy - c_cy[ip], iz1 = iz - c_cz[ip];
+    const int idx1 = in_domain(d, ix1, iy1, iz1) ? iz1 * d.lxy + iy1 * d.lx + ix1 : idx;
+    const int sx = min(max(ix + c_cx[ip], 0), d.lx - 1);
+    const int sy = min(max(iy + c_cy[ip], 0), d.ly - 1);
+    const int sz = min(max(iz + c_cz[ip], 0), d.lz - 1);
+    L.up_owner = owner_prev[idx1];
+    L.here_owner = owner_prev[idx];
+    L.f_here = pop.F(ip, idx);
+    L.f_up = pop.F(ip, idx1);
+    L.f_opp = pop.F(opp, idx);
+    L.g_here = pop.G(ip, idx);
+    L.g_up = pop.G(ip, idx1);
+    L.g_opp = pop.G(opp, idx);
+    L.g_solid = pop.G(opp, sz * d.lxy + sy * d.lx + sx);
+    return L;
+}
+
+// bounce_back_particles_kernel (ibb.cu:19-81) and bounce_back_thermal_kernel (ibb.cu:94-135)
+// for one link, from the post-collision populations in L; returns the values they write into
+// f[opp][idx] and g[opp][idx] plus what the force and heat sums read (ibb.cu:160-180,202-203).
+__device__ inline LinkOut eval_link_at(const LinkLoads &L, int ip, int ix, int iy, int iz,
+                                       double q, int pid, const PView &pv, double density,
+                                       double rho0, const Dims &d) {
+    LinkOut o;
+    const int ix1 = ix - c_cx[ip], iy1 = iy - c_cy[ip], iz1 = iz - c_cz[ip];
     const bool up_in = in_domain(d, ix1, iy1, iz1);
-    const int idx1 = up_in ? iz1 * d.lxy + iy1 * d.lx + ix1 : idx;
-    const bool up_fluid = up_in && owner_prev[idx1] == -1;
-    const bool here_fluid = owner_prev[idx] == -1;
+    const bool up_fluid = up_in && L.up_owner == -1;
+    const bool here_fluid = L.here_owner == -1;
     const double tp = wt(ip);
     // wall velocity at the boundary point (ibb.cu:33-45)
     const double cx = static_cast<double>(ix) + 0.5 + q * c_cx[ip];
@@ -431,15 +458,15 @@ __device__ inline LinkOut eval_link(int ip, int ix, int iy, int iz, double q, in
     const double evel = c_cx[ip] * uwx + c_cy[ip] * uwy + c_cz[ip] * uwz;
     const double RT = 1.0 / 3.0;
     const double momentum_term = 2.0 * tp * rho0 * evel / RT;
-    const double f_here = pop.F(ip, idx);
+    const double f_here = L.f_here;
     o.f_in = f_here;
     o.unhandled_f = false;
     if (up_fluid) {
         if (q < 0.5) {
-            const double f_upstream = pop.F(ip, idx1);
+            const double f_upstream = L.f_up;
             o.f_out = 2.0 * q * f_here + (1.0 - 2.0 * q) * f_upstream - momentum_term;
         } else {
-            const double f_opposite_collide = pop.F(opp, idx);
+            const double f_opposite_collide = L.f_opp;
             const double inv_2q = 0.5 / q;
             o.f_out = (f_here - momentum_term) * inv_2q + (2.0 * q - 1.0) * inv_2q * f_opposite_collide;
         }
@@ -452,15 +479,15 @@ __device__ inline LinkOut eval_link(int ip, int ix, int iy, int iz, double q, in
     // thermal (ibb.cu:109-135)
     const double Tp = pv.temp[pid];
     const double amp = 2.0 * tp * Tp * (1.0 + density);
-    const double g_here = pop.G(ip, idx);
+    const double g_here = L.g_here;
     o.g_in = g_here;
     o.unhandled_g = false;
     if (up_fluid) {
         if (q <= 0.5) {
-            const double g_upstream = pop.G(ip, idx1);
+            const double g_upstream = L.g_up;
             o.g_out = -(2.0 * q * g_here + (1.0 - 2.0 * q) * g_upstream) + amp;
         } else {
-            const double g_opp_here = pop.G(opp, idx);
+            const double g_opp_here = L.g_opp;
             const double inv_2q = 0.5 / q;
             o.g_out = -g_here * inv_2q + amp * inv_2q + (2.0 * q - 1.0) * inv_2q * g_opp_here;
         }
@@ -468,10 +495,19 @@ __device__ inline LinkOut eval_link(int ip, int ix, int iy, int iz, double q, in
         o.g_out = -g_here + amp;
     } else {
         // No write: g[opp][idx] keeps its streamed value, g*_opp(x + c_ip).
-        o.g_out = pop.G(opp, (iz + c_cz[ip]) * d.lxy + (iy + c_cy[ip]) * d.lx + (ix + c_cx[ip]));
+        o.g_out = L.g_solid;
         o.unhandled_g = true;
     }
     return o;
+}
+
+template <class S>
+__device__ inline LinkOut eval_link(int ip, int ix, int iy, int iz, double q, int pid,
+                                    const PView &pv, const PopView<S> &pop,
+                                    const int32_t *owner_prev, double density, double rho0,
+                                    const Dims &d) {
+    return eval_link_at(link_loads<S>(ip, ix, iy, iz, pop, owner_prev, d), ip, ix, iy, iz, q,
+                        pid, pv, density, rho0, d);
 }
 
 // Hydrodynamic force, torque and heat of every particle (compute_particle_forces_kernel,
@@ -512,12 +548,23 @@ __global__ __launch_bounds__(LB) void k_link_sums(PartGrid g) {
             const double z0 = static_cast<double>(iz) + 0.5;
             int own[NPOP];
             neighbour_owners(owner_prev, d, ix, iy, iz, own);
+            // links of this node to p, as a direction mask (q is evaluated again below)
+            unsigned mask = 0u;
             for (int ip = 1; ip < NPOP; ip++) {
-                if (own[ip] != p) continue;
                 double q = 0.0;
-                if (!link_q(ip, x0, y0, z0, X, Y, Z, pd.radius_sq, q)) continue;
-                const LinkOut L = eval_link<S>(ip, ix, iy, iz, q, p, pv, pop, owner_prev,
-                                               rho[idx], pd.rho0, d);
+                if (own[ip] == p && link_q(ip, x0, y0, z0, X, Y, Z, pd.radius_sq, q))
+                    mask |= 1u << ip;
+            }
+            if (mask == 0u) continue;
+            const double density = rho[idx];
+            // links in ascending direction order; each link's lattice loads are issued together
+            while (mask) {
+                const int ip = __builtin_ctz(mask);
+                mask &= mask - 1u;
+                const LinkLoads Ld = link_loads<S>(ip, ix, iy, iz, pop, owner_prev, d);
+                double q = 0.0;
+                link_q(ip, x0, y0, z0, X, Y, Z, pd.radius_sq, q);
+                const LinkOut L = eval_link_at(Ld, ip, ix, iy, iz, q, p, pv, density, pd.rho0, d);
                 nunh_f += L.unhandled_f;
                 nunh_g += L.unhandled_g;
                 const int opp = c_opp[ip];

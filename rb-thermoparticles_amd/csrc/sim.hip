@@ -260,7 +260,7 @@ static int create_impl(const rbtp_case *c, rbtp_sim *s, hipStream_t stream) {
     HIP_TRY(hipEventCreate(&s->ev_k0));
     HIP_TRY(hipEventCreate(&s->ev_k1));
     const size_t n = (size_t)s->dims.n;
-    const size_t pop_bytes = 27 * n * storage_bytes(s->prec);
+    const size_t pop_bytes = pop_buffer_len(n) * storage_bytes(s->prec);
     for (int b = 0; b < 2; b++) {
         HIP_TRY(hipMalloc(&s->f[b], pop_bytes));
         HIP_TRY(hipMalloc(&s->g[b], pop_bytes));
@@ -1017,7 +1017,7 @@ static size_t args_size(int prec) {
 // Device bytes one slot allocates (create_impl, part_alloc).
 static size_t sim_device_bytes(const rbtp_case &c, const rbtp_derived &d) {
     const size_t n = (size_t)c.lx * c.ly * c.lz;
-    size_t b = 4 * 27 * n * storage_bytes(c.precision) + n * aux_bytes(c.precision) +
+    size_t b = 4 * pop_buffer_len(n) * storage_bytes(c.precision) + n * aux_bytes(c.precision) +
                2 * (size_t)c.ly * c.lz * aux_bytes(c.precision) + 5 * n * sizeof(double);
     if (c.activate_particles && d.npart > 0) {
         const size_t np = (size_t)d.npart;
