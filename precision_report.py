@@ -65,7 +65,7 @@ def run(args):
         ppos = np.concatenate([sim.particles("pos_x"), sim.particles("pos_y"), sim.particles("pos_z")])
         sim.set_particles("temp", local_particle_temperature(T, ppos, case.lx, case.ly, case.lz))
     sim.synchronize()
-    samples, ke, steps_at = [], [], []
+    samples, ke, steps_at, traj = [], [], [], []
     dev_ms = 0.0
     wall0 = time.perf_counter()
     done = 0
@@ -78,6 +78,8 @@ def run(args):
         samples.append(nu)
         ke.append(sim.kinetic_energy())
         steps_at.append(done)
+        if sim.np:
+            traj.append(np.stack([sim.particles(k) for k in ("pos_x", "pos_y", "pos_z", "temp")]))
         print(f"[{args.workload} {args.precision}] step {done}: Nu_hot {nu[4]:.5f} Nu_bulk "
               f"{nu[2]:.5f} KE {ke[-1]:.4e} ({dev_ms / done:.3f} ms/step)", flush=True)
         if not np.all(np.isfinite(nu)):
@@ -88,6 +90,7 @@ def run(args):
                                                      "vel_z", "temp")}
     nlinks = len(sim.links()) if sim.np else 0
     np.savez(args.out, nu=np.array(samples), ke=np.array(ke), steps_at=np.array(steps_at),
+             traj=np.array(traj) if traj else np.zeros((0, 4, 0)),
              dev_ms=dev_ms, wall_s=wall, n=sim.n, npart=sim.np, links=nlinks, precision=prec,
              workload=args.workload, **parts)
     print(f"[{args.workload} {args.precision}] {done} steps, {sim.n * done / (dev_ms * 1e-3) / 1e6:.0f} "
@@ -116,8 +119,9 @@ def compare(files):
             lattice_gbs_at_algorithmic_bytes=mlups * BYTES_PER_NODE[prec] / 1e3,
             mlups_vs_fp64=mlups / (float(ref["n"]) * int(ref["steps_at"][-1]) /
                                    (float(ref["dev_ms"]) * 1e-3) / 1e6),
-            nu_hot_mean=float(nu_mean[4]), nu_cold_mean=float(nu_mean[5]), nu_bulk_mean=float(nu_mean[2]),
+            nu_hot_mean=float(nu_mean[4]), nu_cold_mean=float(nu_mean[3]), nu_bulk_mean=float(nu_mean[2]),
             nu_hot_rel_err=float(abs(nu_mean[4] - nu_ref[4]) / abs(nu_ref[4])),
+            nu_cold_rel_err=float(abs(nu_mean[3] - nu_ref[3]) / abs(nu_ref[3])),
             nu_bulk_rel_err=float(abs(nu_mean[2] - nu_ref[2]) / max(abs(nu_ref[2]), 1e-300)),
             ke_final=float(d["ke"][-1]),
             ke_rel_err=float(abs(d["ke"][-1] - ref["ke"][-1]) / abs(ref["ke"][-1])),
@@ -126,6 +130,13 @@ def compare(files):
             particle_rms_temp_err=float(np.sqrt(np.mean((d["temp"] - ref["temp"]) ** 2))),
             particle_temp_spread=float(ref["temp"].std()),
             npart=int(d["npart"]), links=int(d["links"]))
+        if d.get("traj") is not None and ref.get("traj") is not None and d["traj"].size and ref["traj"].size:
+            m = min(len(d["traj"]), len(ref["traj"]))
+            dp = d["traj"][:m, :3] - ref["traj"][:m, :3]
+            row["particle_rms_pos_err_by_sample"] = [float(x) for x in np.sqrt((dp ** 2).sum(axis=1).mean(axis=1))]
+            row["particle_rms_temp_err_by_sample"] = [
+                float(x) for x in np.sqrt(((d["traj"][:m, 3] - ref["traj"][:m, 3]) ** 2).mean(axis=1))]
+            row["sample_steps"] = [int(x) for x in d["steps_at"][:m]]
         out.append(row)
         print(json.dumps(row), flush=True)
     return out

@@ -103,7 +103,7 @@ __global__ __launch_bounds__(DB) void k_means(const double *ux, const double *uy
 // Kinetic energy over the nodes that are fluid in both the current and the previous geometry
 // (every node without particles): per-block partial sums of 0.5|u|^2 and of the node count.
 // Nodes refilled this step are left out: refill_new_fluid_kernel sets their u to momentum over
-// the density deviation sum(f) (refill.cu:224-229), which is not a velocity.
+// the density deviation sum(f) (refill.cu:224-233), which is not a velocity.
 __global__ __launch_bounds__(DB) void k_ke(const double *ux, const double *uy, const double *uz,
                                            const int32_t *owner, const int32_t *owner_prev,
                                            Dims d, double *out) {

@@ -14,16 +14,6 @@ template <class S>
 hipError_t launch_lattice_batched(const StepArgs<S> *slots_dev, const int *tile_start_dev,
                                   int nslots, int total_tiles, int block, int grid, bool strict,
                                   hipStream_t st);
-// Re-collision of the special nodes (grid-stride over the device-side count with `blocks`
-// workgroups of 256), for one lattice or for the lattices of a batch (slot s owns workgroups
-// kstart[s] .. kstart[s+1]-1 of total_blocks).
-template <class S>
-hipError_t launch_special_collide(const StepArgs<S> &a, const SpecialBuf &sp, int blocks,
-                                  bool strict, hipStream_t st);
-template <class S>
-hipError_t launch_special_collide_batched(const SpecialSlot<S> *tab, const int *kstart,
-                                          int nslots, int total_blocks, bool strict,
-                                          hipStream_t st);
 template <class S>
 hipError_t launch_stream_only(const S *fin, const S *gin, const typename Enc<S>::A *rho_wall,
                               double *fout, double *gout, Dims d, double tl0, double tls,

@@ -234,9 +234,22 @@ __device__ inline void node_step(const StepArgs<S> &a, int idx) {
     }
     T f[NPOP], g[NPOP];
     const T tl0 = T(a.k.tl0), tls = T(a.k.tls);
-    gather<S, T, NOSTREAM>(a.fin, a.gin, idx, ix, iy, iz, a.d, rho_w, tl0, tls,
-                           T(a.k.thot), T(a.k.tcold), f, g);
-    const T fx_old = T(a.fx[idx]);
+    // A special node (IBB link node or refilled node) takes the post-stream populations the
+    // particle phase corrected; the first step after a (re)load has none.
+    const int sj = (!NOSTREAM && a.sp_map) ? a.sp_map[idx] : -1;
+    T fx_old;
+    if (sj >= 0) {
+#pragma unroll
+        for (int i = 0; i < NPOP; i++) {
+            f[i] = T(a.sp_pop[(size_t)i * a.sp_cap + sj]);
+            g[i] = T(a.sp_pop[(size_t)(NPOP + i) * a.sp_cap + sj]);
+        }
+        fx_old = T(a.sp_fx_old[sj]);
+    } else {
+        gather<S, T, NOSTREAM>(a.fin, a.gin, idx, ix, iy, iz, a.d, rho_w, tl0, tls,
+                               T(a.k.thot), T(a.k.tcold), f, g);
+        fx_old = T(a.fx[idx]);
+    }
     NodeOut<S> o{a.fout, a.gout};
     T rho, ux, uy, uz, phi;
     const T fx = collide_node<S, T, STRICT>(f, g, fx_old, idx, tl0 + tls * T(ix), o, a.k, rho,
@@ -394,55 +407,6 @@ __global__ __launch_bounds__(BLOCK) void k_init(S *f, S *g, typename Enc<S>::A *
     }
 }
 
-// Re-collide the special nodes from their buffered post-stream populations (IBB- and
-// refill-corrected), overwriting what the lattice kernel wrote there.
-template <class S, class T, bool STRICT>
-__device__ inline void special_collide_node(const StepArgs<S> &a, const SpecialBuf &sp, int j) {
-    const int idx = sp.node[j];
-    const int iz = idx / a.d.lxy;
-    const int rem = idx - iz * a.d.lxy;
-    const int iy = rem / a.d.lx;
-    const int ix = rem - iy * a.d.lx;
-    T f[NPOP], g[NPOP];
-    const size_t cap = (size_t)sp.cap;
-#pragma unroll
-    for (int i = 0; i < NPOP; i++) {
-        f[i] = T(sp.pop[(size_t)i * cap + j]);
-        g[i] = T(sp.pop[(size_t)(NPOP + i) * cap + j]);
-    }
-    NodeOut<S> o{a.fout, a.gout};
-    T rho, ux, uy, uz, phi;
-    const T fx = collide_node<S, T, STRICT>(f, g, T(sp.fx_old[j]), idx,
-                                            T(a.k.tl0) + T(a.k.tls) * T(ix), o, a.k, rho, ux,
-                                            uy, uz, phi);
-    write_node_aux<S, T>(a, idx, ix, iy, iz, fx, rho, ux, uy, uz, phi);
-}
-
-template <class S, class T, bool STRICT>
-__global__ __launch_bounds__(BLOCK) void k_special_collide(const StepArgs<S> a, SpecialBuf sp) {
-    const int count = min(*sp.count, sp.cap);
-    for (int j = blockIdx.x * BLOCK + threadIdx.x; j < count; j += gridDim.x * BLOCK)
-        special_collide_node<S, T, STRICT>(a, sp, j);
-}
-
-// The same over the lattices of a batch: slot s owns workgroups kstart[s] .. kstart[s+1]-1.
-template <class S, class T, bool STRICT>
-__global__ __launch_bounds__(BLOCK) void k_special_collide_batched(const SpecialSlot<S> *tab,
-                                                                   const int *kstart, int nslots) {
-    int lo = 0, hi = nslots - 1;
-    const int u = blockIdx.x;
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (kstart[mid] <= u) lo = mid;
-        else hi = mid - 1;
-    }
-    const SpecialSlot<S> &sl = tab[lo];
-    const int blk = u - kstart[lo], nblk = kstart[lo + 1] - kstart[lo];
-    const int count = min(*sl.sp.count, sl.sp.cap);
-    for (int j = blk * BLOCK + threadIdx.x; j < count; j += nblk * BLOCK)
-        special_collide_node<S, T, STRICT>(sl.a, sl.sp, j);
-}
-
 // forcing (macrovar.cu:36-47) on a loaded phi, and the wall rho from a loaded rho field
 // (restart, initial.cu:206-209).
 template <class S>
@@ -481,40 +445,6 @@ hipError_t launch_lattice_step(const StepArgs<S> &a, bool strict, bool nostream,
         (void)strict;
         if (nostream) hipLaunchKernelGGL((k_lattice_step<S, float, false, true>), nb, BLOCK, 0, st, a);
         else hipLaunchKernelGGL((k_lattice_step<S, float, false, false>), nb, BLOCK, 0, st, a);
-    }
-    return hipGetLastError();
-}
-
-template <class S>
-hipError_t launch_special_collide(const StepArgs<S> &a, const SpecialBuf &sp, int blocks,
-                                  bool strict, hipStream_t st) {
-    if (blocks <= 0) return hipSuccess;
-    if constexpr (std::is_same<S, double>::value) {
-        if (strict) hipLaunchKernelGGL((k_special_collide<S, double, true>), blocks, BLOCK, 0, st, a, sp);
-        else hipLaunchKernelGGL((k_special_collide<S, double, false>), blocks, BLOCK, 0, st, a, sp);
-    } else {
-        (void)strict;
-        hipLaunchKernelGGL((k_special_collide<S, float, false>), blocks, BLOCK, 0, st, a, sp);
-    }
-    return hipGetLastError();
-}
-
-template <class S>
-hipError_t launch_special_collide_batched(const SpecialSlot<S> *tab, const int *kstart,
-                                          int nslots, int total_blocks, bool strict,
-                                          hipStream_t st) {
-    if (nslots <= 0 || total_blocks <= 0) return hipSuccess;
-    if constexpr (std::is_same<S, double>::value) {
-        if (strict)
-            hipLaunchKernelGGL((k_special_collide_batched<S, double, true>), total_blocks, BLOCK, 0,
-                               st, tab, kstart, nslots);
-        else
-            hipLaunchKernelGGL((k_special_collide_batched<S, double, false>), total_blocks, BLOCK, 0,
-                               st, tab, kstart, nslots);
-    } else {
-        (void)strict;
-        hipLaunchKernelGGL((k_special_collide_batched<S, float, false>), total_blocks, BLOCK, 0,
-                           st, tab, kstart, nslots);
     }
     return hipGetLastError();
 }
@@ -599,10 +529,6 @@ hipError_t launch_lattice_batched(const StepArgs<S> *slots_dev, const int *tile_
     template hipError_t launch_lattice_step<S>(const StepArgs<S> &, bool, bool, hipStream_t);   \
     template hipError_t launch_lattice_batched<S>(const StepArgs<S> *, const int *, int, int, int, \
                                                   int, bool, hipStream_t);                        \
-    template hipError_t launch_special_collide<S>(const StepArgs<S> &, const SpecialBuf &, int, \
-                                                  bool, hipStream_t);                           \
-    template hipError_t launch_special_collide_batched<S>(const SpecialSlot<S> *, const int *,  \
-                                                          int, int, bool, hipStream_t);         \
     template hipError_t launch_stream_only<S>(const S *, const S *,                             \
                                               const typename Enc<S>::A *, double *, double *,   \
                                               Dims, double, double, double, double, bool,       \

@@ -70,8 +70,7 @@ struct PartGrid {
     const int *pstart;  // [nslots + 1] particles
     const int *cstart;  // [nslots + 1] pair-search cells
     const int *sstart;  // [nslots + 1] workgroups of the special-node fix-up
-    const int *kstart;  // [nslots + 1] workgroups of the special-node re-collision
-    int nslots, total_p, total_c, total_s, total_k;
+    int nslots, total_p, total_c, total_s;
 };
 
 // Device tables for the two phases of a time step (the geometry generations and the
@@ -84,11 +83,9 @@ struct PartTables {
 
 int part_tables_build(PartTables &t, const std::vector<PartSlot> (&slots)[2], hipStream_t st);
 void part_tables_free(PartTables &t);
-// Workgroups the fix-up (128 threads, 4 nodes each) and the re-collision (256 threads) give to
-// one lattice: from an estimate of its special-node count, which the grid-stride loops only use
-// for parallelism.
+// Workgroups the special-node kernels (128 threads, 4 nodes each) give to one lattice: from an
+// estimate of its special-node count, which the grid-stride loops only use for parallelism.
 int part_fixup_blocks(const PartDev &pd);
-int part_collide_blocks(const PartDev &pd);
 // The particle phase (main.cu:52-84 minus the lattice passes) of every slot of g, in precision
 // form prec.  The caller then advances each slot's geo_cur and sets special_pending.
 int part_phase(const PartGrid &g, int prec, hipStream_t st);

@@ -611,8 +611,19 @@ __global__ __launch_bounds__(LB) void k_link_sums(PartGrid g) {
 // ---- motion -------------------------------------------------------------------------------------
 
 // update_particles_kernel (move.cu:5-103), after saving the state the IBB of this step used.
+// Entries of the previous step's special-node list leave the node map (the lattice kernel that
+// read them ran before this kernel); slot s uses workgroups sstart[s] .. sstart[s+1]-1.
+__global__ __launch_bounds__(128) void k_special_unmap(PartGrid g) {
+    const int slot = slot_of(g.sstart, g.nslots, blockIdx.x);
+    const PartDev &pd = g.tab[slot].pd;
+    const int blk = blockIdx.x - g.sstart[slot];
+    const int nblk = g.sstart[slot + 1] - g.sstart[slot];
+    const int count = min(*pd.sp.count, pd.sp.cap);
+    for (int j = blk * 128 + threadIdx.x; j < count; j += nblk * 128) pd.sp.map[pd.sp.node[j]] = -1;
+}
+
 // The special-node list of the step is rebuilt after the motion: particle 0 clears its count
-// (the re-collision that read it ran before this kernel).
+// (k_special_unmap, the last reader of the old list, ran before this kernel).
 __global__ __launch_bounds__(128) void k_move(PartGrid g) {
     const int u = blockIdx.x * 128 + threadIdx.x;
     if (u >= g.total_p) return;
@@ -688,6 +699,7 @@ __device__ inline void push_special(const PartDev &pd, bool want, int idx, int k
     if (j < pd.sp.cap) {
         pd.sp.node[j] = idx;
         pd.sp_kind[j] = kind;
+        pd.sp.map[idx] = j;
     } else {
         atomicOr(pd.err + ERR_SPECIAL_CAP, 1);
     }
@@ -830,21 +842,26 @@ __global__ __launch_bounds__(128) void k_prefixup(PartGrid g) {
                 double rho_w = 0.0;
                 if (ix == 0) rho_w = double(rho_wall[iz * d.ly + iy]);
                 else if (ix == d.lx - 1) rho_w = double(rho_wall[d.ly * d.lz + iz * d.ly + iy]);
-                pull_one<S>(pop, lane, idx, ix, iy, iz, d, rho_w, pd.thot, pd.tcold, fv, gv);
                 const int ip = c_opp[lane];
                 const int nx = ix + c_cx[ip], ny = iy + c_cy[ip], nz = iz + c_cz[ip];
-                if (lane != 0 && in_domain(d, nx, ny, nz)) {
-                    const int q_owner = owner_prev[nz * d.lxy + ny * d.lx + nx];
+                const bool nin = lane != 0 && in_domain(d, nx, ny, nz);
+                // the solid candidate's owner is loaded with the pull
+                const int q_owner = nin ? owner_prev[nz * d.lxy + ny * d.lx + nx] : -1;
+                pull_one<S>(pop, lane, idx, ix, iy, iz, d, rho_w, pd.thot, pd.tcold, fv, gv);
+                if (q_owner >= 0) {
+                    // this link's lattice values, the sphere's state and the node's density are
+                    // loaded before the link test so that they are in flight together
+                    const LinkLoads Ld = link_loads<S>(ip, ix, iy, iz, pop, owner_prev, d);
+                    const PView pv{pd.old[OLD_PX], pd.old[OLD_PY], pd.old[OLD_PZ], pd.old[OLD_VX],
+                                   pd.old[OLD_VY], pd.old[OLD_VZ], pd.old[OLD_WX], pd.old[OLD_WY],
+                                   pd.old[OLD_WZ], pd.old[OLD_T]};
+                    const double density = macro_rho[idx];
                     double q = 0.0;
-                    if (q_owner >= 0 &&
-                        link_q(ip, static_cast<double>(ix) + 0.5, static_cast<double>(iy) + 0.5,
-                               static_cast<double>(iz) + 0.5, pd.old[OLD_PX][q_owner],
-                               pd.old[OLD_PY][q_owner], pd.old[OLD_PZ][q_owner], pd.radius_sq, q)) {
-                        const PView pv{pd.old[OLD_PX], pd.old[OLD_PY], pd.old[OLD_PZ], pd.old[OLD_VX],
-                                       pd.old[OLD_VY], pd.old[OLD_VZ], pd.old[OLD_WX], pd.old[OLD_WY],
-                                       pd.old[OLD_WZ], pd.old[OLD_T]};
-                        const LinkOut L = eval_link<S>(ip, ix, iy, iz, q, q_owner, pv, pop,
-                                                       owner_prev, macro_rho[idx], pd.rho0, d);
+                    if (link_q(ip, static_cast<double>(ix) + 0.5, static_cast<double>(iy) + 0.5,
+                               static_cast<double>(iz) + 0.5, pv.px[q_owner], pv.py[q_owner],
+                               pv.pz[q_owner], pd.radius_sq, q)) {
+                        const LinkOut L = eval_link_at(Ld, ip, ix, iy, iz, q, q_owner, pv,
+                                                       density, pd.rho0, d);
                         fv = L.f_out;
                         gv = L.g_out;
                     }
@@ -1060,6 +1077,8 @@ int part_alloc(PartDev &pd, const rbtp_case &c, const rbtp_derived &d, const Dim
     PHIP(hipMalloc(&pd.sp.fx_old, cap * sizeof(double)));
     PHIP(hipMalloc(&pd.sp.count, sizeof(int)));
     PHIP(hipMemset(pd.sp.count, 0, sizeof(int)));
+    PHIP(hipMalloc(&pd.sp.map, (size_t)dims.n * sizeof(int32_t)));
+    PHIP(hipMemset(pd.sp.map, 0xFF, (size_t)dims.n * sizeof(int32_t)));
     PHIP(hipMalloc(&pd.err, ERR_COUNT * sizeof(int32_t)));
     PHIP(hipMemset(pd.err, 0, ERR_COUNT * sizeof(int32_t)));
     // Pair-search grid: cells 1% wider than the pair range so that rounding in the binning
@@ -1103,6 +1122,7 @@ void part_free(PartDev &pd) {
     (void)hipFree(pd.sp.pop);
     (void)hipFree(pd.sp.fx_old);
     (void)hipFree(pd.sp.count);
+    (void)hipFree(pd.sp.map);
     (void)hipFree(pd.err);
     std::memset(&pd, 0, sizeof(pd));
 }
@@ -1163,6 +1183,7 @@ int part_load_state(PartDev &pd, const double *host, const Dims &dims, hipStream
     PHIP(hipMemcpyAsync(pd.box[1], pd.box[0], 6 * (size_t)np * sizeof(int32_t), hipMemcpyDeviceToDevice, st));
     PHIP(hipMemsetAsync(pd.err, 0, ERR_COUNT * sizeof(int32_t), st));
     PHIP(hipMemsetAsync(pd.sp.count, 0, sizeof(int), st));
+    PHIP(hipMemsetAsync(pd.sp.map, 0xFF, (size_t)dims.n * sizeof(int32_t), st));
     PHIP(hipStreamSynchronize(st));
     pd.geo_cur = 0;
     pd.special_pending = false;
@@ -1202,13 +1223,6 @@ int part_fixup_blocks(const PartDev &pd) {
     return (int)std::min(std::max(std::ceil(capped / 4.0), 1.0), 16384.0);
 }
 
-int part_collide_blocks(const PartDev &pd) {
-    const double r = pd.radius + 1.0;
-    const double est = (double)pd.np * (4.0 * 3.141592653589793 * r * r * 2.0) + 64.0;
-    const double capped = std::min(est, (double)pd.sp.cap);
-    return (int)std::min(std::max(std::ceil(capped / 256.0), 1.0), 4096.0);
-}
-
 void part_tables_free(PartTables &t) {
     (void)hipFree(t.mem);
     t.mem = nullptr;
@@ -1222,7 +1236,7 @@ int part_tables_build(PartTables &t, const std::vector<PartSlot> (&slots)[2], hi
     const int n = (int)slots[0].size();
     const size_t tab_bytes = ((sizeof(PartSlot) * (size_t)(n > 0 ? n : 1) + 255) / 256) * 256;
     const size_t pre_bytes = ((sizeof(int) * (size_t)(n + 1) + 255) / 256) * 256;
-    const size_t need = 2 * tab_bytes + 4 * pre_bytes;
+    const size_t need = 2 * tab_bytes + 3 * pre_bytes;
     if (t.bytes < need) {
         (void)hipFree(t.mem);
         t.mem = nullptr;
@@ -1230,19 +1244,18 @@ int part_tables_build(PartTables &t, const std::vector<PartSlot> (&slots)[2], hi
         PHIP(hipMalloc(&t.mem, need));
         t.bytes = need;
     }
-    std::vector<int> pre[4];
+    std::vector<int> pre[3];
     for (auto &v : pre) v.assign(n + 1, 0);
     for (int j = 0; j < n; j++) {
         const PartDev &pd = slots[0][j].pd;
         pre[0][j + 1] = pre[0][j] + pd.np;
         pre[1][j + 1] = pre[1][j] + (pd.use_cells ? pd.ncell[0] * pd.ncell[1] * pd.ncell[2] : 0);
         pre[2][j + 1] = pre[2][j] + part_fixup_blocks(pd);
-        pre[3][j + 1] = pre[3][j] + part_collide_blocks(pd);
     }
     std::vector<char> host(need, 0);
     for (int ph = 0; ph < 2; ph++)
         if (n > 0) std::memcpy(host.data() + ph * tab_bytes, slots[ph].data(), sizeof(PartSlot) * n);
-    for (int k = 0; k < 4; k++)
+    for (int k = 0; k < 3; k++)
         std::memcpy(host.data() + 2 * tab_bytes + k * pre_bytes, pre[k].data(), sizeof(int) * (n + 1));
     PHIP(hipMemcpyAsync(t.mem, host.data(), need, hipMemcpyHostToDevice, st));
     PHIP(hipStreamSynchronize(st));
@@ -1252,12 +1265,10 @@ int part_tables_build(PartTables &t, const std::vector<PartSlot> (&slots)[2], hi
         g.pstart = (const int *)(t.mem + 2 * tab_bytes);
         g.cstart = (const int *)(t.mem + 2 * tab_bytes + pre_bytes);
         g.sstart = (const int *)(t.mem + 2 * tab_bytes + 2 * pre_bytes);
-        g.kstart = (const int *)(t.mem + 2 * tab_bytes + 3 * pre_bytes);
         g.nslots = n;
         g.total_p = pre[0][n];
         g.total_c = pre[1][n];
         g.total_s = pre[2][n];
-        g.total_k = pre[3][n];
     }
     return RBTP_OK;
 }
@@ -1281,6 +1292,8 @@ int part_phase(const PartGrid &g, int prec, hipStream_t st) {
         hipLaunchKernelGGL((k_link_sums<S>), tp, LB, 0, st, g);
         return hipGetLastError();
     }));
+    hipLaunchKernelGGL(k_special_unmap, g.total_s, 128, 0, st, g);
+    PHIP(hipGetLastError());
     hipLaunchKernelGGL(k_move, (tp + 127) / 128, 128, 0, st, g);
     PHIP(hipGetLastError());
     hipLaunchKernelGGL(k_geo_clear, tp, PB, 0, st, g);

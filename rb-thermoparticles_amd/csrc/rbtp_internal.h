@@ -76,25 +76,28 @@ template <class S> struct StepArgs {
     typename Enc<S>::A *fx;        // F_x of the previous forcing, updated in place
     typename Enc<S>::A *rho_wall;  // rho at ix=0 (first ly*lz) and ix=lx-1 (next ly*lz), in place
     double *rho_full;              // rho of this step at every node (particle coupling), or null
+    // Special nodes (particle coupling), or null: sp_map[idx] is the node's entry in the
+    // special list (-1 elsewhere); an entry holds the node's corrected post-stream populations
+    // (sp_pop[i*sp_cap + j], f_0..f_26 then g_0..g_26) and the F_x entering its macrovar.
+    const int32_t *sp_map;
+    const double *sp_pop;
+    const double *sp_fx_old;
+    size_t sp_cap;
     double *macro_rho, *macro_ux, *macro_uy, *macro_uz, *macro_phi;  // output fields, or null
     Dims d;
     FluidK k;
 };
 
-// Post-stream populations of the special nodes (IBB link nodes and refilled nodes), computed
-// before the lattice kernel from the post-collision state, and re-collided after it.
+// Post-stream populations of the special nodes (IBB link nodes and refilled nodes), computed by
+// the particle phase from the post-collision state; the next lattice kernel collides them in
+// place of its own pull at those nodes.
 struct SpecialBuf {
     int *node;       // lattice index, [cap]
     double *pop;     // [54][cap]: f_0..f_26 then g_0..g_26 (post-stream, post-IBB/refill)
     double *fx_old;  // F_x entering macrovar at this node, [cap]
     int *count;      // device counter
     int cap;
-};
-
-// One lattice's re-collision of its special nodes inside a batched launch.
-template <class S> struct SpecialSlot {
-    StepArgs<S> a;
-    SpecialBuf sp;
+    int32_t *map;    // [n]: list entry of each node, -1 when it is not special
 };
 
 // Synchronous copies between caller-owned host memory and the device on stream st, staged

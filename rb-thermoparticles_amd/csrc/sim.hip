@@ -5,12 +5,12 @@
 // (main.cu:35-85) and keeps three copies of f and of g.  Here the state between steps is the
 // post-collision pair (f*, g*) in one buffer of a ping-pong pair, and one step is
 //   1. the fused lattice kernel: pull-stream of the previous step (wall rules included), then
-//      macrovar, forcing and both collisions of this step, into the other buffer;
-//   2. the special-node re-collision: nodes whose post-stream populations the particle coupling
-//      replaced (IBB link nodes, refilled nodes) are collided again from their corrected values;
-//   3. the particle phase (particles.hip): force/torque/heat sums, contacts, motion, new
-//      geometry, new-fluid detection, and the corrected post-stream populations of the next
-//      special nodes.
+//      macrovar, forcing and both collisions of this step, into the other buffer; at special
+//      nodes (IBB link nodes, refilled nodes) the post-stream populations come from the
+//      special-node list instead of the pull;
+//   2. the particle phase (particles.hip): force/torque/heat sums, contacts, motion, new
+//      geometry, new-fluid detection, and the special-node list with the corrected post-stream
+//      populations for the next step.
 // Every reference observable at a step boundary is reconstructed by the accessors.
 #include <hip/hip_runtime.h>
 
@@ -325,6 +325,10 @@ static StepArgs<S> step_args(rbtp_sim *s, bool macro_out) {
     a.fx = (typename Enc<S>::A *)s->fx;
     a.rho_wall = (typename Enc<S>::A *)s->rho_wall;
     a.rho_full = s->particles ? s->macro[0] : nullptr;
+    a.sp_map = s->particles ? s->pd.sp.map : nullptr;
+    a.sp_pop = s->particles ? s->pd.sp.pop : nullptr;
+    a.sp_fx_old = s->particles ? s->pd.sp.fx_old : nullptr;
+    a.sp_cap = s->particles ? (size_t)s->pd.sp.cap : 0;
     if (macro_out) {
         a.macro_rho = s->macro[0];
         a.macro_ux = s->macro[1];
@@ -400,8 +404,6 @@ extern "C" int rbtp_init(rbtp_sim *s, const double *temperature, const double *p
 // Time loop
 // ---------------------------------------------------------------------------------------------
 
-// Everything of a time step after the lattice kernel: the special-node re-collision (reads the
-// same StepArgs as the lattice kernel), the buffer flip and the particle phase.
 // The particle phase's view of this lattice when its post-collision populations are in buffer
 // `cur` and the geometry of the step's IBB is generation `geo`.
 static PartSlot part_slot(const rbtp_sim *s, int cur, int geo) {
@@ -420,15 +422,8 @@ static PartSlot part_slot(const rbtp_sim *s, int cur, int geo) {
     return sl;
 }
 
-static int after_lattice(rbtp_sim *s, bool macro_out) {
-    if (s->particles && s->pd.special_pending) {
-        HIP_TRY(dispatch(s->prec, [&](auto *tag) -> hipError_t {
-            using S = std::remove_pointer_t<decltype(tag)>;
-            const StepArgs<S> a = step_args<S>(s, macro_out);
-            return launch_special_collide<S>(a, s->pd.sp, part_collide_blocks(s->pd), s->strict,
-                                             s->stream);
-        }));
-    }
+// Everything of a time step after the lattice kernel: the buffer flip and the particle phase.
+static int after_lattice(rbtp_sim *s) {
     s->cur = 1 - s->cur;
     s->poststream = false;
     if (s->particles) {
@@ -475,7 +470,7 @@ static int one_step(rbtp_sim *s, bool macro_out) {
         s->kernel_ms_sum += ms;
         s->kernel_launches++;
     }
-    return after_lattice(s, macro_out);
+    return after_lattice(s);
 }
 
 static int step_impl(rbtp_sim *s, int nsteps, unsigned flags, bool sync) {
@@ -641,8 +636,10 @@ extern "C" int rbtp_set_populations(rbtp_sim *s, const double *f, const double *
     if (e != hipSuccess) return set_error(RBTP_E_HIP, "rbtp_set_populations: %s", hipGetErrorString(e));
     s->poststream = true;
     if (s->particles) {
+        // the buffered special-node populations no longer belong to this state
         s->pd.special_pending = false;
         HIP_TRY(hipMemsetAsync(s->pd.sp.count, 0, sizeof(int), s->stream));
+        HIP_TRY(hipMemsetAsync(s->pd.sp.map, 0xFF, (size_t)s->dims.n * sizeof(int32_t), s->stream));
         HIP_TRY(hipStreamSynchronize(s->stream));
     }
     return RBTP_OK;
@@ -982,7 +979,6 @@ struct rbtp_batch {
     std::vector<int> geo_at_build;  // each active slot's geometry generation then (0 without particles)
     std::vector<int> part_slots;    // positions in `active` of the slots with particles
     PartTables ptab;                // particle-phase tables over part_slots, per phase
-    void *spec_dev[4];              // SpecialSlot<S>[max_slots] for (parity, macro)
     int total_tiles;
     int block, grid;
     bool tables_dirty;
@@ -995,15 +991,6 @@ struct rbtp_batch {
     double last_ms;
     int64_t graph_launches;
 };
-
-static size_t special_slot_size(int prec) {
-    switch (prec) {
-        case RBTP_PREC_FP64: return sizeof(SpecialSlot<double>);
-        case RBTP_PREC_FP32: return sizeof(SpecialSlot<float>);
-        case RBTP_PREC_FP16S: return sizeof(SpecialSlot<__half>);
-        default: return sizeof(SpecialSlot<__hip_bfloat16>);
-    }
-}
 
 static size_t args_size(int prec) {
     switch (prec) {
@@ -1046,7 +1033,6 @@ extern "C" void rbtp_batch_destroy(rbtp_batch *b) {
     for (rbtp_sim *s : b->slot)
         if (s) rbtp_destroy(s);
     for (void *p : b->args_dev) (void)hipFree(p);
-    for (void *p : b->spec_dev) (void)hipFree(p);
     part_tables_free(b->ptab);
     (void)hipFree(b->tile_start_dev);
     if (b->ev0) (void)hipEventDestroy(b->ev0);
@@ -1089,8 +1075,6 @@ extern "C" int rbtp_batch_create(int precision, int max_slots, double hbm_fracti
     if ((e = hipEventCreate(&b->ev1)) != hipSuccess) return fail(e, "hipEventCreate");
     for (void *&p : b->args_dev)
         if ((e = hipMalloc(&p, args_size(precision) * max_slots)) != hipSuccess) return fail(e, "hipMalloc");
-    for (void *&p : b->spec_dev)
-        if ((e = hipMalloc(&p, special_slot_size(precision) * max_slots)) != hipSuccess) return fail(e, "hipMalloc");
     if ((e = hipMalloc(&b->tile_start_dev, sizeof(int) * (max_slots + 1))) != hipSuccess) return fail(e, "hipMalloc");
     *out = b;
     return RBTP_OK;
@@ -1185,7 +1169,6 @@ static int batch_build_tables(rbtp_batch *b) {
     HIP_TRY(dispatch(b->prec, [&](auto *tag) -> hipError_t {
         using S = std::remove_pointer_t<decltype(tag)>;
         std::vector<StepArgs<S>> h(na > 0 ? na : 1);
-        std::vector<SpecialSlot<S>> sp(nps > 0 ? nps : 1);
         for (int par = 0; par < 2; par++)
             for (int mac = 0; mac < 2; mac++) {
                 for (int j = 0; j < na; j++) {
@@ -1194,16 +1177,8 @@ static int batch_build_tables(rbtp_batch *b) {
                     h[j] = step_args<S>(s, mac != 0);
                     if (par) s->cur = 1 - s->cur;
                 }
-                for (int q = 0; q < nps; q++) {
-                    sp[q].a = h[b->part_slots[q]];
-                    sp[q].sp = b->slot[b->active[b->part_slots[q]]]->pd.sp;
-                }
                 if (na > 0) {
                     hipError_t e = hipMemcpy(b->args_dev[par * 2 + mac], h.data(), sizeof(StepArgs<S>) * na, hipMemcpyHostToDevice);
-                    if (e != hipSuccess) return e;
-                }
-                if (nps > 0) {
-                    hipError_t e = hipMemcpy(b->spec_dev[par * 2 + mac], sp.data(), sizeof(SpecialSlot<S>) * nps, hipMemcpyHostToDevice);
                     if (e != hipSuccess) return e;
                 }
             }
@@ -1234,21 +1209,13 @@ static hipError_t batch_lattice(rbtp_batch *b, bool macro, int block, int grid) 
     });
 }
 
-// One step of every slot: the batched lattice kernel, then one launch of each special-node and
-// particle kernel for all slots with particles.  Special nodes are re-collided from the
-// device-side list count, which is zero until a slot's first particle phase.
+// One step of every slot: the batched lattice kernel (which collides each slot's special nodes
+// from its special-node list), then one launch of each particle kernel for all slots with
+// particles.
 static int batch_step_eager(rbtp_batch *b, bool macro) {
     HIP_TRY(batch_lattice(b, macro, b->block, b->grid));
     const int nps = (int)b->part_slots.size();
     const PartGrid &pg = b->ptab.g[b->phase];
-    if (nps > 0) {
-        HIP_TRY(dispatch(b->prec, [&](auto *tag) {
-            using S = std::remove_pointer_t<decltype(tag)>;
-            return launch_special_collide_batched<S>(
-                (const SpecialSlot<S> *)b->spec_dev[b->phase * 2 + (macro ? 1 : 0)], pg.kstart,
-                nps, pg.total_k, b->strict, b->stream);
-        }));
-    }
     for (int k : b->active) {
         rbtp_sim *s = b->slot[k];
         s->cur = 1 - s->cur;
