@@ -8,8 +8,8 @@
 //   * k_link_sums: one workgroup per particle walks the particle's link box and sums
 //     force, torque and heat in a fixed order (1024 lane partials + pairwise tree), replacing
 //     the reference's FP64 atomics (ibb.cu:182-187,204) with a reproducible reduction;
-//   * k_special_find: compact list of the nodes whose post-stream populations the coupling
-//     replaces (IBB link nodes, refilled nodes);
+//   * k_link_sums also lists the IBB link nodes, k_refill_find the refilled nodes: the compact
+//     list of the nodes whose post-stream populations the coupling replaces;
 //   * k_prefixup: those nodes' corrected post-stream populations (pull + IBB, or refill),
 //     buffered for the next lattice step.
 // Geometry is two owner maps (previous and current generation); each generation is cleared only
@@ -295,6 +295,9 @@ __global__ __launch_bounds__(128) void k_contacts(PartGrid g) {
     const PartDev &pd = g.tab[s].pd;
     const Dims d = g.tab[s].d;
     const int i = u - g.pstart[s];
+    // the special-node list of this step is rebuilt from here on (k_special_unmap, the last
+    // reader of the previous list, ran before this kernel)
+    if (i == 0) *pd.sp.count = 0;
     double *const *P = pd.p;
     for (int a = 0; a < 3; a++) {
         P[RBTP_P_FORCE_PREV2_X + a][i] = P[RBTP_P_FORCE_PREV_X + a][i];
@@ -370,6 +373,30 @@ __global__ __launch_bounds__(128) void k_contacts(PartGrid g) {
     P[RBTP_P_FORCE_X][i] = FX + fx;
     P[RBTP_P_FORCE_Y][i] = FY + fy;
     P[RBTP_P_FORCE_Z][i] = FZ + fz;
+}
+
+// ---- special-node list -------------------------------------------------------------------------
+
+// Appends the nodes of the calling wave's lanes that have `want` set, in lane order, with one
+// atomic per wave: consecutive box nodes stay consecutive in the list, which keeps the later
+// passes over the list coalesced.  Every active lane of the wave must call it.
+__device__ inline void push_special(const PartDev &pd, bool want, int idx, int kind) {
+    const unsigned long long mask = __ballot(want);
+    if (mask == 0) return;
+    const int lane = __lane_id();
+    const int leader = __ffsll((long long)mask) - 1;
+    int base = 0;
+    if (lane == leader) base = atomicAdd(pd.sp.count, __popcll(mask));
+    base = __shfl(base, leader);
+    if (!want) return;
+    const int j = base + __popcll(mask & ((1ull << lane) - 1ull));
+    if (j < pd.sp.cap) {
+        pd.sp.node[j] = idx;
+        pd.sp_kind[j] = kind;
+        pd.sp.map[idx] = j;
+    } else {
+        atomicOr(pd.err + ERR_SPECIAL_CAP, 1);
+    }
 }
 
 // ---- link evaluation -----------------------------------------------------------------------------
@@ -542,50 +569,64 @@ __global__ __launch_bounds__(LB) void k_link_sums(PartGrid g) {
         for (int k = t; k < cnt; k += LB) {
             int ix, iy, iz;
             const int idx = box_node(d, lo, hi, k, ix, iy, iz);
-            if (owner_prev[idx] != -1) continue;
-            const double x0 = static_cast<double>(ix) + 0.5;
-            const double y0 = static_cast<double>(iy) + 0.5;
-            const double z0 = static_cast<double>(iz) + 0.5;
-            int own[NPOP];
-            neighbour_owners(owner_prev, d, ix, iy, iz, own);
-            // links of this node to p, as a direction mask (q is evaluated again below)
-            unsigned mask = 0u;
-            for (int ip = 1; ip < NPOP; ip++) {
-                double q = 0.0;
-                if (own[ip] == p && link_q(ip, x0, y0, z0, X, Y, Z, pd.radius_sq, q))
-                    mask |= 1u << ip;
-            }
-            if (mask == 0u) continue;
-            const double density = rho[idx];
-            // links in ascending direction order; each link's lattice loads are issued together
-            while (mask) {
-                const int ip = __builtin_ctz(mask);
-                mask &= mask - 1u;
-                const LinkLoads Ld = link_loads<S>(ip, ix, iy, iz, pop, owner_prev, d);
-                double q = 0.0;
-                link_q(ip, x0, y0, z0, X, Y, Z, pd.radius_sq, q);
-                const LinkOut L = eval_link_at(Ld, ip, ix, iy, iz, q, p, pv, density, pd.rho0, d);
-                nunh_f += L.unhandled_f;
-                nunh_g += L.unhandled_g;
-                const int opp = c_opp[ip];
-                // Force uses the link record's r = q c (links.cu:134-136, ibb.cu:160-168).
-                const double rx = q * static_cast<double>(c_cx[ip]);
-                const double ry = q * static_cast<double>(c_cy[ip]);
-                const double rz = q * static_cast<double>(c_cz[ip]);
-                const double relx = ix + 0.5 + rx - X;
-                const double rely = iy + 0.5 + ry - Y;
-                const double relz = iz + 0.5 + rz - Z;
-                const double Fx = (c_cx[ip] - L.uwx) * L.f_in - (c_cx[opp] - L.uwx) * L.f_out;
-                const double Fy = (c_cy[ip] - L.uwy) * L.f_in - (c_cy[opp] - L.uwy) * L.f_out;
-                const double Fz = (c_cz[ip] - L.uwz) * L.f_in - (c_cz[opp] - L.uwz) * L.f_out;
-                s[0] += Fx;
-                s[1] += Fy;
-                s[2] += Fz;
-                s[3] += rely * Fz - relz * Fy;
-                s[4] += relz * Fx - relx * Fz;
-                s[5] += relx * Fy - rely * Fx;
-                s[6] += L.g_in - L.g_out;
-            }
+            bool listed = false;  // this workgroup lists idx as an IBB link node
+            do {
+                if (owner_prev[idx] != -1) break;
+                const double x0 = static_cast<double>(ix) + 0.5;
+                const double y0 = static_cast<double>(iy) + 0.5;
+                const double z0 = static_cast<double>(iz) + 0.5;
+                int own[NPOP];
+                neighbour_owners(owner_prev, d, ix, iy, iz, own);
+                // links of this node to p, as a direction mask (q is evaluated again below),
+                // and whether it also links to a sphere of lower pid: a node linked to several
+                // spheres is listed once, by the lowest linked pid
+                unsigned mask = 0u;
+                bool lower = false;
+                for (int ip = 1; ip < NPOP; ip++) {
+                    const int q_owner = own[ip];
+                    double q = 0.0;
+                    if (q_owner == p) {
+                        if (link_q(ip, x0, y0, z0, X, Y, Z, pd.radius_sq, q)) mask |= 1u << ip;
+                    } else if (q_owner >= 0 && q_owner < p) {
+                        if (link_q(ip, x0, y0, z0, pv.px[q_owner], pv.py[q_owner],
+                                   pv.pz[q_owner], pd.radius_sq, q))
+                            lower = true;
+                    }
+                }
+                if (mask == 0u) break;
+                listed = !lower;
+                const double density = rho[idx];
+                // links in ascending direction order; each link's lattice loads are issued together
+                while (mask) {
+                    const int ip = __builtin_ctz(mask);
+                    mask &= mask - 1u;
+                    const LinkLoads Ld = link_loads<S>(ip, ix, iy, iz, pop, owner_prev, d);
+                    double q = 0.0;
+                    link_q(ip, x0, y0, z0, X, Y, Z, pd.radius_sq, q);
+                    const LinkOut L = eval_link_at(Ld, ip, ix, iy, iz, q, p, pv, density, pd.rho0, d);
+                    nunh_f += L.unhandled_f;
+                    nunh_g += L.unhandled_g;
+                    const int opp = c_opp[ip];
+                    // Force uses the link record's r = q c (links.cu:134-136, ibb.cu:160-168).
+                    const double rx = q * static_cast<double>(c_cx[ip]);
+                    const double ry = q * static_cast<double>(c_cy[ip]);
+                    const double rz = q * static_cast<double>(c_cz[ip]);
+                    const double relx = ix + 0.5 + rx - X;
+                    const double rely = iy + 0.5 + ry - Y;
+                    const double relz = iz + 0.5 + rz - Z;
+                    const double Fx = (c_cx[ip] - L.uwx) * L.f_in - (c_cx[opp] - L.uwx) * L.f_out;
+                    const double Fy = (c_cy[ip] - L.uwy) * L.f_in - (c_cy[opp] - L.uwy) * L.f_out;
+                    const double Fz = (c_cz[ip] - L.uwz) * L.f_in - (c_cz[opp] - L.uwz) * L.f_out;
+                    s[0] += Fx;
+                    s[1] += Fy;
+                    s[2] += Fz;
+                    s[3] += rely * Fz - relz * Fy;
+                    s[4] += relz * Fx - relx * Fz;
+                    s[5] += relx * Fy - rely * Fx;
+                    s[6] += L.g_in - L.g_out;
+                }
+            } while (0);
+            push_special(pd, listed, idx, -1);
         }
     }
     if (nunh_f) atomicAdd(pd.err + ERR_UNHANDLED_F, nunh_f);
@@ -612,7 +653,8 @@ __global__ __launch_bounds__(LB) void k_link_sums(PartGrid g) {
 
 // update_particles_kernel (move.cu:5-103), after saving the state the IBB of this step used.
 // Entries of the previous step's special-node list leave the node map (the lattice kernel that
-// read them ran before this kernel); slot s uses workgroups sstart[s] .. sstart[s+1]-1.
+// read them ran before this kernel); slot s uses workgroups sstart[s] .. sstart[s+1]-1.  The
+// list itself is cleared by k_contacts, which runs next.
 __global__ __launch_bounds__(128) void k_special_unmap(PartGrid g) {
     const int slot = slot_of(g.sstart, g.nslots, blockIdx.x);
     const PartDev &pd = g.tab[slot].pd;
@@ -622,15 +664,12 @@ __global__ __launch_bounds__(128) void k_special_unmap(PartGrid g) {
     for (int j = blk * 128 + threadIdx.x; j < count; j += nblk * 128) pd.sp.map[pd.sp.node[j]] = -1;
 }
 
-// The special-node list of the step is rebuilt after the motion: particle 0 clears its count
-// (k_special_unmap, the last reader of the old list, ran before this kernel).
 __global__ __launch_bounds__(128) void k_move(PartGrid g) {
     const int u = blockIdx.x * 128 + threadIdx.x;
     if (u >= g.total_p) return;
     const int s = slot_of(g.pstart, g.nslots, u);
     const PartDev &pd = g.tab[s].pd;
     const int pid = u - g.pstart[s];
-    if (pid == 0) *pd.sp.count = 0;
     double *const *P = pd.p;
     const int src[OLD_COUNT] = {RBTP_P_POS_X, RBTP_P_POS_Y, RBTP_P_POS_Z, RBTP_P_VEL_X,
                                 RBTP_P_VEL_Y, RBTP_P_VEL_Z, RBTP_P_OMEGA_X, RBTP_P_OMEGA_Y,
@@ -683,35 +722,10 @@ __global__ __launch_bounds__(128) void k_move(PartGrid g) {
 
 // ---- special nodes ------------------------------------------------------------------------------
 
-// Appends the nodes of the calling wave's lanes that have `want` set, in lane order, with one
-// atomic per wave: consecutive box nodes stay consecutive in the list, which keeps the later
-// passes over the list coalesced.  Every active lane of the wave must call it.
-__device__ inline void push_special(const PartDev &pd, bool want, int idx, int kind) {
-    const unsigned long long mask = __ballot(want);
-    if (mask == 0) return;
-    const int lane = __lane_id();
-    const int leader = __ffsll((long long)mask) - 1;
-    int base = 0;
-    if (lane == leader) base = atomicAdd(pd.sp.count, __popcll(mask));
-    base = __shfl(base, leader);
-    if (!want) return;
-    const int j = base + __popcll(mask & ((1ull << lane) - 1ull));
-    if (j < pd.sp.cap) {
-        pd.sp.node[j] = idx;
-        pd.sp_kind[j] = kind;
-        pd.sp.map[idx] = j;
-    } else {
-        atomicOr(pd.err + ERR_SPECIAL_CAP, 1);
-    }
-}
-
-// The special-node list of one particle's boxes.
-//   Link nodes of the previous geometry (the nodes bounce_back_particles writes), in the link
-//   box; a node with links to several particles is listed once, by the lowest linked pid.
-//   New-fluid nodes (new_fluid_count_kernel, refill.cu:8-41) in the marking box: solid in the
-//   previous geometry, fluid now, previous owner valid, and within close_thresh of that owner's
-//   new surface.
-__global__ __launch_bounds__(LB) void k_special_find(PartGrid g) {
+// New-fluid nodes of one particle's marking box (new_fluid_count_kernel, refill.cu:8-41): solid
+// in the previous geometry, fluid now, previous owner valid, and within close_thresh of that
+// owner's new surface.  (IBB link nodes were listed by k_link_sums.)
+__global__ __launch_bounds__(LB) void k_refill_find(PartGrid g) {
     const int slot = slot_of(g.pstart, g.nslots, blockIdx.x);
     const PartSlot &sl = g.tab[slot];
     const PartDev &pd = sl.pd;
@@ -721,32 +735,6 @@ __global__ __launch_bounds__(LB) void k_special_find(PartGrid g) {
     const int p = blockIdx.x - g.pstart[slot];
     int lo[3], hi[3];
     if (!load_box(sl.box_prev, p, lo, hi)) return;
-    {
-        int llo[3] = {lo[0], lo[1], lo[2]}, lhi[3] = {hi[0], hi[1], hi[2]};
-        grow_box(d, llo, lhi);
-        const double *OX = pd.old[OLD_PX], *OY = pd.old[OLD_PY], *OZ = pd.old[OLD_PZ];
-        const int cnt = box_count(llo, lhi);
-        for (int k = threadIdx.x; k < cnt; k += LB) {
-            int ix, iy, iz;
-            const int idx = box_node(d, llo, lhi, k, ix, iy, iz);
-            int minpid = INT_MAX;
-            if (owner_prev[idx] == -1) {
-                const double x0 = static_cast<double>(ix) + 0.5;
-                const double y0 = static_cast<double>(iy) + 0.5;
-                const double z0 = static_cast<double>(iz) + 0.5;
-                int own[NPOP];
-                neighbour_owners(owner_prev, d, ix, iy, iz, own);
-                for (int ip = 1; ip < NPOP; ip++) {
-                    const int q_owner = own[ip];
-                    if (q_owner < 0 || q_owner >= minpid) continue;
-                    double q = 0.0;
-                    if (link_q(ip, x0, y0, z0, OX[q_owner], OY[q_owner], OZ[q_owner], pd.radius_sq, q))
-                        minpid = q_owner;
-                }
-            }
-            push_special(pd, minpid == p, idx, -1);
-        }
-    }
     const double PXv = pd.p[RBTP_P_POS_X][p], PYv = pd.p[RBTP_P_POS_Y][p], PZv = pd.p[RBTP_P_POS_Z][p];
     const int cnt = box_count(lo, hi);
     for (int k = threadIdx.x; k < cnt; k += LB) {
@@ -806,7 +794,6 @@ __device__ inline void pull_one(const PopView<S> &pop, int j, int idx, int ix, i
 //              (refill.cu:217-235) in population order and sets F = 0.
 template <class S>
 __global__ __launch_bounds__(128) void k_prefixup(PartGrid g) {
-    __shared__ double sfv[4][32], sgv[4][32];
     const int slot = slot_of(g.sstart, g.nslots, blockIdx.x);
     const PartSlot &sl = g.tab[slot];
     const PartDev &pd = sl.pd;
@@ -824,20 +811,16 @@ __global__ __launch_bounds__(128) void k_prefixup(PartGrid g) {
     const int count = min(*pd.sp.count, pd.sp.cap);
     const size_t cap = (size_t)pd.sp.cap;
     double *out = pd.sp.pop;
-    for (int base = blk * 4; base < count; base += nblk * 4) {
-        const int j = base + grp;
-        const bool live = j < count;
-        int idx = 0, kind = -1, ix = 0, iy = 0, iz = 0;
-        if (live) {
-            idx = pd.sp.node[j];
-            kind = pd.sp_kind[j];
-            iz = idx / d.lxy;
-            const int rem = idx - iz * d.lxy;
-            iy = rem / d.lx;
-            ix = rem - iy * d.lx;
-        }
+    // each 32-lane group walks its own nodes: no workgroup barrier couples the groups
+    for (int j = blk * 4 + grp; j < count; j += nblk * 4) {
+        const int idx = pd.sp.node[j];
+        const int kind = pd.sp_kind[j];
+        const int iz = idx / d.lxy;
+        const int rem = idx - iz * d.lxy;
+        const int iy = rem / d.lx;
+        const int ix = rem - iy * d.lx;
         double fv = 0.0, gv = 0.0;
-        if (live && lane < NPOP) {
+        if (lane < NPOP) {
             if (kind < 0) {
                 double rho_w = 0.0;
                 if (ix == 0) rho_w = double(rho_wall[iz * d.ly + iy]);
@@ -919,22 +902,22 @@ __global__ __launch_bounds__(128) void k_prefixup(PartGrid g) {
             out[(size_t)lane * cap + j] = fv;
             out[(size_t)(NPOP + lane) * cap + j] = gv;
         }
-        sfv[grp][lane] = fv;
-        sgv[grp][lane] = gv;
-        __syncthreads();
-        if (live && lane == 0) {
-            if (kind < 0) {
-                pd.sp.fx_old[j] = double(fx_field[idx]);
-            } else {
-                double rho_loc = 0.0, mom_x = 0.0, mom_y = 0.0, mom_z = 0.0, phi_loc = 0.0;
-                for (int ip = 0; ip < NPOP; ip++) {
-                    const double f_val = sfv[grp][ip];
-                    rho_loc += f_val;
-                    mom_x += c_cx[ip] * f_val;
-                    mom_y += c_cy[ip] * f_val;
-                    mom_z += c_cz[ip] * f_val;
-                    phi_loc += sgv[grp][ip];
-                }
+        if (kind < 0) {
+            if (lane == 0) pd.sp.fx_old[j] = double(fx_field[idx]);
+        } else {
+            // the group's 27 populations, read across its lanes in population order (every
+            // lane of the group is active here: kind is the same for the whole group)
+            double rho_loc = 0.0, mom_x = 0.0, mom_y = 0.0, mom_z = 0.0, phi_loc = 0.0;
+            for (int ip = 0; ip < NPOP; ip++) {
+                const double f_val = __shfl(fv, ip, 32);
+                const double g_val = __shfl(gv, ip, 32);
+                rho_loc += f_val;
+                mom_x += c_cx[ip] * f_val;
+                mom_y += c_cy[ip] * f_val;
+                mom_z += c_cz[ip] * f_val;
+                phi_loc += g_val;
+            }
+            if (lane == 0) {
                 macro_rho[idx] = rho_loc;
                 if (rho_loc > 1e-12) {
                     const double inv_rho = 1.0 / rho_loc;
@@ -951,7 +934,6 @@ __global__ __launch_bounds__(128) void k_prefixup(PartGrid g) {
                 pd.sp.fx_old[j] = 0.0;
             }
         }
-        __syncthreads();
     }
 }
 
@@ -1285,6 +1267,8 @@ int part_phase(const PartGrid &g, int prec, hipStream_t st) {
     }
     hipLaunchKernelGGL(k_cell_bin, (tp + 255) / 256, 256, 0, st, g);
     PHIP(hipGetLastError());
+    hipLaunchKernelGGL(k_special_unmap, g.total_s, 128, 0, st, g);
+    PHIP(hipGetLastError());
     hipLaunchKernelGGL(k_contacts, (tp + 127) / 128, 128, 0, st, g);
     PHIP(hipGetLastError());
     PHIP(dispatch_prec(prec, [&](auto *tag) {
@@ -1292,15 +1276,13 @@ int part_phase(const PartGrid &g, int prec, hipStream_t st) {
         hipLaunchKernelGGL((k_link_sums<S>), tp, LB, 0, st, g);
         return hipGetLastError();
     }));
-    hipLaunchKernelGGL(k_special_unmap, g.total_s, 128, 0, st, g);
-    PHIP(hipGetLastError());
     hipLaunchKernelGGL(k_move, (tp + 127) / 128, 128, 0, st, g);
     PHIP(hipGetLastError());
     hipLaunchKernelGGL(k_geo_clear, tp, PB, 0, st, g);
     PHIP(hipGetLastError());
     hipLaunchKernelGGL(k_geo_mark, tp, PB, 0, st, g);
     PHIP(hipGetLastError());
-    hipLaunchKernelGGL(k_special_find, tp, LB, 0, st, g);
+    hipLaunchKernelGGL(k_refill_find, tp, LB, 0, st, g);
     PHIP(hipGetLastError());
     PHIP(dispatch_prec(prec, [&](auto *tag) {
         using S = std::remove_pointer_t<decltype(tag)>;
