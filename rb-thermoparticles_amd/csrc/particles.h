@@ -13,7 +13,9 @@ namespace rbtp {
 enum { OLD_PX = 0, OLD_PY, OLD_PZ, OLD_VX, OLD_VY, OLD_VZ, OLD_WX, OLD_WY, OLD_WZ, OLD_T, OLD_COUNT };
 
 // Error flags raised by device code (checked after every synchronising call).
-enum { ERR_SPECIAL_CAP = 0, ERR_CONTACT_CAP = 1, ERR_UNHANDLED_F = 2, ERR_UNHANDLED_G = 3, ERR_COUNT = 4 };
+// ERR_UNHANDLED_F/G count links whose bounce-back found neither its node nor the upstream node
+// fluid (the reference's printf cases, ibb.cu:79,130); they are counts, not failures.
+enum { ERR_SPECIAL_CAP = 0, ERR_UNHANDLED_F = 1, ERR_UNHANDLED_G = 2, ERR_COUNT = 3 };
 
 struct PartDev {
     int np;
@@ -29,6 +31,7 @@ struct PartDev {
     int geo_cur;             // generation holding the newest geometry
     SpecialBuf sp;
     int32_t *sp_kind;        // -1: IBB link node; >= 0: refilled node, value = previous owner
+    size_t sp_elem;          // bytes per value of sp.pop (the precision form's auxiliary type)
     bool special_pending;    // sp holds corrected populations for the next lattice step
     int32_t *err;            // ERR_COUNT device flags
     int box_w;               // upper bound of marking-box nodes per axis
@@ -69,8 +72,9 @@ struct PartGrid {
     const PartSlot *tab;
     const int *pstart;  // [nslots + 1] particles
     const int *cstart;  // [nslots + 1] pair-search cells
-    const int *sstart;  // [nslots + 1] workgroups of the special-node fix-up
-    int nslots, total_p, total_c, total_s;
+    const int *sstart;  // [nslots + 1] workgroups of the special-node kernels (4 nodes each)
+    const int *lstart;  // [nslots + 1] workgroups of the link-node fix-up (128 nodes each)
+    int nslots, total_p, total_c, total_s, total_l;
 };
 
 // Device tables for the two phases of a time step (the geometry generations and the
@@ -83,9 +87,10 @@ struct PartTables {
 
 int part_tables_build(PartTables &t, const std::vector<PartSlot> (&slots)[2], hipStream_t st);
 void part_tables_free(PartTables &t);
-// Workgroups the special-node kernels (128 threads, 4 nodes each) give to one lattice: from an
-// estimate of its special-node count, which the grid-stride loops only use for parallelism.
-int part_fixup_blocks(const PartDev &pd);
+// Workgroups of 128 threads the special-node kernels give to one lattice, at `per_block` nodes
+// per workgroup: from an estimate of its special-node count, which the grid-stride loops only
+// use for parallelism.
+int part_fixup_blocks(const PartDev &pd, int per_block);
 // The particle phase (main.cu:52-84 minus the lattice passes) of every slot of g, in precision
 // form prec.  The caller then advances each slot's geo_cur and sets special_pending.
 int part_phase(const PartGrid &g, int prec, hipStream_t st);

@@ -10,8 +10,8 @@
 //     the reference's FP64 atomics (ibb.cu:182-187,204) with a reproducible reduction;
 //   * k_link_sums also lists the IBB link nodes, k_refill_find the refilled nodes: the compact
 //     list of the nodes whose post-stream populations the coupling replaces;
-//   * k_prefixup: those nodes' corrected post-stream populations (pull + IBB, or refill),
-//     buffered for the next lattice step.
+//   * k_fixup_links / k_fixup_refill: those nodes' corrected post-stream populations (pull +
+//     IBB, or refill), buffered for the next lattice step.
 // Geometry is two owner maps (previous and current generation); each generation is cleared only
 // inside the marking boxes it was built from, so a step touches O(P R^3) nodes instead of the
 // reference's full-lattice memsets, scans and host round trips (links.cu:147-256,
@@ -297,7 +297,10 @@ __global__ __launch_bounds__(128) void k_contacts(PartGrid g) {
     const int i = u - g.pstart[s];
     // the special-node list of this step is rebuilt from here on (k_special_unmap, the last
     // reader of the previous list, ran before this kernel)
-    if (i == 0) *pd.sp.count = 0;
+    if (i == 0) {
+        pd.sp.count[0] = 0;
+        pd.sp.count[1] = 0;
+    }
     double *const *P = pd.p;
     for (int a = 0; a < 3; a++) {
         P[RBTP_P_FORCE_PREV2_X + a][i] = P[RBTP_P_FORCE_PREV_X + a][i];
@@ -670,6 +673,8 @@ __global__ __launch_bounds__(128) void k_move(PartGrid g) {
     const int s = slot_of(g.pstart, g.nslots, u);
     const PartDev &pd = g.tab[s].pd;
     const int pid = u - g.pstart[s];
+    // k_link_sums, which ran before, listed the IBB link nodes: they head the list
+    if (pid == 0) pd.sp.count[1] = pd.sp.count[0];
     double *const *P = pd.p;
     const int src[OLD_COUNT] = {RBTP_P_POS_X, RBTP_P_POS_Y, RBTP_P_POS_Z, RBTP_P_VEL_X,
                                 RBTP_P_VEL_Y, RBTP_P_VEL_Z, RBTP_P_OMEGA_X, RBTP_P_OMEGA_Y,
@@ -781,19 +786,86 @@ __device__ inline void pull_one(const PopView<S> &pop, int j, int idx, int ix, i
     }
 }
 
-// Corrected post-stream populations of the special nodes, one 32-lane group per node and one
-// lane per population j.
-//   link node: lane j pulls population j (streaming rules); when the pull source x - c_j is a
-//              solid node linked to x (the link x -> x + c_i with i = opp(j)), the value is the
-//              bounce-back of that link instead (bounce_back_particles /
-//              bounce_back_thermal_particles, ibb.cu:5-136, with the state before this step's
-//              motion);
-//   new fluid: lane j is population j of refill_new_fluid_kernel (refill.cu:121-215) from the
-//              post-collision state at neighbours that were fluid in the previous geometry, with
-//              the moved state for the fallback equilibrium; lane 0 then forms rho, u, phi
-//              (refill.cu:217-235) in population order and sets F = 0.
+// Corrected post-stream populations of the IBB link nodes (the head of the special list), one
+// thread per node: the pull of every population (streaming rules, stored as they arrive), then,
+// for every link of the node (x -> x + c_ip with x + c_ip solid and linked), the bounce-back
+// value of population opp(ip) in place of its pull (bounce_back_particles /
+// bounce_back_thermal_particles, ibb.cu:5-136, with the state before this step's motion).
+// Consecutive list entries are mostly consecutive nodes, so the loads and stores of a wave are
+// contiguous runs.
 template <class S>
-__global__ __launch_bounds__(128) void k_prefixup(PartGrid g) {
+__global__ __launch_bounds__(128) void k_fixup_links(PartGrid g) {
+    const int slot = slot_of(g.lstart, g.nslots, blockIdx.x);
+    const PartSlot &sl = g.tab[slot];
+    const PartDev &pd = sl.pd;
+    const Dims d = sl.d;
+    using A = typename Enc<S>::A;
+    const PopView<S> pop{(const S *)sl.fcur, (const S *)sl.gcur, (size_t)d.n, pd.tl0, pd.tls, d.lx};
+    const int32_t *owner_prev = sl.owner_prev;
+    const A *fx_field = (const A *)sl.fx;
+    const A *rho_wall = (const A *)sl.rho_wall;
+    const double *macro_rho = sl.macro[0];
+    const int blk = blockIdx.x - g.lstart[slot];
+    const int nblk = g.lstart[slot + 1] - g.lstart[slot];
+    const int nlink = min(pd.sp.count[1], pd.sp.cap);
+    const size_t cap = (size_t)pd.sp.cap;
+    A *out = (A *)pd.sp.pop;
+    const PView pv{pd.old[OLD_PX], pd.old[OLD_PY], pd.old[OLD_PZ], pd.old[OLD_VX],
+                   pd.old[OLD_VY], pd.old[OLD_VZ], pd.old[OLD_WX], pd.old[OLD_WY],
+                   pd.old[OLD_WZ], pd.old[OLD_T]};
+    for (int j = blk * 128 + threadIdx.x; j < nlink; j += nblk * 128) {
+        const int idx = pd.sp.node[j];
+        const int iz = idx / d.lxy;
+        const int rem = idx - iz * d.lxy;
+        const int iy = rem / d.lx;
+        const int ix = rem - iy * d.lx;
+        double rho_w = 0.0;
+        if (ix == 0) rho_w = double(rho_wall[iz * d.ly + iy]);
+        else if (ix == d.lx - 1) rho_w = double(rho_wall[d.ly * d.lz + iz * d.ly + iy]);
+        // pulls, nine populations at a time
+        constexpr int CH = 9;
+#pragma unroll
+        for (int c = 0; c < NPOP; c += CH) {
+            double fv[CH], gv[CH];
+#pragma unroll
+            for (int u = 0; u < CH; u++)
+                pull_one<S>(pop, c + u, idx, ix, iy, iz, d, rho_w, pd.thot, pd.tcold, fv[u], gv[u]);
+#pragma unroll
+            for (int u = 0; u < CH; u++) {
+                out[(size_t)(c + u) * cap + j] = A(fv[u]);
+                out[(size_t)(NPOP + c + u) * cap + j] = A(gv[u]);
+            }
+        }
+        pd.sp.fx_old[j] = double(fx_field[idx]);
+        const double x0 = static_cast<double>(ix) + 0.5;
+        const double y0 = static_cast<double>(iy) + 0.5;
+        const double z0 = static_cast<double>(iz) + 0.5;
+        const double density = macro_rho[idx];
+        int own[NPOP];
+        neighbour_owners(owner_prev, d, ix, iy, iz, own);
+        for (int ip = 1; ip < NPOP; ip++) {
+            const int q_owner = own[ip];
+            if (q_owner < 0) continue;
+            double q = 0.0;
+            if (!link_q(ip, x0, y0, z0, pv.px[q_owner], pv.py[q_owner], pv.pz[q_owner],
+                        pd.radius_sq, q))
+                continue;
+            const LinkLoads Ld = link_loads<S>(ip, ix, iy, iz, pop, owner_prev, d);
+            const LinkOut L = eval_link_at(Ld, ip, ix, iy, iz, q, q_owner, pv, density, pd.rho0, d);
+            const int opp = c_opp[ip];
+            out[(size_t)opp * cap + j] = A(L.f_out);
+            out[(size_t)(NPOP + opp) * cap + j] = A(L.g_out);
+        }
+    }
+}
+
+// Populations of the refilled nodes (the tail of the special list), one 32-lane group per node
+// and one lane per population j: refill_new_fluid_kernel (refill.cu:121-215) from the
+// post-collision state at neighbours that were fluid in the previous geometry, with the moved
+// state for the fallback equilibrium; lane 0 then forms rho, u, phi (refill.cu:217-235) in
+// population order and sets F = 0.
+template <class S>
+__global__ __launch_bounds__(128) void k_fixup_refill(PartGrid g) {
     const int slot = slot_of(g.sstart, g.nslots, blockIdx.x);
     const PartSlot &sl = g.tab[slot];
     const PartDev &pd = sl.pd;
@@ -802,149 +874,117 @@ __global__ __launch_bounds__(128) void k_prefixup(PartGrid g) {
     const PopView<S> pop{(const S *)sl.fcur, (const S *)sl.gcur, (size_t)d.n, pd.tl0, pd.tls, d.lx};
     const int32_t *owner_prev = sl.owner_prev;
     A *fx_field = (A *)sl.fx;
-    const A *rho_wall = (const A *)sl.rho_wall;
     double *macro_rho = sl.macro[0], *macro_ux = sl.macro[1], *macro_uy = sl.macro[2];
     double *macro_uz = sl.macro[3], *macro_phi = sl.macro[4];
     const int blk = blockIdx.x - g.sstart[slot];
     const int nblk = g.sstart[slot + 1] - g.sstart[slot];
     const int grp = threadIdx.x >> 5, lane = threadIdx.x & 31;
-    const int count = min(*pd.sp.count, pd.sp.cap);
+    const int count = min(pd.sp.count[0], pd.sp.cap);
+    const int first = min(pd.sp.count[1], pd.sp.cap);
     const size_t cap = (size_t)pd.sp.cap;
-    double *out = pd.sp.pop;
+    A *out = (A *)pd.sp.pop;
+    const double lambda0 = 1.0 - wt(0);
+    const double RT = 1.0 / 3.0;
     // each 32-lane group walks its own nodes: no workgroup barrier couples the groups
-    for (int j = blk * 4 + grp; j < count; j += nblk * 4) {
+    for (int j = first + blk * 4 + grp; j < count; j += nblk * 4) {
         const int idx = pd.sp.node[j];
-        const int kind = pd.sp_kind[j];
+        const int pid = pd.sp_kind[j];
         const int iz = idx / d.lxy;
         const int rem = idx - iz * d.lxy;
         const int iy = rem / d.lx;
         const int ix = rem - iy * d.lx;
         double fv = 0.0, gv = 0.0;
         if (lane < NPOP) {
-            if (kind < 0) {
-                double rho_w = 0.0;
-                if (ix == 0) rho_w = double(rho_wall[iz * d.ly + iy]);
-                else if (ix == d.lx - 1) rho_w = double(rho_wall[d.ly * d.lz + iz * d.ly + iy]);
-                const int ip = c_opp[lane];
-                const int nx = ix + c_cx[ip], ny = iy + c_cy[ip], nz = iz + c_cz[ip];
-                const bool nin = lane != 0 && in_domain(d, nx, ny, nz);
-                // the solid candidate's owner is loaded with the pull
-                const int q_owner = nin ? owner_prev[nz * d.lxy + ny * d.lx + nx] : -1;
-                pull_one<S>(pop, lane, idx, ix, iy, iz, d, rho_w, pd.thot, pd.tcold, fv, gv);
-                if (q_owner >= 0) {
-                    // this link's lattice values, the sphere's state and the node's density are
-                    // loaded before the link test so that they are in flight together
-                    const LinkLoads Ld = link_loads<S>(ip, ix, iy, iz, pop, owner_prev, d);
-                    const PView pv{pd.old[OLD_PX], pd.old[OLD_PY], pd.old[OLD_PZ], pd.old[OLD_VX],
-                                   pd.old[OLD_VY], pd.old[OLD_VZ], pd.old[OLD_WX], pd.old[OLD_WY],
-                                   pd.old[OLD_WZ], pd.old[OLD_T]};
-                    const double density = macro_rho[idx];
-                    double q = 0.0;
-                    if (link_q(ip, static_cast<double>(ix) + 0.5, static_cast<double>(iy) + 0.5,
-                               static_cast<double>(iz) + 0.5, pv.px[q_owner], pv.py[q_owner],
-                               pv.pz[q_owner], pd.radius_sq, q)) {
-                        const LinkOut L = eval_link_at(Ld, ip, ix, iy, iz, q, q_owner, pv,
-                                                       density, pd.rho0, d);
-                        fv = L.f_out;
-                        gv = L.g_out;
+            const int ipop = lane;
+            double temp_f = 0.0, temp_g = 0.0;
+            int sample_count = 0;
+            for (int ip2 = 1; ip2 < NPOP; ++ip2) {
+                const int ix1 = ix + c_cx[ip2], iy1 = iy + c_cy[ip2], iz1 = iz + c_cz[ip2];
+                if (!in_domain(d, ix1, iy1, iz1)) continue;
+                const int idx1 = iz1 * d.lxy + iy1 * d.lx + ix1;
+                if (owner_prev[idx1] != -1) continue;
+                const int ix2 = ix + 2 * c_cx[ip2], iy2 = iy + 2 * c_cy[ip2], iz2 = iz + 2 * c_cz[ip2];
+                if (in_domain(d, ix2, iy2, iz2)) {
+                    const int idx2 = iz2 * d.lxy + iy2 * d.lx + ix2;
+                    if (owner_prev[idx2] == -1) {
+                        temp_f += 2.0 * pop.F(ipop, idx1) - pop.F(ipop, idx2);
+                        temp_g += 2.0 * pop.G(ipop, idx1) - pop.G(ipop, idx2);
+                        ++sample_count;
+                        continue;
                     }
                 }
-            } else {
-                const int ipop = lane, pid = kind;
-                double temp_f = 0.0, temp_g = 0.0;
-                int sample_count = 0;
-                for (int ip2 = 1; ip2 < NPOP; ++ip2) {
-                    const int ix1 = ix + c_cx[ip2], iy1 = iy + c_cy[ip2], iz1 = iz + c_cz[ip2];
-                    if (!in_domain(d, ix1, iy1, iz1)) continue;
-                    const int idx1 = iz1 * d.lxy + iy1 * d.lx + ix1;
-                    if (owner_prev[idx1] != -1) continue;
-                    const int ix2 = ix + 2 * c_cx[ip2], iy2 = iy + 2 * c_cy[ip2], iz2 = iz + 2 * c_cz[ip2];
-                    if (in_domain(d, ix2, iy2, iz2)) {
-                        const int idx2 = iz2 * d.lxy + iy2 * d.lx + ix2;
-                        if (owner_prev[idx2] == -1) {
-                            temp_f += 2.0 * pop.F(ipop, idx1) - pop.F(ipop, idx2);
-                            temp_g += 2.0 * pop.G(ipop, idx1) - pop.G(ipop, idx2);
-                            ++sample_count;
-                            continue;
-                        }
-                    }
-                    temp_f += pop.F(ipop, idx1);
-                    temp_g += pop.G(ipop, idx1);
-                    ++sample_count;
-                }
-                if (sample_count == 0) {
-                    const double lambda0 = 1.0 - wt(0);
-                    const double RT = 1.0 / 3.0;
-                    const double px = pd.p[RBTP_P_POS_X][pid], py = pd.p[RBTP_P_POS_Y][pid], pz = pd.p[RBTP_P_POS_Z][pid];
-                    const double vx = pd.p[RBTP_P_VEL_X][pid], vy = pd.p[RBTP_P_VEL_Y][pid], vz = pd.p[RBTP_P_VEL_Z][pid];
-                    const double omega_x = pd.p[RBTP_P_OMEGA_X][pid], omega_y = pd.p[RBTP_P_OMEGA_Y][pid], omega_z = pd.p[RBTP_P_OMEGA_Z][pid];
-                    const double tempT = pd.p[RBTP_P_TEMP][pid];
-                    const double x = static_cast<double>(ix) + 0.5;
-                    const double y = static_cast<double>(iy) + 0.5;
-                    const double z = static_cast<double>(iz) + 0.5;
-                    const double relx = x - px, rely = y - py, relz = z - pz;
-                    const double u = vx + (relz * omega_y - rely * omega_z);
-                    const double v = vy + (relx * omega_z - relz * omega_x);
-                    const double w = vz + (rely * omega_x - relx * omega_y);
-                    const double r = pd.rho0;
-                    const double eu = c_cx[ipop] * u + c_cy[ipop] * v + c_cz[ipop] * w;
-                    const double uv = (u * u + v * v + w * w) / RT;
-                    temp_f = wt(ipop) * (r + eu + 0.5 * (eu * eu - uv));
-                    temp_g = wt(ipop) * tempT * (1.0 + eu + 0.5 * (eu * eu - uv));
-                    if (ipop == 0) temp_g -= lambda0 * tempT * r;
-                    else temp_g += wt(ipop) * tempT * r;
-                    sample_count = 1;
-                }
-                const double inv_count = 1.0 / static_cast<double>(sample_count);
-                fv = temp_f * inv_count;
-                gv = temp_g * inv_count;
+                temp_f += pop.F(ipop, idx1);
+                temp_g += pop.G(ipop, idx1);
+                ++sample_count;
             }
-            out[(size_t)lane * cap + j] = fv;
-            out[(size_t)(NPOP + lane) * cap + j] = gv;
+            if (sample_count == 0) {
+                const double px = pd.p[RBTP_P_POS_X][pid], py = pd.p[RBTP_P_POS_Y][pid], pz = pd.p[RBTP_P_POS_Z][pid];
+                const double vx = pd.p[RBTP_P_VEL_X][pid], vy = pd.p[RBTP_P_VEL_Y][pid], vz = pd.p[RBTP_P_VEL_Z][pid];
+                const double omega_x = pd.p[RBTP_P_OMEGA_X][pid], omega_y = pd.p[RBTP_P_OMEGA_Y][pid], omega_z = pd.p[RBTP_P_OMEGA_Z][pid];
+                const double tempT = pd.p[RBTP_P_TEMP][pid];
+                const double x = static_cast<double>(ix) + 0.5;
+                const double y = static_cast<double>(iy) + 0.5;
+                const double z = static_cast<double>(iz) + 0.5;
+                const double relx = x - px, rely = y - py, relz = z - pz;
+                const double u = vx + (relz * omega_y - rely * omega_z);
+                const double v = vy + (relx * omega_z - relz * omega_x);
+                const double w = vz + (rely * omega_x - relx * omega_y);
+                const double r = pd.rho0;
+                const double eu = c_cx[ipop] * u + c_cy[ipop] * v + c_cz[ipop] * w;
+                const double uv = (u * u + v * v + w * w) / RT;
+                temp_f = wt(ipop) * (r + eu + 0.5 * (eu * eu - uv));
+                temp_g = wt(ipop) * tempT * (1.0 + eu + 0.5 * (eu * eu - uv));
+                if (ipop == 0) temp_g -= lambda0 * tempT * r;
+                else temp_g += wt(ipop) * tempT * r;
+                sample_count = 1;
+            }
+            const double inv_count = 1.0 / static_cast<double>(sample_count);
+            fv = temp_f * inv_count;
+            gv = temp_g * inv_count;
+            out[(size_t)lane * cap + j] = A(fv);
+            out[(size_t)(NPOP + lane) * cap + j] = A(gv);
         }
-        if (kind < 0) {
-            if (lane == 0) pd.sp.fx_old[j] = double(fx_field[idx]);
-        } else {
-            // the group's 27 populations, read across its lanes in population order (every
-            // lane of the group is active here: kind is the same for the whole group)
-            double rho_loc = 0.0, mom_x = 0.0, mom_y = 0.0, mom_z = 0.0, phi_loc = 0.0;
-            for (int ip = 0; ip < NPOP; ip++) {
-                const double f_val = __shfl(fv, ip, 32);
-                const double g_val = __shfl(gv, ip, 32);
-                rho_loc += f_val;
-                mom_x += c_cx[ip] * f_val;
-                mom_y += c_cy[ip] * f_val;
-                mom_z += c_cz[ip] * f_val;
-                phi_loc += g_val;
+        // the group's 27 populations, read across its lanes in population order (all 32 lanes
+        // of the group are active here)
+        double rho_loc = 0.0, mom_x = 0.0, mom_y = 0.0, mom_z = 0.0, phi_loc = 0.0;
+        for (int ip = 0; ip < NPOP; ip++) {
+            const double f_val = __shfl(fv, ip, 32);
+            const double g_val = __shfl(gv, ip, 32);
+            rho_loc += f_val;
+            mom_x += c_cx[ip] * f_val;
+            mom_y += c_cy[ip] * f_val;
+            mom_z += c_cz[ip] * f_val;
+            phi_loc += g_val;
+        }
+        if (lane == 0) {
+            macro_rho[idx] = rho_loc;
+            if (rho_loc > 1e-12) {
+                const double inv_rho = 1.0 / rho_loc;
+                macro_ux[idx] = mom_x * inv_rho;
+                macro_uy[idx] = mom_y * inv_rho;
+                macro_uz[idx] = mom_z * inv_rho;
+            } else {
+                macro_ux[idx] = 0.0;
+                macro_uy[idx] = 0.0;
+                macro_uz[idx] = 0.0;
             }
-            if (lane == 0) {
-                macro_rho[idx] = rho_loc;
-                if (rho_loc > 1e-12) {
-                    const double inv_rho = 1.0 / rho_loc;
-                    macro_ux[idx] = mom_x * inv_rho;
-                    macro_uy[idx] = mom_y * inv_rho;
-                    macro_uz[idx] = mom_z * inv_rho;
-                } else {
-                    macro_ux[idx] = 0.0;
-                    macro_uy[idx] = 0.0;
-                    macro_uz[idx] = 0.0;
-                }
-                macro_phi[idx] = phi_loc;
-                fx_field[idx] = A(0.0);
-                pd.sp.fx_old[j] = 0.0;
-            }
+            macro_phi[idx] = phi_loc;
+            fx_field[idx] = A(0.0);
+            pd.sp.fx_old[j] = 0.0;
         }
     }
 }
 
+template <class A>
 __global__ __launch_bounds__(256) void k_overwrite_special(PartDev pd, double *f, double *g, Dims d) {
+    const A *pop = (const A *)pd.sp.pop;
     const int count = min(*pd.sp.count, pd.sp.cap);
     const size_t cap = (size_t)pd.sp.cap, n = (size_t)d.n;
     for (int j = blockIdx.x * 256 + threadIdx.x; j < count; j += gridDim.x * 256) {
         const int idx = pd.sp.node[j];
         for (int i = 0; i < NPOP; i++) {
-            f[(size_t)i * n + idx] = pd.sp.pop[(size_t)i * cap + j];
-            g[(size_t)i * n + idx] = pd.sp.pop[(size_t)(NPOP + i) * cap + j];
+            f[(size_t)i * n + idx] = double(pop[(size_t)i * cap + j]);
+            g[(size_t)i * n + idx] = double(pop[(size_t)(NPOP + i) * cap + j]);
         }
     }
 }
@@ -1055,10 +1095,11 @@ int part_alloc(PartDev &pd, const rbtp_case &c, const rbtp_derived &d, const Dim
     pd.sp.cap = (int)cap;
     PHIP(hipMalloc(&pd.sp.node, cap * sizeof(int32_t)));
     PHIP(hipMalloc(&pd.sp_kind, cap * sizeof(int32_t)));
-    PHIP(hipMalloc(&pd.sp.pop, (size_t)2 * NPOP * cap * sizeof(double)));
+    pd.sp_elem = c.precision == RBTP_PREC_FP64 ? sizeof(double) : sizeof(float);
+    PHIP(hipMalloc(&pd.sp.pop, (size_t)2 * NPOP * cap * pd.sp_elem));
     PHIP(hipMalloc(&pd.sp.fx_old, cap * sizeof(double)));
-    PHIP(hipMalloc(&pd.sp.count, sizeof(int)));
-    PHIP(hipMemset(pd.sp.count, 0, sizeof(int)));
+    PHIP(hipMalloc(&pd.sp.count, 2 * sizeof(int)));
+    PHIP(hipMemset(pd.sp.count, 0, 2 * sizeof(int)));
     PHIP(hipMalloc(&pd.sp.map, (size_t)dims.n * sizeof(int32_t)));
     PHIP(hipMemset(pd.sp.map, 0xFF, (size_t)dims.n * sizeof(int32_t)));
     PHIP(hipMalloc(&pd.err, ERR_COUNT * sizeof(int32_t)));
@@ -1164,7 +1205,7 @@ int part_load_state(PartDev &pd, const double *host, const Dims &dims, hipStream
     PHIP(hipMemcpyAsync(pd.owner[1], pd.owner[0], (size_t)dims.n * sizeof(int32_t), hipMemcpyDeviceToDevice, st));
     PHIP(hipMemcpyAsync(pd.box[1], pd.box[0], 6 * (size_t)np * sizeof(int32_t), hipMemcpyDeviceToDevice, st));
     PHIP(hipMemsetAsync(pd.err, 0, ERR_COUNT * sizeof(int32_t), st));
-    PHIP(hipMemsetAsync(pd.sp.count, 0, sizeof(int), st));
+    PHIP(hipMemsetAsync(pd.sp.count, 0, 2 * sizeof(int), st));
     PHIP(hipMemsetAsync(pd.sp.map, 0xFF, (size_t)dims.n * sizeof(int32_t), st));
     PHIP(hipStreamSynchronize(st));
     pd.geo_cur = 0;
@@ -1197,12 +1238,12 @@ int part_init(PartDev &pd, const rbtp_case &c, const rbtp_derived &d, const Dims
     return part_load_state(pd, host.data(), dims, st);
 }
 
-int part_fixup_blocks(const PartDev &pd) {
+int part_fixup_blocks(const PartDev &pd, int per_block) {
     // link nodes lie in a shell about one lattice spacing thick around each sphere
     const double r = pd.radius + 1.0;
     const double est = (double)pd.np * (4.0 * 3.141592653589793 * r * r * 2.0) + 64.0;
     const double capped = std::min(est, (double)pd.sp.cap);
-    return (int)std::min(std::max(std::ceil(capped / 4.0), 1.0), 16384.0);
+    return (int)std::min(std::max(std::ceil(capped / per_block), 1.0), 16384.0);
 }
 
 void part_tables_free(PartTables &t) {
@@ -1218,7 +1259,7 @@ int part_tables_build(PartTables &t, const std::vector<PartSlot> (&slots)[2], hi
     const int n = (int)slots[0].size();
     const size_t tab_bytes = ((sizeof(PartSlot) * (size_t)(n > 0 ? n : 1) + 255) / 256) * 256;
     const size_t pre_bytes = ((sizeof(int) * (size_t)(n + 1) + 255) / 256) * 256;
-    const size_t need = 2 * tab_bytes + 3 * pre_bytes;
+    const size_t need = 2 * tab_bytes + 4 * pre_bytes;
     if (t.bytes < need) {
         (void)hipFree(t.mem);
         t.mem = nullptr;
@@ -1226,18 +1267,19 @@ int part_tables_build(PartTables &t, const std::vector<PartSlot> (&slots)[2], hi
         PHIP(hipMalloc(&t.mem, need));
         t.bytes = need;
     }
-    std::vector<int> pre[3];
+    std::vector<int> pre[4];
     for (auto &v : pre) v.assign(n + 1, 0);
     for (int j = 0; j < n; j++) {
         const PartDev &pd = slots[0][j].pd;
         pre[0][j + 1] = pre[0][j] + pd.np;
         pre[1][j + 1] = pre[1][j] + (pd.use_cells ? pd.ncell[0] * pd.ncell[1] * pd.ncell[2] : 0);
-        pre[2][j + 1] = pre[2][j] + part_fixup_blocks(pd);
+        pre[2][j + 1] = pre[2][j] + part_fixup_blocks(pd, 4);
+        pre[3][j + 1] = pre[3][j] + part_fixup_blocks(pd, 128);
     }
     std::vector<char> host(need, 0);
     for (int ph = 0; ph < 2; ph++)
         if (n > 0) std::memcpy(host.data() + ph * tab_bytes, slots[ph].data(), sizeof(PartSlot) * n);
-    for (int k = 0; k < 3; k++)
+    for (int k = 0; k < 4; k++)
         std::memcpy(host.data() + 2 * tab_bytes + k * pre_bytes, pre[k].data(), sizeof(int) * (n + 1));
     PHIP(hipMemcpyAsync(t.mem, host.data(), need, hipMemcpyHostToDevice, st));
     PHIP(hipStreamSynchronize(st));
@@ -1247,10 +1289,12 @@ int part_tables_build(PartTables &t, const std::vector<PartSlot> (&slots)[2], hi
         g.pstart = (const int *)(t.mem + 2 * tab_bytes);
         g.cstart = (const int *)(t.mem + 2 * tab_bytes + pre_bytes);
         g.sstart = (const int *)(t.mem + 2 * tab_bytes + 2 * pre_bytes);
+        g.lstart = (const int *)(t.mem + 2 * tab_bytes + 3 * pre_bytes);
         g.nslots = n;
         g.total_p = pre[0][n];
         g.total_c = pre[1][n];
         g.total_s = pre[2][n];
+        g.total_l = pre[3][n];
     }
     return RBTP_OK;
 }
@@ -1286,7 +1330,10 @@ int part_phase(const PartGrid &g, int prec, hipStream_t st) {
     PHIP(hipGetLastError());
     PHIP(dispatch_prec(prec, [&](auto *tag) {
         using S = std::remove_pointer_t<decltype(tag)>;
-        hipLaunchKernelGGL((k_prefixup<S>), g.total_s, 128, 0, st, g);
+        hipLaunchKernelGGL((k_fixup_links<S>), g.total_l, 128, 0, st, g);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL((k_fixup_refill<S>), g.total_s, 128, 0, st, g);
         return hipGetLastError();
     }));
     return RBTP_OK;
@@ -1297,15 +1344,18 @@ int part_check_errors(PartDev &pd) {
     PHIP(hipMemcpy(h, pd.err, sizeof(h), hipMemcpyDeviceToHost));
     if (h[ERR_SPECIAL_CAP])
         return report(RBTP_E_CAPACITY, "special-node list overflowed its capacity");
-    if (h[ERR_CONTACT_CAP])
-        return report(RBTP_E_CAPACITY, "contact list overflowed its capacity");
     return RBTP_OK;
 }
 
 hipError_t part_overwrite_special(PartDev &pd, double *f, double *g, const Dims &dims,
                                   hipStream_t st) {
     if (!pd.special_pending) return hipSuccess;
-    hipLaunchKernelGGL(k_overwrite_special, grid_for(pd.sp.cap, 256, 1024), 256, 0, st, pd, f, g, dims);
+    if (pd.sp_elem == sizeof(double))
+        hipLaunchKernelGGL((k_overwrite_special<double>), grid_for(pd.sp.cap, 256, 1024), 256, 0, st,
+                           pd, f, g, dims);
+    else
+        hipLaunchKernelGGL((k_overwrite_special<float>), grid_for(pd.sp.cap, 256, 1024), 256, 0, st,
+                           pd, f, g, dims);
     return hipGetLastError();
 }
 

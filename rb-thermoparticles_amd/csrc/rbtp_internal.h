@@ -78,9 +78,10 @@ template <class S> struct StepArgs {
     double *rho_full;              // rho of this step at every node (particle coupling), or null
     // Special nodes (particle coupling), or null: sp_map[idx] is the node's entry in the
     // special list (-1 elsewhere); an entry holds the node's corrected post-stream populations
-    // (sp_pop[i*sp_cap + j], f_0..f_26 then g_0..g_26) and the F_x entering its macrovar.
+    // (sp_pop[i*sp_cap + j], f_0..f_26 then g_0..g_26, in the auxiliary type) and the F_x
+    // entering its macrovar.
     const int32_t *sp_map;
-    const double *sp_pop;
+    const typename Enc<S>::A *sp_pop;
     const double *sp_fx_old;
     size_t sp_cap;
     double *macro_rho, *macro_ux, *macro_uy, *macro_uz, *macro_phi;  // output fields, or null
@@ -93,9 +94,11 @@ template <class S> struct StepArgs {
 // place of its own pull at those nodes.
 struct SpecialBuf {
     int *node;       // lattice index, [cap]
-    double *pop;     // [54][cap]: f_0..f_26 then g_0..g_26 (post-stream, post-IBB/refill)
+    void *pop;       // [54][cap]: f_0..f_26 then g_0..g_26 (post-stream, post-IBB/refill), in
+                     // the auxiliary type of the precision form (FP64 only for the FP64 form;
+                     // the other forms round them to FP32 when their lattice kernel reads them)
     double *fx_old;  // F_x entering macrovar at this node, [cap]
-    int *count;      // device counter
+    int *count;      // device counters: [0] list length, [1] IBB link nodes at its head
     int cap;
     int32_t *map;    // [n]: list entry of each node, -1 when it is not special
 };

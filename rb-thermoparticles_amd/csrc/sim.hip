@@ -326,7 +326,7 @@ static StepArgs<S> step_args(rbtp_sim *s, bool macro_out) {
     a.rho_wall = (typename Enc<S>::A *)s->rho_wall;
     a.rho_full = s->particles ? s->macro[0] : nullptr;
     a.sp_map = s->particles ? s->pd.sp.map : nullptr;
-    a.sp_pop = s->particles ? s->pd.sp.pop : nullptr;
+    a.sp_pop = s->particles ? (const typename Enc<S>::A *)s->pd.sp.pop : nullptr;
     a.sp_fx_old = s->particles ? s->pd.sp.fx_old : nullptr;
     a.sp_cap = s->particles ? (size_t)s->pd.sp.cap : 0;
     if (macro_out) {
@@ -638,7 +638,7 @@ extern "C" int rbtp_set_populations(rbtp_sim *s, const double *f, const double *
     if (s->particles) {
         // the buffered special-node populations no longer belong to this state
         s->pd.special_pending = false;
-        HIP_TRY(hipMemsetAsync(s->pd.sp.count, 0, sizeof(int), s->stream));
+        HIP_TRY(hipMemsetAsync(s->pd.sp.count, 0, 2 * sizeof(int), s->stream));
         HIP_TRY(hipMemsetAsync(s->pd.sp.map, 0xFF, (size_t)s->dims.n * sizeof(int32_t), s->stream));
         HIP_TRY(hipStreamSynchronize(s->stream));
     }
@@ -1011,8 +1011,9 @@ static size_t sim_device_bytes(const rbtp_case &c, const rbtp_derived &d) {
         const long long w = (long long)std::ceil(2.0 * d.particle_radius) + 1;
         long long cap = (long long)np * ((w + 2) * (w + 2) * (w + 2) + w * w * w);
         if (cap > (long long)n) cap = (long long)n;
-        b += (RBTP_P_COUNT + OLD_COUNT) * np * sizeof(double) + 2 * n * sizeof(int32_t) +
-             12 * np * sizeof(int32_t) + (size_t)cap * (2 * 27 * sizeof(double) + sizeof(double) + 2 * sizeof(int32_t));
+        b += (RBTP_P_COUNT + OLD_COUNT) * np * sizeof(double) + 3 * n * sizeof(int32_t) +
+             12 * np * sizeof(int32_t) +
+             (size_t)cap * (2 * 27 * aux_bytes(c.precision) + sizeof(double) + 2 * sizeof(int32_t));
     }
     return b;
 }
