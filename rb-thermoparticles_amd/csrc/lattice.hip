@@ -234,22 +234,22 @@ __device__ inline void node_step(const StepArgs<S> &a, int idx) {
     }
     T f[NPOP], g[NPOP];
     const T tl0 = T(a.k.tl0), tls = T(a.k.tls);
-    // A special node (IBB link node or refilled node) takes the post-stream populations the
-    // particle phase corrected; the first step after a (re)load has none.
+    gather<S, T, NOSTREAM>(a.fin, a.gin, idx, ix, iy, iz, a.d, rho_w, tl0, tls, T(a.k.thot),
+                           T(a.k.tcold), f, g);
+    // At a special node (IBB link node or refilled node) the populations the particle phase
+    // corrected replace the pulled ones; the first step after a (re)load has none.
     const int sj = (!NOSTREAM && a.sp_map) ? a.sp_map[idx] : -1;
-    T fx_old;
     if (sj >= 0) {
+        const uint32_t m = a.sp_mask[sj];
 #pragma unroll
         for (int i = 0; i < NPOP; i++) {
-            f[i] = T(a.sp_pop[(size_t)i * a.sp_cap + sj]);
-            g[i] = T(a.sp_pop[(size_t)(NPOP + i) * a.sp_cap + sj]);
+            if (m & (1u << i)) {
+                f[i] = T(a.sp_pop[(size_t)i * a.sp_cap + sj]);
+                g[i] = T(a.sp_pop[(size_t)(NPOP + i) * a.sp_cap + sj]);
+            }
         }
-        fx_old = T(a.sp_fx_old[sj]);
-    } else {
-        gather<S, T, NOSTREAM>(a.fin, a.gin, idx, ix, iy, iz, a.d, rho_w, tl0, tls,
-                               T(a.k.thot), T(a.k.tcold), f, g);
-        fx_old = T(a.fx[idx]);
     }
+    const T fx_old = T(a.fx[idx]);
     NodeOut<S> o{a.fout, a.gout};
     T rho, ux, uy, uz, phi;
     const T fx = collide_node<S, T, STRICT>(f, g, fx_old, idx, tl0 + tls * T(ix), o, a.k, rho,

@@ -321,30 +321,38 @@ __global__ __launch_bounds__(128) void k_contacts(PartGrid g) {
             const int cx = cell_coord(xi, pd.cell_scale[0], pd.ncell[0]);
             const int cy = cell_coord(yi, pd.cell_scale[1], pd.ncell[1]);
             const int cz = cell_coord(zi, pd.cell_scale[2], pd.ncell[2]);
-            ncand = 0;
-            for (int z = max(cz - 1, 0); z <= min(cz + 1, pd.ncell[2] - 1) && ncand >= 0; z++)
-                for (int y = max(cy - 1, 0); y <= min(cy + 1, pd.ncell[1] - 1) && ncand >= 0; y++)
-                    for (int x = max(cx - 1, 0); x <= min(cx + 1, pd.ncell[0] - 1) && ncand >= 0; x++) {
-                        const int c = (z * pd.ncell[1] + y) * pd.ncell[0] + x;
-                        const int m = pd.cell_cnt[c];
-                        if (m > CELL_CAP || ncand + m > CAND_MAX) {
-                            ncand = -1;
-                            break;
+            // the (up to) 27 neighbour cells and their counts, loaded together
+            int cell[27], cnt[27];
+            int nc = 0;
+            for (int z = max(cz - 1, 0); z <= min(cz + 1, pd.ncell[2] - 1); z++)
+                for (int y = max(cy - 1, 0); y <= min(cy + 1, pd.ncell[1] - 1); y++)
+                    for (int x = max(cx - 1, 0); x <= min(cx + 1, pd.ncell[0] - 1); x++)
+                        cell[nc++] = (z * pd.ncell[1] + y) * pd.ncell[0] + x;
+            int total = 0;
+            bool full = false;
+            for (int q = 0; q < nc; q++) {
+                cnt[q] = pd.cell_cnt[cell[q]];
+                full = full || cnt[q] > CELL_CAP;
+                total += cnt[q];
+            }
+            if (!full && total <= CAND_MAX) {
+                ncand = 0;
+                for (int q = 0; q < nc; q++)
+                    for (int k = 0; k < cnt[q]; k++) {
+                        // insertion in ascending pid order
+                        const int32_t v = pd.cell_pid[(size_t)cell[q] * CELL_CAP + k];
+                        int pos = ncand;
+                        while (pos > 0 && cand[pos - 1] > v) {
+                            cand[pos] = cand[pos - 1];
+                            --pos;
                         }
-                        for (int k = 0; k < m; k++) {
-                            // insertion in ascending pid order
-                            const int32_t v = pd.cell_pid[(size_t)c * CELL_CAP + k];
-                            int pos = ncand;
-                            while (pos > 0 && cand[pos - 1] > v) {
-                                cand[pos] = cand[pos - 1];
-                                --pos;
-                            }
-                            cand[pos] = v;
-                            ++ncand;
-                        }
+                        cand[pos] = v;
+                        ++ncand;
                     }
+            }
         }
         const int count = ncand < 0 ? pd.np : ncand;
+#pragma unroll 4
         for (int k = 0; k < count; k++) {
             const int j = ncand < 0 ? k : cand[k];
             if (j == i) continue;
@@ -786,13 +794,11 @@ __device__ inline void pull_one(const PopView<S> &pop, int j, int idx, int ix, i
     }
 }
 
-// Corrected post-stream populations of the IBB link nodes (the head of the special list), one
-// thread per node: the pull of every population (streaming rules, stored as they arrive), then,
-// for every link of the node (x -> x + c_ip with x + c_ip solid and linked), the bounce-back
-// value of population opp(ip) in place of its pull (bounce_back_particles /
-// bounce_back_thermal_particles, ibb.cu:5-136, with the state before this step's motion).
-// Consecutive list entries are mostly consecutive nodes, so the loads and stores of a wave are
-// contiguous runs.
+// Bounce-back values of the IBB link nodes (the head of the special list), one thread per node:
+// for every link x -> x + c_ip (x + c_ip solid and linked, with the state before this step's
+// motion) the values bounce_back_particles / bounce_back_thermal_particles write into f[opp][x]
+// and g[opp][x] (ibb.cu:5-136), and the mask of the replaced populations.  The node's other
+// populations are plain pulls, which the next lattice kernel makes itself.
 template <class S>
 __global__ __launch_bounds__(128) void k_fixup_links(PartGrid g) {
     const int slot = slot_of(g.lstart, g.nslots, blockIdx.x);
@@ -802,8 +808,6 @@ __global__ __launch_bounds__(128) void k_fixup_links(PartGrid g) {
     using A = typename Enc<S>::A;
     const PopView<S> pop{(const S *)sl.fcur, (const S *)sl.gcur, (size_t)d.n, pd.tl0, pd.tls, d.lx};
     const int32_t *owner_prev = sl.owner_prev;
-    const A *fx_field = (const A *)sl.fx;
-    const A *rho_wall = (const A *)sl.rho_wall;
     const double *macro_rho = sl.macro[0];
     const int blk = blockIdx.x - g.lstart[slot];
     const int nblk = g.lstart[slot + 1] - g.lstart[slot];
@@ -819,30 +823,13 @@ __global__ __launch_bounds__(128) void k_fixup_links(PartGrid g) {
         const int rem = idx - iz * d.lxy;
         const int iy = rem / d.lx;
         const int ix = rem - iy * d.lx;
-        double rho_w = 0.0;
-        if (ix == 0) rho_w = double(rho_wall[iz * d.ly + iy]);
-        else if (ix == d.lx - 1) rho_w = double(rho_wall[d.ly * d.lz + iz * d.ly + iy]);
-        // pulls, nine populations at a time
-        constexpr int CH = 9;
-#pragma unroll
-        for (int c = 0; c < NPOP; c += CH) {
-            double fv[CH], gv[CH];
-#pragma unroll
-            for (int u = 0; u < CH; u++)
-                pull_one<S>(pop, c + u, idx, ix, iy, iz, d, rho_w, pd.thot, pd.tcold, fv[u], gv[u]);
-#pragma unroll
-            for (int u = 0; u < CH; u++) {
-                out[(size_t)(c + u) * cap + j] = A(fv[u]);
-                out[(size_t)(NPOP + c + u) * cap + j] = A(gv[u]);
-            }
-        }
-        pd.sp.fx_old[j] = double(fx_field[idx]);
         const double x0 = static_cast<double>(ix) + 0.5;
         const double y0 = static_cast<double>(iy) + 0.5;
         const double z0 = static_cast<double>(iz) + 0.5;
         const double density = macro_rho[idx];
         int own[NPOP];
         neighbour_owners(owner_prev, d, ix, iy, iz, own);
+        uint32_t m = 0u;
         for (int ip = 1; ip < NPOP; ip++) {
             const int q_owner = own[ip];
             if (q_owner < 0) continue;
@@ -855,7 +842,9 @@ __global__ __launch_bounds__(128) void k_fixup_links(PartGrid g) {
             const int opp = c_opp[ip];
             out[(size_t)opp * cap + j] = A(L.f_out);
             out[(size_t)(NPOP + opp) * cap + j] = A(L.g_out);
+            m |= 1u << opp;
         }
+        pd.sp.mask[j] = m;
     }
 }
 
@@ -970,11 +959,13 @@ __global__ __launch_bounds__(128) void k_fixup_refill(PartGrid g) {
             }
             macro_phi[idx] = phi_loc;
             fx_field[idx] = A(0.0);
-            pd.sp.fx_old[j] = 0.0;
+            pd.sp.mask[j] = (1u << NPOP) - 1u;
         }
     }
 }
 
+// The replaced populations of the special nodes over a plain pull of the whole lattice (the
+// post-stream accessor): f and g are FP64 in the reference's SoA order.
 template <class A>
 __global__ __launch_bounds__(256) void k_overwrite_special(PartDev pd, double *f, double *g, Dims d) {
     const A *pop = (const A *)pd.sp.pop;
@@ -982,7 +973,9 @@ __global__ __launch_bounds__(256) void k_overwrite_special(PartDev pd, double *f
     const size_t cap = (size_t)pd.sp.cap, n = (size_t)d.n;
     for (int j = blockIdx.x * 256 + threadIdx.x; j < count; j += gridDim.x * 256) {
         const int idx = pd.sp.node[j];
+        const uint32_t m = pd.sp.mask[j];
         for (int i = 0; i < NPOP; i++) {
+            if (!(m & (1u << i))) continue;
             f[(size_t)i * n + idx] = double(pop[(size_t)i * cap + j]);
             g[(size_t)i * n + idx] = double(pop[(size_t)(NPOP + i) * cap + j]);
         }
@@ -1097,7 +1090,7 @@ int part_alloc(PartDev &pd, const rbtp_case &c, const rbtp_derived &d, const Dim
     PHIP(hipMalloc(&pd.sp_kind, cap * sizeof(int32_t)));
     pd.sp_elem = c.precision == RBTP_PREC_FP64 ? sizeof(double) : sizeof(float);
     PHIP(hipMalloc(&pd.sp.pop, (size_t)2 * NPOP * cap * pd.sp_elem));
-    PHIP(hipMalloc(&pd.sp.fx_old, cap * sizeof(double)));
+    PHIP(hipMalloc(&pd.sp.mask, cap * sizeof(uint32_t)));
     PHIP(hipMalloc(&pd.sp.count, 2 * sizeof(int)));
     PHIP(hipMemset(pd.sp.count, 0, 2 * sizeof(int)));
     PHIP(hipMalloc(&pd.sp.map, (size_t)dims.n * sizeof(int32_t)));
@@ -1143,7 +1136,7 @@ void part_free(PartDev &pd) {
     (void)hipFree(pd.sp.node);
     (void)hipFree(pd.sp_kind);
     (void)hipFree(pd.sp.pop);
-    (void)hipFree(pd.sp.fx_old);
+    (void)hipFree(pd.sp.mask);
     (void)hipFree(pd.sp.count);
     (void)hipFree(pd.sp.map);
     (void)hipFree(pd.err);

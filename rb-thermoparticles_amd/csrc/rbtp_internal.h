@@ -76,28 +76,30 @@ template <class S> struct StepArgs {
     typename Enc<S>::A *fx;        // F_x of the previous forcing, updated in place
     typename Enc<S>::A *rho_wall;  // rho at ix=0 (first ly*lz) and ix=lx-1 (next ly*lz), in place
     double *rho_full;              // rho of this step at every node (particle coupling), or null
-    // Special nodes (particle coupling), or null: sp_map[idx] is the node's entry in the
-    // special list (-1 elsewhere); an entry holds the node's corrected post-stream populations
-    // (sp_pop[i*sp_cap + j], f_0..f_26 then g_0..g_26, in the auxiliary type) and the F_x
-    // entering its macrovar.
+    // Special nodes (particle coupling), or null: sp_map[idx] is the node's entry j in the
+    // special list (-1 elsewhere); bit i of sp_mask[j] marks population i as replaced by its
+    // corrected post-stream value (f: sp_pop[i*sp_cap + j], g: sp_pop[(27+i)*sp_cap + j], in the
+    // auxiliary type).
     const int32_t *sp_map;
+    const uint32_t *sp_mask;
     const typename Enc<S>::A *sp_pop;
-    const double *sp_fx_old;
     size_t sp_cap;
     double *macro_rho, *macro_ux, *macro_uy, *macro_uz, *macro_phi;  // output fields, or null
     Dims d;
     FluidK k;
 };
 
-// Post-stream populations of the special nodes (IBB link nodes and refilled nodes), computed by
-// the particle phase from the post-collision state; the next lattice kernel collides them in
-// place of its own pull at those nodes.
+// Post-stream populations of the special nodes (IBB link nodes and refilled nodes) that the
+// particle coupling replaces, computed by the particle phase from the post-collision state; the
+// next lattice kernel uses them in place of its own pull of those populations.  A link node
+// replaces the populations its links bounce back (f[opp][x], ibb.cu:58-71,135); a refilled
+// node replaces all 27 (refill.cu:214-215).
 struct SpecialBuf {
     int *node;       // lattice index, [cap]
     void *pop;       // [54][cap]: f_0..f_26 then g_0..g_26 (post-stream, post-IBB/refill), in
                      // the auxiliary type of the precision form (FP64 only for the FP64 form;
                      // the other forms round them to FP32 when their lattice kernel reads them)
-    double *fx_old;  // F_x entering macrovar at this node, [cap]
+    uint32_t *mask;  // [cap]: bit i set when population i (f and g) is replaced
     int *count;      // device counters: [0] list length, [1] IBB link nodes at its head
     int cap;
     int32_t *map;    // [n]: list entry of each node, -1 when it is not special

@@ -327,7 +327,7 @@ static StepArgs<S> step_args(rbtp_sim *s, bool macro_out) {
     a.rho_full = s->particles ? s->macro[0] : nullptr;
     a.sp_map = s->particles ? s->pd.sp.map : nullptr;
     a.sp_pop = s->particles ? (const typename Enc<S>::A *)s->pd.sp.pop : nullptr;
-    a.sp_fx_old = s->particles ? s->pd.sp.fx_old : nullptr;
+    a.sp_mask = s->particles ? s->pd.sp.mask : nullptr;
     a.sp_cap = s->particles ? (size_t)s->pd.sp.cap : 0;
     if (macro_out) {
         a.macro_rho = s->macro[0];
