@@ -580,19 +580,17 @@ __global__ __launch_bounds__(LB) void k_link_sums(PartGrid g) {
         for (int k = t; k < cnt; k += LB) {
             int ix, iy, iz;
             const int idx = box_node(d, lo, hi, k, ix, iy, iz);
-            bool listed = false;  // this workgroup lists idx as an IBB link node
-            do {
-                if (owner_prev[idx] != -1) break;
-                const double x0 = static_cast<double>(ix) + 0.5;
-                const double y0 = static_cast<double>(iy) + 0.5;
-                const double z0 = static_cast<double>(iz) + 0.5;
+            const double x0 = static_cast<double>(ix) + 0.5;
+            const double y0 = static_cast<double>(iy) + 0.5;
+            const double z0 = static_cast<double>(iz) + 0.5;
+            // links of this fluid node to p, as a direction mask (q is evaluated again below),
+            // and whether it also links to a sphere of lower pid: a node linked to several
+            // spheres is listed once, by the lowest linked pid
+            unsigned mask = 0u;
+            bool lower = false;
+            if (owner_prev[idx] == -1) {
                 int own[NPOP];
                 neighbour_owners(owner_prev, d, ix, iy, iz, own);
-                // links of this node to p, as a direction mask (q is evaluated again below),
-                // and whether it also links to a sphere of lower pid: a node linked to several
-                // spheres is listed once, by the lowest linked pid
-                unsigned mask = 0u;
-                bool lower = false;
                 for (int ip = 1; ip < NPOP; ip++) {
                     const int q_owner = own[ip];
                     double q = 0.0;
@@ -604,40 +602,40 @@ __global__ __launch_bounds__(LB) void k_link_sums(PartGrid g) {
                             lower = true;
                     }
                 }
-                if (mask == 0u) break;
-                listed = !lower;
-                const double density = rho[idx];
-                // links in ascending direction order; each link's lattice loads are issued together
-                while (mask) {
-                    const int ip = __builtin_ctz(mask);
-                    mask &= mask - 1u;
-                    const LinkLoads Ld = link_loads<S>(ip, ix, iy, iz, pop, owner_prev, d);
-                    double q = 0.0;
-                    link_q(ip, x0, y0, z0, X, Y, Z, pd.radius_sq, q);
-                    const LinkOut L = eval_link_at(Ld, ip, ix, iy, iz, q, p, pv, density, pd.rho0, d);
-                    nunh_f += L.unhandled_f;
-                    nunh_g += L.unhandled_g;
-                    const int opp = c_opp[ip];
-                    // Force uses the link record's r = q c (links.cu:134-136, ibb.cu:160-168).
-                    const double rx = q * static_cast<double>(c_cx[ip]);
-                    const double ry = q * static_cast<double>(c_cy[ip]);
-                    const double rz = q * static_cast<double>(c_cz[ip]);
-                    const double relx = ix + 0.5 + rx - X;
-                    const double rely = iy + 0.5 + ry - Y;
-                    const double relz = iz + 0.5 + rz - Z;
-                    const double Fx = (c_cx[ip] - L.uwx) * L.f_in - (c_cx[opp] - L.uwx) * L.f_out;
-                    const double Fy = (c_cy[ip] - L.uwy) * L.f_in - (c_cy[opp] - L.uwy) * L.f_out;
-                    const double Fz = (c_cz[ip] - L.uwz) * L.f_in - (c_cz[opp] - L.uwz) * L.f_out;
-                    s[0] += Fx;
-                    s[1] += Fy;
-                    s[2] += Fz;
-                    s[3] += rely * Fz - relz * Fy;
-                    s[4] += relz * Fx - relx * Fz;
-                    s[5] += relx * Fy - rely * Fx;
-                    s[6] += L.g_in - L.g_out;
-                }
-            } while (0);
-            push_special(pd, listed, idx, -1);
+            }
+            // listed before the sums, so that the list append overlaps the link loads below
+            push_special(pd, mask != 0u && !lower, idx, -1);
+            if (mask == 0u) continue;
+            const double density = rho[idx];
+            // links in ascending direction order; each link's lattice loads are issued together
+            while (mask) {
+                const int ip = __builtin_ctz(mask);
+                mask &= mask - 1u;
+                const LinkLoads Ld = link_loads<S>(ip, ix, iy, iz, pop, owner_prev, d);
+                double q = 0.0;
+                link_q(ip, x0, y0, z0, X, Y, Z, pd.radius_sq, q);
+                const LinkOut L = eval_link_at(Ld, ip, ix, iy, iz, q, p, pv, density, pd.rho0, d);
+                nunh_f += L.unhandled_f;
+                nunh_g += L.unhandled_g;
+                const int opp = c_opp[ip];
+                // Force uses the link record's r = q c (links.cu:134-136, ibb.cu:160-168).
+                const double rx = q * static_cast<double>(c_cx[ip]);
+                const double ry = q * static_cast<double>(c_cy[ip]);
+                const double rz = q * static_cast<double>(c_cz[ip]);
+                const double relx = ix + 0.5 + rx - X;
+                const double rely = iy + 0.5 + ry - Y;
+                const double relz = iz + 0.5 + rz - Z;
+                const double Fx = (c_cx[ip] - L.uwx) * L.f_in - (c_cx[opp] - L.uwx) * L.f_out;
+                const double Fy = (c_cy[ip] - L.uwy) * L.f_in - (c_cy[opp] - L.uwy) * L.f_out;
+                const double Fz = (c_cz[ip] - L.uwz) * L.f_in - (c_cz[opp] - L.uwz) * L.f_out;
+                s[0] += Fx;
+                s[1] += Fy;
+                s[2] += Fz;
+                s[3] += rely * Fz - relz * Fy;
+                s[4] += relz * Fx - relx * Fz;
+                s[5] += relx * Fy - rely * Fx;
+                s[6] += L.g_in - L.g_out;
+            }
         }
     }
     if (nunh_f) atomicAdd(pd.err + ERR_UNHANDLED_F, nunh_f);

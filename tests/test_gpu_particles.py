@@ -186,3 +186,18 @@ def test_crowded_cell_falls_back_to_all_pairs(rb, O):
     check_particles(sim, ora, "crowded")
     check_geometry(sim, ora, "crowded")
     check_fields(sim, ora, "crowded")
+
+
+def test_particles_enabled_with_none(rb, O):
+    """ACTIVATE_PARTICLES with a particle count of zero runs the single-phase step."""
+    kw = particle_case(20, 16, 12, npart=0, diam=4.0, ra=1e5, beta=0.05)
+    case = rb.default_case(**kw)
+    assert rb.derive(case).npart == 0
+    sim, ora = rb.Sim(case), O.Oracle(case)
+    T = rb.linear_temperature(case, 4, 1e-2)
+    sim.init(temperature=T)
+    ora.init(temperature=T)
+    sim.step(6)
+    ora.step(6)
+    check_fields(sim, ora, "no particles")
+    assert len(sim.links()) == 0
