@@ -11,7 +11,8 @@ state resident in HBM before the timed region starts.  The line also carries:
                 populations read and written plus F_x read and written) over its average launch
                 time measured with HIP events on the library's stream;
   cpu_baseline  the FP64 host oracle (OpenMP) timed on this machine's cores on a bounded sample;
-  forms         the FP32-math/FP16-storage form on the same workload, with its Nu deviation.
+  forms         the FP16-storage (FP32 math), FP32 and BF16-storage forms on the same workload,
+                with their Nu and KE deviations, and the 300-sphere FP64 case.
 """
 import argparse
 import json
@@ -154,7 +155,8 @@ def main():
     ap.add_argument("--nu-window", type=int, default=1000,
                     help="untimed steps after the timed run over which Nu is averaged")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    ap.add_argument("--no-forms", action="store_true", help="skip the FP16-storage form")
+    ap.add_argument("--no-forms", action="store_true",
+                    help="skip the FP16-, FP32- and BF16-storage forms")
     ap.add_argument("--no-particles", action="store_true", help="skip the particle-laden case")
     ap.add_argument("--quiet", action="store_true")
     ap.add_argument("--batch-probe", type=int, default=0, metavar="K",
@@ -211,19 +213,26 @@ def main():
     achieved = BYTES_PER_NODE[0] * n / (r64["kernel_ms"] * 1e-3) / 1e9
     forms = {}
     if not args.no_forms:
-        r16 = run_form(rb, config2(precision=2), args.steps, args.warmup, args.timing_steps, log,
-                       args.nu_window)
-        m16 = n * args.steps / (r16["wall_ms"] * 1e-3) / 1e6
-        forms["fp16s"] = dict(
-            mlups=m16, ratio_vs_fp64=m16 / mlups, kernel_ms=r16["kernel_ms"],
-            kernel_gbs=BYTES_PER_NODE[2] * n / (r16["kernel_ms"] * 1e-3) / 1e9,
-            nu_bulk=r16["nu"][2], nu_hot=r16["nu"][4],
-            nu_hot_rel_err_vs_fp64=abs(r16["nu"][4] - r64["nu"][4]) / abs(r64["nu"][4]),
-            nu_hot_window_avg=r16["nu_avg"][4], nu_hot_window_avg_fp64=r64["nu_avg"][4],
-            nu_hot_window_rel_err_vs_fp64=abs(r16["nu_avg"][4] - r64["nu_avg"][4]) / abs(r64["nu_avg"][4]),
-            nu_window_steps=args.nu_window,
-            ke_rel_err_vs_fp64=abs(r16["ke"] - r64["ke"]) / abs(r64["ke"]),
-            bytes_per_node=BYTES_PER_NODE[2])
+        # FP16 storage is the headline low-precision form; FP32 and BF16 storage run the same
+        # workload with the Nu window shortened to keep the default run within minutes
+        for name, prec, window in (("fp16s", 2, args.nu_window), ("fp32", 1, 0), ("bf16s", 3, 0)):
+            r = run_form(rb, config2(precision=prec), args.steps, args.warmup, args.timing_steps,
+                         log, window)
+            m = n * args.steps / (r["wall_ms"] * 1e-3) / 1e6
+            form = dict(
+                mlups=m, ratio_vs_fp64=m / mlups, kernel_ms=r["kernel_ms"],
+                kernel_gbs=BYTES_PER_NODE[prec] * n / (r["kernel_ms"] * 1e-3) / 1e9,
+                nu_bulk=r["nu"][2], nu_hot=r["nu"][4],
+                nu_hot_rel_err_vs_fp64=abs(r["nu"][4] - r64["nu"][4]) / abs(r64["nu"][4]),
+                ke_rel_err_vs_fp64=abs(r["ke"] - r64["ke"]) / abs(r64["ke"]),
+                bytes_per_node=BYTES_PER_NODE[prec])
+            if window:
+                form.update(
+                    nu_hot_window_avg=r["nu_avg"][4], nu_hot_window_avg_fp64=r64["nu_avg"][4],
+                    nu_hot_window_rel_err_vs_fp64=abs(r["nu_avg"][4] - r64["nu_avg"][4]) /
+                    abs(r64["nu_avg"][4]),
+                    nu_window_steps=window)
+            forms[name] = form
     if not args.no_particles:
         forms["fp64_particle_laden"] = run_particles(rb, max(args.steps // 3, 10),
                                                      max(args.warmup // 3, 5), log)
