@@ -149,8 +149,8 @@ def run_serving(rb, nreq, log):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3000)
-    ap.add_argument("--warmup", type=int, default=300)
+    ap.add_argument("--steps", type=int, default=10000)
+    ap.add_argument("--warmup", type=int, default=1000)
     ap.add_argument("--timing-steps", type=int, default=100)
     ap.add_argument("--nu-window", type=int, default=1000,
                     help="untimed steps after the timed run over which Nu is averaged")
@@ -234,8 +234,8 @@ def main():
                     nu_window_steps=window)
             forms[name] = form
     if not args.no_particles:
-        forms["fp64_particle_laden"] = run_particles(rb, max(args.steps // 3, 10),
-                                                     max(args.warmup // 3, 5), log)
+        forms["fp64_particle_laden"] = run_particles(rb, max(args.steps // 2, 10),
+                                                     max(args.warmup // 2, 5), log)
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "lattice_fp64_320x320x160_traffic.json")
     if os.path.exists(tpath):

@@ -9,11 +9,12 @@
 // unspecified (FP64 atomicAdd into per-particle accumulators: ibb.cu:182-187,204 and
 // pforces.cu:32-37) the oracle fixes one deterministic order and documents it at the function:
 //   * pair repulsion: partners j in ascending order, then the six walls, then the link sum;
-//   * link sums (force, torque, heat): per particle, links are dealt round-robin to 1024 partial
-//     sums by their node's position k in the particle's link box (k mod 1024, increasing k, then
-//     increasing direction), and the 1024 partials are combined by a pairwise tree
-//     (stride 512, 256, ..., 1).  The link box is the particle's marking box
-//     floor(p -/+ R - 0.5) (links.cu:164-177) grown by one node per side, clamped to the domain.
+//   * link sums (force, torque, heat): per particle, each node's links are summed first (from 0,
+//     in increasing direction) into the node's share; the shares are dealt round-robin to 64
+//     lane sums by the node's position k in the particle's link box (k mod 64, added from 0 in
+//     increasing k), and the 64 lane sums are combined by a pairwise tree (stride 32, 16, ...,
+//     1).  The link box is the particle's marking box floor(p -/+ R - 0.5) (links.cu:164-177)
+//     grown by one node per side, clamped to the domain.
 //
 // Parity status of each function (how it is pinned independently of this restatement) is listed
 // in the header of oracle/oracle.h.
@@ -21,10 +22,12 @@
 #include "oracle.h"
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <map>
 #include <vector>
 #ifdef _OPENMP
 #include <omp.h>
@@ -47,7 +50,7 @@ double weight(int i) {
     return 1.0 / 216.0;
 }
 
-constexpr int RED_LANES = 1024;  // partial sums per particle for the link reductions
+constexpr int RED_LANES = 64;  // lane sums per particle for the link reductions
 
 }  // namespace
 
@@ -866,6 +869,7 @@ void or_forces(OrState *s) {
     const int LX = s->LX, LXY = s->LXY;
     const int np = s->np;
     std::vector<double> part((size_t)np * 6 * RED_LANES, 0.0);
+    std::map<std::pair<int, long>, std::array<double, 6>> share;  // (pid, box position) -> node share
     for (const rbtp_link &link : s->links) {
         const int ip = link.dir;
         const int opp = OPP[ip];
@@ -887,15 +891,18 @@ void or_forces(OrState *s) {
         const double Fx = (CIX[ip] - uwx) * f_in - (CIX[opp] - uwx) * f_out;
         const double Fy = (CIY[ip] - uwy) * f_in - (CIY[opp] - uwy) * f_out;
         const double Fz = (CIZ[ip] - uwz) * f_in - (CIZ[opp] - uwz) * f_out;
-        const long slot = link_slot(s, link);
-        double *b = &part[((size_t)pid * 6) * RED_LANES];
-        const int lane = (int)(slot % RED_LANES);
-        b[0 * RED_LANES + lane] += Fx;
-        b[1 * RED_LANES + lane] += Fy;
-        b[2 * RED_LANES + lane] += Fz;
-        b[3 * RED_LANES + lane] += rely * Fz - relz * Fy;
-        b[4 * RED_LANES + lane] += relz * Fx - relx * Fz;
-        b[5 * RED_LANES + lane] += relx * Fy - rely * Fx;
+        std::array<double, 6> &v = share[{pid, link_slot(s, link)}];
+        v[0] += Fx;
+        v[1] += Fy;
+        v[2] += Fz;
+        v[3] += rely * Fz - relz * Fy;
+        v[4] += relz * Fx - relx * Fz;
+        v[5] += relx * Fy - rely * Fx;
+    }
+    for (const auto &kv : share) {
+        double *b = &part[((size_t)kv.first.first * 6) * RED_LANES];
+        const int lane = (int)(kv.first.second % RED_LANES);
+        for (int k = 0; k < 6; k++) b[k * RED_LANES + lane] += kv.second[k];
     }
     const int dst[6] = {RBTP_P_FORCE_X, RBTP_P_FORCE_Y, RBTP_P_FORCE_Z,
                         RBTP_P_TORQUE_X, RBTP_P_TORQUE_Y, RBTP_P_TORQUE_Z};
@@ -918,15 +925,17 @@ void or_heat(OrState *s) {
     std::fill(s->p[RBTP_P_HEAT].begin(), s->p[RBTP_P_HEAT].end(), 0.0);
     if (s->links.empty()) return;
     std::vector<double> part((size_t)np * RED_LANES, 0.0);
+    std::map<std::pair<int, long>, double> share;  // (pid, box position) -> node share
     for (const rbtp_link &link : s->links) {
         const int ip = link.dir;
         const int opp = OPP[ip];
         const long idx = (long)link.cell_k * LXY + (long)link.cell_j * LX + link.cell_i;
         const double Gi = s->g_collide[ip * N + idx];
         const double Go = s->g[opp * N + idx];
-        const long slot = link_slot(s, link);
-        part[(size_t)link.pid * RED_LANES + slot % RED_LANES] += Gi - Go;
+        share[{link.pid, link_slot(s, link)}] += Gi - Go;
     }
+    for (const auto &kv : share)
+        part[(size_t)kv.first.first * RED_LANES + kv.first.second % RED_LANES] += kv.second;
     for (int pid = 0; pid < np; pid++) {
         double *b = &part[(size_t)pid * RED_LANES];
         tree_reduce(b);

@@ -46,17 +46,6 @@ template <class S> struct NodeOut {
     S *gout;
 };
 
-// x / d, correctly rounded, from a double-word reciprocal yh + yl of d (yh = RN(1/d)).
-// q0 = RN(x*yh + RN(x*yl)) is within half an ulp plus 2^-100 relative of x/d, hence a faithful
-// rounding; by Markstein's theorem (yh within half an ulp of 1/d) the residual x - q0*d is exact
-// in one FMA and RN(q0 + r*yh) is the correctly rounded quotient.  Four FP64 operations replace
-// the ~10-instruction IEEE division sequence with an identical result.
-__device__ inline double div_rn(double x, double d, double yh, double yl) {
-    const double q0 = __builtin_fma(x, yh, x * yl);
-    const double r = __builtin_fma(-q0, d, x);
-    return __builtin_fma(r, yh, q0);
-}
-
 // Strict FP64 macrovar + forcing + collision_BGK + collision_BGK_scalar for one node.
 // Returns the new F_x; writes the post-collision populations.  Every rounding is the one of
 // the reference's expression (divisions via div_rn, which returns the IEEE quotient).

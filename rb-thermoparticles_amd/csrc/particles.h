@@ -12,10 +12,10 @@ namespace rbtp {
 // Indices of the snapshot of the particle state before the motion of the current step.
 enum { OLD_PX = 0, OLD_PY, OLD_PZ, OLD_VX, OLD_VY, OLD_VZ, OLD_WX, OLD_WY, OLD_WZ, OLD_T, OLD_COUNT };
 
-// Error flags raised by device code (checked after every synchronising call).
-// ERR_UNHANDLED_F/G count links whose bounce-back found neither its node nor the upstream node
-// fluid (the reference's printf cases, ibb.cu:79,130); they are counts, not failures.
-enum { ERR_SPECIAL_CAP = 0, ERR_UNHANDLED_F = 1, ERR_UNHANDLED_G = 2, ERR_COUNT = 3 };
+// Error flags raised by device code (checked after every synchronising call).  The reference's
+// other device-side report, a link whose node is solid (ibb.cu:79,130), cannot occur: links
+// start only at fluid nodes.
+enum { ERR_SPECIAL_CAP = 0, ERR_COUNT = 1 };
 
 struct PartDev {
     int np;
@@ -42,7 +42,20 @@ struct PartDev {
     int32_t *cell_cnt;       // [ncells] particles binned into each cell (may exceed CELL_CAP)
     int32_t *cell_pid;       // [ncells][CELL_CAP]
     bool use_cells;          // pair search through the cell grid (else every pair)
+    // Each link-box node's share of its particle's force, torque and heat sums: lnk_stride
+    // entries per particle (the largest link box, padded to whole waves), 7 planes of np *
+    // lnk_stride values, and one bit per entry (64-bit words) telling whether the node has links.
+    int lnk_stride;
+    double *lnk_part;
+    unsigned long long *lnk_bits;
 };
+
+// Link-box node threads per particle: the largest link box ((box_w + 2)^3 nodes, box_w the
+// largest marking box per axis) rounded up to whole 64-lane waves.
+inline int link_stride(int box_w) {
+    const int b = box_w + 2;
+    return (b * b * b + 63) / 64 * 64;
+}
 
 int report(int code, const char *msg);
 int report_hip(hipError_t e, const char *what);
@@ -74,7 +87,8 @@ struct PartGrid {
     const int *cstart;  // [nslots + 1] pair-search cells
     const int *sstart;  // [nslots + 1] workgroups of the special-node kernels (4 nodes each)
     const int *lstart;  // [nslots + 1] workgroups of the link-node fix-up (128 nodes each)
-    int nslots, total_p, total_c, total_s, total_l;
+    const int *bstart;  // [nslots + 1] link-box node threads (np * lnk_stride per slot)
+    int nslots, total_p, total_c, total_s, total_l, total_b;
 };
 
 // Device tables for the two phases of a time step (the geometry generations and the

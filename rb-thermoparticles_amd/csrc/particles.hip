@@ -5,10 +5,11 @@
 // node owned by p, and the ray from x's centre along c_i meets sphere p at q in [0, 1]
 // (link_count_kernel, links.cu:50-87).  Each consumer re-derives the links it needs from the two
 // owner maps with the same bit-exact test:
-//   * k_link_sums: one workgroup per particle walks the particle's link box and sums
-//     force, torque and heat in a fixed order (1024 lane partials + pairwise tree), replacing
-//     the reference's FP64 atomics (ibb.cu:182-187,204) with a reproducible reduction;
-//   * k_link_sums also lists the IBB link nodes, k_refill_find the refilled nodes: the compact
+//   * k_link_nodes: one thread per node of each particle's link box finds the node's links to
+//     the particle and stores the node's share of force, torque and heat; k_link_reduce sums
+//     the shares of each particle in a fixed order (64 lane partials + pairwise tree),
+//     replacing the reference's FP64 atomics (ibb.cu:182-187,204) with a reproducible reduction;
+//   * k_link_nodes also lists the IBB link nodes, k_refill_find the refilled nodes: the compact
 //     list of the nodes whose post-stream populations the coupling replaces;
 //   * k_fixup_links / k_fixup_refill: those nodes' corrected post-stream populations (pull +
 //     IBB, or refill), buffered for the next lattice step.
@@ -42,8 +43,7 @@ constexpr int PB = 256;   // threads per particle workgroup of the box clear/mar
 constexpr int CELL_CAP = 16;  // particle ids per cell of the pair-search grid
 constexpr int CAND_MAX = 48;  // pair candidates per particle held in LDS
 constexpr int CELLS_MIN_NP = 32;  // particle count from which the pair search uses the grid
-constexpr int LB = 1024;  // threads per particle workgroup of the link sums (also their reduction
-                          // width) and of the special-node search
+constexpr int LB = 1024;  // threads per particle workgroup of the special-node search
 
 __constant__ int c_cx[NPOP] = {0, 1, 0, -1, 0, 0, 0, 1, -1, -1, 1, 1, 0, -1,
                                0, 1, 0, -1, 0, 1, -1, -1, 1, 1, -1, -1, 1};
@@ -72,7 +72,12 @@ __device__ inline bool link_q(int ip, double x0, double y0, double z0, double px
     const double c = dx * dx + dy * dy + dz * dz - radius_sq;
     const double disc = b * b - 4.0 * a * c;
     if (disc <= 0.0) return false;
-    q = (-b - sqrt(disc)) / (2.0 * a);
+    // q = (-b - sqrt(disc)) / (2a), a in {1, 2, 3}: the divisions by 2 and 4 are exact scalings
+    // and the one by 6 is div_rn with RN(1/6) + RN(1/6 - RN(1/6)), the same correctly rounded
+    // quotient in three FMAs
+    const double num = -b - sqrt(disc);
+    q = (a == 3.0) ? div_rn(num, 6.0, 0x1.5555555555555p-3, 0x1.5555555555555p-57)
+                   : num * (a == 1.0 ? 0.5 : 0.25);
     return (q >= 0.0 && q <= 1.0);
 }
 
@@ -390,24 +395,26 @@ __global__ __launch_bounds__(128) void k_contacts(PartGrid g) {
 
 // Appends the nodes of the calling wave's lanes that have `want` set, in lane order, with one
 // atomic per wave: consecutive box nodes stay consecutive in the list, which keeps the later
-// passes over the list coalesced.  Every active lane of the wave must call it.
-__device__ inline void push_special(const PartDev &pd, bool want, int idx, int kind) {
+// passes over the list coalesced.  Every active lane of the wave must call it.  Returns the
+// lane's list position, or -1 (not appended, or the list is full).
+__device__ inline int push_special(const PartDev &pd, bool want, int idx, int kind) {
     const unsigned long long mask = __ballot(want);
-    if (mask == 0) return;
+    if (mask == 0) return -1;
     const int lane = __lane_id();
     const int leader = __ffsll((long long)mask) - 1;
     int base = 0;
     if (lane == leader) base = atomicAdd(pd.sp.count, __popcll(mask));
     base = __shfl(base, leader);
-    if (!want) return;
+    if (!want) return -1;
     const int j = base + __popcll(mask & ((1ull << lane) - 1ull));
     if (j < pd.sp.cap) {
         pd.sp.node[j] = idx;
         pd.sp_kind[j] = kind;
         pd.sp.map[idx] = j;
-    } else {
-        atomicOr(pd.err + ERR_SPECIAL_CAP, 1);
+        return j;
     }
+    atomicOr(pd.err + ERR_SPECIAL_CAP, 1);
+    return -1;
 }
 
 // ---- link evaluation -----------------------------------------------------------------------------
@@ -434,53 +441,48 @@ struct PopView {
 
 struct LinkOut {
     double f_in, f_out, g_in, g_out;
-    double relx, rely, relz, uwx, uwy, uwz;
-    bool unhandled_f, unhandled_g;
+    double uwx, uwy, uwz;
 };
 
-// Everything the bounce-back of link (x, ip) reads from the lattice and the owner map, loaded
-// together so that the loads of several links can be in flight at once.  Every address is valid
-// whichever branch eval_link_at takes: the upstream node x - c_ip stands in as x when it is
-// outside the domain, and the solid node x + c_ip is clamped into it.
+// The lattice values the bounce-back of link (x, ip) reads.  Links start only at fluid nodes,
+// so the last branch of bounce_back_particles_kernel and the no-write branch of
+// bounce_back_thermal_kernel (ibb.cu:77-80,128-131) are never taken, and each distribution
+// needs two values: its own post-collision value at x, and the partner of the branch the link
+// takes, which is known before any population is read (q from the geometry, the upstream
+// node's owner from the owner map):
+//   upstream x - c_ip inside and fluid, q <  1/2 (g: q <= 1/2)  ->  f*_ip(x - c_ip)
+//   upstream x - c_ip inside and fluid, q >= 1/2 (g: q >  1/2)  ->  f*_opp(x)
+//   otherwise                                                   ->  none (x's own value again)
+// All four loads are unconditional, with the address chosen, so that the loads of several links
+// can be in flight together; a repeated read of x's own value touches no further cache line.
 struct LinkLoads {
-    int up_owner, here_owner;
-    double f_here, f_up, f_opp, g_here, g_up, g_opp, g_solid;
+    double f_here, f_pair, g_here, g_pair;
 };
 
 template <class S>
-__device__ inline LinkLoads link_loads(int ip, int ix, int iy, int iz, const PopView<S> &pop,
-                                       const int32_t *owner_prev, const Dims &d) {
-    LinkLoads L;
+__device__ inline LinkLoads link_loads(int ip, int idx, int idx_up, bool up_fluid, double q,
+                                       const PopView<S> &pop) {
     const int opp = c_opp[ip];
-    const int idx = iz * d.lxy + iy * d.lx + ix;
-    const int ix1 = ix - c_cx[ip], iy1 = iy - c_cy[ip], iz1 = iz - c_cz[ip];
-    const int idx1 = in_domain(d, ix1, iy1, iz1) ? iz1 * d.lxy + iy1 * d.lx + ix1 : idx;
-    const int sx = min(max(ix + c_cx[ip], 0), d.lx - 1);
-    const int sy = min(max(iy + c_cy[ip], 0), d.ly - 1);
-    const int sz = min(max(iz + c_cz[ip], 0), d.lz - 1);
-    L.up_owner = owner_prev[idx1];
-    L.here_owner = owner_prev[idx];
+    const int fi = up_fluid ? (q < 0.5 ? ip : opp) : ip;
+    const int fn = up_fluid && q < 0.5 ? idx_up : idx;
+    const int gi = up_fluid ? (q <= 0.5 ? ip : opp) : ip;
+    const int gn = up_fluid && q <= 0.5 ? idx_up : idx;
+    LinkLoads L;
     L.f_here = pop.F(ip, idx);
-    L.f_up = pop.F(ip, idx1);
-    L.f_opp = pop.F(opp, idx);
+    L.f_pair = pop.F(fi, fn);
     L.g_here = pop.G(ip, idx);
-    L.g_up = pop.G(ip, idx1);
-    L.g_opp = pop.G(opp, idx);
-    L.g_solid = pop.G(opp, sz * d.lxy + sy * d.lx + sx);
+    L.g_pair = pop.G(gi, gn);
     return L;
 }
 
 // bounce_back_particles_kernel (ibb.cu:19-81) and bounce_back_thermal_kernel (ibb.cu:94-135)
-// for one link, from the post-collision populations in L; returns the values they write into
-// f[opp][idx] and g[opp][idx] plus what the force and heat sums read (ibb.cu:160-180,202-203).
+// for the link of fluid node (ix,iy,iz) in direction ip, from the post-collision populations in
+// L; returns the values they write into f[opp][idx] and g[opp][idx] plus what the force and
+// heat sums read (ibb.cu:160-180,202-203).
 __device__ inline LinkOut eval_link_at(const LinkLoads &L, int ip, int ix, int iy, int iz,
-                                       double q, int pid, const PView &pv, double density,
-                                       double rho0, const Dims &d) {
+                                       bool up_fluid, double q, int pid, const PView &pv,
+                                       double density, double rho0) {
     LinkOut o;
-    const int ix1 = ix - c_cx[ip], iy1 = iy - c_cy[ip], iz1 = iz - c_cz[ip];
-    const bool up_in = in_domain(d, ix1, iy1, iz1);
-    const bool up_fluid = up_in && L.up_owner == -1;
-    const bool here_fluid = L.here_owner == -1;
     const double tp = wt(ip);
     // wall velocity at the boundary point (ibb.cu:33-45)
     const double cx = static_cast<double>(ix) + 0.5 + q * c_cx[ip];
@@ -491,170 +493,221 @@ __device__ inline LinkOut eval_link_at(const LinkLoads &L, int ip, int ix, int i
     const double uwx = pv.vx[pid] + (relz * wy - rely * wz);
     const double uwy = pv.vy[pid] + (relx * wz - relz * wx);
     const double uwz = pv.vz[pid] + (rely * wx - relx * wy);
-    o.relx = relx; o.rely = rely; o.relz = relz;
     o.uwx = uwx; o.uwy = uwy; o.uwz = uwz;
     const double evel = c_cx[ip] * uwx + c_cy[ip] * uwy + c_cz[ip] * uwz;
     const double RT = 1.0 / 3.0;
-    const double momentum_term = 2.0 * tp * rho0 * evel / RT;
+    // / RT as div_rn with RN(1/RT) = 3 and RN(1/RT - 3) = 3 * 2^-54
+    const double momentum_term = div_rn(2.0 * tp * rho0 * evel, RT, 3.0, 0x1.8p-53);
     const double f_here = L.f_here;
     o.f_in = f_here;
-    o.unhandled_f = false;
     if (up_fluid) {
         if (q < 0.5) {
-            const double f_upstream = L.f_up;
-            o.f_out = 2.0 * q * f_here + (1.0 - 2.0 * q) * f_upstream - momentum_term;
+            o.f_out = 2.0 * q * f_here + (1.0 - 2.0 * q) * L.f_pair - momentum_term;
         } else {
-            const double f_opposite_collide = L.f_opp;
             const double inv_2q = 0.5 / q;
-            o.f_out = (f_here - momentum_term) * inv_2q + (2.0 * q - 1.0) * inv_2q * f_opposite_collide;
+            o.f_out = (f_here - momentum_term) * inv_2q + (2.0 * q - 1.0) * inv_2q * L.f_pair;
         }
-    } else if (here_fluid) {
-        o.f_out = f_here - momentum_term;
     } else {
-        o.f_out = f_here;
-        o.unhandled_f = true;
+        o.f_out = f_here - momentum_term;
     }
     // thermal (ibb.cu:109-135)
     const double Tp = pv.temp[pid];
     const double amp = 2.0 * tp * Tp * (1.0 + density);
     const double g_here = L.g_here;
     o.g_in = g_here;
-    o.unhandled_g = false;
     if (up_fluid) {
         if (q <= 0.5) {
-            const double g_upstream = L.g_up;
-            o.g_out = -(2.0 * q * g_here + (1.0 - 2.0 * q) * g_upstream) + amp;
+            o.g_out = -(2.0 * q * g_here + (1.0 - 2.0 * q) * L.g_pair) + amp;
         } else {
-            const double g_opp_here = L.g_opp;
             const double inv_2q = 0.5 / q;
-            o.g_out = -g_here * inv_2q + amp * inv_2q + (2.0 * q - 1.0) * inv_2q * g_opp_here;
+            o.g_out = -g_here * inv_2q + amp * inv_2q + (2.0 * q - 1.0) * inv_2q * L.g_pair;
         }
-    } else if (here_fluid) {
-        o.g_out = -g_here + amp;
     } else {
-        // No write: g[opp][idx] keeps its streamed value, g*_opp(x + c_ip).
-        o.g_out = L.g_solid;
-        o.unhandled_g = true;
+        o.g_out = -g_here + amp;
     }
     return o;
 }
 
-template <class S>
-__device__ inline LinkOut eval_link(int ip, int ix, int iy, int iz, double q, int pid,
-                                    const PView &pv, const PopView<S> &pop,
-                                    const int32_t *owner_prev, double density, double rho0,
-                                    const Dims &d) {
-    return eval_link_at(link_loads<S>(ip, ix, iy, iz, pop, owner_prev, d), ip, ix, iy, iz, q,
-                        pid, pv, density, rho0, d);
-}
-
 // Hydrodynamic force, torque and heat of every particle (compute_particle_forces_kernel,
-// ibb.cu:138-188; compute_particle_heat_kernel, ibb.cu:190-205).  Lane t of particle p's
-// workgroup takes the nodes k = t, t+1024, ... of p's link box, each node's links in direction
-// order; the 1024 partials are combined pairwise (stride 512 ... 1) and added to the sums left
-// by k_contacts.
+// ibb.cu:138-188; compute_particle_heat_kernel, ibb.cu:190-205), in two launches over every
+// particle of every slot.  k_link_nodes gives one thread to each node of a particle's link box:
+// it finds the node's links to the particle, and stores the node's share of the seven sums (its
+// links' terms in direction order, summed from 0) and a bit telling that the node has links.
+// When the particle is the node's lowest linked pid, it also lists the node as a special node
+// and writes the bounce-back values of its links into the node's entry (ibb.cu:5-136); a node
+// that also links to a sphere of higher pid is marked SP_MULTI and completed by k_fixup_links.
+// k_link_reduce then gives one wave to each particle: lane t adds the shares of the box nodes
+// k = t, t+64, ... in increasing k, and the 64 lane sums are combined pairwise (stride 32 ...
+// 1) and added to the sums left by k_contacts.
+constexpr int RL = 64;  // lanes of the per-particle reduction (one wave)
+constexpr uint32_t SP_MULTI = 1u << 31;  // special-node mask flag: links to several spheres
+
 template <class S>
-__global__ __launch_bounds__(LB) void k_link_sums(PartGrid g) {
-    __shared__ double red[7][LB];
-    const int slot = slot_of(g.pstart, g.nslots, blockIdx.x);
+__global__ __launch_bounds__(256) void k_link_nodes(PartGrid g) {
+    const int u = blockIdx.x * 256 + threadIdx.x;
+    if (u >= g.total_b) return;  // total_b is a whole number of waves
+    // every wave lies in one particle of one slot: slot and particle are wave-uniform
+    const int u0 = __builtin_amdgcn_readfirstlane(u);
+    const int slot = slot_of(g.bstart, g.nslots, u0);
     const PartSlot &sl = g.tab[slot];
     const PartDev &pd = sl.pd;
     const Dims d = sl.d;
-    const PopView<S> pop{(const S *)sl.fcur, (const S *)sl.gcur, (size_t)d.n, pd.tl0, pd.tls, d.lx};
+    const int p = (u0 - g.bstart[slot]) / pd.lnk_stride;
+    const int k = u - g.bstart[slot] - p * pd.lnk_stride;
     const int32_t *owner_prev = sl.owner_prev;
-    const int32_t *box_prev = sl.box_prev;
-    const double *rho = sl.macro[0];
-    const int p = blockIdx.x - g.pstart[slot];
-    const int t = threadIdx.x;
-    double s[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    const PView pv{pd.p[RBTP_P_POS_X], pd.p[RBTP_P_POS_Y], pd.p[RBTP_P_POS_Z],
+                   pd.p[RBTP_P_VEL_X], pd.p[RBTP_P_VEL_Y], pd.p[RBTP_P_VEL_Z],
+                   pd.p[RBTP_P_OMEGA_X], pd.p[RBTP_P_OMEGA_Y], pd.p[RBTP_P_OMEGA_Z],
+                   pd.p[RBTP_P_TEMP]};
     int lo[3], hi[3];
-    int nunh_f = 0, nunh_g = 0;
-    if (load_box(box_prev, p, lo, hi)) {
+    bool in_box = load_box(sl.box_prev, p, lo, hi);
+    bool interior = false;  // the box and its neighbours lie inside the domain
+    if (in_box) {
         grow_box(d, lo, hi);
-        const PView pv{pd.p[RBTP_P_POS_X], pd.p[RBTP_P_POS_Y], pd.p[RBTP_P_POS_Z],
-                       pd.p[RBTP_P_VEL_X], pd.p[RBTP_P_VEL_Y], pd.p[RBTP_P_VEL_Z],
-                       pd.p[RBTP_P_OMEGA_X], pd.p[RBTP_P_OMEGA_Y], pd.p[RBTP_P_OMEGA_Z],
-                       pd.p[RBTP_P_TEMP]};
-        const double X = pv.px[p], Y = pv.py[p], Z = pv.pz[p];
-        const int cnt = box_count(lo, hi);
-        for (int k = t; k < cnt; k += LB) {
-            int ix, iy, iz;
-            const int idx = box_node(d, lo, hi, k, ix, iy, iz);
-            const double x0 = static_cast<double>(ix) + 0.5;
-            const double y0 = static_cast<double>(iy) + 0.5;
-            const double z0 = static_cast<double>(iz) + 0.5;
-            // links of this fluid node to p, as a direction mask (q is evaluated again below),
-            // and whether it also links to a sphere of lower pid: a node linked to several
-            // spheres is listed once, by the lowest linked pid
-            unsigned mask = 0u;
-            bool lower = false;
-            if (owner_prev[idx] == -1) {
-                int own[NPOP];
+        in_box = k < box_count(lo, hi);
+        interior = lo[0] > 0 && lo[1] > 0 && lo[2] > 0 && hi[0] < d.lx - 1 &&
+                   hi[1] < d.ly - 1 && hi[2] < d.lz - 1;
+    }
+    const double X = pv.px[p], Y = pv.py[p], Z = pv.pz[p];
+    int ix = 0, iy = 0, iz = 0, idx = 0;
+    // links of this node to p, as a direction mask (q is evaluated again below), whether it also
+    // links to a sphere of lower pid (a node linked to several spheres is listed once, by the
+    // lowest linked pid) or of higher pid, and which neighbours are fluid
+    unsigned mask = 0u, fluid_nb = 0u;
+    bool lower = false, higher = false;
+    if (in_box) {
+        idx = box_node(d, lo, hi, k, ix, iy, iz);
+        const double x0 = static_cast<double>(ix) + 0.5;
+        const double y0 = static_cast<double>(iy) + 0.5;
+        const double z0 = static_cast<double>(iz) + 0.5;
+        // a link x0 -> x0 + c_ip (|c_ip| <= sqrt 3) reaches into the sphere only from within
+        // R + sqrt 3 of its centre; the margin to R + 1.75 is far above any rounding
+        const double dx = x0 - X, dy = y0 - Y, dz = z0 - Z;
+        const double reach = pd.radius + 1.75;
+        if (dx * dx + dy * dy + dz * dz <= reach * reach && owner_prev[idx] == -1) {
+            int own[NPOP];
+            if (interior) {
+#pragma unroll
+                for (int ip = 1; ip < NPOP; ip++)
+                    own[ip] = owner_prev[idx + c_cx[ip] + c_cy[ip] * d.lx + c_cz[ip] * d.lxy];
+            } else {
                 neighbour_owners(owner_prev, d, ix, iy, iz, own);
-                for (int ip = 1; ip < NPOP; ip++) {
-                    const int q_owner = own[ip];
-                    double q = 0.0;
-                    if (q_owner == p) {
-                        if (link_q(ip, x0, y0, z0, X, Y, Z, pd.radius_sq, q)) mask |= 1u << ip;
-                    } else if (q_owner >= 0 && q_owner < p) {
-                        if (link_q(ip, x0, y0, z0, pv.px[q_owner], pv.py[q_owner],
-                                   pv.pz[q_owner], pd.radius_sq, q))
-                            lower = true;
+            }
+            for (int ip = 1; ip < NPOP; ip++) {
+                const int q_owner = own[ip];
+                double q = 0.0;
+                if (q_owner == -1) fluid_nb |= 1u << ip;
+                if (q_owner == p) {
+                    if (link_q(ip, x0, y0, z0, X, Y, Z, pd.radius_sq, q)) mask |= 1u << ip;
+                } else if (q_owner >= 0) {
+                    if (link_q(ip, x0, y0, z0, pv.px[q_owner], pv.py[q_owner], pv.pz[q_owner],
+                               pd.radius_sq, q)) {
+                        if (q_owner < p) lower = true;
+                        else higher = true;
                     }
                 }
             }
-            // listed before the sums, so that the list append overlaps the link loads below
-            push_special(pd, mask != 0u && !lower, idx, -1);
-            if (mask == 0u) continue;
-            const double density = rho[idx];
-            // links in ascending direction order; each link's lattice loads are issued together
-            while (mask) {
-                const int ip = __builtin_ctz(mask);
-                mask &= mask - 1u;
-                const LinkLoads Ld = link_loads<S>(ip, ix, iy, iz, pop, owner_prev, d);
-                double q = 0.0;
-                link_q(ip, x0, y0, z0, X, Y, Z, pd.radius_sq, q);
-                const LinkOut L = eval_link_at(Ld, ip, ix, iy, iz, q, p, pv, density, pd.rho0, d);
-                nunh_f += L.unhandled_f;
-                nunh_g += L.unhandled_g;
-                const int opp = c_opp[ip];
-                // Force uses the link record's r = q c (links.cu:134-136, ibb.cu:160-168).
-                const double rx = q * static_cast<double>(c_cx[ip]);
-                const double ry = q * static_cast<double>(c_cy[ip]);
-                const double rz = q * static_cast<double>(c_cz[ip]);
-                const double relx = ix + 0.5 + rx - X;
-                const double rely = iy + 0.5 + ry - Y;
-                const double relz = iz + 0.5 + rz - Z;
-                const double Fx = (c_cx[ip] - L.uwx) * L.f_in - (c_cx[opp] - L.uwx) * L.f_out;
-                const double Fy = (c_cy[ip] - L.uwy) * L.f_in - (c_cy[opp] - L.uwy) * L.f_out;
-                const double Fz = (c_cz[ip] - L.uwz) * L.f_in - (c_cz[opp] - L.uwz) * L.f_out;
-                s[0] += Fx;
-                s[1] += Fy;
-                s[2] += Fz;
-                s[3] += rely * Fz - relz * Fy;
-                s[4] += relz * Fx - relx * Fz;
-                s[5] += relx * Fy - rely * Fx;
-                s[6] += L.g_in - L.g_out;
-            }
         }
     }
-    if (nunh_f) atomicAdd(pd.err + ERR_UNHANDLED_F, nunh_f);
-    if (nunh_g) atomicAdd(pd.err + ERR_UNHANDLED_G, nunh_g);
-    for (int c = 0; c < 7; c++) red[c][t] = s[c];
-    __syncthreads();
-    for (int stride = LB / 2; stride >= 1; stride >>= 1) {
-        if (t < stride)
-            for (int c = 0; c < 7; c++) red[c][t] = red[c][t] + red[c][t + stride];
-        __syncthreads();
+    // every lane of the wave is here
+    const int j = push_special(pd, mask != 0u && !lower, idx, -1);
+    const unsigned long long has = __ballot(mask != 0u);
+    if (__lane_id() == 0) pd.lnk_bits[(size_t)p * (pd.lnk_stride / 64) + k / 64] = has;
+    if (mask == 0u) return;
+    const PopView<S> pop{(const S *)sl.fcur, (const S *)sl.gcur, (size_t)d.n, pd.tl0, pd.tls, d.lx};
+    using A = typename Enc<S>::A;
+    A *out = (A *)pd.sp.pop;
+    const size_t cap = (size_t)pd.sp.cap;
+    const double density = sl.macro[0][idx];
+    const double x0 = static_cast<double>(ix) + 0.5;
+    const double y0 = static_cast<double>(iy) + 0.5;
+    const double z0 = static_cast<double>(iz) + 0.5;
+    double v[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    uint32_t replaced = 0u;
+    // links in ascending direction order; each link's lattice loads are issued together
+    while (mask) {
+        const int ip = __builtin_ctz(mask);
+        mask &= mask - 1u;
+        const int opp = c_opp[ip];
+        const bool up_fluid = (fluid_nb >> opp) & 1u;  // x - c_ip = x + c_opp
+        const int idx_up = idx - (c_cx[ip] + c_cy[ip] * d.lx + c_cz[ip] * d.lxy);
+        double q = 0.0;
+        link_q(ip, x0, y0, z0, X, Y, Z, pd.radius_sq, q);
+        const LinkLoads Ld = link_loads<S>(ip, idx, idx_up, up_fluid, q, pop);
+        const LinkOut L = eval_link_at(Ld, ip, ix, iy, iz, up_fluid, q, p, pv, density, pd.rho0);
+        if (j >= 0) {
+            out[(size_t)opp * cap + j] = A(L.f_out);
+            out[(size_t)(NPOP + opp) * cap + j] = A(L.g_out);
+            replaced |= 1u << opp;
+        }
+        // Force uses the link record's r = q c (links.cu:134-136, ibb.cu:160-168).
+        const double rx = q * static_cast<double>(c_cx[ip]);
+        const double ry = q * static_cast<double>(c_cy[ip]);
+        const double rz = q * static_cast<double>(c_cz[ip]);
+        const double relx = ix + 0.5 + rx - X;
+        const double rely = iy + 0.5 + ry - Y;
+        const double relz = iz + 0.5 + rz - Z;
+        const double Fx = (c_cx[ip] - L.uwx) * L.f_in - (c_cx[opp] - L.uwx) * L.f_out;
+        const double Fy = (c_cy[ip] - L.uwy) * L.f_in - (c_cy[opp] - L.uwy) * L.f_out;
+        const double Fz = (c_cz[ip] - L.uwz) * L.f_in - (c_cz[opp] - L.uwz) * L.f_out;
+        v[0] += Fx;
+        v[1] += Fy;
+        v[2] += Fz;
+        v[3] += rely * Fz - relz * Fy;
+        v[4] += relz * Fx - relx * Fz;
+        v[5] += relx * Fy - rely * Fx;
+        v[6] += L.g_in - L.g_out;
     }
-    if (t == 0) {
-        pd.p[RBTP_P_FORCE_X][p] += red[0][0];
-        pd.p[RBTP_P_FORCE_Y][p] += red[1][0];
-        pd.p[RBTP_P_FORCE_Z][p] += red[2][0];
-        pd.p[RBTP_P_TORQUE_X][p] += red[3][0];
-        pd.p[RBTP_P_TORQUE_Y][p] += red[4][0];
-        pd.p[RBTP_P_TORQUE_Z][p] += red[5][0];
-        pd.p[RBTP_P_HEAT][p] += red[6][0];
+    if (j >= 0) pd.sp.mask[j] = replaced | (higher ? SP_MULTI : 0u);
+    const size_t plane = (size_t)pd.np * pd.lnk_stride;
+    const size_t e = (size_t)p * pd.lnk_stride + k;
+    for (int c = 0; c < 7; c++) pd.lnk_part[c * plane + e] = v[c];
+}
+
+__global__ __launch_bounds__(256) void k_link_reduce(PartGrid g) {
+    const int w = blockIdx.x * (256 / RL) + (threadIdx.x / RL);
+    if (w >= g.total_p) return;  // whole waves
+    const int lane = threadIdx.x % RL;
+    const int w0 = __builtin_amdgcn_readfirstlane(w);
+    const int slot = slot_of(g.pstart, g.nslots, w0);
+    const PartSlot &sl = g.tab[slot];
+    const PartDev &pd = sl.pd;
+    const int p = w0 - g.pstart[slot];
+    double s[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    int lo[3], hi[3];
+    if (load_box(sl.box_prev, p, lo, hi)) {
+        grow_box(sl.d, lo, hi);
+        const int nw = (box_count(lo, hi) + RL - 1) / RL;  // words of node bits
+        const unsigned long long *bits = pd.lnk_bits + (size_t)p * (pd.lnk_stride / 64);
+        const size_t base = (size_t)p * pd.lnk_stride + lane;
+        const size_t plane = (size_t)pd.np * pd.lnk_stride;
+        constexpr int U = 4;  // shares of up to U nodes loaded together, then added in order
+        for (int w1 = 0; w1 < nw; w1 += U) {
+            bool has[U];
+            double sh[U][7];
+#pragma unroll
+            for (int i = 0; i < U; i++) {
+                const unsigned long long b = (w1 + i < nw) ? bits[w1 + i] : 0ull;
+                has[i] = (b >> lane) & 1ull;
+                for (int c = 0; c < 7; c++)
+                    sh[i][c] = has[i] ? pd.lnk_part[c * plane + base + (size_t)(w1 + i) * RL] : 0.0;
+            }
+#pragma unroll
+            for (int i = 0; i < U; i++)
+                if (has[i])
+                    for (int c = 0; c < 7; c++) s[c] += sh[i][c];
+        }
+    }
+    for (int stride = RL / 2; stride >= 1; stride >>= 1)
+        for (int c = 0; c < 7; c++) s[c] = s[c] + __shfl_down(s[c], stride, RL);
+    if (lane == 0) {
+        pd.p[RBTP_P_FORCE_X][p] += s[0];
+        pd.p[RBTP_P_FORCE_Y][p] += s[1];
+        pd.p[RBTP_P_FORCE_Z][p] += s[2];
+        pd.p[RBTP_P_TORQUE_X][p] += s[3];
+        pd.p[RBTP_P_TORQUE_Y][p] += s[4];
+        pd.p[RBTP_P_TORQUE_Z][p] += s[5];
+        pd.p[RBTP_P_HEAT][p] += s[6];
     }
 }
 
@@ -679,7 +732,7 @@ __global__ __launch_bounds__(128) void k_move(PartGrid g) {
     const int s = slot_of(g.pstart, g.nslots, u);
     const PartDev &pd = g.tab[s].pd;
     const int pid = u - g.pstart[s];
-    // k_link_sums, which ran before, listed the IBB link nodes: they head the list
+    // k_link_nodes, which ran before, listed the IBB link nodes: they head the list
     if (pid == 0) pd.sp.count[1] = pd.sp.count[0];
     double *const *P = pd.p;
     const int src[OLD_COUNT] = {RBTP_P_POS_X, RBTP_P_POS_Y, RBTP_P_POS_Z, RBTP_P_VEL_X,
@@ -735,7 +788,7 @@ __global__ __launch_bounds__(128) void k_move(PartGrid g) {
 
 // New-fluid nodes of one particle's marking box (new_fluid_count_kernel, refill.cu:8-41): solid
 // in the previous geometry, fluid now, previous owner valid, and within close_thresh of that
-// owner's new surface.  (IBB link nodes were listed by k_link_sums.)
+// owner's new surface.  (IBB link nodes were listed by k_link_nodes.)
 __global__ __launch_bounds__(LB) void k_refill_find(PartGrid g) {
     const int slot = slot_of(g.pstart, g.nslots, blockIdx.x);
     const PartSlot &sl = g.tab[slot];
@@ -792,7 +845,8 @@ __device__ inline void pull_one(const PopView<S> &pop, int j, int idx, int ix, i
     }
 }
 
-// Bounce-back values of the IBB link nodes (the head of the special list), one thread per node:
+// Bounce-back values of the IBB link nodes that link to more than one sphere (SP_MULTI entries at
+// the head of the special list; k_link_nodes wrote every other link node), one thread per node:
 // for every link x -> x + c_ip (x + c_ip solid and linked, with the state before this step's
 // motion) the values bounce_back_particles / bounce_back_thermal_particles write into f[opp][x]
 // and g[opp][x] (ibb.cu:5-136), and the mask of the replaced populations.  The node's other
@@ -816,6 +870,7 @@ __global__ __launch_bounds__(128) void k_fixup_links(PartGrid g) {
                    pd.old[OLD_VY], pd.old[OLD_VZ], pd.old[OLD_WX], pd.old[OLD_WY],
                    pd.old[OLD_WZ], pd.old[OLD_T]};
     for (int j = blk * 128 + threadIdx.x; j < nlink; j += nblk * 128) {
+        if (!(pd.sp.mask[j] & SP_MULTI)) continue;  // k_link_nodes wrote this node's links
         const int idx = pd.sp.node[j];
         const int iz = idx / d.lxy;
         const int rem = idx - iz * d.lxy;
@@ -835,9 +890,12 @@ __global__ __launch_bounds__(128) void k_fixup_links(PartGrid g) {
             if (!link_q(ip, x0, y0, z0, pv.px[q_owner], pv.py[q_owner], pv.pz[q_owner],
                         pd.radius_sq, q))
                 continue;
-            const LinkLoads Ld = link_loads<S>(ip, ix, iy, iz, pop, owner_prev, d);
-            const LinkOut L = eval_link_at(Ld, ip, ix, iy, iz, q, q_owner, pv, density, pd.rho0, d);
             const int opp = c_opp[ip];
+            const bool up_fluid = own[opp] == -1;  // x - c_ip = x + c_opp, inside and fluid
+            const int idx_up = idx - (c_cx[ip] + c_cy[ip] * d.lx + c_cz[ip] * d.lxy);
+            const LinkLoads Ld = link_loads<S>(ip, idx, idx_up, up_fluid, q, pop);
+            const LinkOut L = eval_link_at(Ld, ip, ix, iy, iz, up_fluid, q, q_owner, pv, density,
+                                           pd.rho0);
             out[(size_t)opp * cap + j] = A(L.f_out);
             out[(size_t)(NPOP + opp) * cap + j] = A(L.g_out);
             m |= 1u << opp;
@@ -1095,6 +1153,11 @@ int part_alloc(PartDev &pd, const rbtp_case &c, const rbtp_derived &d, const Dim
     PHIP(hipMemset(pd.sp.map, 0xFF, (size_t)dims.n * sizeof(int32_t)));
     PHIP(hipMalloc(&pd.err, ERR_COUNT * sizeof(int32_t)));
     PHIP(hipMemset(pd.err, 0, ERR_COUNT * sizeof(int32_t)));
+    pd.lnk_stride = link_stride(pd.box_w);
+    if ((long long)pd.np * pd.lnk_stride > INT_MAX)
+        return report(RBTP_E_CAPACITY, "too many particles for the link-box thread grid");
+    PHIP(hipMalloc(&pd.lnk_part, 7 * np * (size_t)pd.lnk_stride * sizeof(double)));
+    PHIP(hipMalloc(&pd.lnk_bits, np * (size_t)(pd.lnk_stride / 64) * sizeof(unsigned long long)));
     // Pair-search grid: cells 1% wider than the pair range so that rounding in the binning
     // never separates a pair in range by more than one cell; coarsened when it would hold more
     // than 8 cells per particle.
@@ -1138,6 +1201,8 @@ void part_free(PartDev &pd) {
     (void)hipFree(pd.sp.count);
     (void)hipFree(pd.sp.map);
     (void)hipFree(pd.err);
+    (void)hipFree(pd.lnk_part);
+    (void)hipFree(pd.lnk_bits);
     std::memset(&pd, 0, sizeof(pd));
 }
 
@@ -1250,7 +1315,7 @@ int part_tables_build(PartTables &t, const std::vector<PartSlot> (&slots)[2], hi
     const int n = (int)slots[0].size();
     const size_t tab_bytes = ((sizeof(PartSlot) * (size_t)(n > 0 ? n : 1) + 255) / 256) * 256;
     const size_t pre_bytes = ((sizeof(int) * (size_t)(n + 1) + 255) / 256) * 256;
-    const size_t need = 2 * tab_bytes + 4 * pre_bytes;
+    const size_t need = 2 * tab_bytes + 5 * pre_bytes;
     if (t.bytes < need) {
         (void)hipFree(t.mem);
         t.mem = nullptr;
@@ -1258,7 +1323,7 @@ int part_tables_build(PartTables &t, const std::vector<PartSlot> (&slots)[2], hi
         PHIP(hipMalloc(&t.mem, need));
         t.bytes = need;
     }
-    std::vector<int> pre[4];
+    std::vector<int> pre[5];
     for (auto &v : pre) v.assign(n + 1, 0);
     for (int j = 0; j < n; j++) {
         const PartDev &pd = slots[0][j].pd;
@@ -1266,11 +1331,12 @@ int part_tables_build(PartTables &t, const std::vector<PartSlot> (&slots)[2], hi
         pre[1][j + 1] = pre[1][j] + (pd.use_cells ? pd.ncell[0] * pd.ncell[1] * pd.ncell[2] : 0);
         pre[2][j + 1] = pre[2][j] + part_fixup_blocks(pd, 4);
         pre[3][j + 1] = pre[3][j] + part_fixup_blocks(pd, 128);
+        pre[4][j + 1] = pre[4][j] + pd.np * pd.lnk_stride;
     }
     std::vector<char> host(need, 0);
     for (int ph = 0; ph < 2; ph++)
         if (n > 0) std::memcpy(host.data() + ph * tab_bytes, slots[ph].data(), sizeof(PartSlot) * n);
-    for (int k = 0; k < 4; k++)
+    for (int k = 0; k < 5; k++)
         std::memcpy(host.data() + 2 * tab_bytes + k * pre_bytes, pre[k].data(), sizeof(int) * (n + 1));
     PHIP(hipMemcpyAsync(t.mem, host.data(), need, hipMemcpyHostToDevice, st));
     PHIP(hipStreamSynchronize(st));
@@ -1281,11 +1347,13 @@ int part_tables_build(PartTables &t, const std::vector<PartSlot> (&slots)[2], hi
         g.cstart = (const int *)(t.mem + 2 * tab_bytes + pre_bytes);
         g.sstart = (const int *)(t.mem + 2 * tab_bytes + 2 * pre_bytes);
         g.lstart = (const int *)(t.mem + 2 * tab_bytes + 3 * pre_bytes);
+        g.bstart = (const int *)(t.mem + 2 * tab_bytes + 4 * pre_bytes);
         g.nslots = n;
         g.total_p = pre[0][n];
         g.total_c = pre[1][n];
         g.total_s = pre[2][n];
         g.total_l = pre[3][n];
+        g.total_b = pre[4][n];
     }
     return RBTP_OK;
 }
@@ -1308,7 +1376,10 @@ int part_phase(const PartGrid &g, int prec, hipStream_t st) {
     PHIP(hipGetLastError());
     PHIP(dispatch_prec(prec, [&](auto *tag) {
         using S = std::remove_pointer_t<decltype(tag)>;
-        hipLaunchKernelGGL((k_link_sums<S>), tp, LB, 0, st, g);
+        hipLaunchKernelGGL((k_link_nodes<S>), (g.total_b + 255) / 256, 256, 0, st, g);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL(k_link_reduce, (tp + 256 / RL - 1) / (256 / RL), 256, 0, st, g);
         return hipGetLastError();
     }));
     hipLaunchKernelGGL(k_move, (tp + 127) / 128, 128, 0, st, g);

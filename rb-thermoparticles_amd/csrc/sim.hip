@@ -1013,7 +1013,8 @@ static size_t sim_device_bytes(const rbtp_case &c, const rbtp_derived &d) {
         if (cap > (long long)n) cap = (long long)n;
         b += (RBTP_P_COUNT + OLD_COUNT) * np * sizeof(double) + 3 * n * sizeof(int32_t) +
              12 * np * sizeof(int32_t) +
-             (size_t)cap * (2 * 27 * aux_bytes(c.precision) + sizeof(double) + 2 * sizeof(int32_t));
+             (size_t)cap * (2 * 27 * aux_bytes(c.precision) + sizeof(double) + 2 * sizeof(int32_t)) +
+             np * (size_t)link_stride((int)w) * (7 * sizeof(double) + 1);  // shares + bits (rounded up)
     }
     return b;
 }
