@@ -7,6 +7,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <cstdint>
+
 namespace rbtp {
 
 constexpr int NPOP = 27;
@@ -21,6 +23,32 @@ struct D3Q27 {
     static constexpr int opp[NPOP] = {0, 3, 4, 1, 2, 6, 5, 9, 10, 7, 8, 17, 18, 15,
                                       16, 13, 14, 11, 12, 25, 26, 23, 24, 21, 22, 19, 20};
 };
+
+// The tables packed into integer constants, for a direction index that differs between lanes:
+// a lookup is then a shift and a mask instead of a memory load.  Components are stored as c + 1
+// in 2 bits; opposites in 5 bits, 12 to a word.
+namespace packed {
+constexpr uint64_t comp(const int (&c)[NPOP]) {
+    uint64_t v = 0;
+    for (int i = 0; i < NPOP; i++) v |= (uint64_t)(c[i] + 1) << (2 * i);
+    return v;
+}
+constexpr uint64_t opp(int first) {
+    uint64_t v = 0;
+    for (int i = first; i < first + 12 && i < NPOP; i++) v |= (uint64_t)D3Q27::opp[i] << (5 * (i - first));
+    return v;
+}
+constexpr uint64_t CX = comp(D3Q27::cx), CY = comp(D3Q27::cy), CZ = comp(D3Q27::cz);
+constexpr uint64_t OPP0 = opp(0), OPP1 = opp(12), OPP2 = opp(24);
+}  // namespace packed
+__host__ __device__ inline int dir_cx(int i) { return (int)((packed::CX >> (2 * i)) & 3u) - 1; }
+__host__ __device__ inline int dir_cy(int i) { return (int)((packed::CY >> (2 * i)) & 3u) - 1; }
+__host__ __device__ inline int dir_cz(int i) { return (int)((packed::CZ >> (2 * i)) & 3u) - 1; }
+__host__ __device__ inline int dir_opp(int i) {
+    const uint64_t w = i < 12 ? packed::OPP0 : (i < 24 ? packed::OPP1 : packed::OPP2);
+    const int sh = 5 * (i < 12 ? i : (i < 24 ? i - 12 : i - 24));
+    return (int)((w >> sh) & 31u);
+}
 
 // Storage order of a buffer of 27 populations.  Nodes are taken in runs of POP_RUN consecutive
 // indices; a run keeps population 0 of its nodes, then population 1, ..., then population 26.

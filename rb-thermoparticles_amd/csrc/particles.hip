@@ -45,15 +45,6 @@ constexpr int CAND_MAX = 48;  // pair candidates per particle held in LDS
 constexpr int CELLS_MIN_NP = 32;  // particle count from which the pair search uses the grid
 constexpr int LB = 1024;  // threads per particle workgroup of the special-node search
 
-__constant__ int c_cx[NPOP] = {0, 1, 0, -1, 0, 0, 0, 1, -1, -1, 1, 1, 0, -1,
-                               0, 1, 0, -1, 0, 1, -1, -1, 1, 1, -1, -1, 1};
-__constant__ int c_cy[NPOP] = {0, 0, 1, 0, -1, 0, 0, 1, 1, -1, -1, 0, 1, 0,
-                               -1, 0, 1, 0, -1, 1, 1, -1, -1, 1, 1, -1, -1};
-__constant__ int c_cz[NPOP] = {0, 0, 0, 0, 0, 1, -1, 0, 0, 0, 0, 1, 1, 1,
-                               1, -1, -1, -1, -1, 1, 1, 1, 1, -1, -1, -1, -1};
-__constant__ int c_opp[NPOP] = {0, 3, 4, 1, 2, 6, 5, 9, 10, 7, 8, 17, 18, 15,
-                                16, 13, 14, 11, 12, 25, 26, 23, 24, 21, 22, 19, 20};
-
 __device__ inline double wt(int i) {
     return i == 0 ? 8.0 / 27.0 : (i <= 6 ? 2.0 / 27.0 : (i <= 18 ? 1.0 / 54.0 : 1.0 / 216.0));
 }
@@ -61,9 +52,9 @@ __device__ inline double wt(int i) {
 // compute_link_q (links.cu:7-23)
 __device__ inline bool link_q(int ip, double x0, double y0, double z0, double px, double py,
                               double pz, double radius_sq, double &q) {
-    const double cx_dir = static_cast<double>(c_cx[ip]);
-    const double cy_dir = static_cast<double>(c_cy[ip]);
-    const double cz_dir = static_cast<double>(c_cz[ip]);
+    const double cx_dir = static_cast<double>(dir_cx(ip));
+    const double cy_dir = static_cast<double>(dir_cy(ip));
+    const double cz_dir = static_cast<double>(dir_cz(ip));
     const double dx = x0 - px;
     const double dy = y0 - py;
     const double dz = z0 - pz;
@@ -136,20 +127,40 @@ __device__ inline bool in_domain(const Dims &d, int x, int y, int z) {
     return x >= 0 && x < d.lx && y >= 0 && y < d.ly && z >= 0 && z < d.lz;
 }
 
-// Owners of the 26 neighbours of (ix, iy, iz), all loads issued together (addresses clamped
-// into the domain); -2 where the neighbour is outside the domain.
-__device__ inline void neighbour_owners(const int32_t *owner, const Dims &d, int ix, int iy,
-                                        int iz, int (&own)[NPOP]) {
+// Owners of the 26 neighbours of (ix, iy, iz), all loads issued together (addresses clamped into
+// the domain; -2 where the neighbour is outside it).  With `interior` (the node and its
+// neighbours inside the domain) there is no clamping at all.
+__device__ inline void neighbour_owners(gptr<const int32_t> owner, const Dims &d, int ix,
+                                        int iy, int iz, bool interior, int (&own)[NPOP]) {
     own[0] = -2;
+    if (interior) {
+        const int idx = iz * d.lxy + iy * d.lx + ix;
+#pragma unroll
+        for (int ip = 1; ip < NPOP; ip++)
+            own[ip] = owner[idx + D3Q27::cx[ip] + D3Q27::cy[ip] * d.lx + D3Q27::cz[ip] * d.lxy];
+        return;
+    }
 #pragma unroll
     for (int ip = 1; ip < NPOP; ip++) {
-        const int nx = ix + c_cx[ip], ny = iy + c_cy[ip], nz = iz + c_cz[ip];
+        const int nx = ix + D3Q27::cx[ip], ny = iy + D3Q27::cy[ip], nz = iz + D3Q27::cz[ip];
         const int cx = min(max(nx, 0), d.lx - 1);
         const int cy = min(max(ny, 0), d.ly - 1);
         const int cz = min(max(nz, 0), d.lz - 1);
-        const int v = owner[cz * d.lxy + cy * d.lx + cx];
-        own[ip] = in_domain(d, nx, ny, nz) ? v : -2;
+        own[ip] = owner[cz * d.lxy + cy * d.lx + cx];
     }
+    // An empty statement that takes the 26 values as read and written: the compiler cannot
+    // move a load past it (into the branch of the select below), so the loads stay one batch
+    // with a single wait.
+    asm volatile(""
+                 : "+v"(own[1]), "+v"(own[2]), "+v"(own[3]), "+v"(own[4]), "+v"(own[5]),
+                   "+v"(own[6]), "+v"(own[7]), "+v"(own[8]), "+v"(own[9]), "+v"(own[10]),
+                   "+v"(own[11]), "+v"(own[12]), "+v"(own[13]), "+v"(own[14]), "+v"(own[15]),
+                   "+v"(own[16]), "+v"(own[17]), "+v"(own[18]), "+v"(own[19]), "+v"(own[20]),
+                   "+v"(own[21]), "+v"(own[22]), "+v"(own[23]), "+v"(own[24]), "+v"(own[25]),
+                   "+v"(own[26]));
+#pragma unroll
+    for (int ip = 1; ip < NPOP; ip++)
+        if (!in_domain(d, ix + D3Q27::cx[ip], iy + D3Q27::cy[ip], iz + D3Q27::cz[ip])) own[ip] = -2;
 }
 
 // ---- slot lookup -------------------------------------------------------------------------------
@@ -403,14 +414,16 @@ __device__ inline int push_special(const PartDev &pd, bool want, int idx, int ki
     const int lane = __lane_id();
     const int leader = __ffsll((long long)mask) - 1;
     int base = 0;
-    if (lane == leader) base = atomicAdd(pd.sp.count, __popcll(mask));
+    if (lane == leader)
+        base = __hip_atomic_fetch_add(gbl(pd.sp.count), __popcll(mask), __ATOMIC_RELAXED,
+                                      __HIP_MEMORY_SCOPE_AGENT);
     base = __shfl(base, leader);
     if (!want) return -1;
     const int j = base + __popcll(mask & ((1ull << lane) - 1ull));
     if (j < pd.sp.cap) {
-        pd.sp.node[j] = idx;
-        pd.sp_kind[j] = kind;
-        pd.sp.map[idx] = j;
+        gbl(pd.sp.node)[j] = idx;
+        gbl(pd.sp_kind)[j] = kind;
+        gbl(pd.sp.map)[idx] = j;
         return j;
     }
     atomicOr(pd.err + ERR_SPECIAL_CAP, 1);
@@ -421,12 +434,30 @@ __device__ inline int push_special(const PartDev &pd, bool want, int idx, int ki
 
 // Particle state used by the IBB of a link.
 struct PView {
-    const double *px, *py, *pz, *vx, *vy, *vz, *wx, *wy, *wz, *temp;
+    gptr<const double> px, py, pz, vx, vy, vz, wx, wy, wz, temp;
 };
+// The view of the state arrays `a` (pd.p or pd.old) with the given indices.
+template <class Arr>
+__device__ inline PView pview(const Arr &a, int ipx, int ipy, int ipz, int ivx, int ivy, int ivz,
+                              int iwx, int iwy, int iwz, int it) {
+    return PView{gbl((const double *)a[ipx]), gbl((const double *)a[ipy]),
+                 gbl((const double *)a[ipz]), gbl((const double *)a[ivx]),
+                 gbl((const double *)a[ivy]), gbl((const double *)a[ivz]),
+                 gbl((const double *)a[iwx]), gbl((const double *)a[iwy]),
+                 gbl((const double *)a[iwz]), gbl((const double *)a[it])};
+}
+__device__ inline PView pview_now(const PartDev &pd) {
+    return pview(pd.p, RBTP_P_POS_X, RBTP_P_POS_Y, RBTP_P_POS_Z, RBTP_P_VEL_X, RBTP_P_VEL_Y,
+                 RBTP_P_VEL_Z, RBTP_P_OMEGA_X, RBTP_P_OMEGA_Y, RBTP_P_OMEGA_Z, RBTP_P_TEMP);
+}
+__device__ inline PView pview_old(const PartDev &pd) {
+    return pview(pd.old, OLD_PX, OLD_PY, OLD_PZ, OLD_VX, OLD_VY, OLD_VZ, OLD_WX, OLD_WY, OLD_WZ,
+                 OLD_T);
+}
 
 template <class S>
 struct PopView {
-    const S *f, *g;
+    gptr<const S> f, g;
     size_t n;
     double tl0, tls;  // shift of the 16-bit g encoding (conduction profile)
     int lx;
@@ -462,7 +493,7 @@ struct LinkLoads {
 template <class S>
 __device__ inline LinkLoads link_loads(int ip, int idx, int idx_up, bool up_fluid, double q,
                                        const PopView<S> &pop) {
-    const int opp = c_opp[ip];
+    const int opp = dir_opp(ip);
     const int fi = up_fluid ? (q < 0.5 ? ip : opp) : ip;
     const int fn = up_fluid && q < 0.5 ? idx_up : idx;
     const int gi = up_fluid ? (q <= 0.5 ? ip : opp) : ip;
@@ -485,16 +516,16 @@ __device__ inline LinkOut eval_link_at(const LinkLoads &L, int ip, int ix, int i
     LinkOut o;
     const double tp = wt(ip);
     // wall velocity at the boundary point (ibb.cu:33-45)
-    const double cx = static_cast<double>(ix) + 0.5 + q * c_cx[ip];
-    const double cy = static_cast<double>(iy) + 0.5 + q * c_cy[ip];
-    const double cz = static_cast<double>(iz) + 0.5 + q * c_cz[ip];
+    const double cx = static_cast<double>(ix) + 0.5 + q * dir_cx(ip);
+    const double cy = static_cast<double>(iy) + 0.5 + q * dir_cy(ip);
+    const double cz = static_cast<double>(iz) + 0.5 + q * dir_cz(ip);
     const double relx = cx - pv.px[pid], rely = cy - pv.py[pid], relz = cz - pv.pz[pid];
     const double wx = pv.wx[pid], wy = pv.wy[pid], wz = pv.wz[pid];
     const double uwx = pv.vx[pid] + (relz * wy - rely * wz);
     const double uwy = pv.vy[pid] + (relx * wz - relz * wx);
     const double uwz = pv.vz[pid] + (rely * wx - relx * wy);
     o.uwx = uwx; o.uwy = uwy; o.uwz = uwz;
-    const double evel = c_cx[ip] * uwx + c_cy[ip] * uwy + c_cz[ip] * uwz;
+    const double evel = dir_cx(ip) * uwx + dir_cy(ip) * uwy + dir_cz(ip) * uwz;
     const double RT = 1.0 / 3.0;
     // / RT as div_rn with RN(1/RT) = 3 and RN(1/RT - 3) = 3 * 2^-54
     const double momentum_term = div_rn(2.0 * tp * rho0 * evel, RT, 3.0, 0x1.8p-53);
@@ -554,11 +585,8 @@ __global__ __launch_bounds__(256) void k_link_nodes(PartGrid g) {
     const Dims d = sl.d;
     const int p = (u0 - g.bstart[slot]) / pd.lnk_stride;
     const int k = u - g.bstart[slot] - p * pd.lnk_stride;
-    const int32_t *owner_prev = sl.owner_prev;
-    const PView pv{pd.p[RBTP_P_POS_X], pd.p[RBTP_P_POS_Y], pd.p[RBTP_P_POS_Z],
-                   pd.p[RBTP_P_VEL_X], pd.p[RBTP_P_VEL_Y], pd.p[RBTP_P_VEL_Z],
-                   pd.p[RBTP_P_OMEGA_X], pd.p[RBTP_P_OMEGA_Y], pd.p[RBTP_P_OMEGA_Z],
-                   pd.p[RBTP_P_TEMP]};
+    const gptr<const int32_t> owner_prev = gbl(sl.owner_prev);
+    const PView pv = pview_now(pd);
     int lo[3], hi[3];
     bool in_box = load_box(sl.box_prev, p, lo, hi);
     bool interior = false;  // the box and its neighbours lie inside the domain
@@ -575,31 +603,38 @@ __global__ __launch_bounds__(256) void k_link_nodes(PartGrid g) {
     // lowest linked pid) or of higher pid, and which neighbours are fluid
     unsigned mask = 0u, fluid_nb = 0u;
     bool lower = false, higher = false;
+    double x0 = 0.0, y0 = 0.0, z0 = 0.0;
     if (in_box) {
         idx = box_node(d, lo, hi, k, ix, iy, iz);
-        const double x0 = static_cast<double>(ix) + 0.5;
-        const double y0 = static_cast<double>(iy) + 0.5;
-        const double z0 = static_cast<double>(iz) + 0.5;
+        x0 = static_cast<double>(ix) + 0.5;
+        y0 = static_cast<double>(iy) + 0.5;
+        z0 = static_cast<double>(iz) + 0.5;
         // a link x0 -> x0 + c_ip (|c_ip| <= sqrt 3) reaches into the sphere only from within
         // R + sqrt 3 of its centre; the margin to R + 1.75 is far above any rounding
         const double dx = x0 - X, dy = y0 - Y, dz = z0 - Z;
         const double reach = pd.radius + 1.75;
-        if (dx * dx + dy * dy + dz * dz <= reach * reach && owner_prev[idx] == -1) {
+        if (dx * dx + dy * dy + dz * dz <= reach * reach) {
+            // the node's owner and its 26 neighbours' in one batch of loads
+            const int self = owner_prev[idx];
             int own[NPOP];
-            if (interior) {
+            neighbour_owners(owner_prev, d, ix, iy, iz, interior, own);
+            if (self == -1) {
+                unsigned other = 0u;
 #pragma unroll
-                for (int ip = 1; ip < NPOP; ip++)
-                    own[ip] = owner_prev[idx + c_cx[ip] + c_cy[ip] * d.lx + c_cz[ip] * d.lxy];
-            } else {
-                neighbour_owners(owner_prev, d, ix, iy, iz, own);
-            }
-            for (int ip = 1; ip < NPOP; ip++) {
-                const int q_owner = own[ip];
-                double q = 0.0;
-                if (q_owner == -1) fluid_nb |= 1u << ip;
-                if (q_owner == p) {
-                    if (link_q(ip, x0, y0, z0, X, Y, Z, pd.radius_sq, q)) mask |= 1u << ip;
-                } else if (q_owner >= 0) {
+                for (int ip = 1; ip < NPOP; ip++) {
+                    if (own[ip] == -1) fluid_nb |= 1u << ip;
+                    if (own[ip] >= 0 && own[ip] != p) other |= 1u << ip;
+                    double q;
+                    // direction constants fold into the unrolled code; no loads
+                    if (own[ip] == p && link_q(ip, x0, y0, z0, X, Y, Z, pd.radius_sq, q))
+                        mask |= 1u << ip;
+                }
+                // neighbours owned by other spheres (only near another sphere)
+                while (other) {
+                    const int ip = __builtin_ctz(other);
+                    other &= other - 1u;
+                    const int q_owner = own[ip];
+                    double q;
                     if (link_q(ip, x0, y0, z0, pv.px[q_owner], pv.py[q_owner], pv.pz[q_owner],
                                pd.radius_sq, q)) {
                         if (q_owner < p) lower = true;
@@ -612,25 +647,24 @@ __global__ __launch_bounds__(256) void k_link_nodes(PartGrid g) {
     // every lane of the wave is here
     const int j = push_special(pd, mask != 0u && !lower, idx, -1);
     const unsigned long long has = __ballot(mask != 0u);
-    if (__lane_id() == 0) pd.lnk_bits[(size_t)p * (pd.lnk_stride / 64) + k / 64] = has;
+    if (__lane_id() == 0) gbl(pd.lnk_bits)[(size_t)p * (pd.lnk_stride / 64) + k / 64] = has;
     if (mask == 0u) return;
-    const PopView<S> pop{(const S *)sl.fcur, (const S *)sl.gcur, (size_t)d.n, pd.tl0, pd.tls, d.lx};
+    const PopView<S> pop{gbl((const S *)sl.fcur), gbl((const S *)sl.gcur), (size_t)d.n, pd.tl0,
+                         pd.tls, d.lx};
     using A = typename Enc<S>::A;
-    A *out = (A *)pd.sp.pop;
+    const gptr<A> out = gbl((A *)pd.sp.pop);
     const size_t cap = (size_t)pd.sp.cap;
-    const double density = sl.macro[0][idx];
-    const double x0 = static_cast<double>(ix) + 0.5;
-    const double y0 = static_cast<double>(iy) + 0.5;
-    const double z0 = static_cast<double>(iz) + 0.5;
+    const double density = gbl((const double *)sl.macro[0])[idx];
     double v[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     uint32_t replaced = 0u;
     // links in ascending direction order; each link's lattice loads are issued together
     while (mask) {
         const int ip = __builtin_ctz(mask);
         mask &= mask - 1u;
-        const int opp = c_opp[ip];
+        const int opp = dir_opp(ip);
+        const int cx = dir_cx(ip), cy = dir_cy(ip), cz = dir_cz(ip);
         const bool up_fluid = (fluid_nb >> opp) & 1u;  // x - c_ip = x + c_opp
-        const int idx_up = idx - (c_cx[ip] + c_cy[ip] * d.lx + c_cz[ip] * d.lxy);
+        const int idx_up = idx - (cx + cy * d.lx + cz * d.lxy);
         double q = 0.0;
         link_q(ip, x0, y0, z0, X, Y, Z, pd.radius_sq, q);
         const LinkLoads Ld = link_loads<S>(ip, idx, idx_up, up_fluid, q, pop);
@@ -641,15 +675,15 @@ __global__ __launch_bounds__(256) void k_link_nodes(PartGrid g) {
             replaced |= 1u << opp;
         }
         // Force uses the link record's r = q c (links.cu:134-136, ibb.cu:160-168).
-        const double rx = q * static_cast<double>(c_cx[ip]);
-        const double ry = q * static_cast<double>(c_cy[ip]);
-        const double rz = q * static_cast<double>(c_cz[ip]);
+        const double rx = q * static_cast<double>(cx);
+        const double ry = q * static_cast<double>(cy);
+        const double rz = q * static_cast<double>(cz);
         const double relx = ix + 0.5 + rx - X;
         const double rely = iy + 0.5 + ry - Y;
         const double relz = iz + 0.5 + rz - Z;
-        const double Fx = (c_cx[ip] - L.uwx) * L.f_in - (c_cx[opp] - L.uwx) * L.f_out;
-        const double Fy = (c_cy[ip] - L.uwy) * L.f_in - (c_cy[opp] - L.uwy) * L.f_out;
-        const double Fz = (c_cz[ip] - L.uwz) * L.f_in - (c_cz[opp] - L.uwz) * L.f_out;
+        const double Fx = (cx - L.uwx) * L.f_in - (-cx - L.uwx) * L.f_out;
+        const double Fy = (cy - L.uwy) * L.f_in - (-cy - L.uwy) * L.f_out;
+        const double Fz = (cz - L.uwz) * L.f_in - (-cz - L.uwz) * L.f_out;
         v[0] += Fx;
         v[1] += Fy;
         v[2] += Fz;
@@ -658,10 +692,11 @@ __global__ __launch_bounds__(256) void k_link_nodes(PartGrid g) {
         v[5] += relx * Fy - rely * Fx;
         v[6] += L.g_in - L.g_out;
     }
-    if (j >= 0) pd.sp.mask[j] = replaced | (higher ? SP_MULTI : 0u);
+    if (j >= 0) gbl(pd.sp.mask)[j] = replaced | (higher ? SP_MULTI : 0u);
     const size_t plane = (size_t)pd.np * pd.lnk_stride;
     const size_t e = (size_t)p * pd.lnk_stride + k;
-    for (int c = 0; c < 7; c++) pd.lnk_part[c * plane + e] = v[c];
+    const gptr<double> part = gbl(pd.lnk_part);
+    for (int c = 0; c < 7; c++) part[c * plane + e] = v[c];
 }
 
 __global__ __launch_bounds__(256) void k_link_reduce(PartGrid g) {
@@ -824,7 +859,7 @@ template <class S>
 __device__ inline void pull_one(const PopView<S> &pop, int j, int idx, int ix, int iy, int iz,
                                 const Dims &d, double rho_w, double thot, double tcold,
                                 double &fv, double &gv) {
-    const int cx = c_cx[j], cy = c_cy[j], cz = c_cz[j];
+    const int cx = dir_cx(j), cy = dir_cy(j), cz = dir_cz(j);
     const bool outx = (cx == 1 && ix == 0) || (cx == -1 && ix == d.lx - 1);
     const bool outy = (cy == 1 && iy == 0) || (cy == -1 && iy == d.ly - 1);
     const bool outz = (cz == 1 && iz == 0) || (cz == -1 && iz == d.lz - 1);
@@ -834,7 +869,7 @@ __device__ inline void pull_one(const PopView<S> &pop, int j, int idx, int ix, i
         gv = pop.G(j, src);
         return;
     }
-    const int op = c_opp[j];
+    const int op = dir_opp(j);
     fv = pop.F(op, idx);
     const double g_opp = pop.G(op, idx);
     if (outz || outy) {
@@ -858,19 +893,19 @@ __global__ __launch_bounds__(128) void k_fixup_links(PartGrid g) {
     const PartDev &pd = sl.pd;
     const Dims d = sl.d;
     using A = typename Enc<S>::A;
-    const PopView<S> pop{(const S *)sl.fcur, (const S *)sl.gcur, (size_t)d.n, pd.tl0, pd.tls, d.lx};
-    const int32_t *owner_prev = sl.owner_prev;
-    const double *macro_rho = sl.macro[0];
+    const PopView<S> pop{gbl((const S *)sl.fcur), gbl((const S *)sl.gcur), (size_t)d.n, pd.tl0,
+                         pd.tls, d.lx};
+    const gptr<const int32_t> owner_prev = gbl(sl.owner_prev);
+    const gptr<const double> macro_rho = gbl((const double *)sl.macro[0]);
     const int blk = blockIdx.x - g.lstart[slot];
     const int nblk = g.lstart[slot + 1] - g.lstart[slot];
     const int nlink = min(pd.sp.count[1], pd.sp.cap);
     const size_t cap = (size_t)pd.sp.cap;
-    A *out = (A *)pd.sp.pop;
-    const PView pv{pd.old[OLD_PX], pd.old[OLD_PY], pd.old[OLD_PZ], pd.old[OLD_VX],
-                   pd.old[OLD_VY], pd.old[OLD_VZ], pd.old[OLD_WX], pd.old[OLD_WY],
-                   pd.old[OLD_WZ], pd.old[OLD_T]};
+    const gptr<A> out = gbl((A *)pd.sp.pop);
+    const gptr<uint32_t> spmask = gbl(pd.sp.mask);
+    const PView pv = pview_old(pd);
     for (int j = blk * 128 + threadIdx.x; j < nlink; j += nblk * 128) {
-        if (!(pd.sp.mask[j] & SP_MULTI)) continue;  // k_link_nodes wrote this node's links
+        if (!(spmask[j] & SP_MULTI)) continue;  // k_link_nodes wrote this node's links
         const int idx = pd.sp.node[j];
         const int iz = idx / d.lxy;
         const int rem = idx - iz * d.lxy;
@@ -881,7 +916,7 @@ __global__ __launch_bounds__(128) void k_fixup_links(PartGrid g) {
         const double z0 = static_cast<double>(iz) + 0.5;
         const double density = macro_rho[idx];
         int own[NPOP];
-        neighbour_owners(owner_prev, d, ix, iy, iz, own);
+        neighbour_owners(owner_prev, d, ix, iy, iz, false, own);
         uint32_t m = 0u;
         for (int ip = 1; ip < NPOP; ip++) {
             const int q_owner = own[ip];
@@ -890,9 +925,9 @@ __global__ __launch_bounds__(128) void k_fixup_links(PartGrid g) {
             if (!link_q(ip, x0, y0, z0, pv.px[q_owner], pv.py[q_owner], pv.pz[q_owner],
                         pd.radius_sq, q))
                 continue;
-            const int opp = c_opp[ip];
+            const int opp = dir_opp(ip);
             const bool up_fluid = own[opp] == -1;  // x - c_ip = x + c_opp, inside and fluid
-            const int idx_up = idx - (c_cx[ip] + c_cy[ip] * d.lx + c_cz[ip] * d.lxy);
+            const int idx_up = idx - (dir_cx(ip) + dir_cy(ip) * d.lx + dir_cz(ip) * d.lxy);
             const LinkLoads Ld = link_loads<S>(ip, idx, idx_up, up_fluid, q, pop);
             const LinkOut L = eval_link_at(Ld, ip, ix, iy, iz, up_fluid, q, q_owner, pv, density,
                                            pd.rho0);
@@ -900,7 +935,7 @@ __global__ __launch_bounds__(128) void k_fixup_links(PartGrid g) {
             out[(size_t)(NPOP + opp) * cap + j] = A(L.g_out);
             m |= 1u << opp;
         }
-        pd.sp.mask[j] = m;
+        spmask[j] = m;
     }
 }
 
@@ -916,7 +951,7 @@ __global__ __launch_bounds__(128) void k_fixup_refill(PartGrid g) {
     const PartDev &pd = sl.pd;
     const Dims d = sl.d;
     using A = typename Enc<S>::A;
-    const PopView<S> pop{(const S *)sl.fcur, (const S *)sl.gcur, (size_t)d.n, pd.tl0, pd.tls, d.lx};
+    const PopView<S> pop{gbl((const S *)sl.fcur), gbl((const S *)sl.gcur), (size_t)d.n, pd.tl0, pd.tls, d.lx};
     const int32_t *owner_prev = sl.owner_prev;
     A *fx_field = (A *)sl.fx;
     double *macro_rho = sl.macro[0], *macro_ux = sl.macro[1], *macro_uy = sl.macro[2];
@@ -944,11 +979,11 @@ __global__ __launch_bounds__(128) void k_fixup_refill(PartGrid g) {
             double temp_f = 0.0, temp_g = 0.0;
             int sample_count = 0;
             for (int ip2 = 1; ip2 < NPOP; ++ip2) {
-                const int ix1 = ix + c_cx[ip2], iy1 = iy + c_cy[ip2], iz1 = iz + c_cz[ip2];
+                const int ix1 = ix + dir_cx(ip2), iy1 = iy + dir_cy(ip2), iz1 = iz + dir_cz(ip2);
                 if (!in_domain(d, ix1, iy1, iz1)) continue;
                 const int idx1 = iz1 * d.lxy + iy1 * d.lx + ix1;
                 if (owner_prev[idx1] != -1) continue;
-                const int ix2 = ix + 2 * c_cx[ip2], iy2 = iy + 2 * c_cy[ip2], iz2 = iz + 2 * c_cz[ip2];
+                const int ix2 = ix + 2 * dir_cx(ip2), iy2 = iy + 2 * dir_cy(ip2), iz2 = iz + 2 * dir_cz(ip2);
                 if (in_domain(d, ix2, iy2, iz2)) {
                     const int idx2 = iz2 * d.lxy + iy2 * d.lx + ix2;
                     if (owner_prev[idx2] == -1) {
@@ -975,7 +1010,7 @@ __global__ __launch_bounds__(128) void k_fixup_refill(PartGrid g) {
                 const double v = vy + (relx * omega_z - relz * omega_x);
                 const double w = vz + (rely * omega_x - relx * omega_y);
                 const double r = pd.rho0;
-                const double eu = c_cx[ipop] * u + c_cy[ipop] * v + c_cz[ipop] * w;
+                const double eu = dir_cx(ipop) * u + dir_cy(ipop) * v + dir_cz(ipop) * w;
                 const double uv = (u * u + v * v + w * w) / RT;
                 temp_f = wt(ipop) * (r + eu + 0.5 * (eu * eu - uv));
                 temp_g = wt(ipop) * tempT * (1.0 + eu + 0.5 * (eu * eu - uv));
@@ -996,9 +1031,9 @@ __global__ __launch_bounds__(128) void k_fixup_refill(PartGrid g) {
             const double f_val = __shfl(fv, ip, 32);
             const double g_val = __shfl(gv, ip, 32);
             rho_loc += f_val;
-            mom_x += c_cx[ip] * f_val;
-            mom_y += c_cy[ip] * f_val;
-            mom_z += c_cz[ip] * f_val;
+            mom_x += dir_cx(ip) * f_val;
+            mom_y += dir_cy(ip) * f_val;
+            mom_z += dir_cz(ip) * f_val;
             phi_loc += g_val;
         }
         if (lane == 0) {
@@ -1056,7 +1091,7 @@ __global__ __launch_bounds__(PB) void k_collect_links(PartDev pd, const int32_t 
         const double y0 = static_cast<double>(iy) + 0.5;
         const double z0 = static_cast<double>(iz) + 0.5;
         for (int ip = 1; ip < NPOP; ip++) {
-            const int nx = ix + c_cx[ip], ny = iy + c_cy[ip], nz = iz + c_cz[ip];
+            const int nx = ix + dir_cx(ip), ny = iy + dir_cy(ip), nz = iz + dir_cz(ip);
             if (!in_domain(d, nx, ny, nz)) continue;
             if (owner[nz * d.lxy + ny * d.lx + nx] != p) continue;
             double q = 0.0;
@@ -1066,9 +1101,9 @@ __global__ __launch_bounds__(PB) void k_collect_links(PartDev pd, const int32_t 
             rbtp_link L;
             L.cell_i = ix; L.cell_j = iy; L.cell_k = iz; L.dir = ip; L.pid = p; L.pad = 0;
             L.q = q;
-            L.rx = q * static_cast<double>(c_cx[ip]);
-            L.ry = q * static_cast<double>(c_cy[ip]);
-            L.rz = q * static_cast<double>(c_cz[ip]);
+            L.rx = q * static_cast<double>(dir_cx(ip));
+            L.ry = q * static_cast<double>(dir_cy(ip));
+            L.rz = q * static_cast<double>(dir_cz(ip));
             out[j] = L;
         }
     }

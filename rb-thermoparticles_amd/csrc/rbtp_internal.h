@@ -41,29 +41,44 @@ struct FluidK {
 //   double/float: the reference's value.
 //   half/bf16:    value/w_i for f; value/w_i - T_cond(ix) for g (shifted and scaled), where
 //                 T_cond is the linear conduction profile at the node that holds the value.
+// Pointer into global memory.  Loads and stores through it compile to global_* instructions;
+// through a generic pointer the compiler cannot place (one read from a table in memory) they
+// become flat_* instructions, which also count against the scalar-load counter and so make
+// every wait for a scalar load wait for them too.
+template <class T> using gptr = __attribute__((address_space(1))) T *;
+template <class T> __device__ inline gptr<T> gbl(T *p) { return (gptr<T>)p; }
+
 template <class S> struct Enc;
 template <> struct Enc<double> {
     using A = double;  // auxiliary per-node field type (F_x, wall rho)
     static constexpr bool scaled = false;
     __device__ static inline double ld(const double *p) { return *p; }
+    __device__ static inline double ld(gptr<const double> p) { return *p; }
     __device__ static inline void st(double *p, double v) { *p = v; }
 };
 template <> struct Enc<float> {
     using A = float;
     static constexpr bool scaled = false;
     __device__ static inline float ld(const float *p) { return *p; }
+    __device__ static inline float ld(gptr<const float> p) { return *p; }
     __device__ static inline void st(float *p, float v) { *p = v; }
 };
 template <> struct Enc<__half> {
     using A = float;
     static constexpr bool scaled = true;
     __device__ static inline float ld(const __half *p) { return __half2float(*p); }
+    __device__ static inline float ld(gptr<const __half> p) {
+        return __half2float(__ushort_as_half(*(gptr<const unsigned short>)p));
+    }
     __device__ static inline void st(__half *p, float v) { *p = __float2half_rn(v); }
 };
 template <> struct Enc<__hip_bfloat16> {
     using A = float;
     static constexpr bool scaled = true;
     __device__ static inline float ld(const __hip_bfloat16 *p) { return __bfloat162float(*p); }
+    __device__ static inline float ld(gptr<const __hip_bfloat16> p) {
+        return __bfloat162float(__ushort_as_bfloat16(*(gptr<const unsigned short>)p));
+    }
     __device__ static inline void st(__hip_bfloat16 *p, float v) { *p = __float2bfloat16(v); }
 };
 
