@@ -28,17 +28,6 @@ namespace rbtp {
 namespace {
 
 constexpr int BLOCK = 256;
-constexpr int NUM_XCD = 8;
-
-// Workgroups are dispatched round-robin over the 8 XCDs.  Remap the hardware workgroup id so
-// that each XCD sweeps one contiguous range of the lattice: the partially shared cache lines at
-// workgroup seams then stay in one XCD's L2.
-__device__ inline int xcd_block(int nblocks) {
-    const int bid = blockIdx.x;
-    const int q = nblocks / NUM_XCD, r = nblocks % NUM_XCD;
-    const int xcd = bid % NUM_XCD, local = bid / NUM_XCD;
-    return xcd * q + (xcd < r ? xcd : r) + local;
-}
 
 // Node outputs shared by the lattice kernel and the special-node kernel.
 template <class S> struct NodeOut {

@@ -17,6 +17,17 @@ __device__ inline double div_rn(double x, double d, double yh, double yl) {
     return __builtin_fma(r, yh, q0);
 }
 
+// Workgroups are dispatched round-robin over the 8 XCDs, each with its own L2.  The remapped id
+// gives each XCD one contiguous range of logical workgroups (a bijection of [0, nblocks)), so
+// that neighbouring workgroups, which share cache lines, share an L2.
+constexpr int NUM_XCD = 8;
+__device__ inline int xcd_block(int nblocks) {
+    const int bid = blockIdx.x;
+    const int q = nblocks / NUM_XCD, r = nblocks % NUM_XCD;
+    const int xcd = bid % NUM_XCD, local = bid / NUM_XCD;
+    return xcd * q + (xcd < r ? xcd : r) + local;
+}
+
 // ---- populations in and out ------------------------------------------------------------------
 
 template <class S, class T>

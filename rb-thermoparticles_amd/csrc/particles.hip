@@ -72,6 +72,27 @@ __device__ inline bool link_q(int ip, double x0, double y0, double z0, double px
     return (q >= 0.0 && q <= 1.0);
 }
 
+// Whether link_q finds a link from a node x0 outside the sphere to its neighbour x0 + c_ip
+// inside it.  The quadratic of link_q is then positive at q = 0 and negative at q = 1, so its
+// smaller root lies in (0, 1); when both values clear a margin far above the rounding of
+// link_q (1e-6 against errors near 1e-14 for spheres of a few lattice units), the root link_q
+// computes does too, and no square root is needed.  Otherwise link_q itself decides.
+__device__ inline bool link_exists(int ip, double x0, double y0, double z0, double px,
+                                   double py, double pz, double radius_sq) {
+    const double cx_dir = static_cast<double>(dir_cx(ip));
+    const double cy_dir = static_cast<double>(dir_cy(ip));
+    const double cz_dir = static_cast<double>(dir_cz(ip));
+    const double dx = x0 - px;
+    const double dy = y0 - py;
+    const double dz = z0 - pz;
+    const double a = cx_dir * cx_dir + cy_dir * cy_dir + cz_dir * cz_dir;
+    const double b = 2.0 * (cx_dir * dx + cy_dir * dy + cz_dir * dz);
+    const double c = dx * dx + dy * dy + dz * dz - radius_sq;
+    if (c > 1e-6 && (a + b) + c < -1e-6) return true;
+    double q;
+    return link_q(ip, x0, y0, z0, px, py, pz, radius_sq, q);
+}
+
 // Marking box of a particle (links.cu:164-177); false when empty.
 __host__ __device__ inline bool marking_box(double px, double py, double pz, double R,
                                             const Dims &d, int lo[3], int hi[3]) {
@@ -575,7 +596,8 @@ constexpr uint32_t SP_MULTI = 1u << 31;  // special-node mask flag: links to sev
 
 template <class S>
 __global__ __launch_bounds__(256) void k_link_nodes(PartGrid g) {
-    const int u = blockIdx.x * 256 + threadIdx.x;
+    // a particle's waves are consecutive: on one XCD, they share its L2
+    const int u = xcd_block(gridDim.x) * 256 + threadIdx.x;
     if (u >= g.total_b) return;  // total_b is a whole number of waves
     // every wave lies in one particle of one slot: slot and particle are wave-uniform
     const int u0 = __builtin_amdgcn_readfirstlane(u);
@@ -624,9 +646,8 @@ __global__ __launch_bounds__(256) void k_link_nodes(PartGrid g) {
                 for (int ip = 1; ip < NPOP; ip++) {
                     if (own[ip] == -1) fluid_nb |= 1u << ip;
                     if (own[ip] >= 0 && own[ip] != p) other |= 1u << ip;
-                    double q;
                     // direction constants fold into the unrolled code; no loads
-                    if (own[ip] == p && link_q(ip, x0, y0, z0, X, Y, Z, pd.radius_sq, q))
+                    if (own[ip] == p && link_exists(ip, x0, y0, z0, X, Y, Z, pd.radius_sq))
                         mask |= 1u << ip;
                 }
                 // neighbours owned by other spheres (only near another sphere)
