@@ -1,4 +1,4 @@
-"""Precision forms against FP64 on the particle-laden production cases.
+"""Precision forms against FP64 on the production cases.
 
     python precision_report.py run --workload c5 --precision fp64 --steps 20000 --out c5_fp64.npz
     python precision_report.py run --workload c5 --precision fp16s --steps 20000 --out c5_fp16s.npz
@@ -7,8 +7,15 @@
 Workloads (synthetic initial state: linear temperature + 1e-3 seeded noise, fluid at rest,
 spheres placed by the reference's glibc-rand rejection sampler, each sphere starting at the
 fluid temperature of its centre cell):
+  c1: 64x64x32, Ra=1e5, Pr=0.71, no spheres (laminar: the flow settles to a steady state);
+  c2: 320x320x160, Ra=1e8, Pr=0.71, no spheres;
   c3: 320x320x160, Ra=1e8, Pr=0.71, 300 spheres of diameter 10;
   c5: 512x512x256, Ra=1e9, Pr=0.71, 5000 spheres of diameter 8.
+
+--perturb EPS adds EPS times a second seeded noise field to the initial temperature.  An FP64
+run with a tiny EPS (1e-12) against the plain FP64 run measures how far the observables of a
+chaotic flow drift from a perturbation far below any storage rounding: the floor against which
+the precision forms' deviations are read.
 
 `run` advances one precision form, sampling Nu (the seven numbers of output_nu) and the kinetic
 energy every --sample steps, and stores the samples, the device time and the final particle
@@ -46,8 +53,10 @@ def load_rbtp():
 
 
 def workload(name, precision):
-    from cases import config3, config5
-    return {"c3": config3, "c5": config5}[name](precision=precision)
+    from cases import config1, config2, config3, config5
+    if name == "c1":
+        return dict(config1(), precision=precision)
+    return {"c2": config2, "c3": config3, "c5": config5}[name](precision=precision)
 
 
 def run(args):
@@ -60,6 +69,9 @@ def run(args):
     case = rb.default_case(**kw)
     sim = rb.Sim(case)
     T = rb.linear_temperature(case, 2024, 1e-3)
+    if args.perturb:
+        T = T + args.perturb * (rb.linear_temperature(case, 77, 1.0) -
+                                rb.linear_temperature(case, 77, 0.0))
     sim.init(temperature=T)
     if sim.np:
         ppos = np.concatenate([sim.particles("pos_x"), sim.particles("pos_y"), sim.particles("pos_z")])
@@ -92,7 +104,7 @@ def run(args):
     np.savez(args.out, nu=np.array(samples), ke=np.array(ke), steps_at=np.array(steps_at),
              traj=np.array(traj) if traj else np.zeros((0, 4, 0)),
              dev_ms=dev_ms, wall_s=wall, n=sim.n, npart=sim.np, links=nlinks, precision=prec,
-             workload=args.workload, **parts)
+             workload=args.workload, perturb=args.perturb, **parts)
     print(f"[{args.workload} {args.precision}] {done} steps, {sim.n * done / (dev_ms * 1e-3) / 1e6:.0f} "
           f"MLUPS (device), {nlinks} links", flush=True)
     sim.close()
@@ -111,11 +123,9 @@ def compare(files):
         half = len(d["nu"]) // 2
         nu_mean = d["nu"][half:].mean(axis=0)
         nu_ref = ref["nu"][half:].mean(axis=0)
-        dpos = np.sqrt((d["pos_x"] - ref["pos_x"]) ** 2 + (d["pos_y"] - ref["pos_y"]) ** 2 +
-                       (d["pos_z"] - ref["pos_z"]) ** 2)
         row = dict(
             workload=str(d["workload"]), precision=[k for k, v in PRECISIONS.items() if v == prec][0],
-            steps=steps, mlups=mlups,
+            perturb=float(d.get("perturb", 0.0)), steps=steps, mlups=mlups,
             lattice_gbs_at_algorithmic_bytes=mlups * BYTES_PER_NODE[prec] / 1e3,
             mlups_vs_fp64=mlups / (float(ref["n"]) * int(ref["steps_at"][-1]) /
                                    (float(ref["dev_ms"]) * 1e-3) / 1e6),
@@ -123,13 +133,18 @@ def compare(files):
             nu_hot_rel_err=float(abs(nu_mean[4] - nu_ref[4]) / abs(nu_ref[4])),
             nu_cold_rel_err=float(abs(nu_mean[3] - nu_ref[3]) / abs(nu_ref[3])),
             nu_bulk_rel_err=float(abs(nu_mean[2] - nu_ref[2]) / max(abs(nu_ref[2]), 1e-300)),
+            nu_hot_final=float(d["nu"][-1][4]),
+            nu_hot_final_rel_err=float(abs(d["nu"][-1][4] - ref["nu"][-1][4]) / abs(ref["nu"][-1][4])),
             ke_final=float(d["ke"][-1]),
             ke_rel_err=float(abs(d["ke"][-1] - ref["ke"][-1]) / abs(ref["ke"][-1])),
-            particle_rms_pos_err=float(np.sqrt(np.mean(dpos ** 2))),
-            particle_max_pos_err=float(dpos.max()),
-            particle_rms_temp_err=float(np.sqrt(np.mean((d["temp"] - ref["temp"]) ** 2))),
-            particle_temp_spread=float(ref["temp"].std()),
             npart=int(d["npart"]), links=int(d["links"]))
+        if int(d["npart"]) > 0:
+            dpos = np.sqrt((d["pos_x"] - ref["pos_x"]) ** 2 + (d["pos_y"] - ref["pos_y"]) ** 2 +
+                           (d["pos_z"] - ref["pos_z"]) ** 2)
+            row.update(particle_rms_pos_err=float(np.sqrt(np.mean(dpos ** 2))),
+                       particle_max_pos_err=float(dpos.max()),
+                       particle_rms_temp_err=float(np.sqrt(np.mean((d["temp"] - ref["temp"]) ** 2))),
+                       particle_temp_spread=float(ref["temp"].std()))
         if d.get("traj") is not None and ref.get("traj") is not None and d["traj"].size and ref["traj"].size:
             m = min(len(d["traj"]), len(ref["traj"]))
             dp = d["traj"][:m, :3] - ref["traj"][:m, :3]
@@ -146,12 +161,14 @@ def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     sub = ap.add_subparsers(dest="cmd", required=True)
     r = sub.add_parser("run")
-    r.add_argument("--workload", choices=["c3", "c5"], default="c5")
+    r.add_argument("--workload", choices=["c1", "c2", "c3", "c5"], default="c5")
     r.add_argument("--precision", choices=list(PRECISIONS), default="fp64")
     r.add_argument("--steps", type=int, default=20000)
     r.add_argument("--sample", type=int, default=500)
     r.add_argument("--out", required=True)
     r.add_argument("--no-particles", action="store_true", help="the lattice alone")
+    r.add_argument("--perturb", type=float, default=0.0,
+                   help="amplitude of a second seeded noise added to the initial temperature")
     c = sub.add_parser("compare")
     c.add_argument("files", nargs="+")
     args = ap.parse_args()

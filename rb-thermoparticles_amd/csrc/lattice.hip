@@ -15,9 +15,10 @@
 //     and tauc, no fused multiply-add (the translation unit is compiled with
 //     -ffp-contract=off).  Terms whose velocity component is zero are skipped; adding or
 //     multiplying an exact zero changes at most the sign of a zero, never a value.
-//   * fast (collide_fast<T>): the same model with divisions replaced by reciprocal
-//     multiplications, in FP64 or FP32, contraction allowed.  Used by the FP32 and
-//     FP16/BF16-storage forms.
+//   * FP32 (collide_shifted): the same model in FP32 arithmetic with divisions replaced by
+//     reciprocal multiplications and contraction allowed, on g shifted by the node's conduction
+//     temperature and scaled by 1/w_i (the variables the FP32 and FP16/BF16 storage forms
+//     hold).  Used by the FP32 and FP16/BF16-storage forms.
 #include "lattice.h"
 #include "lattice_device.h"
 
@@ -112,57 +113,73 @@ __device__ inline double collide_strict(double (&f)[NPOP], double (&g)[NPOP], do
     return fx;
 }
 
-// Fast form of the same node update (reciprocals instead of divisions), in T = float or double.
+// The same node update in FP32 arithmetic (reciprocals instead of divisions, contraction
+// allowed), for the shifted encodings.  g enters and leaves in the shifted variables of
+// gather_shifted, p_i = g_i / w_i - T_s; with phi' = sum w_i p_i (so phi = T_s + phi') and
+// A_i = eu_i + (eu_i^2 - u.u/RT)/2 (so geq_i = w_i phi (1 + A_i)), the non-equilibrium part of
+// collision_BGK_scalar (collision.cu:55-87) is
+//   (g_i - geqnew_i) / w_i = p_i - phi' - phi (A_i + s_i rho),  s_0 = -(1 - w_0)/w_0, s_i = 1,
+// and the update p_i <- p_i - that / tauc + (source)_i / w_i.  Every term is a deviation of
+// size |T - T_s| or |u|, never the O(T) value of g itself.
 template <class S, class T>
-__device__ inline T collide_fast(T (&f)[NPOP], T (&g)[NPOP], T fx_old, int idx, T gshift,
-                                 const NodeOut<S> &o, const FluidK &k, T &rho, T &ux, T &uy,
-                                 T &uz, T &phi) {
+__device__ inline T collide_shifted(T (&f)[NPOP], T (&p)[NPOP], T fx_old, int idx, T ts,
+                                    const NodeOut<S> &o, const FluidK &k, T &rho, T &ux, T &uy,
+                                    T &uz, T &phi) {
 #pragma clang fp contract(fast)
-    moments<T>(f, g, rho, ux, uy, uz, phi);
+    T phis = T(0);
+    rho = T(0); ux = T(0); uy = T(0); uz = T(0);
+#pragma unroll
+    for (int i = 0; i < NPOP; i++) {
+        rho += f[i];
+        if (D3Q27::cx[i] == 1) ux += f[i];
+        if (D3Q27::cx[i] == -1) ux -= f[i];
+        if (D3Q27::cy[i] == 1) uy += f[i];
+        if (D3Q27::cy[i] == -1) uy -= f[i];
+        if (D3Q27::cz[i] == 1) uz += f[i];
+        if (D3Q27::cz[i] == -1) uz -= f[i];
+        phis += T(weight(i)) * p[i];
+    }
+    phi = ts + phis;
     ux = ux + T(0.5) * fx_old;
-    const T t0 = T(k.t0);
-    const T fx = T(k.gb) * (phi - t0);
+    const T fx = T(k.gb) * (phi - T(k.t0));
     const T inv_tau = T(k.inv_tau), inv_tauc = T(k.inv_tauc);
     const T uv = T(3) * (ux * ux + uy * uy + uz * uz);
     const T fc3 = T(3) * T(k.force_coeff) * fx;
+    const T s0 = T(-k.lambda0 / weight(0));
     T gx = T(0), gy = T(0), gz = T(0);
-    const T lam = T(k.lambda0) * phi * rho;
 #pragma unroll
     for (int i = 0; i < NPOP; i++) {
         const T tp = T(weight(i));
         const T eu = (i == 0) ? T(0) : T(3) * cdot<T>(i, ux, uy, uz);
         const T h = T(0.5) * (eu * eu - uv);
-        const T Xi = (T(1) + eu) + h;
+        const T ai = eu + h;
         const T feq = tp * ((rho + eu) + h);
         const T fpop = f[i];
-        const T fnew = fpop + (feq - fpop) * inv_tau + (T(D3Q27::cx[i]) - ux) * fc3 * tp * Xi;
+        const T fnew = fpop + (feq - fpop) * inv_tau + (T(D3Q27::cx[i]) - ux) * fc3 * tp * (T(1) + ai);
         enc_f<S, T>(o.fout + pop_at(i, idx), fnew, i);
-        const T geq = tp * phi * Xi;
-        const T geqnew = (i == 0) ? (geq - lam) : (geq + tp * phi * rho);
-        f[i] = geqnew;  // reuse the register for g's equilibrium
-        const T diff = g[i] - geqnew;
-        if (D3Q27::cx[i] == 1) gx += diff;
-        if (D3Q27::cx[i] == -1) gx -= diff;
-        if (D3Q27::cy[i] == 1) gy += diff;
-        if (D3Q27::cy[i] == -1) gy -= diff;
-        if (D3Q27::cz[i] == 1) gz += diff;
-        if (D3Q27::cz[i] == -1) gz -= diff;
+        const T dneq = p[i] - phis - phi * (ai + ((i == 0) ? s0 : T(1)) * rho);
+        f[i] = dneq;  // reuse the register for the non-equilibrium part
+        const T wd = tp * dneq;
+        if (D3Q27::cx[i] == 1) gx += wd;
+        if (D3Q27::cx[i] == -1) gx -= wd;
+        if (D3Q27::cy[i] == 1) gy += wd;
+        if (D3Q27::cy[i] == -1) gy -= wd;
+        if (D3Q27::cz[i] == 1) gz += wd;
+        if (D3Q27::cz[i] == -1) gz -= wd;
     }
     const T inv_gd = T(1) / (rho * T(1.0 / 3.0) + T(k.c2t));
     const T rc = T(k.ralph_coeff);
 #pragma unroll
     for (int i = 0; i < NPOP; i++) {
-        const T tp = T(weight(i));
-        const T gpop = g[i];
-        T gnew = gpop + (f[i] - gpop) * inv_tauc;
+        T pnew = p[i] - f[i] * inv_tauc;
         if (i != 0) {
             const int cx = D3Q27::cx[i];
             const T gneq = cdot<T>(i, gx, gy, gz);
             const T eb = cx == 0 ? T(0) : (cx > 0 ? fx : -fx);
             const T gradt = -(T(2) * gneq + eb * phi) * inv_gd;
-            gnew += tp * rc * (gradt * rho + T(3) * phi * eb);
+            pnew += rc * (gradt * rho + T(3) * phi * eb);
         }
-        enc_g<S, T>(o.gout + pop_at(i, idx), gnew, i, gshift);
+        Enc<S>::st(o.gout + pop_at(i, idx), float(pnew));
     }
     return fx;
 }
@@ -176,7 +193,7 @@ __device__ inline T collide_node(T (&f)[NPOP], T (&g)[NPOP], T fx_old, int idx, 
         (void)gshift;
         return collide_strict<S>(f, g, fx_old, idx, o, k, rho, ux, uy, uz, phi);
     } else {
-        return collide_fast<S, T>(f, g, fx_old, idx, gshift, o, k, rho, ux, uy, uz, phi);
+        return collide_shifted<S, T>(f, g, fx_old, idx, gshift, o, k, rho, ux, uy, uz, phi);
     }
 }
 
@@ -210,10 +227,16 @@ __device__ inline void node_step(const StepArgs<S> &a, int idx) {
         if (ix == 0) rho_w = T(a.rho_wall[iz * a.d.ly + iy]);
         else if (ix == a.d.lx - 1) rho_w = T(a.rho_wall[a.d.ly * a.d.lz + iz * a.d.ly + iy]);
     }
+    // g: the post-stream value (strict FP64) or its shifted form (FP32 arithmetic)
     T f[NPOP], g[NPOP];
     const T tl0 = T(a.k.tl0), tls = T(a.k.tls);
-    gather<S, T, NOSTREAM>(a.fin, a.gin, idx, ix, iy, iz, a.d, rho_w, tl0, tls, T(a.k.thot),
-                           T(a.k.tcold), f, g);
+    const T ts = tl0 + tls * T(ix);
+    if constexpr (STRICT)
+        gather<S, T, NOSTREAM>(a.fin, a.gin, idx, ix, iy, iz, a.d, rho_w, tl0, tls, T(a.k.thot),
+                               T(a.k.tcold), f, g);
+    else
+        gather_shifted<S, T, NOSTREAM>(a.fin, a.gin, idx, ix, iy, iz, a.d, rho_w, ts, tls,
+                                       T(a.k.thot), T(a.k.tcold), f, g);
     // At a special node (IBB link node or refilled node) the populations the particle phase
     // corrected replace the pulled ones; the first step after a (re)load has none.
     const int sj = (!NOSTREAM && a.sp_map) ? a.sp_map[idx] : -1;
@@ -223,15 +246,16 @@ __device__ inline void node_step(const StepArgs<S> &a, int idx) {
         for (int i = 0; i < NPOP; i++) {
             if (m & (1u << i)) {
                 f[i] = T(a.sp_pop[(size_t)i * a.sp_cap + sj]);
-                g[i] = T(a.sp_pop[(size_t)(NPOP + i) * a.sp_cap + sj]);
+                const T gv = T(a.sp_pop[(size_t)(NPOP + i) * a.sp_cap + sj]);
+                if constexpr (STRICT) g[i] = gv;
+                else g[i] = gv * T(1.0 / weight(i)) - ts;
             }
         }
     }
     const T fx_old = T(a.fx[idx]);
     NodeOut<S> o{a.fout, a.gout};
     T rho, ux, uy, uz, phi;
-    const T fx = collide_node<S, T, STRICT>(f, g, fx_old, idx, tl0 + tls * T(ix), o, a.k, rho,
-                                            ux, uy, uz, phi);
+    const T fx = collide_node<S, T, STRICT>(f, g, fx_old, idx, ts, o, a.k, rho, ux, uy, uz, phi);
     write_node_aux<S, T>(a, idx, ix, iy, iz, fx, rho, ux, uy, uz, phi);
 }
 
@@ -309,8 +333,8 @@ __global__ __launch_bounds__(BLOCK) void k_decode(const S *__restrict__ in, doub
     const double w = i == 0 ? 8.0 / 27.0 : (i <= 6 ? 2.0 / 27.0 : (i <= 18 ? 1.0 / 54.0 : 1.0 / 216.0));
     const T s = T(Enc<S>::ld(in + pop_at(i, (int)(j % n))));
     if constexpr (Enc<S>::scaled) {
-        const T shift = T(tl0 + tls * double((j % n) % (size_t)lx));
-        out[j] = is_g ? double((s + shift) * T(w)) : double(s * T(w));
+        const double shift = tl0 + tls * double((j % n) % (size_t)lx);
+        out[j] = is_g ? (double(s) + shift) * w : double(s) * w;
     } else {
         out[j] = double(s);
     }
@@ -412,13 +436,10 @@ hipError_t launch_lattice_step(const StepArgs<S> &a, bool strict, bool nostream,
                                hipStream_t st) {
     const int nb = blocks_for((size_t)a.d.n);
     if constexpr (std::is_same<S, double>::value) {
-        if (strict) {
-            if (nostream) hipLaunchKernelGGL((k_lattice_step<S, double, true, true>), nb, BLOCK, 0, st, a);
-            else hipLaunchKernelGGL((k_lattice_step<S, double, true, false>), nb, BLOCK, 0, st, a);
-        } else {
-            if (nostream) hipLaunchKernelGGL((k_lattice_step<S, double, false, true>), nb, BLOCK, 0, st, a);
-            else hipLaunchKernelGGL((k_lattice_step<S, double, false, false>), nb, BLOCK, 0, st, a);
-        }
+        // FP64 storage has one arithmetic form: the strict one
+        if (!strict) return hipErrorInvalidValue;
+        if (nostream) hipLaunchKernelGGL((k_lattice_step<S, double, true, true>), nb, BLOCK, 0, st, a);
+        else hipLaunchKernelGGL((k_lattice_step<S, double, true, false>), nb, BLOCK, 0, st, a);
     } else {
         (void)strict;
         if (nostream) hipLaunchKernelGGL((k_lattice_step<S, float, false, true>), nb, BLOCK, 0, st, a);
@@ -476,23 +497,18 @@ hipError_t launch_lattice_batched(const StepArgs<S> *slots_dev, const int *tile_
                                   int nslots, int total_tiles, int block, int grid, bool strict,
                                   hipStream_t st) {
     if (nslots <= 0 || total_tiles <= 0) return hipSuccess;
-    const bool dbl = std::is_same<S, double>::value;
 #define RBTP_BATCHED(T, STRICT_, BLK)                                                          \
     hipLaunchKernelGGL((k_lattice_batched<S, T, STRICT_, BLK>), grid, BLK, 0, st, slots_dev,   \
                        tile_start_dev, nslots, total_tiles)
-    if (dbl && strict) {
+    if constexpr (std::is_same<S, double>::value) {
+        if (!strict) return hipErrorInvalidValue;
         switch (block) {
             case 64: RBTP_BATCHED(double, true, 64); break;
             case 128: RBTP_BATCHED(double, true, 128); break;
             default: RBTP_BATCHED(double, true, 256); break;
         }
-    } else if (dbl) {
-        switch (block) {
-            case 64: RBTP_BATCHED(double, false, 64); break;
-            case 128: RBTP_BATCHED(double, false, 128); break;
-            default: RBTP_BATCHED(double, false, 256); break;
-        }
     } else {
+        (void)strict;
         switch (block) {
             case 64: RBTP_BATCHED(float, false, 64); break;
             case 128: RBTP_BATCHED(float, false, 128); break;

@@ -89,6 +89,56 @@ __device__ inline void gather(const S *__restrict__ fin, const S *__restrict__ g
     }
 }
 
+// gather() for the shifted encodings (float, half, bf16), with g returned in the variables the
+// FP32 collision works in: p_i = g_i / w_i - T_s, T_s = tl0 + tls*ix the conduction temperature
+// of the node itself.  A value pulled from x - c_i is stored relative to that node's T_s, which
+// differs by -tls*c_x; the wall rules of streaming_scalar become
+//   z and y walls (adiabatic):  p_i = p*_opp(x)
+//   x walls:                    p_i = -p*_opp(x) + 2 (T_wall - T_s) + 2 T_wall rho(x)
+// so that no O(T) quantity enters the arithmetic on g, only its deviation from T_s.
+template <class S, class T, bool NOSTREAM>
+__device__ inline void gather_shifted(const S *__restrict__ fin, const S *__restrict__ gin,
+                                      int idx, int ix, int iy, int iz, const Dims &d, T rho_w,
+                                      T ts, T tls, T thot, T tcold, T (&f)[NPOP], T (&p)[NPOP]) {
+    static_assert(Enc<S>::scaled, "shifted gather needs a shifted encoding");
+    // all 54 loads first, then the arithmetic, so that they are in flight together
+    T sf[NPOP], sg[NPOP];
+#pragma unroll
+    for (int i = 0; i < NPOP; i++) {
+        const int cx = D3Q27::cx[i], cy = D3Q27::cy[i], cz = D3Q27::cz[i];
+        size_t src = pop_at(i, idx);
+        if (!NOSTREAM) {
+            const bool outx = (cx == 1 && ix == 0) || (cx == -1 && ix == d.lx - 1);
+            const bool outy = (cy == 1 && iy == 0) || (cy == -1 && iy == d.ly - 1);
+            const bool outz = (cz == 1 && iz == 0) || (cz == -1 && iz == d.lz - 1);
+            const bool inside = !(outx || outy || outz);
+            const int off = cx + cy * d.lx + cz * d.lxy;
+            src = inside ? pop_at(i, idx - off) : pop_at(D3Q27::opp[i], idx);
+        }
+        sf[i] = T(Enc<S>::ld(fin + src));
+        sg[i] = T(Enc<S>::ld(gin + src));
+    }
+#pragma unroll
+    for (int i = 0; i < NPOP; i++) {
+        const int cx = D3Q27::cx[i], cy = D3Q27::cy[i], cz = D3Q27::cz[i];
+        f[i] = sf[i] * T(weight(i));
+        if (NOSTREAM || cx == 0) {
+            p[i] = sg[i];
+            continue;
+        }
+        const bool outx = (cx == 1 && ix == 0) || (cx == -1 && ix == d.lx - 1);
+        const bool outy = (cy == 1 && iy == 0) || (cy == -1 && iy == d.ly - 1);
+        const bool outz = (cz == 1 && iz == 0) || (cz == -1 && iz == d.lz - 1);
+        const bool inside = !(outx || outy || outz);
+        const T sv = sg[i];
+        const T pulled = cx > 0 ? sv - (inside ? tls : T(0)) : sv + (inside ? tls : T(0));
+        // x wall only: hot plate at x=0 (c_x=+1), cold plate at x=LX (c_x=-1).
+        const T twall = (cx == 1) ? thot : tcold;
+        const T walled = -sv + T(2.0) * (twall - ts) + T(2.0) * twall * rho_w;
+        p[i] = (outx && !outy && !outz) ? walled : pulled;
+    }
+}
+
 // Moments of macrovar (macrovar.cu:18-33): sequential sums in population order; terms with a
 // zero velocity component are skipped (they add an exact zero).
 template <class T>

@@ -32,15 +32,16 @@ struct FluidK {
     // Double-word reciprocals (hi = RN(1/d), lo = RN((1 - d*hi)/d)) of the constant divisors of
     // the strict form: RT = 1.0/3.0, 3.0, tau, tauc.
     double rt_h, rt_l, three_h, three_l, tau_h, tau_l, tauc_h, tauc_l;
-    // Shift of the 16-bit g encoding: the conduction profile T(ix) = tl0 + tls*ix between the
-    // hot and cold walls at cell centres.
+    // Shift of the reduced-precision g encoding: the conduction profile T(ix) = tl0 + tls*ix
+    // between the hot and cold walls at cell centres.
     double tl0, tls;
 };
 
 // Storage encodings of the populations.
-//   double/float: the reference's value.
-//   half/bf16:    value/w_i for f; value/w_i - T_cond(ix) for g (shifted and scaled), where
-//                 T_cond is the linear conduction profile at the node that holds the value.
+//   double:           the reference's value.
+//   float/half/bf16:  value/w_i for f; value/w_i - T_cond(ix) for g (shifted and scaled), where
+//                     T_cond is the linear conduction profile at the node that holds the value.
+//                     The FP32-arithmetic collision works in the same shifted variables.
 // Pointer into global memory.  Loads and stores through it compile to global_* instructions;
 // through a generic pointer the compiler cannot place (one read from a table in memory) they
 // become flat_* instructions, which also count against the scalar-load counter and so make
@@ -58,7 +59,7 @@ template <> struct Enc<double> {
 };
 template <> struct Enc<float> {
     using A = float;
-    static constexpr bool scaled = false;
+    static constexpr bool scaled = true;
     __device__ static inline float ld(const float *p) { return *p; }
     __device__ static inline float ld(gptr<const float> p) { return *p; }
     __device__ static inline void st(float *p, float v) { *p = v; }
