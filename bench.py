@@ -9,7 +9,8 @@ Prints one JSON line.  value = lattice updates per second of the whole job (MLUP
 state resident in HBM before the timed region starts.  The line also carries:
   roofline      the fused lattice kernel's algorithmic HBM bytes per launch (880 B/node: 54 FP64
                 populations read and written plus F_x read and written) over its average launch
-                time measured with HIP events on the library's stream;
+                time measured with HIP events on the library's stream, beside two measured
+                ceilings: a streaming copy and the same access pattern without arithmetic;
   cpu_baseline  the FP64 host oracle (OpenMP) timed on this machine's cores on a bounded sample;
   forms         the FP16-storage (FP32 math), FP32 and BF16-storage forms on the same workload,
                 with their Nu and KE deviations, and the 300-sphere FP64 case.
@@ -207,6 +208,8 @@ def main():
         return
     copy_gbs = rb.bandwidth_probe(2 << 30, 20)
     log(f"copy probe: {copy_gbs:.0f} GB/s")
+    stencil_gbs = rb.stencil_probe(kw64["lx"], kw64["ly"], kw64["lz"], 20)
+    log(f"stencil-pattern probe: {stencil_gbs:.0f} GB/s")
     r64 = run_form(rb, kw64, args.steps, args.warmup, args.timing_steps, log, args.nu_window)
     n = r64["n"]
     mlups = n * args.steps / (r64["wall_ms"] * 1e-3) / 1e6
@@ -263,6 +266,8 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "k_lattice_step<double,double,strict>",
                      "copy_probe_gbs": copy_gbs, "frac_of_copy_probe": achieved / copy_gbs,
+                     "stencil_probe_gbs": stencil_gbs,
+                     "frac_of_stencil_probe": achieved / stencil_gbs,
                      "kernel_ms": r64["kernel_ms"], "bytes_per_node": BYTES_PER_NODE[0]},
         "cpu_baseline": cpu,
         "forms": forms,

@@ -38,7 +38,7 @@ extern "C" {
 
 /* Storage/arithmetic forms of the lattice. */
 #define RBTP_PREC_FP64 0   /* FP64 storage and arithmetic, operation order of the reference */
-#define RBTP_PREC_FP32 1   /* FP32 storage and arithmetic */
+#define RBTP_PREC_FP32 1   /* FP32 arithmetic, populations stored as FP32 (shifted, 1/w scaled) */
 #define RBTP_PREC_FP16S 2  /* FP32 arithmetic, populations stored as FP16 (shifted, 1/w scaled) */
 #define RBTP_PREC_BF16S 3  /* FP32 arithmetic, populations stored as BF16 (shifted, 1/w scaled) */
 
@@ -226,6 +226,11 @@ int rbtp_kinetic_energy(rbtp_sim *s, double *out);
  * 16-byte vectors) timed over `reps` launches; *gbs = 2*bytes*reps/time in GB/s.  No reference
  * counterpart; used to put the roofline fractions in context. */
 int rbtp_bandwidth_probe(size_t bytes, int reps, double *gbs);
+/* The HBM access pattern of the FP64 lattice step without its arithmetic, on an lx x ly x lz
+ * lattice: the same pull of 54 populations under the wall rules, F_x read and written, and 54
+ * stores, timed over `reps` launches; *gbs = 880 B x nodes x reps / time in GB/s.  No reference
+ * counterpart; it is the ceiling of the access pattern itself on this device. */
+int rbtp_stencil_probe(int lx, int ly, int lz, int reps, double *gbs);
 
 /*
  * Output-side fields and files (output/output_flow.cu, output/output_part.cu,

@@ -137,6 +137,7 @@ def _load():
         "rbtp_kinetic_energy": (I, [P, C.POINTER(D)]),
         "rbtp_case_from_parameters_h": (I, [C.c_char_p, C.POINTER(Case)]),
         "rbtp_bandwidth_probe": (I, [C.c_size_t, I, C.POINTER(D)]),
+        "rbtp_stencil_probe": (I, [I, I, I, I, C.POINTER(D)]),
         "rbtp_get_output_fields": (I, [P, dp]),
         "rbtp_profile": (I, [P, dp]),
         "rbtp_write_fg": (I, [P, C.c_char_p]),
@@ -189,6 +190,14 @@ def bandwidth_probe(nbytes: int = 2 << 30, reps: int = 20) -> float:
     """Achievable HBM bandwidth (GB/s) of a streaming copy on the current device."""
     v = C.c_double()
     _check(lib.rbtp_bandwidth_probe(int(nbytes), int(reps), C.byref(v)))
+    return v.value
+
+
+def stencil_probe(lx: int, ly: int, lz: int, reps: int = 20) -> float:
+    """GB/s of the FP64 lattice step's access pattern alone (rbtp_stencil_probe), at 880 B per
+    node: the ceiling of the pattern on this device."""
+    v = C.c_double()
+    _check(lib.rbtp_stencil_probe(int(lx), int(ly), int(lz), int(reps), C.byref(v)))
     return v.value
 
 

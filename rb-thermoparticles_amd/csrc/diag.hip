@@ -4,6 +4,7 @@
 // pairwise tree), so results are reproducible run to run; the host only combines the LX plane
 // values in the reference's order.  Nothing but a few kilobytes crosses to the host, instead of
 // the reference's 2 x 8N + 4N bytes per call.
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -234,20 +235,21 @@ __global__ __launch_bounds__(DB) void k_profile(const double *fields, Dims d, do
 
 // Streaming copy of 16-byte vectors: the reference kernel for the achievable HBM bandwidth and
 // for calibrating the memory counters (it reads and writes exactly `count` vectors).  Each
-// thread moves CP_UNROLL vectors 256 apart, all loads issued before the stores; one workgroup
-// per 256*CP_UNROLL vectors.
-constexpr int CP_UNROLL = 4;
+// thread moves U vectors 256 apart, all loads issued before the stores; one workgroup per 256*U
+// vectors.  The probe runs U = 1, 2 and 4 and keeps the fastest: which is best differs between
+// devices.
+template <int U>
 __global__ __launch_bounds__(256) void k_copy_probe(const double2 *__restrict__ src,
                                                     double2 *__restrict__ dst, size_t count) {
-    const size_t base = (size_t)blockIdx.x * (256 * CP_UNROLL) + threadIdx.x;
-    double2 v[CP_UNROLL];
+    const size_t base = (size_t)blockIdx.x * (256 * U) + threadIdx.x;
+    double2 v[U];
 #pragma unroll
-    for (int u = 0; u < CP_UNROLL; u++) {
+    for (int u = 0; u < U; u++) {
         const size_t i = base + (size_t)u * 256;
         if (i < count) v[u] = src[i];
     }
 #pragma unroll
-    for (int u = 0; u < CP_UNROLL; u++) {
+    for (int u = 0; u < U; u++) {
         const size_t i = base + (size_t)u * 256;
         if (i < count) dst[i] = v[u];
     }
@@ -401,6 +403,7 @@ int diag_profile(const double *dev_fields, const Dims &dims, hipStream_t st, dou
 int diag_copy_probe(size_t bytes, int reps, double *gbs) {
     const size_t count = bytes / sizeof(double2);
     if (count == 0 || reps < 1) return report(RBTP_E_ARG, "copy probe: bytes >= 16 and reps >= 1");
+    if ((count + 255) / 256 > 0x7fffffffu) return report(RBTP_E_ARG, "copy probe: too many bytes");
     double2 *a = nullptr, *b = nullptr;
     hipStream_t st = nullptr;
     hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -410,23 +413,28 @@ int diag_copy_probe(size_t bytes, int reps, double *gbs) {
     if (e == hipSuccess) e = hipEventCreate(&e0);
     if (e == hipSuccess) e = hipEventCreate(&e1);
     if (e == hipSuccess) e = hipMemsetAsync(a, 0, count * sizeof(double2), st);
-    const size_t grid_sz = (count + 256 * CP_UNROLL - 1) / (256 * CP_UNROLL);
-    const unsigned grid = (unsigned)grid_sz;
-    if (e == hipSuccess && (grid_sz == 0 || grid_sz > 0x7fffffffu)) e = hipErrorInvalidValue;
-    if (e == hipSuccess) {
-        hipLaunchKernelGGL(k_copy_probe, grid, 256, 0, st, a, b, count);  // warm-up
+    double best = 0.0;
+    auto run = [&](auto kernel, int unroll) {
+        const unsigned grid = (unsigned)((count + 256 * (size_t)unroll - 1) / (256 * (size_t)unroll));
+        if (e != hipSuccess) return;
+        hipLaunchKernelGGL(kernel, grid, 256, 0, st, a, b, count);  // warm-up
         e = hipGetLastError();
-    }
-    if (e == hipSuccess) e = hipEventRecord(e0, st);
-    for (int r = 0; r < reps && e == hipSuccess; r++) {
-        hipLaunchKernelGGL(k_copy_probe, grid, 256, 0, st, a, b, count);
-        e = hipGetLastError();
-    }
-    if (e == hipSuccess) e = hipEventRecord(e1, st);
-    if (e == hipSuccess) e = hipEventSynchronize(e1);
-    float ms = 0.f;
-    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
-    if (e == hipSuccess) *gbs = 2.0 * (double)(count * sizeof(double2)) * reps / (ms * 1e-3) / 1e9;
+        if (e == hipSuccess) e = hipEventRecord(e0, st);
+        for (int r = 0; r < reps && e == hipSuccess; r++) {
+            hipLaunchKernelGGL(kernel, grid, 256, 0, st, a, b, count);
+            e = hipGetLastError();
+        }
+        if (e == hipSuccess) e = hipEventRecord(e1, st);
+        if (e == hipSuccess) e = hipEventSynchronize(e1);
+        float ms = 0.f;
+        if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+        if (e == hipSuccess && ms > 0.f)
+            best = std::max(best, 2.0 * (double)(count * sizeof(double2)) * reps / (ms * 1e-3) / 1e9);
+    };
+    run(k_copy_probe<1>, 1);
+    run(k_copy_probe<2>, 2);
+    run(k_copy_probe<4>, 4);
+    if (e == hipSuccess) *gbs = best;
     (void)hipFree(a);
     (void)hipFree(b);
     if (e0) (void)hipEventDestroy(e0);

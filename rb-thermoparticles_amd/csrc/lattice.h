@@ -30,6 +30,9 @@ hipError_t launch_init(S *f, S *g, typename Enc<S>::A *fx, typename Enc<S>::A *r
                        double beta, double tl0, double tls, Dims d, double *mr, double *mux,
                        double *muy, double *muz, double *mphi, hipStream_t st);
 
+// Time the FP64 lattice step's access pattern alone on an lx x ly x lz lattice (rbtp_stencil_probe).
+int lattice_stencil_probe(int lx, int ly, int lz, int reps, double *gbs);
+
 template <class S>
 hipError_t launch_forcing_field(typename Enc<S>::A *fx, typename Enc<S>::A *rho_wall,
                                 const double *rho, const double *phi, double gb, double t0,

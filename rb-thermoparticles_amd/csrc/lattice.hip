@@ -20,6 +20,7 @@
 //     temperature and scaled by 1/w_i (the variables the FP32 and FP16/BF16 storage forms
 //     hold).  Used by the FP32 and FP16/BF16-storage forms.
 #include "lattice.h"
+#include "../../include/rbtp.h"
 #include "lattice_device.h"
 
 #include <type_traits>
@@ -427,7 +428,76 @@ __global__ __launch_bounds__(BLOCK) void k_forcing_field(typename Enc<S>::A *fx,
 
 inline int blocks_for(size_t n) { return (int)((n + BLOCK - 1) / BLOCK); }
 
+// The FP64 lattice kernel's memory traffic without its arithmetic: the same launch shape, the
+// same pull of 54 populations under the wall rules (gather), the same F_x read and write, and
+// the 54 values stored unchanged into the other buffer.  Its time is the ceiling the access
+// pattern itself reaches on the device.
+__global__ __launch_bounds__(BLOCK) void k_stencil_probe(const double *__restrict__ fin,
+                                                         const double *__restrict__ gin,
+                                                         double *__restrict__ fout,
+                                                         double *__restrict__ gout,
+                                                         double *__restrict__ fx, Dims d) {
+    const int nblocks = (d.n + BLOCK - 1) / BLOCK;
+    const int idx = xcd_block(nblocks) * BLOCK + threadIdx.x;
+    if (idx >= d.n) return;
+    const int iz = idx / d.lxy;
+    const int rem = idx - iz * d.lxy;
+    const int iy = rem / d.lx;
+    const int ix = rem - iy * d.lx;
+    double f[NPOP], g[NPOP];
+    gather<double, double, false>(fin, gin, idx, ix, iy, iz, d, 0.0, 0.0, 0.0, 0.0, 0.0, f, g);
+#pragma unroll
+    for (int i = 0; i < NPOP; i++) {
+        fout[pop_at(i, idx)] = f[i];
+        gout[pop_at(i, idx)] = g[i];
+    }
+    fx[idx] = fx[idx] + f[0];
+}
+
 }  // namespace
+
+int lattice_stencil_probe(int lx, int ly, int lz, int reps, double *gbs) {
+    if (lx < 2 || ly < 2 || lz < 2 || reps < 1 || (long long)lx * ly * lz > 0x7fffffffLL)
+        return report(RBTP_E_ARG, "stencil probe: lattice of 2..2^31 nodes, reps >= 1");
+    Dims d;
+    d.lx = lx; d.ly = ly; d.lz = lz; d.lxy = lx * ly; d.n = lx * ly * lz;
+    const size_t len = pop_buffer_len((size_t)d.n);
+    double *buf[4] = {nullptr, nullptr, nullptr, nullptr};
+    double *fx = nullptr;
+    hipStream_t st = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipError_t e = hipSuccess;
+    for (int k = 0; k < 4 && e == hipSuccess; k++) e = hipMalloc(&buf[k], len * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc(&fx, (size_t)d.n * sizeof(double));
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    for (int k = 0; k < 2 && e == hipSuccess; k++) e = hipMemsetAsync(buf[k], 0, len * sizeof(double), st);
+    if (e == hipSuccess) e = hipMemsetAsync(fx, 0, (size_t)d.n * sizeof(double), st);
+    const int nb = blocks_for((size_t)d.n);
+    int cur = 0;
+    auto launch = [&]() {
+        hipLaunchKernelGGL(k_stencil_probe, nb, BLOCK, 0, st, buf[cur], buf[cur + 1],
+                           buf[2 - cur], buf[3 - cur], fx, d);
+        cur = 2 - cur;
+        return hipGetLastError();
+    };
+    if (e == hipSuccess) e = launch();  // warm-up
+    if (e == hipSuccess) e = hipEventRecord(e0, st);
+    for (int r = 0; r < reps && e == hipSuccess; r++) e = launch();
+    if (e == hipSuccess) e = hipEventRecord(e1, st);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    float ms = 0.f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    if (e == hipSuccess) *gbs = 880.0 * (double)d.n * reps / (ms * 1e-3) / 1e9;
+    for (double *p : buf) (void)hipFree(p);
+    (void)hipFree(fx);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (st) (void)hipStreamDestroy(st);
+    if (e != hipSuccess) return report_hip(e, "stencil probe");
+    return RBTP_OK;
+}
 
 // ---- host launchers --------------------------------------------------------------------------
 

@@ -57,8 +57,6 @@ inline int link_stride(int box_w) {
     return (b * b * b + 63) / 64 * 64;
 }
 
-int report(int code, const char *msg);
-int report_hip(hipError_t e, const char *what);
 
 int part_alloc(PartDev &pd, const rbtp_case &c, const rbtp_derived &d, const Dims &dims);
 void part_free(PartDev &pd);

@@ -12,6 +12,10 @@
 namespace rbtp {
 
 // Lattice shape.  n = lx*ly*lz must stay below 2^31 (checked by rbtp_create).
+// Record an error for rbtp_last_error() and return its code (defined with the C-ABI).
+int report(int code, const char *msg);
+int report_hip(hipError_t e, const char *what);
+
 struct Dims {
     int lx, ly, lz, lxy;
     int n;

@@ -748,6 +748,11 @@ extern "C" int rbtp_bandwidth_probe(size_t bytes, int reps, double *gbs) {
     return diag_copy_probe(bytes, reps, gbs);
 }
 
+extern "C" int rbtp_stencil_probe(int lx, int ly, int lz, int reps, double *gbs) {
+    if (!gbs) return set_error(RBTP_E_ARG, "null argument");
+    return lattice_stencil_probe(lx, ly, lz, reps, gbs);
+}
+
 extern "C" int rbtp_kinetic_energy(rbtp_sim *s, double *out) {
     if (!s || !out) return set_error(RBTP_E_ARG, "null argument");
     if (!s->macro_valid) return set_error(RBTP_E_STATE, "run the last step with RBTP_STEP_MACRO_LAST first");

@@ -102,7 +102,9 @@ def test_uniform_temperature_rest_state(rb, O):
 
 @pytest.mark.parametrize("prec", [1, 2, 3])
 def test_low_precision_forms_track_fp64(rb, prec):
-    """FP32 math with FP32/FP16/BF16 storage stays close to FP64 over a short run."""
+    """FP32 math with FP32/FP16/BF16 storage stays close to FP64 over a short run.  The bounds
+    sit about 10x above the deviations measured with the collision on shifted g (phi after 200
+    steps: 6e-8 FP32, 3e-5 FP16, 5e-4 BF16)."""
     kw = single_phase(32, 32, 16, ra=1e5)
     ref = rb.Sim(rb.default_case(**kw))
     low = rb.Sim(rb.default_case(**kw, precision=prec))
@@ -111,7 +113,7 @@ def test_low_precision_forms_track_fp64(rb, prec):
     low.init(temperature=T)
     ref.step(200)
     low.step(200)
-    tol = {1: 1e-5, 2: 2e-3, 3: 2e-2}[prec]
+    tol = {1: 1e-6, 2: 3e-4, 3: 5e-3}[prec]
     assert rel_err(low.field("phi"), ref.field("phi")) <= tol
     nu_r, nu_l = ref.nu(), low.nu()
     assert abs(nu_l[4] - nu_r[4]) <= 10 * tol * abs(nu_r[4])

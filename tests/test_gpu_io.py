@@ -194,3 +194,15 @@ def test_kinetic_energy_over_fluid_nodes(rb, O):
     expect = np.mean(0.5 * (u[0] ** 2 + u[1] ** 2 + u[2] ** 2))
     assert sim.kinetic_energy() == pytest.approx(expect, rel=1e-12)
     assert fluid.sum() < fluid.size
+
+
+def test_measurement_probes(rb):
+    """The copy and stencil-pattern probes return a positive rate and reject bad arguments."""
+    copy = rb.bandwidth_probe(64 << 20, 2)
+    assert 0.0 < copy < 1e5
+    sten = rb.stencil_probe(64, 48, 40, 2)
+    assert 0.0 < sten < 1e5
+    with pytest.raises(rb.RbtpError):
+        rb.stencil_probe(1, 48, 40, 2)
+    with pytest.raises(rb.RbtpError):
+        rb.stencil_probe(64, 48, 40, 0)
