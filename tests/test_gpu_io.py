@@ -97,6 +97,36 @@ def test_single_phase_restart_is_bit_continuous(rb, tmp_path):
     assert np.array_equal(again.field("phi"), full.field("phi"))
 
 
+@pytest.mark.parametrize("prec", [1, 2])
+def test_reduced_precision_restart_continues(rb, tmp_path, prec):
+    """An FP32- or FP16-storage run written out (populations decoded to FP64) and read back
+    (encoded again into the shifted storage form) continues within the storage rounding.  F_x
+    is recomputed from the stored phi on load (initial.cu:207) instead of carried over in FP32,
+    so the two runs differ by single roundings that the storage then rounds again: measured
+    1.1e-6 (FP32) and 1.1e-3 (FP16) after 18 more steps; a broken encoding would be O(1) off."""
+    kw = single_phase(24, 20, 16, ra=1e6, precision=prec)
+    case = rb.default_case(**kw)
+    T = rb.linear_temperature(case, 5, 1e-2)
+    full = rb.Sim(case)
+    full.init(temperature=T)
+    full.step(30)
+    part = rb.Sim(case)
+    part.init(temperature=T)
+    part.step(12)
+    d = tmp_path / "Ra"
+    d.mkdir()
+    part.write_fg(str(d / "fg000000012.dat"))
+    part.write_flow(str(d / "000000012.dat"))
+    again = rb.Sim(rb.default_case(**kw, continuous_steps=12))
+    again.restart_load(str(d), 12)
+    assert again.current_step == 12
+    again.step(18)
+    tol = {1: 1e-5, 2: 5e-3}[prec]
+    for name in ("f", "g"):
+        assert rel_err(again.populations(name), full.populations(name)) <= tol, name
+    assert rel_err(again.field("phi"), full.field("phi")) <= tol
+
+
 def test_restart_with_particles_reads_pos_file(rb, tmp_path):
     kw = particle_case(20, 16, 14, npart=2, diam=5.0, ra=1e5, beta=0.05)
     case = rb.default_case(**kw)
