@@ -874,33 +874,6 @@ __global__ __launch_bounds__(LB) void k_refill_find(PartGrid g) {
     }
 }
 
-// Post-stream value of population j at node (ix,iy,iz) under the wall rules of streaming and
-// streaming_scalar (the single-direction form of gather()).
-template <class S>
-__device__ inline void pull_one(const PopView<S> &pop, int j, int idx, int ix, int iy, int iz,
-                                const Dims &d, double rho_w, double thot, double tcold,
-                                double &fv, double &gv) {
-    const int cx = dir_cx(j), cy = dir_cy(j), cz = dir_cz(j);
-    const bool outx = (cx == 1 && ix == 0) || (cx == -1 && ix == d.lx - 1);
-    const bool outy = (cy == 1 && iy == 0) || (cy == -1 && iy == d.ly - 1);
-    const bool outz = (cz == 1 && iz == 0) || (cz == -1 && iz == d.lz - 1);
-    if (!(outx || outy || outz)) {
-        const int src = idx - (cx + cy * d.lx + cz * d.lxy);
-        fv = pop.F(j, src);
-        gv = pop.G(j, src);
-        return;
-    }
-    const int op = dir_opp(j);
-    fv = pop.F(op, idx);
-    const double g_opp = pop.G(op, idx);
-    if (outz || outy) {
-        gv = g_opp;
-    } else {
-        const double twall = (cx == 1) ? thot : tcold;
-        gv = -g_opp + 2.0 * wt(j) * twall * (1.0 + rho_w);
-    }
-}
-
 // Bounce-back values of the IBB link nodes that link to more than one sphere (SP_MULTI entries at
 // the head of the special list; k_link_nodes wrote every other link node), one thread per node:
 // for every link x -> x + c_ip (x + c_ip solid and linked, with the state before this step's
