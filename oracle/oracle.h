@@ -41,6 +41,11 @@
  *                         library (same call, same glibc).
  *
  * Summation order: see the header of oracle.cpp (deterministic per-particle reductions).
+ *
+ * Drift: tests/golden/ holds the oracle's own output on three small cases (fluid only, one
+ * sphere, two moving spheres with refill), written by tests/golden/make_fixtures.py; the oracle
+ * must reproduce them bit for bit (tests/test_golden.py).  These fixtures pin the oracle against
+ * its own past, not against the reference.
  */
 #ifndef RBTP_ORACLE_H
 #define RBTP_ORACLE_H
