@@ -8,6 +8,10 @@ from golden_check import FIXTURES, compare, load, replay
 pytestmark = pytest.mark.gpu
 
 
+def test_fixtures_present():
+    assert set(FIXTURES) >= {"fluid_8x8x4", "sphere_12x10x10", "moving_spheres_16x12x12"}
+
+
 @pytest.mark.parametrize("name", FIXTURES)
 def test_library_reproduces_fixture(rb, name):
     fx = load(name)
