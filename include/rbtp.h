@@ -290,6 +290,13 @@ int rbtp_last_step_ms(rbtp_sim *s, double *ms);
 int rbtp_set_kernel_timing(rbtp_sim *s, int enable);
 int rbtp_kernel_ms(rbtp_sim *s, double *avg_ms, int64_t *launches);
 
+/* Reduced-precision storage keeps g_i/w_i relative to a per-x-column shift profile, the
+ * conduction profile after rbtp_init and rbtp_restart_load.  Every `steps` steps of rbtp_step*
+ * the profile is moved to the current plane-mean temperature and the stored g re-encoded
+ * (0 never; default 500; ignored for FP64 storage and for batch slots).  No counterpart in the
+ * reference, whose FP64 storage has no encoding. */
+int rbtp_set_shift_interval(rbtp_sim *s, int steps);
+
 #ifdef __cplusplus
 }
 #endif

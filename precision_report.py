@@ -68,6 +68,8 @@ def run(args):
         kw.update(activate_particles=0, particle_fraction=0.0)
     case = rb.default_case(**kw)
     sim = rb.Sim(case)
+    if args.shift_interval is not None:
+        sim.set_shift_interval(args.shift_interval)
     T = rb.linear_temperature(case, 2024, 1e-3)
     if args.perturb:
         T = T + args.perturb * (rb.linear_temperature(case, 77, 1.0) -
@@ -104,7 +106,8 @@ def run(args):
     np.savez(args.out, nu=np.array(samples), ke=np.array(ke), steps_at=np.array(steps_at),
              traj=np.array(traj) if traj else np.zeros((0, 4, 0)),
              dev_ms=dev_ms, wall_s=wall, n=sim.n, npart=sim.np, links=nlinks, precision=prec,
-             workload=args.workload, perturb=args.perturb, **parts)
+             workload=args.workload, perturb=args.perturb,
+             shift_interval=-1 if args.shift_interval is None else args.shift_interval, **parts)
     print(f"[{args.workload} {args.precision}] {done} steps, {sim.n * done / (dev_ms * 1e-3) / 1e6:.0f} "
           f"MLUPS (device), {nlinks} links", flush=True)
     sim.close()
@@ -125,7 +128,8 @@ def compare(files):
         nu_ref = ref["nu"][half:].mean(axis=0)
         row = dict(
             workload=str(d["workload"]), precision=[k for k, v in PRECISIONS.items() if v == prec][0],
-            perturb=float(d.get("perturb", 0.0)), steps=steps, mlups=mlups,
+            perturb=float(d.get("perturb", 0.0)), shift_interval=int(d.get("shift_interval", -1)),
+            steps=steps, mlups=mlups,
             lattice_gbs_at_algorithmic_bytes=mlups * BYTES_PER_NODE[prec] / 1e3,
             mlups_vs_fp64=mlups / (float(ref["n"]) * int(ref["steps_at"][-1]) /
                                    (float(ref["dev_ms"]) * 1e-3) / 1e6),
@@ -169,6 +173,9 @@ def main():
     r.add_argument("--no-particles", action="store_true", help="the lattice alone")
     r.add_argument("--perturb", type=float, default=0.0,
                    help="amplitude of a second seeded noise added to the initial temperature")
+    r.add_argument("--shift-interval", type=int, default=None,
+                   help="steps between refreshes of the reduced-precision shift profile "
+                        "(0: keep the conduction profile; default: the library's 500)")
     c = sub.add_parser("compare")
     c.add_argument("files", nargs="+")
     args = ap.parse_args()

@@ -157,6 +157,7 @@ def _load():
         "rbtp_last_step_ms": (I, [P, C.POINTER(D)]),
         "rbtp_set_kernel_timing": (I, [P, I]),
         "rbtp_kernel_ms": (I, [P, C.POINTER(D), C.POINTER(C.c_int64)]),
+        "rbtp_set_shift_interval": (I, [P, I]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -377,6 +378,10 @@ class Sim:
     # -- timing ------------------------------------------------------------------------------------
     def set_kernel_timing(self, enable: bool) -> None:
         _check(lib.rbtp_set_kernel_timing(self._h, 1 if enable else 0))
+
+    def set_shift_interval(self, steps: int) -> None:
+        """Steps between refreshes of the reduced-precision shift profile (0: never)."""
+        _check(lib.rbtp_set_shift_interval(self._h, int(steps)))
 
     def timing(self) -> StepTiming:
         ms = C.c_double()

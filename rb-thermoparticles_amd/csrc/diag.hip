@@ -171,7 +171,8 @@ __global__ __launch_bounds__(DB) void k_output_fields(const S *fc, const S *gc,
         const double geqnew = (ip == 0) ? (geq - lambda0 * phi_local * rho_local)
                                         : (geq + tp * phi_local * rho_local);
         const double diff = dec_g<S, double>(Enc<S>::ld(gc + pop_at(ip, idx)), ip,
-                                             k.tl0 + k.tls * double(idx % d.lx)) - geqnew;
+                                             Enc<S>::scaled ? double(k.gshift[idx % d.lx]) : 0.0) -
+                            geqnew;
         gneqx += cx * diff;
         gneqy += cy * diff;
         gneqz += cz * diff;

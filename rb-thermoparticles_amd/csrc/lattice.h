@@ -16,22 +16,29 @@ hipError_t launch_lattice_batched(const StepArgs<S> *slots_dev, const int *tile_
                                   hipStream_t st);
 template <class S>
 hipError_t launch_stream_only(const S *fin, const S *gin, const typename Enc<S>::A *rho_wall,
-                              double *fout, double *gout, Dims d, double tl0, double tls,
+                              double *fout, double *gout, Dims d, const float *gshift,
                               double thot, double tcold, bool nostream, hipStream_t st);
 template <class S>
-hipError_t launch_decode(const S *in, double *out, size_t n, int is_g, double tl0, double tls,
+hipError_t launch_decode(const S *in, double *out, size_t n, int is_g, const float *gshift,
                          int lx, hipStream_t st);
 template <class S>
-hipError_t launch_encode(const double *in, S *out, size_t n, int is_g, double tl0, double tls,
+hipError_t launch_encode(const double *in, S *out, size_t n, int is_g, const float *gshift,
                          int lx, hipStream_t st);
 template <class S>
 hipError_t launch_init(S *f, S *g, typename Enc<S>::A *fx, typename Enc<S>::A *rho_wall,
                        const double *temperature, double rho0, double t0, double grav0,
-                       double beta, double tl0, double tls, Dims d, double *mr, double *mux,
+                       double beta, const float *gshift, Dims d, double *mr, double *mux,
                        double *muy, double *muz, double *mphi, hipStream_t st);
 
 // Time the FP64 lattice step's access pattern alone on an lx x ly x lz lattice (rbtp_stencil_probe).
 int lattice_stencil_probe(int lx, int ly, int lz, int reps, double *gbs);
+
+// Refresh the shift profile of the reduced-precision g encoding to the plane-mean temperature
+// and re-encode the stored g (buffer g, the state the next step reads) under it.  partial holds
+// lx*lz doubles, delta lx floats.  No-op for FP64 storage.
+template <class S>
+hipError_t launch_shift_refresh(S *g, Dims d, float *gshift, double *partial, float *delta,
+                                hipStream_t st);
 
 template <class S>
 hipError_t launch_forcing_field(typename Enc<S>::A *fx, typename Enc<S>::A *rho_wall,

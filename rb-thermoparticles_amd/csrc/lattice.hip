@@ -230,14 +230,17 @@ __device__ inline void node_step(const StepArgs<S> &a, int idx) {
     }
     // g: the post-stream value (strict FP64) or its shifted form (FP32 arithmetic)
     T f[NPOP], g[NPOP];
-    const T tl0 = T(a.k.tl0), tls = T(a.k.tls);
-    const T ts = tl0 + tls * T(ix);
-    if constexpr (STRICT)
-        gather<S, T, NOSTREAM>(a.fin, a.gin, idx, ix, iy, iz, a.d, rho_w, tl0, tls, T(a.k.thot),
+    T ts = T(0);
+    if constexpr (STRICT) {
+        gather<S, T, NOSTREAM>(a.fin, a.gin, idx, ix, iy, iz, a.d, rho_w, a.gshift, T(a.k.thot),
                                T(a.k.tcold), f, g);
-    else
-        gather_shifted<S, T, NOSTREAM>(a.fin, a.gin, idx, ix, iy, iz, a.d, rho_w, ts, tls,
+    } else {
+        ts = T(a.gshift[ix]);
+        const T dm = T(a.gshift[ix > 0 ? ix - 1 : ix]) - ts;
+        const T dp = T(a.gshift[ix < a.d.lx - 1 ? ix + 1 : ix]) - ts;
+        gather_shifted<S, T, NOSTREAM>(a.fin, a.gin, idx, ix, iy, iz, a.d, rho_w, ts, dm, dp,
                                        T(a.k.thot), T(a.k.tcold), f, g);
+    }
     // At a special node (IBB link node or refilled node) the populations the particle phase
     // corrected replace the pulled ones; the first step after a (re)load has none.
     const int sj = (!NOSTREAM && a.sp_map) ? a.sp_map[idx] : -1;
@@ -296,7 +299,7 @@ __global__ __launch_bounds__(BLOCK) void k_stream_only(const S *__restrict__ fin
                                                        const S *__restrict__ gin,
                                                        const typename Enc<S>::A *rho_wall,
                                                        double *fout, double *gout, Dims d,
-                                                       double tl0, double tls, double thot,
+                                                       const float *gshift, double thot,
                                                        double tcold, int nostream) {
     const int idx = blockIdx.x * BLOCK + threadIdx.x;
     if (idx >= d.n) return;
@@ -310,11 +313,9 @@ __global__ __launch_bounds__(BLOCK) void k_stream_only(const S *__restrict__ fin
     else if (ix == d.lx - 1) rho_w = T(rho_wall[d.ly * d.lz + iz * d.ly + iy]);
     T f[NPOP], g[NPOP];
     if (nostream)
-        gather<S, T, true>(fin, gin, idx, ix, iy, iz, d, rho_w, T(tl0), T(tls),
-                           T(thot), T(tcold), f, g);
+        gather<S, T, true>(fin, gin, idx, ix, iy, iz, d, rho_w, gshift, T(thot), T(tcold), f, g);
     else
-        gather<S, T, false>(fin, gin, idx, ix, iy, iz, d, rho_w, T(tl0), T(tls),
-                            T(thot), T(tcold), f, g);
+        gather<S, T, false>(fin, gin, idx, ix, iy, iz, d, rho_w, gshift, T(thot), T(tcold), f, g);
 #pragma unroll
     for (int i = 0; i < NPOP; i++) {
         fout[(size_t)i * d.n + idx] = double(f[i]);
@@ -325,7 +326,7 @@ __global__ __launch_bounds__(BLOCK) void k_stream_only(const S *__restrict__ fin
 // Decode a population buffer to FP64 (post-collision accessor).
 template <class S>
 __global__ __launch_bounds__(BLOCK) void k_decode(const S *__restrict__ in, double *out, size_t n,
-                                                  int is_g, double tl0, double tls, int lx) {
+                                                  int is_g, const float *gshift, int lx) {
     const size_t j = (size_t)blockIdx.x * BLOCK + threadIdx.x;
     if (j >= (size_t)NPOP * n) return;
     const int i = (int)(j / n);
@@ -334,7 +335,7 @@ __global__ __launch_bounds__(BLOCK) void k_decode(const S *__restrict__ in, doub
     const double w = i == 0 ? 8.0 / 27.0 : (i <= 6 ? 2.0 / 27.0 : (i <= 18 ? 1.0 / 54.0 : 1.0 / 216.0));
     const T s = T(Enc<S>::ld(in + pop_at(i, (int)(j % n))));
     if constexpr (Enc<S>::scaled) {
-        const double shift = tl0 + tls * double((j % n) % (size_t)lx);
+        const double shift = double(gshift[(j % n) % (size_t)lx]);
         out[j] = is_g ? (double(s) + shift) * w : double(s) * w;
     } else {
         out[j] = double(s);
@@ -344,14 +345,14 @@ __global__ __launch_bounds__(BLOCK) void k_decode(const S *__restrict__ in, doub
 // Encode FP64 populations into the storage form (restart load, set_populations).
 template <class S>
 __global__ __launch_bounds__(BLOCK) void k_encode(const double *__restrict__ in, S *out, size_t n,
-                                                  int is_g, double tl0, double tls, int lx) {
+                                                  int is_g, const float *gshift, int lx) {
     const size_t j = (size_t)blockIdx.x * BLOCK + threadIdx.x;
     if (j >= (size_t)NPOP * n) return;
     const int i = (int)(j / n);
     const double w = i == 0 ? 8.0 / 27.0 : (i <= 6 ? 2.0 / 27.0 : (i <= 18 ? 1.0 / 54.0 : 1.0 / 216.0));
     const double v = in[j];
     if constexpr (Enc<S>::scaled) {
-        const double shift = tl0 + tls * double((j % n) % (size_t)lx);
+        const double shift = double(gshift[(j % n) % (size_t)lx]);
         Enc<S>::st(out + pop_at(i, (int)(j % n)), is_g ? float(v * (1.0 / w) - shift) : float(v * (1.0 / w)));
     } else {
         Enc<S>::st(out + pop_at(i, (int)(j % n)), (S)v);
@@ -366,7 +367,7 @@ __global__ __launch_bounds__(BLOCK) void k_init(S *f, S *g, typename Enc<S>::A *
                                                 typename Enc<S>::A *rho_wall,
                                                 const double *temperature, double rho0,
                                                 double t0, double grav0, double beta,
-                                                double tl0, double tls, Dims d,
+                                                const float *gshift, Dims d,
                                                 double *macro_rho, double *macro_ux,
                                                 double *macro_uy, double *macro_uz,
                                                 double *macro_phi) {
@@ -395,7 +396,7 @@ __global__ __launch_bounds__(BLOCK) void k_init(S *f, S *g, typename Enc<S>::A *
         double geq = weight(i) * tempT * (1.0 + eu + 0.5 * (eu * eu - uv));
         if (i == 0) geq -= lambda0 * tempT * r;
         else geq += weight(i) * tempT * r;
-        enc_g<S, double>(g + pop_at(i, idx), geq, i, tl0 + tls * double(ix));
+        enc_g<S, double>(g + pop_at(i, idx), geq, i, Enc<S>::scaled ? double(gshift[ix]) : 0.0);
     }
     using A = typename Enc<S>::A;
     fx_field[idx] = A(grav0 * beta * (tempT - t0));
@@ -428,6 +429,55 @@ __global__ __launch_bounds__(BLOCK) void k_forcing_field(typename Enc<S>::A *fx,
 
 inline int blocks_for(size_t n) { return (int)((n + BLOCK - 1) / BLOCK); }
 
+// Plane means of the temperature for the shift profile of the reduced-precision encodings, from
+// the stored g (T = gshift[ix] + sum_i w_i p_i at every node), in a fixed order: workgroup z
+// sums its lattice plane's rows iy = 0, 1, ... for each column ix (thread ix), then
+// k_shift_update adds the z partials in increasing z.
+template <class S>
+__global__ __launch_bounds__(BLOCK) void k_shift_partial(const S *__restrict__ g, Dims d,
+                                                         double *__restrict__ partial) {
+    const int iz = blockIdx.y;
+    const int ix = blockIdx.x * BLOCK + threadIdx.x;
+    if (ix >= d.lx) return;
+    double acc = 0.0;
+    for (int iy = 0; iy < d.ly; iy++) {
+        const int idx = iz * d.lxy + iy * d.lx + ix;
+        double t = 0.0;
+#pragma unroll
+        for (int i = 0; i < NPOP; i++) t += weight(i) * double(Enc<S>::ld(g + pop_at(i, idx)));
+        acc += t;
+    }
+    partial[(size_t)iz * d.lx + ix] = acc;
+}
+
+// New shift profile = the plane means; delta[ix] = old - new, by which every stored g value of
+// column ix changes (k_shift_reencode).
+__global__ __launch_bounds__(BLOCK) void k_shift_update(const double *__restrict__ partial, Dims d,
+                                                        float *gshift, float *delta) {
+    const int ix = blockIdx.x * BLOCK + threadIdx.x;
+    if (ix >= d.lx) return;
+    double sum = 0.0;
+    for (int iz = 0; iz < d.lz; iz++) sum += partial[(size_t)iz * d.lx + ix];
+    const float old = gshift[ix];
+    const float now = float(double(old) + sum / (double(d.ly) * double(d.lz)));
+    gshift[ix] = now;
+    delta[ix] = old - now;
+}
+
+// p_i <- p_i + delta[ix] for every stored g value: the same g under the new shift profile.
+template <class S>
+__global__ __launch_bounds__(BLOCK) void k_shift_reencode(S *__restrict__ g, Dims d,
+                                                          const float *__restrict__ delta) {
+    const int idx = blockIdx.x * BLOCK + threadIdx.x;
+    if (idx >= d.n) return;
+    const float dl = delta[idx % d.lx];
+#pragma unroll
+    for (int i = 0; i < NPOP; i++) {
+        S *q = g + pop_at(i, idx);
+        Enc<S>::st(q, float(Enc<S>::ld(q)) + dl);
+    }
+}
+
 // The FP64 lattice kernel's memory traffic without its arithmetic: the same launch shape, the
 // same pull of 54 populations under the wall rules (gather), the same F_x read and write, and
 // the 54 values stored unchanged into the other buffer.  Its time is the ceiling the access
@@ -445,7 +495,7 @@ __global__ __launch_bounds__(BLOCK) void k_stencil_probe(const double *__restric
     const int iy = rem / d.lx;
     const int ix = rem - iy * d.lx;
     double f[NPOP], g[NPOP];
-    gather<double, double, false>(fin, gin, idx, ix, iy, iz, d, 0.0, 0.0, 0.0, 0.0, 0.0, f, g);
+    gather<double, double, false>(fin, gin, idx, ix, iy, iz, d, 0.0, nullptr, 0.0, 0.0, f, g);
 #pragma unroll
     for (int i = 0; i < NPOP; i++) {
         fout[pop_at(i, idx)] = f[i];
@@ -520,37 +570,57 @@ hipError_t launch_lattice_step(const StepArgs<S> &a, bool strict, bool nostream,
 
 template <class S>
 hipError_t launch_stream_only(const S *fin, const S *gin, const typename Enc<S>::A *rho_wall,
-                              double *fout, double *gout, Dims d, double tl0, double tls,
+                              double *fout, double *gout, Dims d, const float *gshift,
                               double thot, double tcold, bool nostream, hipStream_t st) {
     hipLaunchKernelGGL((k_stream_only<S>), blocks_for((size_t)d.n), BLOCK, 0, st, fin, gin,
-                       rho_wall, fout, gout, d, tl0, tls, thot, tcold, nostream ? 1 : 0);
+                       rho_wall, fout, gout, d, gshift, thot, tcold, nostream ? 1 : 0);
     return hipGetLastError();
 }
 
 template <class S>
-hipError_t launch_decode(const S *in, double *out, size_t n, int is_g, double tl0, double tls,
+hipError_t launch_decode(const S *in, double *out, size_t n, int is_g, const float *gshift,
                          int lx, hipStream_t st) {
     hipLaunchKernelGGL((k_decode<S>), blocks_for((size_t)NPOP * n), BLOCK, 0, st, in, out, n, is_g,
-                       tl0, tls, lx);
+                       gshift, lx);
     return hipGetLastError();
 }
 
 template <class S>
-hipError_t launch_encode(const double *in, S *out, size_t n, int is_g, double tl0, double tls,
+hipError_t launch_encode(const double *in, S *out, size_t n, int is_g, const float *gshift,
                          int lx, hipStream_t st) {
     hipLaunchKernelGGL((k_encode<S>), blocks_for((size_t)NPOP * n), BLOCK, 0, st, in, out, n, is_g,
-                       tl0, tls, lx);
+                       gshift, lx);
     return hipGetLastError();
 }
 
 template <class S>
 hipError_t launch_init(S *f, S *g, typename Enc<S>::A *fx, typename Enc<S>::A *rho_wall,
                        const double *temperature, double rho0, double t0, double grav0,
-                       double beta, double tl0, double tls, Dims d, double *mr, double *mux,
+                       double beta, const float *gshift, Dims d, double *mr, double *mux,
                        double *muy, double *muz, double *mphi, hipStream_t st) {
     hipLaunchKernelGGL((k_init<S>), blocks_for((size_t)d.n), BLOCK, 0, st, f, g, fx, rho_wall,
-                       temperature, rho0, t0, grav0, beta, tl0, tls, d, mr, mux, muy, muz, mphi);
+                       temperature, rho0, t0, grav0, beta, gshift, d, mr, mux, muy, muz, mphi);
     return hipGetLastError();
+}
+
+template <class S>
+hipError_t launch_shift_refresh(S *g, Dims d, float *gshift, double *partial, float *delta,
+                                hipStream_t st) {
+    if constexpr (!Enc<S>::scaled) {
+        return hipSuccess;  // FP64 storage is not shifted
+    } else {
+        hipLaunchKernelGGL((k_shift_partial<S>), dim3(blocks_for((size_t)d.lx), d.lz), BLOCK, 0, st,
+                           (const S *)g, d, partial);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL(k_shift_update, blocks_for((size_t)d.lx), BLOCK, 0, st,
+                           (const double *)partial, d, gshift, delta);
+        e = hipGetLastError();
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL((k_shift_reencode<S>), blocks_for((size_t)d.n), BLOCK, 0, st, g, d,
+                           (const float *)delta);
+        return hipGetLastError();
+    }
 }
 
 template <class S>
@@ -595,16 +665,18 @@ hipError_t launch_lattice_batched(const StepArgs<S> *slots_dev, const int *tile_
                                                   int, bool, hipStream_t);                        \
     template hipError_t launch_stream_only<S>(const S *, const S *,                             \
                                               const typename Enc<S>::A *, double *, double *,   \
-                                              Dims, double, double, double, double, bool,       \
+                                              Dims, const float *, double, double, bool,        \
                                               hipStream_t);                                      \
-    template hipError_t launch_decode<S>(const S *, double *, size_t, int, double, double, int, \
+    template hipError_t launch_decode<S>(const S *, double *, size_t, int, const float *, int,   \
                                          hipStream_t);                                          \
-    template hipError_t launch_encode<S>(const double *, S *, size_t, int, double, double, int, \
+    template hipError_t launch_encode<S>(const double *, S *, size_t, int, const float *, int,  \
                                          hipStream_t);                                          \
     template hipError_t launch_init<S>(S *, S *, typename Enc<S>::A *, typename Enc<S>::A *,    \
-                                       const double *, double, double, double, double, double,  \
-                                       double, Dims, double *, double *, double *, double *,    \
-                                       double *, hipStream_t);                                            \
+                                       const double *, double, double, double, double,          \
+                                       const float *, Dims, double *, double *, double *,       \
+                                       double *, double *, hipStream_t);                         \
+    template hipError_t launch_shift_refresh<S>(S *, Dims, float *, double *, float *,          \
+                                                hipStream_t);                                    \
     template hipError_t launch_forcing_field<S>(typename Enc<S>::A *, typename Enc<S>::A *,     \
                                                 const double *, const double *, double, double, \
                                                 Dims, hipStream_t);

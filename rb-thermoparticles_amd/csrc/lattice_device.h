@@ -55,11 +55,19 @@ __device__ inline void enc_g(S *p, T v, int i, T shift) {
 //   f: f_i(x) = f*_i(x - c_i), or f*_opp(x) when x - c_i is outside (streaming.cu:10-21).
 //   g: as f inside; outside, the z then y walls reflect (adiabatic) and the x walls apply
 //      -g*_opp(x) + 2 w_i T_wall (1 + rho(x)) (streaming.cu:37-61).
+// gshift: the shift profile of the reduced-precision encodings (one value per x column; unused
+// for FP64 storage).
 template <class S, class T, bool NOSTREAM>
 __device__ inline void gather(const S *__restrict__ fin, const S *__restrict__ gin, int idx,
-                              int ix, int iy, int iz, const Dims &d, T rho_w, T tl0, T tls,
-                              T thot, T tcold, T (&f)[NPOP], T (&g)[NPOP]) {
-    const T own_shift = tl0 + tls * T(ix);
+                              int ix, int iy, int iz, const Dims &d, T rho_w,
+                              const float *__restrict__ gshift, T thot, T tcold, T (&f)[NPOP],
+                              T (&g)[NPOP]) {
+    T own_shift = T(0), shift_m = T(0), shift_p = T(0);  // at ix, ix - 1, ix + 1
+    if constexpr (Enc<S>::scaled) {
+        own_shift = T(gshift[ix]);
+        shift_m = T(gshift[ix > 0 ? ix - 1 : ix]);
+        shift_p = T(gshift[ix < d.lx - 1 ? ix + 1 : ix]);
+    }
 #pragma unroll
     for (int i = 0; i < NPOP; i++) {
         const int cx = D3Q27::cx[i], cy = D3Q27::cy[i], cz = D3Q27::cz[i];
@@ -76,8 +84,8 @@ __device__ inline void gather(const S *__restrict__ fin, const S *__restrict__ g
         const int off = cx + cy * d.lx + cz * d.lxy;
         const size_t src = inside ? pop_at(i, idx - off) : pop_at(op, idx);
         const T fv = dec_f<S, T>(Enc<S>::ld(fin + src), i);
-        const T gv = dec_g<S, T>(Enc<S>::ld(gin + src), i,
-                                 inside ? tl0 + tls * T(ix - cx) : own_shift);
+        const T src_shift = (inside && cx == 1) ? shift_m : ((inside && cx == -1) ? shift_p : own_shift);
+        const T gv = dec_g<S, T>(Enc<S>::ld(gin + src), i, src_shift);
         f[i] = fv;
         if (inside || outz || outy) {
             g[i] = gv;
@@ -90,16 +98,18 @@ __device__ inline void gather(const S *__restrict__ fin, const S *__restrict__ g
 }
 
 // gather() for the shifted encodings (float, half, bf16), with g returned in the variables the
-// FP32 collision works in: p_i = g_i / w_i - T_s, T_s = tl0 + tls*ix the conduction temperature
-// of the node itself.  A value pulled from x - c_i is stored relative to that node's T_s, which
-// differs by -tls*c_x; the wall rules of streaming_scalar become
+// FP32 collision works in: p_i = g_i / w_i - T_s, T_s = gshift[ix] the shift of the node's own
+// column.  A value pulled from x - c_i is stored relative to the shift of that node's column, so
+// it gains dm = gshift[ix-1] - T_s (c_x = +1) or dp = gshift[ix+1] - T_s (c_x = -1); the wall
+// rules of streaming_scalar become
 //   z and y walls (adiabatic):  p_i = p*_opp(x)
 //   x walls:                    p_i = -p*_opp(x) + 2 (T_wall - T_s) + 2 T_wall rho(x)
 // so that no O(T) quantity enters the arithmetic on g, only its deviation from T_s.
 template <class S, class T, bool NOSTREAM>
 __device__ inline void gather_shifted(const S *__restrict__ fin, const S *__restrict__ gin,
                                       int idx, int ix, int iy, int iz, const Dims &d, T rho_w,
-                                      T ts, T tls, T thot, T tcold, T (&f)[NPOP], T (&p)[NPOP]) {
+                                      T ts, T dm, T dp, T thot, T tcold, T (&f)[NPOP],
+                                      T (&p)[NPOP]) {
     static_assert(Enc<S>::scaled, "shifted gather needs a shifted encoding");
     // all 54 loads first, then the arithmetic, so that they are in flight together
     T sf[NPOP], sg[NPOP];
@@ -131,7 +141,7 @@ __device__ inline void gather_shifted(const S *__restrict__ fin, const S *__rest
         const bool outz = (cz == 1 && iz == 0) || (cz == -1 && iz == d.lz - 1);
         const bool inside = !(outx || outy || outz);
         const T sv = sg[i];
-        const T pulled = cx > 0 ? sv - (inside ? tls : T(0)) : sv + (inside ? tls : T(0));
+        const T pulled = cx > 0 ? sv + (inside ? dm : T(0)) : sv + (inside ? dp : T(0));
         // x wall only: hot plate at x=0 (c_x=+1), cold plate at x=LX (c_x=-1).
         const T twall = (cx == 1) ? thot : tcold;
         const T walled = -sv + T(2.0) * (twall - ts) + T(2.0) * twall * rho_w;

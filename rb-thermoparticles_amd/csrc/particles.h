@@ -22,7 +22,7 @@ struct PartDev {
     // Case constants (globals.cu:126-148, pforces.cu:121-124).
     double radius, radius_sq, mass, inertia, heat_capacity, rho_ratio, grav0, coeff;
     double min_gap, min_gap_w, stf0, stf1, stf0_w, stf1_w, rho0, t0, thot, tcold;
-    double tl0, tls;  // shift of the 16-bit g encoding (conduction profile T = tl0 + tls*ix)
+    const float *gshift;     // shift profile of the reduced-precision g encoding, [lx] (the sim's)
     double *base;            // one allocation for every P-length FP64 array
     double *p[RBTP_P_COUNT]; // particle state, indexed by RBTP_P_*
     double *old[OLD_COUNT];  // state before this step's motion
@@ -115,7 +115,8 @@ int part_collect_new_fluid(PartDev &pd, hipStream_t st,
 
 // Constants of the output-field kernels (compute_Ked / compute_ted, overwrite_particle_nodes).
 struct OutK {
-    double tau, tauc, visc, diff, t0, rho0, tl0, tls;
+    double tau, tauc, visc, diff, t0, rho0;
+    const float *gshift;  // shift profile of the reduced-precision g encoding, [lx]
 };
 
 int diag_output_fields(int prec, const void *fc, const void *gc, const void *fx,

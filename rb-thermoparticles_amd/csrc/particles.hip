@@ -480,14 +480,14 @@ template <class S>
 struct PopView {
     gptr<const S> f, g;
     size_t n;
-    double tl0, tls;  // shift of the 16-bit g encoding (conduction profile)
+    gptr<const float> gshift;  // shift profile of the reduced-precision g encoding, [lx]
     int lx;
     __device__ inline double F(int i, int idx) const {
         return dec_f<S, double>(Enc<S>::ld(f + pop_at(i, idx)), i);
     }
     __device__ inline double G(int i, int idx) const {
         return dec_g<S, double>(Enc<S>::ld(g + pop_at(i, idx)), i,
-                                tl0 + tls * double(idx % lx));
+                                Enc<S>::scaled ? double(gshift[idx % lx]) : 0.0);
     }
 };
 
@@ -670,8 +670,8 @@ __global__ __launch_bounds__(256) void k_link_nodes(PartGrid g) {
     const unsigned long long has = __ballot(mask != 0u);
     if (__lane_id() == 0) gbl(pd.lnk_bits)[(size_t)p * (pd.lnk_stride / 64) + k / 64] = has;
     if (mask == 0u) return;
-    const PopView<S> pop{gbl((const S *)sl.fcur), gbl((const S *)sl.gcur), (size_t)d.n, pd.tl0,
-                         pd.tls, d.lx};
+    const PopView<S> pop{gbl((const S *)sl.fcur), gbl((const S *)sl.gcur), (size_t)d.n,
+                         gbl(pd.gshift), d.lx};
     using A = typename Enc<S>::A;
     const gptr<A> out = gbl((A *)pd.sp.pop);
     const size_t cap = (size_t)pd.sp.cap;
@@ -887,8 +887,8 @@ __global__ __launch_bounds__(128) void k_fixup_links(PartGrid g) {
     const PartDev &pd = sl.pd;
     const Dims d = sl.d;
     using A = typename Enc<S>::A;
-    const PopView<S> pop{gbl((const S *)sl.fcur), gbl((const S *)sl.gcur), (size_t)d.n, pd.tl0,
-                         pd.tls, d.lx};
+    const PopView<S> pop{gbl((const S *)sl.fcur), gbl((const S *)sl.gcur), (size_t)d.n,
+                         gbl(pd.gshift), d.lx};
     const gptr<const int32_t> owner_prev = gbl(sl.owner_prev);
     const gptr<const double> macro_rho = gbl((const double *)sl.macro[0]);
     const int blk = blockIdx.x - g.lstart[slot];
@@ -945,7 +945,8 @@ __global__ __launch_bounds__(128) void k_fixup_refill(PartGrid g) {
     const PartDev &pd = sl.pd;
     const Dims d = sl.d;
     using A = typename Enc<S>::A;
-    const PopView<S> pop{gbl((const S *)sl.fcur), gbl((const S *)sl.gcur), (size_t)d.n, pd.tl0, pd.tls, d.lx};
+    const PopView<S> pop{gbl((const S *)sl.fcur), gbl((const S *)sl.gcur), (size_t)d.n,
+                         gbl(pd.gshift), d.lx};
     const int32_t *owner_prev = sl.owner_prev;
     A *fx_field = (A *)sl.fx;
     double *macro_rho = sl.macro[0], *macro_ux = sl.macro[1], *macro_uy = sl.macro[2];
@@ -1150,8 +1151,6 @@ int part_alloc(PartDev &pd, const rbtp_case &c, const rbtp_derived &d, const Dim
     pd.stf1_w = c.rep_stf1_w;
     pd.rho0 = c.rho0;
     pd.t0 = c.t_ref;
-    pd.tls = (c.t_cold - c.t_hot) / (double)c.lx;
-    pd.tl0 = c.t_hot + 0.5 * pd.tls;
     pd.thot = c.t_hot;
     pd.tcold = c.t_cold;
     const size_t np = (size_t)pd.np;

@@ -36,16 +36,17 @@ struct FluidK {
     // Double-word reciprocals (hi = RN(1/d), lo = RN((1 - d*hi)/d)) of the constant divisors of
     // the strict form: RT = 1.0/3.0, 3.0, tau, tauc.
     double rt_h, rt_l, three_h, three_l, tau_h, tau_l, tauc_h, tauc_l;
-    // Shift of the reduced-precision g encoding: the conduction profile T(ix) = tl0 + tls*ix
-    // between the hot and cold walls at cell centres.
+    // The conduction profile T(ix) = tl0 + tls*ix between the hot and cold walls at cell
+    // centres: the initial shift profile of the reduced-precision g encoding.
     double tl0, tls;
 };
 
 // Storage encodings of the populations.
 //   double:           the reference's value.
-//   float/half/bf16:  value/w_i for f; value/w_i - T_cond(ix) for g (shifted and scaled), where
-//                     T_cond is the linear conduction profile at the node that holds the value.
-//                     The FP32-arithmetic collision works in the same shifted variables.
+//   float/half/bf16:  value/w_i for f; value/w_i - gshift[ix] for g (shifted and scaled), where
+//                     gshift is a per-column temperature profile: the linear conduction profile
+//                     at first, then the plane-mean temperature, refreshed every few hundred
+//                     steps.  The FP32-arithmetic collision works in the same shifted variables.
 // Pointer into global memory.  Loads and stores through it compile to global_* instructions;
 // through a generic pointer the compiler cannot place (one read from a table in memory) they
 // become flat_* instructions, which also count against the scalar-load counter and so make
@@ -105,6 +106,7 @@ template <class S> struct StepArgs {
     const typename Enc<S>::A *sp_pop;
     size_t sp_cap;
     double *macro_rho, *macro_ux, *macro_uy, *macro_uz, *macro_phi;  // output fields, or null
+    const float *gshift;  // [lx] shift profile of the reduced-precision g encoding
     Dims d;
     FluidK k;
 };

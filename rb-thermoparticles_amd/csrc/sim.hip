@@ -91,6 +91,13 @@ struct rbtp_sim {
     void *rho_wall;  // 2*ly*lz (aux type)
     double *macro[5];  // rho, ux, uy, uz, phi
     double *temp_dev;  // initial temperature upload
+    // Shift profile of the reduced-precision g encoding ([lx], the conduction profile after
+    // init/restart), refreshed to the plane-mean temperature every shift_interval steps
+    // (0: never); shift_partial [lx*lz] and shift_delta [lx] are the refresh's scratch.
+    float *gshift;
+    double *shift_partial;
+    float *shift_delta;
+    int shift_interval;
     int cur;
     bool poststream;   // cur holds post-stream populations (after init / restart / set)
     bool macro_valid;
@@ -208,6 +215,9 @@ extern "C" void rbtp_destroy(rbtp_sim *s) {
     (void)hipFree(s->rho_wall);
     for (double *m : s->macro) (void)hipFree(m);
     (void)hipFree(s->temp_dev);
+    (void)hipFree(s->gshift);
+    (void)hipFree(s->shift_partial);
+    (void)hipFree(s->shift_delta);
     part_free(s->pd);
     part_tables_free(s->ptab);
     if (s->ev_start) (void)hipEventDestroy(s->ev_start);
@@ -216,6 +226,20 @@ extern "C" void rbtp_destroy(rbtp_sim *s) {
     if (s->ev_k1) (void)hipEventDestroy(s->ev_k1);
     if (s->stream && s->own_stream) (void)hipStreamDestroy(s->stream);
     delete s;
+}
+
+// Steps between refreshes of the shift profile of a reduced-precision sim (rbtp_step).
+constexpr int SHIFT_INTERVAL = 500;
+
+// The shift profile back to the conduction profile T(ix) = tl0 + tls*ix (init and restart
+// encode against it).
+static int reset_shift(rbtp_sim *s) {
+    std::vector<float> h((size_t)s->c.lx);
+    for (int ix = 0; ix < s->c.lx; ix++) h[ix] = float(s->k.tl0 + s->k.tls * double(ix));
+    HIP_TRY(hipMemcpyAsync(s->gshift, h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice,
+                           s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    return RBTP_OK;
 }
 
 static int create_impl(const rbtp_case *c, rbtp_sim *s, hipStream_t stream) {
@@ -268,8 +292,16 @@ static int create_impl(const rbtp_case *c, rbtp_sim *s, hipStream_t stream) {
     HIP_TRY(hipMalloc(&s->fx, n * aux_bytes(s->prec)));
     HIP_TRY(hipMalloc(&s->rho_wall, 2 * (size_t)c->ly * c->lz * aux_bytes(s->prec)));
     for (double *&m : s->macro) HIP_TRY(hipMalloc(&m, n * sizeof(double)));
+    HIP_TRY(hipMalloc(&s->gshift, (size_t)c->lx * sizeof(float)));
+    HIP_TRY(hipMalloc(&s->shift_partial, (size_t)c->lx * c->lz * sizeof(double)));
+    HIP_TRY(hipMalloc(&s->shift_delta, (size_t)c->lx * sizeof(float)));
+    RB_TRY(reset_shift(s));
+    s->shift_interval = s->prec == RBTP_PREC_FP64 ? 0 : SHIFT_INTERVAL;
     s->particles = c->activate_particles && s->d.npart > 0;
-    if (s->particles) RB_TRY(part_alloc(s->pd, *c, s->d, s->dims));
+    if (s->particles) {
+        RB_TRY(part_alloc(s->pd, *c, s->d, s->dims));
+        s->pd.gshift = s->gshift;
+    }
     return RBTP_OK;
 }
 
@@ -325,6 +357,7 @@ static StepArgs<S> step_args(rbtp_sim *s, bool macro_out) {
     a.fx = (typename Enc<S>::A *)s->fx;
     a.rho_wall = (typename Enc<S>::A *)s->rho_wall;
     a.rho_full = s->particles ? s->macro[0] : nullptr;
+    a.gshift = s->gshift;
     a.sp_map = s->particles ? s->pd.sp.map : nullptr;
     a.sp_pop = s->particles ? (const typename Enc<S>::A *)s->pd.sp.pop : nullptr;
     a.sp_mask = s->particles ? s->pd.sp.mask : nullptr;
@@ -381,11 +414,12 @@ extern "C" int rbtp_init(rbtp_sim *s, const double *temperature, const double *p
         tdev = s->temp_dev;
     }
     s->cur = 0;
+    RB_TRY(reset_shift(s));
     HIP_TRY(dispatch(s->prec, [&](auto *tag) {
         using S = std::remove_pointer_t<decltype(tag)>;
         return launch_init<S>((S *)s->f[0], (S *)s->g[0], (typename Enc<S>::A *)s->fx,
                               (typename Enc<S>::A *)s->rho_wall, tdev, s->c.rho0, s->c.t_ref,
-                              s->d.grav0, s->c.beta, s->k.tl0, s->k.tls, s->dims, s->macro[0], s->macro[1],
+                              s->d.grav0, s->c.beta, s->gshift, s->dims, s->macro[0], s->macro[1],
                               s->macro[2], s->macro[3], s->macro[4], s->stream);
     }));
     if (s->particles) {
@@ -442,6 +476,14 @@ static int after_lattice(rbtp_sim *s) {
         s->pd.special_pending = true;
     }
     s->step++;
+    if (s->shift_interval > 0 && s->step % s->shift_interval == 0) {
+        // after the particle phase, which read this state under the old profile
+        HIP_TRY(dispatch(s->prec, [&](auto *tag) {
+            using S = std::remove_pointer_t<decltype(tag)>;
+            return launch_shift_refresh<S>((S *)s->g[s->cur], s->dims, s->gshift,
+                                           s->shift_partial, s->shift_delta, s->stream);
+        }));
+    }
     return RBTP_OK;
 }
 
@@ -533,6 +575,12 @@ extern "C" int rbtp_set_kernel_timing(rbtp_sim *s, int enable) {
     s->kernel_timing = enable != 0;
     return RBTP_OK;
 }
+extern "C" int rbtp_set_shift_interval(rbtp_sim *s, int steps) {
+    if (!s) return set_error(RBTP_E_ARG, "null handle");
+    if (steps < 0) return set_error(RBTP_E_ARG, "negative shift interval %d", steps);
+    s->shift_interval = s->prec == RBTP_PREC_FP64 ? 0 : steps;
+    return RBTP_OK;
+}
 extern "C" int rbtp_kernel_ms(rbtp_sim *s, double *avg_ms, int64_t *launches) {
     if (!s) return set_error(RBTP_E_ARG, "null handle");
     if (avg_ms) *avg_ms = s->kernel_launches ? s->kernel_ms_sum / s->kernel_launches : 0.0;
@@ -591,7 +639,7 @@ extern "C" int rbtp_get_populations(rbtp_sim *s, int which, double *out) {
         e = dispatch(s->prec, [&](auto *tag) {
             using S = std::remove_pointer_t<decltype(tag)>;
             const void *src = want_g ? s->g[s->cur] : s->f[s->cur];
-            return launch_decode<S>((const S *)src, dev, n, want_g ? 1 : 0, s->k.tl0, s->k.tls,
+            return launch_decode<S>((const S *)src, dev, n, want_g ? 1 : 0, s->gshift,
                                     s->dims.lx, s->stream);
         });
     } else {
@@ -599,7 +647,7 @@ extern "C" int rbtp_get_populations(rbtp_sim *s, int which, double *out) {
             using S = std::remove_pointer_t<decltype(tag)>;
             return launch_stream_only<S>((const S *)s->f[s->cur], (const S *)s->g[s->cur],
                                          (const typename Enc<S>::A *)s->rho_wall, dev,
-                                         dev + 27 * n, s->dims, s->k.tl0, s->k.tls, s->c.t_hot,
+                                         dev + 27 * n, s->dims, s->gshift, s->c.t_hot,
                                          s->c.t_cold, false, s->stream);
         });
         if (e == hipSuccess && s->particles)
@@ -628,7 +676,7 @@ extern "C" int rbtp_set_populations(rbtp_sim *s, const double *f, const double *
         e = dispatch(s->prec, [&](auto *tag) {
             using S = std::remove_pointer_t<decltype(tag)>;
             return launch_encode<S>(dev, (S *)(which ? s->g[s->cur] : s->f[s->cur]), n, which,
-                                    s->k.tl0, s->k.tls, s->dims.lx, s->stream);
+                                    s->gshift, s->dims.lx, s->stream);
         });
     }
     if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
@@ -777,8 +825,7 @@ static OutK out_consts(const rbtp_sim *s) {
     k.diff = s->d.diff;
     k.t0 = s->c.t_ref;
     k.rho0 = s->c.rho0;
-    k.tl0 = s->k.tl0;
-    k.tls = s->k.tls;
+    k.gshift = s->gshift;
     return k;
 }
 
@@ -925,6 +972,8 @@ extern "C" int rbtp_restart_load(rbtp_sim *s, const char *dir, int step) {
         fclose(fp);
         RB_TRY(part_load_state(s->pd, h.data(), s->dims, s->stream));
     }
+    // the encodings restart from the conduction profile, as after rbtp_init
+    RB_TRY(reset_shift(s));
     RB_TRY(rbtp_set_populations(s, f.data(), g.data()));
     // load_flow_fields (initial.cu:32-55) and forcing on the loaded phi (initial.cu:207)
     snprintf(path, sizeof(path), "%s/%09d.dat", dir, step);
@@ -1111,6 +1160,9 @@ extern "C" int rbtp_batch_admit(rbtp_batch *b, const rbtp_case *c, const double 
     HIP_TRY(hipStreamSynchronize(b->stream));
     rbtp_sim *s = nullptr;
     RB_TRY(create_on_stream(c, &s, b->stream));
+    // Batched steps have no per-slot refresh of the shift profile; the admission step, taken
+    // through the single-sim path, must not refresh it either.
+    s->shift_interval = 0;
     int r = rbtp_init(s, temperature, ppos, ptemp);
     if (r == RBTP_OK) r = step_impl(s, 1, 0, true);  // first step reads the post-stream start
     if (r != RBTP_OK) {

@@ -119,6 +119,51 @@ def test_low_precision_forms_track_fp64(rb, prec):
     assert abs(nu_l[4] - nu_r[4]) <= 10 * tol * abs(nu_r[4])
 
 
+@pytest.mark.parametrize("prec", [1, 2, 3])
+def test_shift_refresh_keeps_the_state(rb, prec):
+    """Moving the shift profile to the plane-mean temperature (every 7 steps here) re-encodes
+    the stored g without changing what it decodes to beyond one storage rounding: the run stays
+    within the bounds of test_low_precision_forms_track_fp64, and its g is no further from FP64
+    than that of the same form kept on the conduction profile (within a factor 2)."""
+    kw = single_phase(32, 32, 16, ra=1e5)
+    ref = rb.Sim(rb.default_case(**kw))
+    T = rb.linear_temperature(ref.case, 5, 1e-3)
+    ref.init(temperature=T)
+    ref.step(200)
+    tol = {1: 1e-6, 2: 3e-4, 3: 5e-3}[prec]
+    err_g = {}
+    for interval in (0, 7):
+        low = rb.Sim(rb.default_case(**kw, precision=prec))
+        low.set_shift_interval(interval)
+        low.init(temperature=T)
+        low.step(200)
+        assert rel_err(low.field("phi"), ref.field("phi")) <= tol, interval
+        err_g[interval] = rel_err(low.populations("g"), ref.populations("g"))
+    assert err_g[7] <= 2 * err_g[0], err_g
+
+
+def test_shift_refresh_follows_the_temperature(rb):
+    """From a uniform start at the hot-wall temperature the conduction profile is up to
+    T_hot - T_cold away from the stored temperature; FP16 storage refreshed to the plane means
+    every step tracks FP64 more closely (RMS of phi) than storage left on the conduction
+    profile."""
+    kw = single_phase(32, 32, 16, ra=1e5)
+    ref = rb.Sim(rb.default_case(**kw))
+    n = 32 * 32 * 16
+    T = np.full(n, ref.case.t_hot)
+    ref.init(temperature=T)
+    ref.step(100)
+    phi_ref = ref.field("phi")
+    rms = {}
+    for interval in (0, 1):
+        low = rb.Sim(rb.default_case(**kw, precision=2))
+        low.set_shift_interval(interval)
+        low.init(temperature=T)
+        low.step(100)
+        rms[interval] = float(np.sqrt(np.mean((low.field("phi") - phi_ref) ** 2)))
+    assert rms[1] < 0.5 * rms[0], rms
+
+
 def test_synchronize_before_and_after_async_steps(rb, O):
     """rbtp_synchronize with no step in flight leaves no error behind; an asynchronous run is
     timed at the synchronisation and gives the same state as a synchronous one."""
