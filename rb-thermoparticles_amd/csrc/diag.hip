@@ -237,13 +237,14 @@ __global__ __launch_bounds__(DB) void k_profile(const double *fields, Dims d, do
 // Streaming copy of 16-byte vectors: the reference kernel for the achievable HBM bandwidth and
 // for calibrating the memory counters (it reads and writes exactly `count` vectors).  Each
 // thread moves U vectors 256 apart, all loads issued before the stores; one workgroup per 256*U
-// vectors.  The probe runs U = 1, 2 and 4 and keeps the fastest: which is best differs between
-// devices.
-template <int U>
-__global__ __launch_bounds__(256) void k_copy_probe(const double2 *__restrict__ src,
-                                                    double2 *__restrict__ dst, size_t count) {
+// vectors.  The probe runs U = 1, 2 and 4, with ordinary and with nontemporal stores (as the
+// lattice kernels use), and keeps the fastest: which is best differs between devices.
+using dvec2 = double __attribute__((ext_vector_type(2)));
+template <int U, bool NT>
+__global__ __launch_bounds__(256) void k_copy_probe(const dvec2 *__restrict__ src,
+                                                    dvec2 *__restrict__ dst, size_t count) {
     const size_t base = (size_t)blockIdx.x * (256 * U) + threadIdx.x;
-    double2 v[U];
+    dvec2 v[U];
 #pragma unroll
     for (int u = 0; u < U; u++) {
         const size_t i = base + (size_t)u * 256;
@@ -252,7 +253,10 @@ __global__ __launch_bounds__(256) void k_copy_probe(const double2 *__restrict__ 
 #pragma unroll
     for (int u = 0; u < U; u++) {
         const size_t i = base + (size_t)u * 256;
-        if (i < count) dst[i] = v[u];
+        if (i < count) {
+            if constexpr (NT) __builtin_nontemporal_store(v[u], dst + i);
+            else dst[i] = v[u];
+        }
     }
 }
 
@@ -402,18 +406,18 @@ int diag_profile(const double *dev_fields, const Dims &dims, hipStream_t st, dou
 
 
 int diag_copy_probe(size_t bytes, int reps, double *gbs) {
-    const size_t count = bytes / sizeof(double2);
+    const size_t count = bytes / sizeof(dvec2);
     if (count == 0 || reps < 1) return report(RBTP_E_ARG, "copy probe: bytes >= 16 and reps >= 1");
     if ((count + 255) / 256 > 0x7fffffffu) return report(RBTP_E_ARG, "copy probe: too many bytes");
-    double2 *a = nullptr, *b = nullptr;
+    dvec2 *a = nullptr, *b = nullptr;
     hipStream_t st = nullptr;
     hipEvent_t e0 = nullptr, e1 = nullptr;
-    hipError_t e = hipMalloc(&a, count * sizeof(double2));
-    if (e == hipSuccess) e = hipMalloc(&b, count * sizeof(double2));
+    hipError_t e = hipMalloc(&a, count * sizeof(dvec2));
+    if (e == hipSuccess) e = hipMalloc(&b, count * sizeof(dvec2));
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipEventCreate(&e0);
     if (e == hipSuccess) e = hipEventCreate(&e1);
-    if (e == hipSuccess) e = hipMemsetAsync(a, 0, count * sizeof(double2), st);
+    if (e == hipSuccess) e = hipMemsetAsync(a, 0, count * sizeof(dvec2), st);
     double best = 0.0;
     auto run = [&](auto kernel, int unroll) {
         const unsigned grid = (unsigned)((count + 256 * (size_t)unroll - 1) / (256 * (size_t)unroll));
@@ -430,11 +434,14 @@ int diag_copy_probe(size_t bytes, int reps, double *gbs) {
         float ms = 0.f;
         if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
         if (e == hipSuccess && ms > 0.f)
-            best = std::max(best, 2.0 * (double)(count * sizeof(double2)) * reps / (ms * 1e-3) / 1e9);
+            best = std::max(best, 2.0 * (double)(count * sizeof(dvec2)) * reps / (ms * 1e-3) / 1e9);
     };
-    run(k_copy_probe<1>, 1);
-    run(k_copy_probe<2>, 2);
-    run(k_copy_probe<4>, 4);
+    run(k_copy_probe<1, false>, 1);
+    run(k_copy_probe<2, false>, 2);
+    run(k_copy_probe<4, false>, 4);
+    run(k_copy_probe<1, true>, 1);
+    run(k_copy_probe<2, true>, 2);
+    run(k_copy_probe<4, true>, 4);
     if (e == hipSuccess) *gbs = best;
     (void)hipFree(a);
     (void)hipFree(b);

@@ -498,8 +498,8 @@ __global__ __launch_bounds__(BLOCK) void k_stencil_probe(const double *__restric
     gather<double, double, false>(fin, gin, idx, ix, iy, iz, d, 0.0, nullptr, 0.0, 0.0, f, g);
 #pragma unroll
     for (int i = 0; i < NPOP; i++) {
-        fout[pop_at(i, idx)] = f[i];
-        gout[pop_at(i, idx)] = g[i];
+        Enc<double>::st(fout + pop_at(i, idx), f[i]);
+        Enc<double>::st(gout + pop_at(i, idx), g[i]);
     }
     fx[idx] = fx[idx] + f[0];
 }
