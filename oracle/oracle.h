@@ -13,11 +13,13 @@
  *   or_macrovar           moments of random populations against numpy sums (rho, u+F/2, phi).
  *   or_collision_f        mass conservation, momentum change = Guo source first moment, rest
  *                         state with u=F=0 is a fixed point (f=0), feq moments.
- *   or_collision_g        zeroth moment of g after collision equals sum of geqnew (= phi) plus
- *                         sum of the source; rest state g = w*T (uniform T, u=F=0) stays put
- *                         bitwise over many full steps.  The source term's coefficients
- *                         (collision.cu:119-125) have no independent closed form beyond this:
- *                         PARITY UNPINNED for the Ralph term's value at nonzero gradients.
+ *   or_collision_g        zeroth moment of g after collision equals sum of geqnew (= phi); the
+ *                         first moment against its closed form G(1 - 1/tauc) + Rc[phi F -
+ *                         rho(2G + phi F)/(3D)] on random states; per population, what remains
+ *                         after the relaxation is w_i c_i.A with A three times that first-moment
+ *                         source, so every population of the source term (collision.cu:55-87)
+ *                         is fixed; rest state g = w*T (uniform T, u=F=0) stays put bitwise
+ *                         over many full steps.
  *   or_stream_f/g         single-population delta moves to x+c, reflects into opp at walls, the
  *                         x-wall anti-bounce-back value 2 w T_wall (1+rho) - g_opp, z/y/x
  *                         precedence at edges; mass conservation of f over a full step.
@@ -26,15 +28,21 @@
  *                         sampling of the segment on random spheres.
  *   or_build_links        brute-force enumeration on tiny grids; sphere centred on a cell centre
  *                         gives a symmetric link set; lowest pid keeps shared nodes.
- *   or_ibb_f / or_ibb_g   a sphere at rest in fluid at rest: zero force and torque by symmetry;
- *                         q<0.5 / q>=0.5 branches against the Bouzidi formulas evaluated in
- *                         numpy on the same inputs (link-by-link).
+ *   or_ibb_f, or_forces   a sphere at rest in fluid at rest: zero force and torque by symmetry.
+ *                         A sphere moving through fluid at rest (f* = 0): each link writes
+ *                         -mt or -mt/(2q), so force and torque are closed-form sums over the
+ *                         link list, which exercise both q branches (the (2q-1)/(2q) f*_opp term
+ *                         is zero there).
+ *   or_ibb_g, or_heat     a sphere at T_p in fluid held at the fixed point g = w T_f: closed-form
+ *                         heat over the link list, both q branches including the g*_opp term.
  *   or_repulsion          two spheres and a sphere at a wall at known gaps (closed form).
  *   or_move               constant force and heat: closed-form trapezoid sequences; free fall.
- *   or_refill             an isolated new-fluid node with no prev-fluid neighbour gives the
- *                         closed-form fallback equilibrium.  The extrapolation average
- *                         (refill.cu:121-153) is checked against a numpy restatement on random
- *                         populations; no closed form exists: PARITY UNPINNED beyond that.
+ *   or_refill             a new-fluid node with no prev-fluid neighbour (a sphere that jumps 3
+ *                         nodes): the fallback's moments are rho = 1 - 4|u|^2/3, momentum u/3
+ *                         (e.u not divided by RT), phi = T_p (1 - 4|u|^2/3), F = 0.  Linear
+ *                         pre-stream fields make every second-order extrapolation exact, so the
+ *                         average (refill.cu:121-153) has a closed form in the first-order
+ *                         directions.  rho, u, phi written consistently with the populations.
  *   or_nu                 exact linear conduction profile with u=0: wall Nu = 1 and
  *                         Nu_bulk = 1 - phi_p.
  *   place_particles       non-overlap and window constraints; same glibc rand sequence as the

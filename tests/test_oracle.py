@@ -144,6 +144,14 @@ def test_collision_g_moments(O):
     Fv = np.vstack([F, 0 * F, 0 * F])
     expect = (C.T @ g0) - G / tauc + Rc * (phi * Fv - rho * (2 * G + phi * Fv) / (3 * D))
     assert np.allclose(C.T @ g1, expect, rtol=0, atol=1e-13)
+    # Per population: with the relaxation -(g - geqnew)/tauc taken out, what remains is the
+    # source S_i = w_i Rc (gradt_i rho + 3 phi c_i.F), gradt_i = -(2 c_i.G + phi c_i.F)/D', which
+    # is w_i c_i . A for one vector A per node.  Its first moment is A/3, so the first-moment
+    # closed form above fixes every population of the source.
+    S = g1 - (g0 - (g0 - geq) / tauc)
+    A = 3.0 * (C.T @ S)
+    assert np.allclose(S, W[:, None] * (C @ A), rtol=0, atol=1e-13)
+    assert np.abs(A).max() > 1e-6
 
 
 def test_streaming_delta_and_walls(O):
@@ -340,6 +348,100 @@ def test_sphere_at_rest_has_no_net_force(O):
         assert abs(ora.particles(n)[0]) < 1e-14, n
 
 
+def test_moving_sphere_first_step_force_closed_form(O):
+    """A sphere translating with velocity V through fluid at rest (f = f* = 0 everywhere, no
+    buoyancy).  bounce_back_particles_kernel (ibb.cu:5-82) then writes, for a link (x, i, q),
+    f_opp(x) = -mt (q < 1/2, or x - c_i not fluid) or -mt/(2q) (q >= 1/2), with
+    mt = 2 w_i RHO0 (c_i . V)/RT, and compute_particle_forces_kernel (ibb.cu:138-188) adds
+    (c_i - V) f*_i(x) - (c_opp - V) f_opp(x) = (c_i + V) f_opp(x), with torque
+    (x + 1/2 + q c_i - p) x that force.  The sums are formed here from the oracle's link list
+    (itself pinned by brute force) and compared with the oracle's accumulators."""
+    L = (15, 15, 15)
+    kw = dict(single_phase(*L, beta=0.0, t_hot=0.1, t_cold=0.1, t_ref=0.1), activate_particles=1,
+              npart_override=1, particle_diam_frac=6.0 / 15)
+    ora = O.Oracle(O.default_case(**kw))
+    p = np.array([7.3, 7.6, 7.45])
+    ora.init(ppos=p)
+    V = np.array([0.013, -0.007, 0.004])
+    for k, v in zip(("vel_x", "vel_y", "vel_z"), V):
+        ora.set_particles(k, [v])
+    links = ora.links()
+    ib, _ = ora.flags()
+    assert len(links) > 500
+    assert np.all(ora.populations("f") == 0.0)
+    for s in ("macrovar", "forcing", "collide_f", "stream_f", "ibb_f", "collide_g", "stream_g",
+              "ibb_g", "prepare", "repulsion", "forces"):
+        ora.stage(s)
+    RT = 1.0 / 3.0
+    F = np.zeros(3)
+    T = np.zeros(3)
+    n_half = 0
+    for lk in links:
+        i, q = int(lk["dir"]), float(lk["q"])
+        c = C[i].astype(np.float64)
+        x = np.array([lk["cell_i"], lk["cell_j"], lk["cell_k"]])
+        xm = x - C[i]
+        inside = bool(np.all(xm >= 0) and np.all(xm < np.array(L)))
+        up_fluid = inside and ib[xm[2] * L[0] * L[1] + xm[1] * L[0] + xm[0]] == 0
+        mt = 2.0 * W[i] * 1.0 * float(c @ V) / RT
+        if up_fluid and q >= 0.5:
+            fopp = -mt / (2.0 * q)
+            n_half += 1
+        else:
+            fopp = -mt
+        f_link = (c + V) * fopp
+        F += f_link
+        T += np.cross(x + 0.5 + q * c - p, f_link)
+    assert n_half > 0  # both branches of the IBB rule are exercised
+    got_F = np.array([ora.particles(k)[0] for k in ("force_x", "force_y", "force_z")])
+    got_T = np.array([ora.particles(k)[0] for k in ("torque_x", "torque_y", "torque_z")])
+    assert np.abs(F).max() > 1e-3
+    np.testing.assert_allclose(got_F, F, rtol=1e-12, atol=1e-15 * np.abs(F).max())
+    np.testing.assert_allclose(got_T, T, rtol=1e-10, atol=1e-12 * np.abs(F).max())
+
+
+def test_sphere_heat_closed_form(O):
+    """A sphere at T_p in fluid at rest held at the fixed point g_i = w_i T_f (f = 0, so
+    delta rho = 0).  bounce_back_thermal_kernel (ibb.cu:84-136) writes, with amp = 2 w_i T_p:
+    g_opp = -(2q g*_i(x) + (1 - 2q) g*_i(x - c)) + amp = w_i (2 T_p - T_f) for q <= 1/2 (or
+    -g*_i + amp when x - c is not fluid), and -g*_i/(2q) + amp/(2q) + (2q - 1)/(2q) g*_opp(x)
+    otherwise; compute_particle_heat_kernel (ibb.cu:190-205) sums g*_i(x) - g_opp(x)."""
+    L = (15, 15, 15)
+    Tf, Tp = 0.1, 0.35
+    kw = dict(single_phase(*L, beta=0.0, t_hot=Tf, t_cold=Tf, t_ref=Tf), activate_particles=1,
+              npart_override=1, particle_diam_frac=6.0 / 15)
+    ora = O.Oracle(O.default_case(**kw))
+    p = np.array([7.3, 7.6, 7.45])
+    ora.init(ppos=p)
+    ora.set_particles("temp", [Tp])
+    n = L[0] * L[1] * L[2]
+    g_rest = np.repeat((W * Tf)[:, None], n, axis=1)
+    ora.set_populations("g", g_rest)
+    ora.set_populations("g_collide", g_rest)
+    links = ora.links()
+    ib, _ = ora.flags()
+    for s in ("macrovar", "forcing", "collide_f", "stream_f", "ibb_f", "collide_g", "stream_g",
+              "ibb_g", "prepare", "repulsion", "forces", "heat"):
+        ora.stage(s)
+    heat = 0.0
+    for lk in links:
+        i, q = int(lk["dir"]), float(lk["q"])
+        x = np.array([lk["cell_i"], lk["cell_j"], lk["cell_k"]])
+        xm = x - C[i]
+        inside = bool(np.all(xm >= 0) and np.all(xm < np.array(L)))
+        up_fluid = inside and ib[xm[2] * L[0] * L[1] + xm[1] * L[0] + xm[0]] == 0
+        gs, amp = W[i] * Tf, 2.0 * W[i] * Tp
+        if not up_fluid:
+            gopp = -gs + amp
+        elif q <= 0.5:
+            gopp = -(2 * q * gs + (1 - 2 * q) * gs) + amp
+        else:
+            gopp = -gs / (2 * q) + amp / (2 * q) + (2 * q - 1) / (2 * q) * gs
+        heat += gs - gopp
+    assert heat < -1e-3  # a hotter sphere gives heat to the fluid: the sum is negative
+    assert ora.particles("heat")[0] == pytest.approx(heat, rel=1e-12)
+
+
 def test_repulsion_closed_form(O):
     """repulsive_pair_kernel / repulsive_wall_kernel (pforces.cu:6-76) at known gaps."""
     L = (40, 30, 30)
@@ -400,6 +502,114 @@ def test_reference_placement_constraints(O):
         assert np.all(P[:, a] >= 1.5 * R) and np.all(P[:, a] <= Lm - 1.5 * R)
     D = np.sqrt(((P[:, None] - P[None]) ** 2).sum(-1)) + np.eye(30) * 1e9
     assert np.all(D - 2 * R >= R / 3)
+
+
+def _refill_setup(O, L, R, p0, shift):
+    """One step up to build_links of a sphere that then sits `shift` nodes further along x;
+    returns the oracle (before its refill stage) and the previous solid flags."""
+    kw = dict(single_phase(*L, beta=0.0), activate_particles=1, npart_override=1,
+              particle_diam_frac=2.0 * R / L[0])
+    ora = O.Oracle(O.default_case(**kw))
+    ora.init(ppos=np.array(p0), ptemp=np.array([0.3]))
+    for s in ("macrovar", "forcing", "collide_f", "stream_f", "ibb_f", "collide_g", "stream_g",
+              "ibb_g", "prepare", "repulsion", "forces", "heat", "move"):
+        ora.stage(s)
+    ora.set_particles("pos_x", [p0[0] + shift])
+    ora.set_particles("pos_y", [p0[1]])
+    ora.set_particles("pos_z", [p0[2]])
+    ora.stage("build_links")
+    ib_prev, _ = ora.flags(prev=True)
+    return ora, ib_prev
+
+
+def _neighbours(L, idx):
+    iz, rem = divmod(idx, L[0] * L[1])
+    iy, ix = divmod(rem, L[0])
+    return np.array([ix, iy, iz])
+
+
+def _inside(L, x):
+    return bool(np.all(x >= 0) and np.all(x < np.array(L)))
+
+
+def _at(L, x):
+    return int(x[2] * L[0] * L[1] + x[1] * L[0] + x[0])
+
+
+def test_refill_fallback_moments_closed_form(O):
+    """refill_new_fluid_kernel (refill.cu:154-195) at a new-fluid node with no previous-fluid
+    neighbour: the populations come from the owner's surface velocity u and temperature T_p,
+    with e.u not divided by RT and r = RHO0 = 1.  Their moments are then fixed:
+    rho = sum f = 1 - 4|u|^2/3, momentum = sum c f = u/3, phi = sum g = T_p (1 - 4|u|^2/3),
+    and F = 0.  A sphere that jumps 3 nodes leaves such nodes behind."""
+    L, R, p0 = (24, 16, 16), 4.0, [9.0, 8.0, 8.0]
+    ora, ib_prev = _refill_setup(O, L, R, p0, 3.0)
+    V, Om = np.array([0.02, -0.01, 0.005]), np.array([0.004, -0.002, 0.01])
+    for k, v in zip(("vel_x", "vel_y", "vel_z", "omega_x", "omega_y", "omega_z"),
+                    np.concatenate([V, Om])):
+        ora.set_particles(k, [v])
+    Tp = float(ora.particles("temp")[0])  # after this step's move
+    ora.stage("refill")
+    nodes, pids = ora.new_fluid()
+    p = np.array([p0[0] + 3.0, p0[1], p0[2]])
+    checked = 0
+    for idx, pid in zip(nodes, pids):
+        x = _neighbours(L, int(idx))
+        nb = [x + C[d] for d in range(1, 27)]
+        if any(_inside(L, y) and ib_prev[_at(L, y)] == 0 for y in nb):
+            continue
+        assert pid == 0
+        u = V + np.cross(Om, x + 0.5 - p)
+        u2 = float(u @ u)
+        rho = ora.field("rho")[idx]
+        assert rho == pytest.approx(1.0 - 4.0 * u2 / 3.0, rel=1e-14)
+        for k, comp in zip(("ux", "uy", "uz"), u / 3.0):
+            assert ora.field(k)[idx] == pytest.approx(comp / rho, rel=1e-12, abs=1e-18)
+        assert ora.field("phi")[idx] == pytest.approx(Tp * (1.0 - 4.0 * u2 / 3.0), rel=1e-14)
+        assert ora.field("fx")[idx] == 0.0
+        checked += 1
+    assert checked > 0
+
+
+def test_refill_extrapolation_exact_for_linear_fields(O):
+    """refill_new_fluid_kernel (refill.cu:121-153) averages, over the directions d whose node
+    x + c_d was fluid, 2 F(x + c_d) - F(x + 2 c_d) (when x + 2 c_d was fluid too) or
+    F(x + c_d), of the pre-stream populations F = f*, g*.  For fields linear in position,
+    F_ip(x) = a_ip + b_ip . x, every second-order sample equals F_ip(x) and a first-order one
+    adds b_ip . c_d, so the written value is F_ip(x) + b_ip . (sum of first-order c_d) / count."""
+    L, R, p0 = (24, 16, 16), 4.0, [9.0, 8.0, 8.0]
+    ora, ib_prev = _refill_setup(O, L, R, p0, 1.0)
+    n = L[0] * L[1] * L[2]
+    xs = np.stack([_neighbours(L, i) for i in range(n)]).astype(np.float64)  # (n, 3)
+    rng = np.random.default_rng(7)
+    lin = {}
+    for name in ("f_collide", "g_collide"):
+        a, b = rng.normal(size=27), rng.normal(size=(27, 3)) * 0.01
+        lin[name] = (a, b)
+        ora.set_populations(name, a[:, None] + b @ xs.T)
+    ora.stage("refill")
+    nodes, _ = ora.new_fluid()
+    f, g = ora.populations("f"), ora.populations("g")
+    checked = 0
+    for idx in nodes:
+        x = _neighbours(L, int(idx))
+        first, count = np.zeros(3), 0
+        for d in range(1, 27):
+            y = x + C[d]
+            if not (_inside(L, y) and ib_prev[_at(L, y)] == 0):
+                continue
+            count += 1
+            y2 = x + 2 * C[d]
+            if not (_inside(L, y2) and ib_prev[_at(L, y2)] == 0):
+                first += C[d]
+        if count == 0:
+            continue
+        for pops, name in ((f, "f_collide"), (g, "g_collide")):
+            a, b = lin[name]
+            want = a + b @ x + b @ first / count
+            np.testing.assert_allclose(pops[:, idx], want, rtol=0, atol=1e-13)
+        checked += 1
+    assert checked > 3
 
 
 def test_refill_writes_consistent_moments(O):
