@@ -171,7 +171,7 @@ def main():
     args = ap.parse_args()
 
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+        print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported "
               "(the multi-GPU slab decomposition is a design, not code)", file=sys.stderr)
         sys.exit(2)
 
@@ -210,6 +210,11 @@ def main():
     log(f"copy probe: {copy_gbs:.0f} GB/s")
     stencil_gbs = rb.stencil_probe(kw64["lx"], kw64["ly"], kw64["lz"], 20)
     log(f"stencil-pattern probe: {stencil_gbs:.0f} GB/s")
+    # the alternative streaming scheme (in-place AA) as access patterns on the same lattice
+    aa_even = rb.streaming_probe(1, kw64["lx"], kw64["ly"], kw64["lz"], 20)
+    aa_odd = rb.streaming_probe(2, kw64["lx"], kw64["ly"], kw64["lz"], 20)
+    aa_pair = 2.0 / (1.0 / aa_even + 1.0 / aa_odd)  # mean rate over an even + odd step pair
+    log(f"AA-pattern probes: even {aa_even:.0f}, odd {aa_odd:.0f}, pair {aa_pair:.0f} GB/s")
     r64 = run_form(rb, kw64, args.steps, args.warmup, args.timing_steps, log, args.nu_window)
     n = r64["n"]
     mlups = n * args.steps / (r64["wall_ms"] * 1e-3) / 1e6
@@ -268,6 +273,7 @@ def main():
                      "copy_probe_gbs": copy_gbs, "frac_of_copy_probe": achieved / copy_gbs,
                      "stencil_probe_gbs": stencil_gbs,
                      "frac_of_stencil_probe": achieved / stencil_gbs,
+                     "aa_pattern_probe_gbs": {"even": aa_even, "odd": aa_odd, "pair": aa_pair},
                      "kernel_ms": r64["kernel_ms"], "bytes_per_node": BYTES_PER_NODE[0]},
         "cpu_baseline": cpu,
         "forms": forms,

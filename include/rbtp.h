@@ -231,6 +231,11 @@ int rbtp_bandwidth_probe(size_t bytes, int reps, double *gbs);
  * stores, timed over `reps` launches; *gbs = 880 B x nodes x reps / time in GB/s.  No reference
  * counterpart; it is the ceiling of the access pattern itself on this device. */
 int rbtp_stencil_probe(int lx, int ly, int lz, int reps, double *gbs);
+/* Streaming schemes as access patterns, same sizes and units as rbtp_stencil_probe: scheme 0 is
+ * the pull between two buffers (rbtp_stencil_probe), 1 and 2 the even and odd steps of in-place
+ * AA streaming on one buffer per distribution.  No reference counterpart; it measures the
+ * alternative to the pull on this device. */
+int rbtp_streaming_probe(int scheme, int lx, int ly, int lz, int reps, double *gbs);
 
 /*
  * Output-side fields and files (output/output_flow.cu, output/output_part.cu,

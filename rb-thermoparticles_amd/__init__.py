@@ -158,6 +158,7 @@ def _load():
         "rbtp_set_kernel_timing": (I, [P, I]),
         "rbtp_kernel_ms": (I, [P, C.POINTER(D), C.POINTER(C.c_int64)]),
         "rbtp_set_shift_interval": (I, [P, I]),
+        "rbtp_streaming_probe": (I, [I, I, I, I, I, C.POINTER(D)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -191,6 +192,14 @@ def bandwidth_probe(nbytes: int = 2 << 30, reps: int = 20) -> float:
     """Achievable HBM bandwidth (GB/s) of a streaming copy on the current device."""
     v = C.c_double()
     _check(lib.rbtp_bandwidth_probe(int(nbytes), int(reps), C.byref(v)))
+    return v.value
+
+
+def streaming_probe(scheme: int, lx: int, ly: int, lz: int, reps: int = 20) -> float:
+    """GB/s at 880 B per node of a streaming scheme's access pattern (rbtp_streaming_probe):
+    0 the pull between two buffers, 1 and 2 the even and odd steps of in-place AA streaming."""
+    v = C.c_double()
+    _check(lib.rbtp_streaming_probe(int(scheme), int(lx), int(ly), int(lz), int(reps), C.byref(v)))
     return v.value
 
 

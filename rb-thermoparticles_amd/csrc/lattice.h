@@ -32,6 +32,9 @@ hipError_t launch_init(S *f, S *g, typename Enc<S>::A *fx, typename Enc<S>::A *r
 
 // Time the FP64 lattice step's access pattern alone on an lx x ly x lz lattice (rbtp_stencil_probe).
 int lattice_stencil_probe(int lx, int ly, int lz, int reps, double *gbs);
+// scheme 0: the pull (as lattice_stencil_probe); 1 and 2: the even and odd steps of in-place
+// AA streaming.  GB/s at 880 B per node.
+int lattice_streaming_probe(int scheme, int lx, int ly, int lz, int reps, double *gbs);
 
 // Refresh the shift profile of the reduced-precision g encoding to the plane-mean temperature
 // and re-encode the stored g (buffer g, the state the next step reads) under it.  partial holds

@@ -506,9 +506,71 @@ __global__ __launch_bounds__(BLOCK) void k_stencil_probe(const double *__restric
 
 }  // namespace
 
+// The two steps of in-place (AA) streaming as access patterns, for comparison with the pull
+// above: one buffer per distribution, 54 loads and 54 stores per node plus F_x, the same launch
+// shape.  Even step: a node reads its own slots i and writes its own slots opp(i).  Odd step: a
+// node reads slot i at x + c_i (its own slot opp(i) where x + c_i is outside) and writes the
+// same locations; each location belongs to exactly one node, so both steps are race-free.  A
+// value is stored with 1 added, so that no store can be dropped as rewriting what was read.
+__global__ __launch_bounds__(BLOCK) void k_aa_even_probe(double *__restrict__ f,
+                                                         double *__restrict__ g,
+                                                         double *__restrict__ fx, Dims d) {
+    const int nblocks = (d.n + BLOCK - 1) / BLOCK;
+    const int idx = xcd_block(nblocks) * BLOCK + threadIdx.x;
+    if (idx >= d.n) return;
+    double fv[NPOP], gv[NPOP];
+#pragma unroll
+    for (int i = 0; i < NPOP; i++) {
+        fv[i] = f[pop_at(i, idx)];
+        gv[i] = g[pop_at(i, idx)];
+    }
+#pragma unroll
+    for (int i = 0; i < NPOP; i++) {
+        Enc<double>::st(f + pop_at(D3Q27::opp[i], idx), fv[i] + 1.0);
+        Enc<double>::st(g + pop_at(D3Q27::opp[i], idx), gv[i] + 1.0);
+    }
+    fx[idx] = fx[idx] + fv[0];
+}
+
+__global__ __launch_bounds__(BLOCK) void k_aa_odd_probe(double *__restrict__ f,
+                                                        double *__restrict__ g,
+                                                        double *__restrict__ fx, Dims d) {
+    const int nblocks = (d.n + BLOCK - 1) / BLOCK;
+    const int idx = xcd_block(nblocks) * BLOCK + threadIdx.x;
+    if (idx >= d.n) return;
+    const int iz = idx / d.lxy;
+    const int rem = idx - iz * d.lxy;
+    const int iy = rem / d.lx;
+    const int ix = rem - iy * d.lx;
+    double fv[NPOP], gv[NPOP];
+    size_t loc[NPOP];
+#pragma unroll
+    for (int i = 0; i < NPOP; i++) {
+        const int cx = D3Q27::cx[i], cy = D3Q27::cy[i], cz = D3Q27::cz[i];
+        const bool inside = ix + cx >= 0 && ix + cx < d.lx && iy + cy >= 0 && iy + cy < d.ly &&
+                            iz + cz >= 0 && iz + cz < d.lz;
+        const int off = cx + cy * d.lx + cz * d.lxy;
+        loc[i] = inside ? pop_at(i, idx + off) : pop_at(D3Q27::opp[i], idx);
+        fv[i] = f[loc[i]];
+        gv[i] = g[loc[i]];
+    }
+#pragma unroll
+    for (int i = 0; i < NPOP; i++) {
+        Enc<double>::st(f + loc[i], fv[D3Q27::opp[i]] + 1.0);
+        Enc<double>::st(g + loc[i], gv[D3Q27::opp[i]] + 1.0);
+    }
+    fx[idx] = fx[idx] + fv[0];
+}
+
 int lattice_stencil_probe(int lx, int ly, int lz, int reps, double *gbs) {
+    return lattice_streaming_probe(0, lx, ly, lz, reps, gbs);
+}
+
+int lattice_streaming_probe(int scheme, int lx, int ly, int lz, int reps, double *gbs) {
     if (lx < 2 || ly < 2 || lz < 2 || reps < 1 || (long long)lx * ly * lz > 0x7fffffffLL)
         return report(RBTP_E_ARG, "stencil probe: lattice of 2..2^31 nodes, reps >= 1");
+    if (scheme < 0 || scheme > 2)
+        return report(RBTP_E_ARG, "streaming probe: scheme 0 (pull), 1 (AA even) or 2 (AA odd)");
     Dims d;
     d.lx = lx; d.ly = ly; d.lz = lz; d.lxy = lx * ly; d.n = lx * ly * lz;
     const size_t len = pop_buffer_len((size_t)d.n);
@@ -527,9 +589,15 @@ int lattice_stencil_probe(int lx, int ly, int lz, int reps, double *gbs) {
     const int nb = blocks_for((size_t)d.n);
     int cur = 0;
     auto launch = [&]() {
-        hipLaunchKernelGGL(k_stencil_probe, nb, BLOCK, 0, st, buf[cur], buf[cur + 1],
-                           buf[2 - cur], buf[3 - cur], fx, d);
-        cur = 2 - cur;
+        if (scheme == 1) {
+            hipLaunchKernelGGL(k_aa_even_probe, nb, BLOCK, 0, st, buf[0], buf[1], fx, d);
+        } else if (scheme == 2) {
+            hipLaunchKernelGGL(k_aa_odd_probe, nb, BLOCK, 0, st, buf[0], buf[1], fx, d);
+        } else {
+            hipLaunchKernelGGL(k_stencil_probe, nb, BLOCK, 0, st, buf[cur], buf[cur + 1],
+                               buf[2 - cur], buf[3 - cur], fx, d);
+            cur = 2 - cur;
+        }
         return hipGetLastError();
     };
     if (e == hipSuccess) e = launch();  // warm-up

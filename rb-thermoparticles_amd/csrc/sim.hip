@@ -801,6 +801,11 @@ extern "C" int rbtp_stencil_probe(int lx, int ly, int lz, int reps, double *gbs)
     return lattice_stencil_probe(lx, ly, lz, reps, gbs);
 }
 
+extern "C" int rbtp_streaming_probe(int scheme, int lx, int ly, int lz, int reps, double *gbs) {
+    if (!gbs) return set_error(RBTP_E_ARG, "null argument");
+    return lattice_streaming_probe(scheme, lx, ly, lz, reps, gbs);
+}
+
 extern "C" int rbtp_kinetic_energy(rbtp_sim *s, double *out) {
     if (!s || !out) return set_error(RBTP_E_ARG, "null argument");
     if (!s->macro_valid) return set_error(RBTP_E_STATE, "run the last step with RBTP_STEP_MACRO_LAST first");

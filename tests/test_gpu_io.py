@@ -236,3 +236,7 @@ def test_measurement_probes(rb):
         rb.stencil_probe(1, 48, 40, 2)
     with pytest.raises(rb.RbtpError):
         rb.stencil_probe(64, 48, 40, 0)
+    for scheme in (0, 1, 2):  # pull, AA even, AA odd
+        assert 0.0 < rb.streaming_probe(scheme, 64, 48, 40, 2) < 1e5
+    with pytest.raises(rb.RbtpError):
+        rb.streaming_probe(3, 64, 48, 40, 2)
