@@ -41,7 +41,10 @@ namespace {
 
 constexpr int PB = 256;   // threads per particle workgroup of the box clear/mark/collect kernels
 constexpr int CELL_CAP = 16;  // particle ids per cell of the pair-search grid
-constexpr int CAND_MAX = 48;  // pair candidates per particle held in LDS
+// Pair candidates per particle held in LDS.  A 3x3x3 block of cells, each 2R + REP_MIN_GAP
+// wide, holds at most about 74 non-overlapping spheres, so clustered spheres stay on the cell
+// path; beyond 96 a particle checks every partner.
+constexpr int CAND_MAX = 96;
 constexpr int CELLS_MIN_NP = 32;  // particle count from which the pair search uses the grid
 constexpr int LB = 1024;  // threads per particle workgroup of the special-node search
 
@@ -316,90 +319,109 @@ __device__ inline bool pair_term(const PartDev &pd, const double *X, const doubl
 }
 
 // prepare_particle_forces_step (pforces.cu:80-116), the heat history of compute_particle_heat
-// (ibb.cu:260-261), and apply_repulsive_forces (pforces.cu:118-148).  Pair terms are summed per
-// particle in ascending partner order; the term on the higher pid of a pair is evaluated from its
-// own side, which is bit-for-bit the negation the reference applies (pforces.cu:35-37).
-// With the cell grid, the partners are the particles binned in the 27 cells around i's cell,
-// sorted by pid; every pair within range is among them, and the pairs left out are ones the
-// reference skips, so the sum is the same.  When a cell overflowed or i has more than
-// CAND_MAX candidates, i falls back to every partner.
-__global__ __launch_bounds__(128) void k_contacts(PartGrid g) {
-    __shared__ int32_t cand_all[128][CAND_MAX];
-    int32_t *cand = cand_all[threadIdx.x];
-    const int u = blockIdx.x * 128 + threadIdx.x;
-    if (u >= g.total_p) return;
+// (ibb.cu:260-261), and apply_repulsive_forces (pforces.cu:118-148), one wave per particle.
+// Pair terms are summed per particle in ascending partner order; the term on the higher pid of a
+// pair is evaluated from its own side, which is bit-for-bit the negation the reference applies
+// (pforces.cu:35-37).  With the cell grid, the partners are the particles binned in the 27 cells
+// around i's cell, ranked by pid; every pair within range is among them, and the pairs left out
+// are ones the reference skips, so the sum is the same.  When a cell overflowed or i has more
+// than CAND_MAX candidates, i falls back to every partner.  The lanes evaluate the terms of 64
+// consecutive partners at a time; every lane then adds them in partner order (shuffles), so the
+// sum is the sequential one.
+constexpr int CW = 64;  // lanes per particle in k_contacts (one wave)
+__global__ __launch_bounds__(CW) void k_contacts(PartGrid g) {
+    __shared__ int32_t cand[CAND_MAX];    // candidates in cell order
+    __shared__ int32_t sorted[CAND_MAX];  // the same, ranked by pid
+    const int u = blockIdx.x;
+    if (u >= g.total_p) return;  // whole workgroup (one wave)
+    const int lane = threadIdx.x;
     const int s = slot_of(g.pstart, g.nslots, u);
     const PartDev &pd = g.tab[s].pd;
     const Dims d = g.tab[s].d;
     const int i = u - g.pstart[s];
-    // the special-node list of this step is rebuilt from here on (k_special_unmap, the last
-    // reader of the previous list, ran before this kernel)
-    if (i == 0) {
-        pd.sp.count[0] = 0;
-        pd.sp.count[1] = 0;
-    }
     double *const *P = pd.p;
-    for (int a = 0; a < 3; a++) {
-        P[RBTP_P_FORCE_PREV2_X + a][i] = P[RBTP_P_FORCE_PREV_X + a][i];
-        P[RBTP_P_TORQUE_PREV2_X + a][i] = P[RBTP_P_TORQUE_PREV_X + a][i];
-        P[RBTP_P_FORCE_PREV_X + a][i] = P[RBTP_P_FORCE_X + a][i];
-        P[RBTP_P_TORQUE_PREV_X + a][i] = P[RBTP_P_TORQUE_X + a][i];
-        P[RBTP_P_TORQUE_X + a][i] = 0.0;
+    if (lane == 0) {
+        // the special-node list of this step is rebuilt from here on (k_special_unmap, the last
+        // reader of the previous list, ran before this kernel)
+        if (i == 0) {
+            pd.sp.count[0] = 0;
+            pd.sp.count[1] = 0;
+        }
+        for (int a = 0; a < 3; a++) {
+            P[RBTP_P_FORCE_PREV2_X + a][i] = P[RBTP_P_FORCE_PREV_X + a][i];
+            P[RBTP_P_TORQUE_PREV2_X + a][i] = P[RBTP_P_TORQUE_PREV_X + a][i];
+            P[RBTP_P_FORCE_PREV_X + a][i] = P[RBTP_P_FORCE_X + a][i];
+            P[RBTP_P_TORQUE_PREV_X + a][i] = P[RBTP_P_TORQUE_X + a][i];
+            P[RBTP_P_TORQUE_X + a][i] = 0.0;
+        }
+        P[RBTP_P_HEAT_PREV][i] = P[RBTP_P_HEAT][i];
+        P[RBTP_P_HEAT][i] = 0.0;
     }
-    P[RBTP_P_HEAT_PREV][i] = P[RBTP_P_HEAT][i];
-    P[RBTP_P_HEAT][i] = 0.0;
     const double *X = P[RBTP_P_POS_X], *Y = P[RBTP_P_POS_Y], *Z = P[RBTP_P_POS_Z];
     const double xi = X[i], yi = Y[i], zi = Z[i];
     double FX = 0.0, FY = 0.0, FZ = 0.0;
     const double radius = pd.radius, coeff = pd.coeff;
     if (pd.np > 1) {
-        int ncand = -1;  // -1: every partner
+        int count = pd.np;     // partners to walk
+        bool ranked = false;   // partners taken from `sorted`
         if (pd.use_cells) {
             const int cx = cell_coord(xi, pd.cell_scale[0], pd.ncell[0]);
             const int cy = cell_coord(yi, pd.cell_scale[1], pd.ncell[1]);
             const int cz = cell_coord(zi, pd.cell_scale[2], pd.ncell[2]);
-            // the (up to) 27 neighbour cells and their counts, loaded together
-            int cell[27], cnt[27];
+            // the (up to) 27 neighbour cells, their counts and where their ids start in `cand`
+            int cell[27], start[28];
             int nc = 0;
             for (int z = max(cz - 1, 0); z <= min(cz + 1, pd.ncell[2] - 1); z++)
                 for (int y = max(cy - 1, 0); y <= min(cy + 1, pd.ncell[1] - 1); y++)
                     for (int x = max(cx - 1, 0); x <= min(cx + 1, pd.ncell[0] - 1); x++)
                         cell[nc++] = (z * pd.ncell[1] + y) * pd.ncell[0] + x;
-            int total = 0;
             bool full = false;
+            start[0] = 0;
             for (int q = 0; q < nc; q++) {
-                cnt[q] = pd.cell_cnt[cell[q]];
-                full = full || cnt[q] > CELL_CAP;
-                total += cnt[q];
+                const int c = pd.cell_cnt[cell[q]];
+                full = full || c > CELL_CAP;
+                start[q + 1] = start[q] + c;
             }
+            const int total = start[nc];
             if (!full && total <= CAND_MAX) {
-                ncand = 0;
-                for (int q = 0; q < nc; q++)
-                    for (int k = 0; k < cnt[q]; k++) {
-                        // insertion in ascending pid order
-                        const int32_t v = pd.cell_pid[(size_t)cell[q] * CELL_CAP + k];
-                        int pos = ncand;
-                        while (pos > 0 && cand[pos - 1] > v) {
-                            cand[pos] = cand[pos - 1];
-                            --pos;
-                        }
-                        cand[pos] = v;
-                        ++ncand;
-                    }
+                for (int t = lane; t < total; t += CW) {
+                    int q = 0;
+                    while (start[q + 1] <= t) q++;
+                    cand[t] = pd.cell_pid[(size_t)cell[q] * CELL_CAP + (t - start[q])];
+                }
+                __syncthreads();
+                // rank by pid (every particle is binned once: the ids are distinct)
+                for (int t = lane; t < total; t += CW) {
+                    const int32_t v = cand[t];
+                    int r = 0;
+                    for (int m = 0; m < total; m++) r += cand[m] < v ? 1 : 0;
+                    sorted[r] = v;
+                }
+                __syncthreads();
+                count = total;
+                ranked = true;
             }
         }
-        const int count = ncand < 0 ? pd.np : ncand;
-#pragma unroll 4
-        for (int k = 0; k < count; k++) {
-            const int j = ncand < 0 ? k : cand[k];
-            if (j == i) continue;
-            double fx, fy, fz;
-            if (!pair_term(pd, X, Y, Z, xi, yi, zi, j, fx, fy, fz)) continue;
-            FX += fx;
-            FY += fy;
-            FZ += fz;
+        for (int k0 = 0; k0 < count; k0 += CW) {
+            const int k = k0 + lane;
+            double fx = 0.0, fy = 0.0, fz = 0.0;
+            bool ok = false;
+            if (k < count) {
+                const int j = ranked ? sorted[k] : k;
+                if (j != i) ok = pair_term(pd, X, Y, Z, xi, yi, zi, j, fx, fy, fz);
+            }
+            const unsigned long long okm = __ballot(ok);
+            for (int m = 0; m < CW; m++) {
+                const double tx = __shfl(fx, m), ty = __shfl(fy, m), tz = __shfl(fz, m);
+                if ((okm >> m) & 1ull) {
+                    FX += tx;
+                    FY += ty;
+                    FZ += tz;
+                }
+            }
         }
     }
+    if (lane != 0) return;
     // repulsive_wall_kernel (pforces.cu:40-76)
     double fx = 0.0, fy = 0.0, fz = 0.0;
     const double mgw = pd.min_gap_w;
@@ -1400,7 +1422,7 @@ int part_phase(const PartGrid &g, int prec, hipStream_t st) {
     PHIP(hipGetLastError());
     hipLaunchKernelGGL(k_special_unmap, g.total_s, 128, 0, st, g);
     PHIP(hipGetLastError());
-    hipLaunchKernelGGL(k_contacts, (tp + 127) / 128, 128, 0, st, g);
+    hipLaunchKernelGGL(k_contacts, tp, CW, 0, st, g);
     PHIP(hipGetLastError());
     PHIP(dispatch_prec(prec, [&](auto *tag) {
         using S = std::remove_pointer_t<decltype(tag)>;

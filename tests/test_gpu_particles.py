@@ -188,6 +188,26 @@ def test_crowded_cell_falls_back_to_all_pairs(rb, O):
     check_fields(sim, ora, "crowded")
 
 
+@pytest.mark.parametrize("side,spacing", [(4, 2.4), (5, 1.9)])
+def test_dense_cluster_candidates(rb, O, side, spacing):
+    """A side^3 block of small spheres, no cell over its capacity: the spheres near the block's
+    centre see 64 (side 4) or 125 (side 5) pair candidates in their 27 cells, the first within
+    the candidate list, the second beyond it (the all-pairs fallback); both agree with the
+    oracle bit for bit."""
+    centre = np.array([20.0, 16.0, 12.0])
+    offs = (np.arange(side) - (side - 1) / 2.0) * spacing
+    rng = np.random.default_rng(side)
+    pos = [tuple(centre + np.array([a, b, c]) + rng.uniform(-0.05, 0.05, 3))
+           for a in offs for b in offs for c in offs]
+    kw = particle_case(40, 32, 24, npart=len(pos), diam=1.2, ra=1e4, beta=0.05)
+    sim, ora = make_pair(rb, O, kw, pos)
+    sim.step(2)
+    ora.step(2)
+    check_particles(sim, ora, f"cluster {side}")
+    check_geometry(sim, ora, f"cluster {side}")
+    check_fields(sim, ora, f"cluster {side}")
+
+
 def test_particles_enabled_with_none(rb, O):
     """ACTIVATE_PARTICLES with a particle count of zero runs the single-phase step."""
     kw = particle_case(20, 16, 12, npart=0, diam=4.0, ra=1e5, beta=0.05)
