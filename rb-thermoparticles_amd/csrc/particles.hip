@@ -897,11 +897,13 @@ __global__ __launch_bounds__(LB) void k_refill_find(PartGrid g) {
 }
 
 // Bounce-back values of the IBB link nodes that link to more than one sphere (SP_MULTI entries at
-// the head of the special list; k_link_nodes wrote every other link node), one thread per node:
-// for every link x -> x + c_ip (x + c_ip solid and linked, with the state before this step's
-// motion) the values bounce_back_particles / bounce_back_thermal_particles write into f[opp][x]
-// and g[opp][x] (ibb.cu:5-136), and the mask of the replaced populations.  The node's other
-// populations are plain pulls, which the next lattice kernel makes itself.
+// the head of the special list; k_link_nodes wrote every other link node): for every link
+// x -> x + c_ip (x + c_ip solid and linked, with the state before this step's motion) the values
+// bounce_back_particles / bounce_back_thermal_particles write into f[opp][x] and g[opp][x]
+// (ibb.cu:5-136), and the mask of the replaced populations.  The node's other populations are
+// plain pulls, which the next lattice kernel makes itself.  Each wave reads the masks of 64
+// consecutive entries at once and completes their SP_MULTI entries one after another, one lane
+// per direction; the links of a node are independent (no sum), so their order does not matter.
 template <class S>
 __global__ __launch_bounds__(128) void k_fixup_links(PartGrid g) {
     const int slot = slot_of(g.lstart, g.nslots, blockIdx.x);
@@ -913,45 +915,59 @@ __global__ __launch_bounds__(128) void k_fixup_links(PartGrid g) {
                          gbl(pd.gshift), d.lx};
     const gptr<const int32_t> owner_prev = gbl(sl.owner_prev);
     const gptr<const double> macro_rho = gbl((const double *)sl.macro[0]);
-    const int blk = blockIdx.x - g.lstart[slot];
-    const int nblk = g.lstart[slot + 1] - g.lstart[slot];
+    const int lane = __lane_id();
+    const int wave = (blockIdx.x - g.lstart[slot]) * 2 + (threadIdx.x >> 6);
+    const int nwave = (g.lstart[slot + 1] - g.lstart[slot]) * 2;
     const int nlink = min(pd.sp.count[1], pd.sp.cap);
     const size_t cap = (size_t)pd.sp.cap;
     const gptr<A> out = gbl((A *)pd.sp.pop);
     const gptr<uint32_t> spmask = gbl(pd.sp.mask);
     const PView pv = pview_old(pd);
-    for (int j = blk * 128 + threadIdx.x; j < nlink; j += nblk * 128) {
-        if (!(spmask[j] & SP_MULTI)) continue;  // k_link_nodes wrote this node's links
-        const int idx = pd.sp.node[j];
-        const int iz = idx / d.lxy;
-        const int rem = idx - iz * d.lxy;
-        const int iy = rem / d.lx;
-        const int ix = rem - iy * d.lx;
-        const double x0 = static_cast<double>(ix) + 0.5;
-        const double y0 = static_cast<double>(iy) + 0.5;
-        const double z0 = static_cast<double>(iz) + 0.5;
-        const double density = macro_rho[idx];
-        int own[NPOP];
-        neighbour_owners(owner_prev, d, ix, iy, iz, false, own);
-        uint32_t m = 0u;
-        for (int ip = 1; ip < NPOP; ip++) {
-            const int q_owner = own[ip];
-            if (q_owner < 0) continue;
-            double q = 0.0;
-            if (!link_q(ip, x0, y0, z0, pv.px[q_owner], pv.py[q_owner], pv.pz[q_owner],
-                        pd.radius_sq, q))
-                continue;
-            const int opp = dir_opp(ip);
-            const bool up_fluid = own[opp] == -1;  // x - c_ip = x + c_opp, inside and fluid
-            const int idx_up = idx - (dir_cx(ip) + dir_cy(ip) * d.lx + dir_cz(ip) * d.lxy);
-            const LinkLoads Ld = link_loads<S>(ip, idx, idx_up, up_fluid, q, pop);
-            const LinkOut L = eval_link_at(Ld, ip, ix, iy, iz, up_fluid, q, q_owner, pv, density,
-                                           pd.rho0);
-            out[(size_t)opp * cap + j] = A(L.f_out);
-            out[(size_t)(NPOP + opp) * cap + j] = A(L.g_out);
-            m |= 1u << opp;
+    const int ip = lane;  // the direction of this lane (1..26 take part)
+    for (int w0 = wave * 64; w0 < nlink; w0 += nwave * 64) {
+        const int jl = w0 + lane;
+        const uint32_t ml = jl < nlink ? spmask[jl] : 0u;
+        unsigned long long multi = __ballot((ml & SP_MULTI) != 0u);
+        while (multi) {
+            const int j = w0 + __builtin_ctzll(multi);
+            multi &= multi - 1ull;
+            const int idx = pd.sp.node[j];
+            const int iz = idx / d.lxy;
+            const int rem = idx - iz * d.lxy;
+            const int iy = rem / d.lx;
+            const int ix = rem - iy * d.lx;
+            // this lane's neighbour owner (-2 outside the domain and for lanes without a
+            // direction), and the upstream neighbour's (lane opp) by a shuffle
+            int own = -2;
+            if (ip >= 1 && ip < NPOP) {
+                const int nx = ix + dir_cx(ip), ny = iy + dir_cy(ip), nz = iz + dir_cz(ip);
+                if (in_domain(d, nx, ny, nz)) own = owner_prev[nz * d.lxy + ny * d.lx + nx];
+            }
+            const int own_up = __shfl(own, (ip >= 1 && ip < NPOP) ? dir_opp(ip) : 0);
+            uint32_t bit = 0u;
+            if (own >= 0) {
+                const double x0 = static_cast<double>(ix) + 0.5;
+                const double y0 = static_cast<double>(iy) + 0.5;
+                const double z0 = static_cast<double>(iz) + 0.5;
+                double q = 0.0;
+                if (link_q(ip, x0, y0, z0, pv.px[own], pv.py[own], pv.pz[own], pd.radius_sq, q)) {
+                    const int opp = dir_opp(ip);
+                    const bool up_fluid = own_up == -1;  // x - c_ip = x + c_opp, inside and fluid
+                    const int idx_up = idx - (dir_cx(ip) + dir_cy(ip) * d.lx + dir_cz(ip) * d.lxy);
+                    const double density = macro_rho[idx];
+                    const LinkLoads Ld = link_loads<S>(ip, idx, idx_up, up_fluid, q, pop);
+                    const LinkOut L = eval_link_at(Ld, ip, ix, iy, iz, up_fluid, q, own, pv,
+                                                   density, pd.rho0);
+                    out[(size_t)opp * cap + j] = A(L.f_out);
+                    out[(size_t)(NPOP + opp) * cap + j] = A(L.g_out);
+                    bit = 1u << opp;
+                }
+            }
+            // OR of the lanes' bits: the replaced populations of entry j
+#pragma unroll
+            for (int o = 32; o >= 1; o >>= 1) bit |= (uint32_t)__shfl_xor((int)bit, o);
+            if (lane == 0) spmask[j] = bit;
         }
-        spmask[j] = m;
     }
 }
 
