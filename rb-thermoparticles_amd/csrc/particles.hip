@@ -383,6 +383,8 @@ __global__ __launch_bounds__(CW) void k_contacts(PartGrid g) {
                 start[q + 1] = start[q] + c;
             }
             const int total = start[nc];
+            // every lane read the same counts: this branch, and the barriers in it, are uniform
+            // over the workgroup (one wave)
             if (!full && total <= CAND_MAX) {
                 for (int t = lane; t < total; t += CW) {
                     int q = 0;
