@@ -157,6 +157,8 @@ def test_shift_refresh_follows_the_temperature(rb):
     rms = {}
     for interval in (0, 1):
         low = rb.Sim(rb.default_case(**kw, precision=2))
+        with pytest.raises(rb.RbtpError):
+            low.set_shift_interval(-1)
         low.set_shift_interval(interval)
         low.init(temperature=T)
         low.step(100)
