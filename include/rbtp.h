@@ -111,6 +111,20 @@ int rbtp_init(rbtp_sim *s, const double *temperature, const double *ppos, const 
  * x=0 to t_cold at x=lx at cell centres, plus amp * U[-1,1) from splitmix64(seed) in idx order. */
 int rbtp_linear_temperature(const rbtp_case *c, uint64_t seed, double amp, double *out_n);
 
+/* generate_initial_positions (pinit.cu:97-144) alone, on the host (no device is touched): the
+ * rejection placement with glibc srand(particle_rand_seed)/rand that rbtp_init runs when ppos is
+ * NULL.  out_3p receives x[P], y[P], z[P] for P = rbtp_case_derive's npart.  Returns
+ * RBTP_E_PLACEMENT where the reference prints "Can't distribute the particles" and exits
+ * (pinit.cu:126-131), after 10^6 rejections. */
+int rbtp_place_particles(const rbtp_case *c, double *out_3p);
+
+/* The particle part of load_particles_from_restart (initial.cu:57-192) alone, on the host: read
+ * np lines of a pos_%09d.txt file (pid followed by the 33 values of output_part.cu:101-113) into
+ * out (RBTP_P_COUNT arrays of np doubles, indexed by RBTP_P_*; arrays the file does not hold are
+ * zero).  A missing file, a short or unparsable line, or a pid outside [0, np) is RBTP_E_IO (the
+ * reference exits, initial.cu:63-66,99-101, or writes out of bounds). */
+int rbtp_read_particle_file(const char *path, int np, double *out);
+
 /*
  * Advance nsteps time steps of main.cu:32-86 (macrovar -> forcing -> collision_BGK -> streaming
  * -> bounce_back_particles -> collision_BGK_scalar -> streaming_scalar ->
@@ -231,6 +245,23 @@ int rbtp_bandwidth_probe(size_t bytes, int reps, double *gbs);
  * stores, timed over `reps` launches; *gbs = 880 B x nodes x reps / time in GB/s.  No reference
  * counterpart; it is the ceiling of the access pattern itself on this device. */
 int rbtp_stencil_probe(int lx, int ly, int lz, int reps, double *gbs);
+/* The FP64 divisions and square root of the strict kernels on the device, for bitwise checks
+ * against IEEE host arithmetic (no reference counterpart: the reference divides with the CUDA
+ * compiler's own sequence, which this library replaces by div_rn, a correctly rounded quotient
+ * from a double-word reciprocal in three fused multiply-adds).  out[j] = a[j] op b[j] for n
+ * host operand pairs (b unused for the one-operand ops):
+ *   RBTP_ARITH_DIV            a / b, reciprocal pair formed on the device (grad_denom,
+ *                             collision.cu:73-77)
+ *   RBTP_ARITH_DIV_HOSTRECIP  a / b, reciprocal pair formed on the host (RT, 3, tau, tauc)
+ *   RBTP_ARITH_DIV6           a / 6 (compute_link_q for a diagonal direction, links.cu:21)
+ *   RBTP_ARITH_DIV_RT         a / (1.0/3.0) (IBB momentum term, ibb.cu:48-49)
+ *   RBTP_ARITH_SQRT           sqrt(a) (links.cu:21, refill.cu:38, pforces.cu:20) */
+#define RBTP_ARITH_DIV 0
+#define RBTP_ARITH_DIV_HOSTRECIP 1
+#define RBTP_ARITH_DIV6 2
+#define RBTP_ARITH_DIV_RT 3
+#define RBTP_ARITH_SQRT 4
+int rbtp_arith_probe(int op, const double *a, const double *b, int64_t n, double *out);
 /* Streaming schemes as access patterns, same sizes and units as rbtp_stencil_probe: scheme 0 is
  * the pull between two buffers (rbtp_stencil_probe), 1 and 2 the even and odd steps of in-place
  * AA streaming on one buffer per distribution.  No reference counterpart; it measures the

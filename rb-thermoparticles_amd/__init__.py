@@ -159,6 +159,9 @@ def _load():
         "rbtp_kernel_ms": (I, [P, C.POINTER(D), C.POINTER(C.c_int64)]),
         "rbtp_set_shift_interval": (I, [P, I]),
         "rbtp_streaming_probe": (I, [I, I, I, I, I, C.POINTER(D)]),
+        "rbtp_arith_probe": (I, [I, dp, P, C.c_int64, dp]),
+        "rbtp_place_particles": (I, [C.POINTER(Case), dp]),
+        "rbtp_read_particle_file": (I, [C.c_char_p, I, dp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -209,6 +212,41 @@ def stencil_probe(lx: int, ly: int, lz: int, reps: int = 20) -> float:
     v = C.c_double()
     _check(lib.rbtp_stencil_probe(int(lx), int(ly), int(lz), int(reps), C.byref(v)))
     return v.value
+
+
+ARITH = {"div": 0, "div_hostrecip": 1, "div6": 2, "div_rt": 3, "sqrt": 4}
+
+
+def arith_probe(op: str, a, b=None) -> np.ndarray:
+    """The strict kernels' FP64 division or square root evaluated on the device
+    (rbtp_arith_probe): op is one of ARITH; b is the divisor for "div" and "div_hostrecip"."""
+    a = np.ascontiguousarray(a, dtype=np.float64).reshape(-1)
+    bp = None
+    if b is not None:
+        b = np.ascontiguousarray(b, dtype=np.float64).reshape(-1)
+        if b.size != a.size:
+            raise ValueError("a and b must have the same length")
+        bp = b.ctypes.data_as(C.c_void_p)
+    out = np.empty_like(a)
+    _check(lib.rbtp_arith_probe(ARITH[op], a, bp, a.size, out))
+    return out
+
+
+def place_particles(case: Case) -> np.ndarray:
+    """Reference rejection placement (pinit.cu:97-144) on the host: shape (3, npart)."""
+    npart = derive(case).npart
+    out = np.zeros(3 * max(npart, 0), dtype=np.float64)
+    if npart > 0:
+        _check(lib.rbtp_place_particles(C.byref(case), out))
+    return out.reshape(3, max(npart, 0))
+
+
+def read_particle_file(path: str, npart: int) -> dict:
+    """A pos_%09d.txt file (initial.cu:57-192) as a dict of particle arrays (host only)."""
+    out = np.zeros(len(PARTICLE_ARRAYS) * max(npart, 0), dtype=np.float64)
+    _check(lib.rbtp_read_particle_file(os.fsencode(path), int(npart), out))
+    out = out.reshape(len(PARTICLE_ARRAYS), max(npart, 0))
+    return {name: out[k] for k, name in enumerate(PARTICLE_ARRAYS)}
 
 
 def derive(case: Case) -> Derived:
@@ -304,8 +342,12 @@ class Sim:
         return out.reshape(27, self.n)
 
     def set_populations(self, f: np.ndarray, g: np.ndarray) -> None:
-        _check(lib.rbtp_set_populations(self._h, np.ascontiguousarray(f, np.float64).reshape(-1),
-                                        np.ascontiguousarray(g, np.float64).reshape(-1)))
+        f = np.ascontiguousarray(f, np.float64).reshape(-1)
+        g = np.ascontiguousarray(g, np.float64).reshape(-1)
+        if f.size != 27 * self.n or g.size != 27 * self.n:
+            raise ValueError(f"f and g must hold 27*n = {27 * self.n} values each "
+                             f"(got {f.size} and {g.size})")
+        _check(lib.rbtp_set_populations(self._h, f, g))
 
     def flags(self):
         ib = np.empty(self.n, dtype=np.int32)
@@ -320,8 +362,11 @@ class Sim:
         return out
 
     def set_particles(self, name: str, values) -> None:
-        v = np.ascontiguousarray(values, dtype=np.float64)
-        _check(lib.rbtp_set_particles(self._h, PARTICLE_ARRAYS.index(name), v))
+        v = np.ascontiguousarray(values, dtype=np.float64).reshape(-1)
+        if v.size != self.np:
+            raise ValueError(f"{name} must hold npart = {self.np} values (got {v.size})")
+        if self.np:
+            _check(lib.rbtp_set_particles(self._h, PARTICLE_ARRAYS.index(name), v))
 
     def links(self) -> np.ndarray:
         cnt = C.c_int64()

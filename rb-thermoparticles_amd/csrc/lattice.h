@@ -30,6 +30,9 @@ hipError_t launch_init(S *f, S *g, typename Enc<S>::A *fx, typename Enc<S>::A *r
                        double beta, const float *gshift, Dims d, double *mr, double *mux,
                        double *muy, double *muz, double *mphi, hipStream_t st);
 
+// rbtp_arith_probe: op RBTP_ARITH_* over n host operand pairs, result into host out.
+int lattice_arith_probe(int op, const double *a, const double *b, size_t n, double *out);
+
 // Time the FP64 lattice step's access pattern alone on an lx x ly x lz lattice (rbtp_stencil_probe).
 int lattice_stencil_probe(int lx, int ly, int lz, int reps, double *gbs);
 // scheme 0: the pull (as lattice_stencil_probe); 1 and 2: the even and odd steps of in-place

@@ -24,6 +24,7 @@
 #include "lattice_device.h"
 
 #include <type_traits>
+#include <vector>
 
 namespace rbtp {
 
@@ -504,7 +505,71 @@ __global__ __launch_bounds__(BLOCK) void k_stencil_probe(const double *__restric
     fx[idx] = fx[idx] + f[0];
 }
 
+// The FP64 divisions and square roots of the strict kernels, one operand pair per thread, with
+// the same inline functions the kernels use (lattice_arith_probe).
+__global__ __launch_bounds__(BLOCK) void k_arith_probe(int op, const double *__restrict__ a,
+                                                       const double *__restrict__ b,
+                                                       const double *__restrict__ bh,
+                                                       const double *__restrict__ bl, size_t n,
+                                                       double *__restrict__ out) {
+    const size_t j = (size_t)blockIdx.x * BLOCK + threadIdx.x;
+    if (j >= n) return;
+    const double x = a[j];
+    double r = 0.0;
+    if (op == RBTP_ARITH_DIV) {
+        // the pair formed on the device, as collide_strict does for grad_denom
+        const double d = b[j];
+        const double h = 1.0 / d;
+        const double l = __builtin_fma(-d, h, 1.0) * h;
+        r = div_rn(x, d, h, l);
+    } else if (op == RBTP_ARITH_DIV_HOSTRECIP) {
+        r = div_rn(x, b[j], bh[j], bl[j]);
+    } else if (op == RBTP_ARITH_DIV6) {
+        r = div_rn(x, 6.0, RECIP6_HI, RECIP6_LO);
+    } else if (op == RBTP_ARITH_DIV_RT) {
+        r = div_rn(x, 1.0 / 3.0, RECIP_RT_HI, RECIP_RT_LO);
+    } else {
+        r = sqrt(x);
+    }
+    out[j] = r;
+}
+
 }  // namespace
+
+int lattice_arith_probe(int op, const double *a, const double *b, size_t n, double *out) {
+    if (op < RBTP_ARITH_DIV || op > RBTP_ARITH_SQRT)
+        return report(RBTP_E_ARG, "arith probe: unknown operation");
+    if (n == 0) return RBTP_OK;
+    const bool needs_b = op == RBTP_ARITH_DIV || op == RBTP_ARITH_DIV_HOSTRECIP;
+    if (!a || !out || (needs_b && !b)) return report(RBTP_E_ARG, "arith probe: null operand");
+    const size_t bytes = n * sizeof(double);
+    std::vector<double> hh, hl;
+    if (op == RBTP_ARITH_DIV_HOSTRECIP) {
+        hh.resize(n);
+        hl.resize(n);
+        for (size_t j = 0; j < n; j++) dw_recip(b[j], hh[j], hl[j]);
+    }
+    double *buf = nullptr;
+    hipStream_t st = nullptr;
+    hipError_t e = hipMalloc(&buf, 5 * bytes);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
+    double *da = buf, *db = buf + n, *dh = buf + 2 * n, *dl = buf + 3 * n, *dout = buf + 4 * n;
+    if (e == hipSuccess) e = copy_to_device(da, a, bytes, st);
+    if (e == hipSuccess && needs_b) e = copy_to_device(db, b, bytes, st);
+    if (e == hipSuccess && !hh.empty()) e = copy_to_device(dh, hh.data(), bytes, st);
+    if (e == hipSuccess && !hl.empty()) e = copy_to_device(dl, hl.data(), bytes, st);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_arith_probe, (unsigned)((n + BLOCK - 1) / BLOCK), BLOCK, 0, st, op,
+                           (const double *)da, (const double *)db, (const double *)dh,
+                           (const double *)dl, n, dout);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = copy_to_host(out, dout, bytes, st);
+    if (st) (void)hipStreamDestroy(st);
+    (void)hipFree(buf);
+    if (e != hipSuccess) return report_hip(e, "arith probe");
+    return RBTP_OK;
+}
 
 // The two steps of in-place (AA) streaming as access patterns, for comparison with the pull
 // above: one buffer per distribution, 54 loads and 54 stores per node plus F_x, the same launch

@@ -58,6 +58,10 @@ inline int link_stride(int box_w) {
 }
 
 
+// generate_initial_positions (pinit.cu:97-144) on the host: d.npart positions into X, Y, Z
+// (resized by the caller); RBTP_E_PLACEMENT after 10^6 rejections.
+int place_particles(const rbtp_case &c, const rbtp_derived &d, std::vector<double> &X,
+                    std::vector<double> &Y, std::vector<double> &Z);
 int part_alloc(PartDev &pd, const rbtp_case &c, const rbtp_derived &d, const Dims &dims);
 void part_free(PartDev &pd);
 int part_init(PartDev &pd, const rbtp_case &c, const rbtp_derived &d, const Dims &dims,

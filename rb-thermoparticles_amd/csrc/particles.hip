@@ -70,8 +70,7 @@ __device__ inline bool link_q(int ip, double x0, double y0, double z0, double px
     // and the one by 6 is div_rn with RN(1/6) + RN(1/6 - RN(1/6)), the same correctly rounded
     // quotient in three FMAs
     const double num = -b - sqrt(disc);
-    q = (a == 3.0) ? div_rn(num, 6.0, 0x1.5555555555555p-3, 0x1.5555555555555p-57)
-                   : num * (a == 1.0 ? 0.5 : 0.25);
+    q = (a == 3.0) ? div_rn(num, 6.0, RECIP6_HI, RECIP6_LO) : num * (a == 1.0 ? 0.5 : 0.25);
     return (q >= 0.0 && q <= 1.0);
 }
 
@@ -573,7 +572,7 @@ __device__ inline LinkOut eval_link_at(const LinkLoads &L, int ip, int ix, int i
     const double evel = dir_cx(ip) * uwx + dir_cy(ip) * uwy + dir_cz(ip) * uwz;
     const double RT = 1.0 / 3.0;
     // / RT as div_rn with RN(1/RT) = 3 and RN(1/RT - 3) = 3 * 2^-54
-    const double momentum_term = div_rn(2.0 * tp * rho0 * evel, RT, 3.0, 0x1.8p-53);
+    const double momentum_term = div_rn(2.0 * tp * rho0 * evel, RT, RECIP_RT_HI, RECIP_RT_LO);
     const double f_here = L.f_here;
     o.f_in = f_here;
     if (up_fluid) {
@@ -1112,6 +1111,8 @@ __global__ __launch_bounds__(256) void k_overwrite_special(PartDev pd, double *f
 }
 
 // Link records of the newest geometry in the reference layout (links.cu:89-139), unordered.
+// A record is listed when the hot path's own test (link_exists, as in k_link_nodes) accepts the
+// link, so the list compared with the reference criterion is the set the coupling uses.
 __global__ __launch_bounds__(PB) void k_collect_links(PartDev pd, const int32_t *owner,
                                                       const int32_t *box, Dims d,
                                                       rbtp_link *out, int cap, int *count) {
@@ -1132,8 +1133,10 @@ __global__ __launch_bounds__(PB) void k_collect_links(PartDev pd, const int32_t 
             const int nx = ix + dir_cx(ip), ny = iy + dir_cy(ip), nz = iz + dir_cz(ip);
             if (!in_domain(d, nx, ny, nz)) continue;
             if (owner[nz * d.lxy + ny * d.lx + nx] != p) continue;
+            // the decision k_link_nodes takes (link_exists), then q as it computes it
+            if (!link_exists(ip, x0, y0, z0, X, Y, Z, pd.radius_sq)) continue;
             double q = 0.0;
-            if (!link_q(ip, x0, y0, z0, X, Y, Z, pd.radius_sq, q)) continue;
+            link_q(ip, x0, y0, z0, X, Y, Z, pd.radius_sq, q);
             const int j = atomicAdd(count, 1);
             if (j >= cap) continue;
             rbtp_link L;
@@ -1278,8 +1281,8 @@ void part_free(PartDev &pd) {
 }
 
 // generate_initial_positions (pinit.cu:97-144): glibc srand/rand rejection sampling.
-static int place_particles(const rbtp_case &c, const rbtp_derived &d, std::vector<double> &X,
-                           std::vector<double> &Y, std::vector<double> &Z) {
+int place_particles(const rbtp_case &c, const rbtp_derived &d, std::vector<double> &X,
+                    std::vector<double> &Y, std::vector<double> &Z) {
     const int np = d.npart;
     const double radius = d.particle_radius;
     std::srand(c.particle_rand_seed);

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 #include <hip/hip_fp16.h>
 #include <hip/hip_bf16.h>
+#include <cmath>
 #include <cstdint>
 #include <cstddef>
 
@@ -11,11 +12,23 @@
 
 namespace rbtp {
 
-// Lattice shape.  n = lx*ly*lz must stay below 2^31 (checked by rbtp_create).
 // Record an error for rbtp_last_error() and return its code (defined with the C-ABI).
 int report(int code, const char *msg);
 int report_hip(hipError_t e, const char *what);
 
+// Double-word reciprocal of a divisor d for div_rn (lattice_device.h), formed on the host:
+// hi = RN(1/d), lo = RN((1 - d*hi)/d); the residual 1 - d*hi is exact in one fused multiply-add.
+inline void dw_recip(double d, double &hi, double &lo) {
+    hi = 1.0 / d;
+    lo = std::fma(-d, hi, 1.0) / d;
+}
+// The pairs of the two constant divisors of the particle kernels, as dw_recip forms them:
+// 6 (compute_link_q's 2a for a diagonal direction, links.cu:21) and RT = RN(1/3) (the IBB
+// momentum term, ibb.cu:48-49), whose RN(1/RT) is exactly 3.
+constexpr double RECIP6_HI = 0x1.5555555555555p-3, RECIP6_LO = 0x1.5555555555555p-57;
+constexpr double RECIP_RT_HI = 3.0, RECIP_RT_LO = 0x1.8p-53;
+
+// Lattice shape.  n = lx*ly*lz must stay below 2^31 (checked by rbtp_create).
 struct Dims {
     int lx, ly, lz, lxy;
     int n;

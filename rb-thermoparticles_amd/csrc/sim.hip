@@ -259,14 +259,10 @@ static int create_impl(const rbtp_case *c, rbtp_sim *s, hipStream_t stream) {
     s->k.c2t = 2.0 * tauc / 3.0;
     s->k.inv_tau = 1.0 / tau;
     s->k.inv_tauc = 1.0 / tauc;
-    auto dw = [](double d, double &h, double &l) {
-        h = 1.0 / d;
-        l = std::fma(-d, h, 1.0) / d;
-    };
-    dw(1.0 / 3.0, s->k.rt_h, s->k.rt_l);
-    dw(3.0, s->k.three_h, s->k.three_l);
-    dw(tau, s->k.tau_h, s->k.tau_l);
-    dw(tauc, s->k.tauc_h, s->k.tauc_l);
+    dw_recip(1.0 / 3.0, s->k.rt_h, s->k.rt_l);
+    dw_recip(3.0, s->k.three_h, s->k.three_l);
+    dw_recip(tau, s->k.tau_h, s->k.tau_l);
+    dw_recip(tauc, s->k.tauc_h, s->k.tauc_l);
     s->k.tls = (c->t_cold - c->t_hot) / (double)c->lx;
     s->k.tl0 = c->t_hot + 0.5 * s->k.tls;
     s->prec = c->precision;
@@ -801,6 +797,12 @@ extern "C" int rbtp_stencil_probe(int lx, int ly, int lz, int reps, double *gbs)
     return lattice_stencil_probe(lx, ly, lz, reps, gbs);
 }
 
+extern "C" int rbtp_arith_probe(int op, const double *a, const double *b, int64_t n,
+                                double *out) {
+    if (n < 0) return set_error(RBTP_E_ARG, "negative operand count");
+    return lattice_arith_probe(op, a, b, (size_t)n, out);
+}
+
 extern "C" int rbtp_streaming_probe(int scheme, int lx, int ly, int lz, int reps, double *gbs) {
     if (!gbs) return set_error(RBTP_E_ARG, "null argument");
     return lattice_streaming_probe(scheme, lx, ly, lz, reps, gbs);
@@ -932,6 +934,47 @@ extern "C" int rbtp_write_particles(rbtp_sim *s, const char *path) {
     return RBTP_OK;
 }
 
+// The particle loop of load_particles_from_restart (initial.cu:57-192): np lines of pid and 33
+// values in the pos_%09d.txt column order.  The reference exits on a short or unreadable line
+// and indexes its arrays with the pid as read; here both, and a pid outside [0, np), are
+// RBTP_E_IO.  Host only.
+extern "C" int rbtp_read_particle_file(const char *path, int np, double *out) {
+    if (!path || (np > 0 && !out)) return set_error(RBTP_E_ARG, "null argument");
+    if (np < 0) return set_error(RBTP_E_ARG, "negative particle count %d", np);
+    FILE *fp = fopen(path, "r");
+    if (!fp) return set_error(RBTP_E_IO, "Failed to open restart particle file %s", path);
+    std::fill(out, out + (size_t)RBTP_P_COUNT * np, 0.0);
+    for (int line = 0; line < np; line++) {
+        int pid = -1;
+        double v[33];
+        bool ok = fscanf(fp, "%d", &pid) == 1;
+        for (int c = 0; ok && c < 33; c++) ok = fscanf(fp, "%lf", &v[c]) == 1;
+        if (!ok || pid < 0 || pid >= np) {
+            fclose(fp);
+            return set_error(RBTP_E_IO, "Failed to read particle restart data from %s (line %d)",
+                             path, line + 1);
+        }
+        for (int c = 0; c < 33; c++) out[(size_t)kPosColumns[c] * np + pid] = v[c];
+    }
+    fclose(fp);
+    return RBTP_OK;
+}
+
+// generate_initial_positions (pinit.cu:97-144) alone, on the host: the positions rbtp_init
+// places when it is given none.
+extern "C" int rbtp_place_particles(const rbtp_case *c, double *out_3p) {
+    if (!c || !out_3p) return set_error(RBTP_E_ARG, "null argument");
+    rbtp_derived d;
+    rbtp_case_derive(c, &d);
+    const int np = d.npart;
+    std::vector<double> X(np), Y(np), Z(np);
+    RB_TRY(place_particles(*c, d, X, Y, Z));
+    std::copy(X.begin(), X.end(), out_3p);
+    std::copy(Y.begin(), Y.end(), out_3p + np);
+    std::copy(Z.begin(), Z.end(), out_3p + 2 * (size_t)np);
+    return RBTP_OK;
+}
+
 // load_restart_state (initial.cu:194-236).
 extern "C" int rbtp_restart_load(rbtp_sim *s, const char *dir, int step) {
     if (!s || !dir) return set_error(RBTP_E_ARG, "null argument");
@@ -960,21 +1003,8 @@ extern "C" int rbtp_restart_load(rbtp_sim *s, const char *dir, int step) {
     if (s->particles) {
         const int np = s->pd.np;
         snprintf(path, sizeof(path), "%s/pos/pos_%09d.txt", dir, step);
-        FILE *fp = fopen(path, "r");
-        if (!fp) return set_error(RBTP_E_IO, "Failed to open restart particle file %s", path);
         std::vector<double> h((size_t)RBTP_P_COUNT * np, 0.0);
-        for (int line = 0; line < np; line++) {
-            int pid = -1;
-            double v[33];
-            bool ok = fscanf(fp, "%d", &pid) == 1;
-            for (int c = 0; ok && c < 33; c++) ok = fscanf(fp, "%lf", &v[c]) == 1;
-            if (!ok || pid < 0 || pid >= np) {
-                fclose(fp);
-                return set_error(RBTP_E_IO, "Failed to read particle restart data from %s", path);
-            }
-            for (int c = 0; c < 33; c++) h[(size_t)kPosColumns[c] * np + pid] = v[c];
-        }
-        fclose(fp);
+        RB_TRY(rbtp_read_particle_file(path, np, h.data()));
         RB_TRY(part_load_state(s->pd, h.data(), s->dims, s->stream));
     }
     // the encodings restart from the conduction profile, as after rbtp_init
