@@ -9,12 +9,15 @@ namespace rbtp {
 // x / d, correctly rounded, from a double-word reciprocal yh + yl of d (yh = RN(1/d)).
 // q0 = RN(x*yh + RN(x*yl)) is within half an ulp plus 2^-100 relative of x/d, hence a faithful
 // rounding; by Markstein's theorem (yh within half an ulp of 1/d) the residual x - q0*d is exact
-// in one FMA and RN(q0 + r*yh) is the correctly rounded quotient.  Four FP64 operations replace
-// the ~10-instruction IEEE division sequence with an identical result.
+// in one FMA and RN(q0 + r*yh) is the correctly rounded quotient.  For x = +-0 the sum of signed
+// zeros can lose the quotient's sign (+0 for 0 / negative d when RN(x*yl) and x*yh differ in
+// sign), so v_div_fixup_f64 gives the result the IEEE sign (and the special cases), as the
+// compiler's own division sequence does last.  Five FP64 operations replace the ~10-instruction
+// IEEE division sequence with an identical result.
 __device__ inline double div_rn(double x, double d, double yh, double yl) {
     const double q0 = __builtin_fma(x, yh, x * yl);
     const double r = __builtin_fma(-q0, d, x);
-    return __builtin_fma(r, yh, q0);
+    return __builtin_amdgcn_div_fixup(__builtin_fma(r, yh, q0), d, x);
 }
 
 // Workgroups are dispatched round-robin over the 8 XCDs, each with its own L2.  The remapped id
