@@ -234,6 +234,12 @@ int rbtp_get_new_fluid(rbtp_sim *s, int32_t *nodes, int32_t *pids, int64_t capac
  *                        without particles); no reference counterpart.
  */
 int rbtp_nu(rbtp_sim *s, double out[7]);
+/* rbtp_nu with no device allocation.  No counterpart in the reference: it is for callers that
+ * cannot allocate (tight serving loops, harnesses that forbid hipMalloc).  Same kernels and the
+ * same result as rbtp_nu, with the partial sums in a fixed static device array of the library
+ * (4*lx + 11 doubles, lx at most 8192; RBTP_E_CAPACITY beyond) that every call on a device
+ * shares: calls that overlap on one device race. */
+int rbtp_nu_noalloc(rbtp_sim *s, double out[7]);
 int rbtp_means(rbtp_sim *s, double out[4]);
 int rbtp_kinetic_energy(rbtp_sim *s, double *out);
 /* Achievable HBM bandwidth on the current device: a streaming copy of `bytes` (read + write,

@@ -133,6 +133,7 @@ def _load():
         "rbtp_get_links": (I, [P, P, C.c_int64, C.POINTER(C.c_int64)]),
         "rbtp_get_new_fluid": (I, [P, P, P, C.c_int64, C.POINTER(C.c_int64)]),
         "rbtp_nu": (I, [P, dp]),
+        "rbtp_nu_noalloc": (I, [P, dp]),
         "rbtp_means": (I, [P, dp]),
         "rbtp_kinetic_energy": (I, [P, C.POINTER(D)]),
         "rbtp_case_from_parameters_h": (I, [C.c_char_p, C.POINTER(Case)]),
@@ -392,6 +393,13 @@ class Sim:
     def nu(self) -> np.ndarray:
         out = np.zeros(7, dtype=np.float64)
         _check(lib.rbtp_nu(self._h, out))
+        return out
+
+    def nu_noalloc(self) -> np.ndarray:
+        """rbtp_nu_noalloc: output_nu with the partial sums in the library's static device
+        scratch (no allocation)."""
+        out = np.zeros(7, dtype=np.float64)
+        _check(lib.rbtp_nu_noalloc(self._h, out))
         return out
 
     def means(self) -> np.ndarray:

@@ -6,7 +6,9 @@
 // the reference's 2 x 8N + 4N bytes per call.
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "particles.h"
@@ -262,6 +264,61 @@ __global__ __launch_bounds__(256) void k_copy_probe(const dvec2 *__restrict__ sr
 
 constexpr int RED_BLOCKS = 1024;
 
+// ---- allocation-free Nu (rbtp_nu_noalloc) -------------------------------------------------------
+//
+// The plane sums (4 per x-plane), the two wall sums and the seven results of output_nu live in a
+// file-scope device array instead of a per-call allocation.  Every call on one device shares it
+// and it keeps the last call's contents; each call writes every entry it reads before reading
+// it, so nothing needs clearing.  Calls that overlap on one device race on it.
+constexpr int NU_NOALLOC_MAX_LX = 8192;
+constexpr size_t NU_SCRATCH_DOUBLES = 4 * (size_t)NU_NOALLOC_MAX_LX + 4 + 7;
+static __device__ __align__(256) double scratch[NU_SCRATCH_DOUBLES];
+
+// The host-side combination of diag_nu (output_statis.cu:133-219) on the device, one thread,
+// same operations in the same order.
+__global__ void k_nu_combine(const double *sums, int lx, int ly, int lz, double volume_fraction,
+                             double cp_particle_ratio, double diff, double thot, double tcold,
+                             int step, double *out) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    const double fluid_fraction = fmax(0.0, 1.0 - volume_fraction);
+    double plane_avg_uT_fluid = 0.0, plane_avg_uT_part = 0.0;
+    for (int ix = 0; ix < lx; ++ix) {
+        const double sf = sums[ix * 4], sp = sums[ix * 4 + 1];
+        const double cf = sums[ix * 4 + 2], cp = sums[ix * 4 + 3];
+        if (cf > 0) plane_avg_uT_fluid += sf / cf;
+        if (cp > 0) plane_avg_uT_part += sp / cp;
+    }
+    plane_avg_uT_fluid /= static_cast<double>(lx);
+    plane_avg_uT_part /= static_cast<double>(lx);
+    const double conductive_flux = diff * (thot - tcold) / static_cast<double>(lx);
+    double Nu_bulk_fluid = fluid_fraction, Nu_bulk_part = 0.0;
+    if (conductive_flux != 0.0) {
+        Nu_bulk_fluid += fluid_fraction * (plane_avg_uT_fluid / conductive_flux);
+        Nu_bulk_part = volume_fraction * cp_particle_ratio * (plane_avg_uT_part / conductive_flux);
+    }
+    const double Nu_bulk = Nu_bulk_fluid + Nu_bulk_part;
+    const double bottom = sums[4 * lx], bottom_count = sums[4 * lx + 1];
+    const double top = sums[4 * lx + 2], top_count = sums[4 * lx + 3];
+    const double section_area = static_cast<double>(ly) * static_cast<double>(lz);
+    const double wall_scale = static_cast<double>(lx) / (thot - tcold);
+    double Tbottom = 0.0, Ttop = 0.0;
+    if (bottom_count > 0) Tbottom = (bottom / section_area) * wall_scale;
+    if (top_count > 0) Ttop = (top / section_area) * wall_scale;
+    out[0] = (double)step;
+    out[1] = volume_fraction;
+    out[2] = Nu_bulk;
+    out[3] = Ttop;
+    out[4] = Tbottom;
+    out[5] = Nu_bulk_fluid;
+    out[6] = Nu_bulk_part;
+}
+
+// Device address of `scratch` on each device, looked up once per device with that device
+// current (the array's name is a host-side stand-in, not a device address).
+constexpr int NU_MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+double *g_scratch_dev[NU_MAX_DEVICES] = {};
+
 #define DHIP(call)                                         \
     do {                                                   \
         hipError_t e_ = (call);                            \
@@ -320,6 +377,46 @@ int diag_nu(const rbtp_case &c, const rbtp_derived &d, const Dims &dims, const d
     out[4] = Tbottom;
     out[5] = Nu_bulk_fluid;
     out[6] = Nu_bulk_part;
+    return RBTP_OK;
+}
+
+// rbtp_nu without any allocation: the kernels of diag_nu on the device's scratch array, the
+// combination on the device, and the seven results copied into the caller's out.
+int diag_nu_noalloc(const rbtp_case &c, const rbtp_derived &d, const Dims &dims, const double *ux,
+                    const double *phi, const int32_t *owner, int step, hipStream_t st,
+                    double out[7]) {
+    const int LX = dims.lx;
+    if (LX > NU_NOALLOC_MAX_LX) {
+        char msg[160];
+        snprintf(msg, sizeof(msg),
+                 "rbtp_nu_noalloc: lx = %d exceeds the scratch limit of %d x-planes", LX,
+                 NU_NOALLOC_MAX_LX);
+        return report(RBTP_E_CAPACITY, msg);
+    }
+    int dev = 0;
+    DHIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= NU_MAX_DEVICES) return report(RBTP_E_ARG, "rbtp_nu_noalloc: device id out of range");
+    double *buf = nullptr;
+    {
+        std::lock_guard<std::mutex> lock(g_scratch_mu);
+        if (!g_scratch_dev[dev]) {
+            void *p = nullptr;
+            DHIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+            g_scratch_dev[dev] = (double *)p;
+        }
+        buf = g_scratch_dev[dev];
+    }
+    hipLaunchKernelGGL(k_nu_planes, LX, DB, 0, st, ux, phi, owner, dims, buf);
+    DHIP(hipGetLastError());
+    hipLaunchKernelGGL(k_nu_walls, 2, DB, 0, st, phi, owner, dims, c.t_hot, c.t_cold, buf + 4 * LX);
+    DHIP(hipGetLastError());
+    double *res = buf + 4 * (size_t)LX + 4;
+    hipLaunchKernelGGL(k_nu_combine, 1, 1, 0, st, (const double *)buf, LX, dims.ly, dims.lz,
+                       c.particle_fraction, c.cp_particle_ratio, d.diff, c.t_hot, c.t_cold, step,
+                       res);
+    DHIP(hipGetLastError());
+    DHIP(hipMemcpyAsync(out, res, 7 * sizeof(double), hipMemcpyDeviceToHost, st));
+    DHIP(hipStreamSynchronize(st));
     return RBTP_OK;
 }
 

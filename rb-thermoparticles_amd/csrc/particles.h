@@ -129,6 +129,9 @@ int diag_output_fields(int prec, const void *fc, const void *gc, const void *fx,
 int diag_profile(const double *dev_fields, const Dims &dims, hipStream_t st, double *out);
 int diag_nu(const rbtp_case &c, const rbtp_derived &d, const Dims &dims, const double *ux,
             const double *phi, const int32_t *owner, int step, hipStream_t st, double out[7]);
+int diag_nu_noalloc(const rbtp_case &c, const rbtp_derived &d, const Dims &dims, const double *ux,
+                    const double *phi, const int32_t *owner, int step, hipStream_t st,
+                    double out[7]);
 int diag_means(const Dims &dims, double *const *macro, const PartDev *pd, hipStream_t st,
                double out[4]);
 int diag_copy_probe(size_t bytes, int reps, double *gbs);

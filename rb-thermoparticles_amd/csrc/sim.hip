@@ -779,6 +779,16 @@ extern "C" int rbtp_nu(rbtp_sim *s, double out[7]) {
     return RBTP_OK;
 }
 
+extern "C" int rbtp_nu_noalloc(rbtp_sim *s, double out[7]) {
+    if (!s || !out) return set_error(RBTP_E_ARG, "null argument");
+    if (!s->macro_valid) return set_error(RBTP_E_STATE, "run the last step with RBTP_STEP_MACRO_LAST first");
+    HIP_TRY(hipSetDevice(s->device));
+    const int32_t *owner = s->particles ? s->pd.owner[s->pd.geo_cur] : nullptr;
+    RB_TRY(diag_nu_noalloc(s->c, s->d, s->dims, s->macro[1], s->macro[4], owner, s->step,
+                           s->stream, out));
+    return RBTP_OK;
+}
+
 extern "C" int rbtp_means(rbtp_sim *s, double out[4]) {
     if (!s || !out) return set_error(RBTP_E_ARG, "null argument");
     if (!s->macro_valid) return set_error(RBTP_E_STATE, "run the last step with RBTP_STEP_MACRO_LAST first");
