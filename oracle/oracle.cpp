@@ -60,8 +60,9 @@ struct OrState {
     int LX, LY, LZ, LXY;
     long N;
     double tp[NPOP];
-    // fluid (initial.cu:245-263)
-    std::vector<double> f, g, f_temp, g_temp, f_collide, g_collide;
+    // fluid (initial.cu:245-263); the streaming temporaries f_temp/g_temp are not needed (the
+    // pull goes from the f_collide copy straight into f)
+    std::vector<double> f, g, f_collide, g_collide;
     std::vector<double> rho, ux, uy, uz, phi, fx, fy, fz;
     // geometry (globals.cu:74-75)
     std::vector<int> ibnode, ibnode_prev, owner, owner_prev;
@@ -152,7 +153,7 @@ OrState *or_create(const rbtp_case *c) {
     s->N = (long)s->LXY * c->lz;
     for (int i = 0; i < NPOP; i++) s->tp[i] = weight(i);
     const long N = s->N;
-    for (auto *v : {&s->f, &s->g, &s->f_temp, &s->g_temp, &s->f_collide, &s->g_collide})
+    for (auto *v : {&s->f, &s->g, &s->f_collide, &s->g_collide})
         v->assign(NPOP * N, 0.0);
     for (auto *v : {&s->rho, &s->ux, &s->uy, &s->uz, &s->phi, &s->fx, &s->fy, &s->fz})
         v->assign(N, 0.0);

@@ -28,15 +28,26 @@
  *                         sampling of the segment on random spheres.
  *   or_build_links        brute-force enumeration on tiny grids; sphere centred on a cell centre
  *                         gives a symmetric link set; lowest pid keeps shared nodes.
- *   or_ibb_f, or_forces   a sphere at rest in fluid at rest: zero force and torque by symmetry.
+ *   or_ibb_f              Bouzidi's linear interpolation on post-collision fields linear in
+ *                         position with 27 distinct coefficients (f*_i != f*_opp), moving and
+ *                         rotating spheres: q < 1/2 is f*_i at x - (1-2q) c_i minus mt, the
+ *                         other branches their closed forms; every branch is hit, including
+ *                         upstream solid (between two spheres) and outside the domain
+ *                         (tests/test_oracle_pins.py).
+ *   or_forces             a sphere at rest in fluid at rest: zero force and torque by symmetry.
  *                         A sphere moving through fluid at rest (f* = 0): each link writes
  *                         -mt or -mt/(2q), so force and torque are closed-form sums over the
- *                         link list, which exercise both q branches (the (2q-1)/(2q) f*_opp term
- *                         is zero there).
- *   or_ibb_g, or_heat     a sphere at T_p in fluid held at the fixed point g = w T_f: closed-form
+ *                         link list, which exercise both q branches.
+ *   or_ibb_g              the same on linear g* fields with a non-uniform density deviation
+ *                         (the (1 + rho(x)) factor), per-sphere T_p, and a previous geometry that
+ *                         differs from the current one (ibnode_prev decides 'upstream fluid').
+ *   or_heat               a sphere at T_p in fluid held at the fixed point g = w T_f: closed-form
  *                         heat over the link list, both q branches including the g*_opp term.
  *   or_repulsion          two spheres and a sphere at a wall at known gaps (closed form).
  *   or_move               constant force and heat: closed-form trapezoid sequences; free fall.
+ *   new-fluid predicate   brute force over every node of a lattice where one sphere jumps 3
+ *                         nodes and another leaves a node exactly on close_thresh (radius
+ *                         1.75 + 2^-52 so that the distance minus R is exactly the literal).
  *   or_refill             a new-fluid node with no prev-fluid neighbour (a sphere that jumps 3
  *                         nodes): the fallback's moments are rho = 1 - 4|u|^2/3, momentum u/3
  *                         (e.u not divided by RT), phi = T_p (1 - 4|u|^2/3), F = 0.  Linear
@@ -44,9 +55,23 @@
  *                         average (refill.cu:121-153) has a closed form in the first-order
  *                         directions.  rho, u, phi written consistently with the populations.
  *   or_nu                 exact linear conduction profile with u=0: wall Nu = 1 and
- *                         Nu_bulk = 1 - phi_p.
+ *                         Nu_bulk = 1 - phi_p; with particle nodes, fields constant on the fluid
+ *                         and on the particle nodes of each plane give closed-form plane averages,
+ *                         bulk and wall Nu (PARTICLE_FRACTION, current flags, full-section
+ *                         wall normalisation).
+ *   or_output_fields      compute_Ked / compute_ted on equilibrium plus a known non-equilibrium
+ *                         part with a force: closed forms from the D3Q27 fourth moments;
+ *                         overwrite_particle_nodes: rigid-body velocity, T_p, RHO0, zeros.
+ *   or_means, or_profile  lattice and plane means of the overwritten fields.
  *   place_particles       non-overlap and window constraints; same glibc rand sequence as the
  *                         library (same call, same glibc).
+ *
+ * Each pin in tests/test_oracle_pins.py also asserts that the mistakes it targets (x and x - c
+ * or i and opp swapped, the (1 + rho) factor dropped, ibnode for ibnode_prev, < for <=, the old
+ * centre or the current owner in the new-fluid test, the previous flags or a fluid-count wall
+ * normalisation in Nu, the force term of compute_Ked, rho + 1 in compute_ted, the rotation in the
+ * overwrite) move the result far beyond its tolerance; planted one at a time in this file on the
+ * CPU, each of them fails its test.
  *
  * Summation order: see the header of oracle.cpp (deterministic per-particle reductions).
  *
