@@ -28,6 +28,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <map>
+#include <utility>
 #include <vector>
 #ifdef _OPENMP
 #include <omp.h>
@@ -52,6 +53,34 @@ double weight(int i) {
 
 constexpr int RED_LANES = 64;  // lane sums per particle for the link reductions
 
+// Lattice-sized arrays: an allocator that leaves new elements uninitialised, so that the arrays
+// are first touched (and zeroed) by all threads in parallel instead of page by page on one
+// thread (58 GB of populations at 512x512x256).
+template <class T>
+struct UninitAlloc : std::allocator<T> {
+    template <class U>
+    struct rebind {
+        using other = UninitAlloc<U>;
+    };
+    UninitAlloc() = default;
+    template <class U>
+    UninitAlloc(const UninitAlloc<U> &) {}
+    template <class U>
+    void construct(U *) noexcept {}
+    template <class U, class... A>
+    void construct(U *p, A &&...a) {
+        ::new ((void *)p) U(std::forward<A>(a)...);
+    }
+};
+using Field = std::vector<double, UninitAlloc<double>>;
+
+void fill_par(Field &v, size_t n, double value) {
+    v.resize(n);
+    const long m = (long)n;
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < m; i++) v[i] = value;
+}
+
 }  // namespace
 
 struct OrState {
@@ -62,8 +91,8 @@ struct OrState {
     double tp[NPOP];
     // fluid (initial.cu:245-263); the streaming temporaries f_temp/g_temp are not needed (the
     // pull goes from the f_collide copy straight into f)
-    std::vector<double> f, g, f_collide, g_collide;
-    std::vector<double> rho, ux, uy, uz, phi, fx, fy, fz;
+    Field f, g, f_collide, g_collide;
+    Field rho, ux, uy, uz, phi, fx, fy, fz;
     // geometry (globals.cu:74-75)
     std::vector<int> ibnode, ibnode_prev, owner, owner_prev;
     // particles (globals.cu:64-103): indexed by RBTP_P_*
@@ -153,10 +182,9 @@ OrState *or_create(const rbtp_case *c) {
     s->N = (long)s->LXY * c->lz;
     for (int i = 0; i < NPOP; i++) s->tp[i] = weight(i);
     const long N = s->N;
-    for (auto *v : {&s->f, &s->g, &s->f_collide, &s->g_collide})
-        v->assign(NPOP * N, 0.0);
+    for (auto *v : {&s->f, &s->g, &s->f_collide, &s->g_collide}) fill_par(*v, NPOP * N, 0.0);
     for (auto *v : {&s->rho, &s->ux, &s->uy, &s->uz, &s->phi, &s->fx, &s->fy, &s->fz})
-        v->assign(N, 0.0);
+        fill_par(*v, N, 0.0);
     s->ibnode.assign(N, 0); s->ibnode_prev.assign(N, 0);
     s->owner.assign(N, -1); s->owner_prev.assign(N, -1);
     s->np = c->activate_particles ? s->d.npart : 0;
@@ -386,7 +414,7 @@ void or_collision_g(OrState *s) {
 // of the node itself (the z/y walls come first in streaming_scalar's precedence); inside rows
 // copy with the x shift, and only their end nodes can meet the x walls.  thermal selects the
 // x-wall rule of streaming_scalar (g) over plain bounce-back (f).
-static void stream_into(OrState *s, const std::vector<double> &src, std::vector<double> &dst,
+static void stream_into(OrState *s, const Field &src, Field &dst,
                         bool thermal) {
     const long N = s->N;
     const int LX = s->LX, LY = s->LY, LZ = s->LZ, LXY = s->LXY;
@@ -428,7 +456,7 @@ static void stream_into(OrState *s, const std::vector<double> &src, std::vector<
     }
 }
 
-static void copy_par(const std::vector<double> &src, std::vector<double> &dst) {
+static void copy_par(const Field &src, Field &dst) {
     const long n = (long)src.size();
 #pragma omp parallel for schedule(static)
     for (long i = 0; i < n; i++) dst[i] = src[i];
@@ -452,6 +480,8 @@ void or_stream_g(OrState *s) {
 static void init_fluid(OrState *s, const double *temperature) {
     const long N = s->N;
     const double t0 = s->c.t_ref, grav0 = s->d.grav0, beta = s->c.beta, RHO0 = s->c.rho0;
+    // every node is independent: the loops run in parallel
+#pragma omp parallel for schedule(static)
     for (long idx = 0; idx < N; idx++) {
         // init_f
         const double r = 0.0, u = 0.0, v = 0.0, w = 0.0;
@@ -465,6 +495,7 @@ static void init_fluid(OrState *s, const double *temperature) {
             s->f[ip * N + idx] = feq;
         }
     }
+#pragma omp parallel for schedule(static)
     for (long idx = 0; idx < N; idx++) {
         // init_g
         const double r = s->rho[idx];
@@ -559,37 +590,52 @@ void or_build_links(OrState *s) {
                     s->ibnode[idx] = 1;
                 }
     }
-    // link_count_kernel + ExclusiveSum + link_fill_kernel: one ordered pass gives the same list.
+    // link_count_kernel + ExclusiveSum + link_fill_kernel: the list in (idx, dir) order.  Each
+    // thread lists the links of one contiguous range of nodes, and the ranges are joined in
+    // order, which is what the scan's offsets do.
     s->links.clear();
-    for (long idx = 0; idx < N; idx++) {
-        if (s->ibnode[idx] == 1) continue;
-        const int iz = (int)(idx / LXY);
-        const int rem = (int)(idx - (long)iz * LXY);
-        const int iy = rem / LX;
-        const int ix = rem - iy * LX;
-        const double x0 = static_cast<double>(ix) + 0.5;
-        const double y0 = static_cast<double>(iy) + 0.5;
-        const double z0 = static_cast<double>(iz) + 0.5;
-        for (int ip = 1; ip < NPOP; ++ip) {
-            const int nx = ix + CIX[ip], ny = iy + CIY[ip], nz = iz + CIZ[ip];
-            if (nx < 0 || nx >= LX || ny < 0 || ny >= LY || nz < 0 || nz >= LZ) continue;
-            const long nidx = (long)nz * LXY + (long)ny * LX + nx;
-            const int pid = s->owner[nidx];
-            if (pid < 0) continue;
-            double q = 0.0;
-            if (!compute_link_q(ip, x0, y0, z0, PX[pid], PY[pid], PZ[pid], radius_sq, q)) continue;
-            rbtp_link L;
-            L.cell_i = ix; L.cell_j = iy; L.cell_k = iz; L.dir = ip; L.pid = pid; L.pad = 0;
-            L.q = q;
-            L.rx = q * static_cast<double>(CIX[ip]);
-            L.ry = q * static_cast<double>(CIY[ip]);
-            L.rz = q * static_cast<double>(CIZ[ip]);
-            s->links.push_back(L);
+    const int nchunk = 256;
+    std::vector<std::vector<rbtp_link>> part(nchunk);
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int ch = 0; ch < nchunk; ch++) {
+        const long idx_lo = N * ch / nchunk, idx_hi = N * (ch + 1) / nchunk;
+        std::vector<rbtp_link> &out = part[ch];
+        for (long idx = idx_lo; idx < idx_hi; idx++) {
+            if (s->ibnode[idx] == 1) continue;
+            const int iz = (int)(idx / LXY);
+            const int rem = (int)(idx - (long)iz * LXY);
+            const int iy = rem / LX;
+            const int ix = rem - iy * LX;
+            const double x0 = static_cast<double>(ix) + 0.5;
+            const double y0 = static_cast<double>(iy) + 0.5;
+            const double z0 = static_cast<double>(iz) + 0.5;
+            for (int ip = 1; ip < NPOP; ++ip) {
+                const int nx = ix + CIX[ip], ny = iy + CIY[ip], nz = iz + CIZ[ip];
+                if (nx < 0 || nx >= LX || ny < 0 || ny >= LY || nz < 0 || nz >= LZ) continue;
+                const long nidx = (long)nz * LXY + (long)ny * LX + nx;
+                const int pid = s->owner[nidx];
+                if (pid < 0) continue;
+                double q = 0.0;
+                if (!compute_link_q(ip, x0, y0, z0, PX[pid], PY[pid], PZ[pid], radius_sq, q)) continue;
+                rbtp_link L;
+                L.cell_i = ix; L.cell_j = iy; L.cell_k = iz; L.dir = ip; L.pid = pid; L.pad = 0;
+                L.q = q;
+                L.rx = q * static_cast<double>(CIX[ip]);
+                L.ry = q * static_cast<double>(CIY[ip]);
+                L.rz = q * static_cast<double>(CIZ[ip]);
+                out.push_back(L);
+            }
         }
     }
+    for (const auto &v : part) s->links.insert(s->links.end(), v.begin(), v.end());
 }
 
-// build_initial_ibnode (pinit.cu:146-179), sequential over nodes and pids.
+// build_initial_ibnode (pinit.cu:146-179).  The reference walks every node and gives it to the
+// first pid (in increasing order) whose sphere contains it, O(N P).  Walking the pids in
+// increasing order over their marking boxes floor(p -/+ R - 0.5) (links.cu:164-177) and keeping
+// a node's first owner gives the same map: a node centre i + 1/2 within R of p lies in
+// [p - R, p + R], so i lies in [p - R - 1/2, p + R - 1/2] and inside the box.  The distance test
+// is the reference's expression.
 static void initial_ibnode(OrState *s) {
     const long N = s->N;
     const int LX = s->LX, LXY = s->LXY;
@@ -599,23 +645,24 @@ static void initial_ibnode(OrState *s) {
         s->ibnode[idx] = 0;
         s->owner[idx] = -1;
     }
-    for (long idx = 0; idx < N; ++idx) {
-        const int iz = (int)(idx / LXY);
-        const int iy = (int)((idx - (long)iz * LXY) / LX);
-        const int ix = (int)(idx - (long)iz * LXY - (long)iy * LX);
-        const double cx = static_cast<double>(ix) + 0.5;
-        const double cy = static_cast<double>(iy) + 0.5;
-        const double cz = static_cast<double>(iz) + 0.5;
-        for (int p = 0; p < s->np; ++p) {
-            const double dx = cx - s->p[RBTP_P_POS_X][p];
-            const double dy = cy - s->p[RBTP_P_POS_Y][p];
-            const double dz = cz - s->p[RBTP_P_POS_Z][p];
-            if (dx * dx + dy * dy + dz * dz <= radius_sq) {
-                s->ibnode[idx] = 1;
-                s->owner[idx] = p;
-                break;
-            }
-        }
+    for (int p = 0; p < s->np; ++p) {
+        const double px = s->p[RBTP_P_POS_X][p], py = s->p[RBTP_P_POS_Y][p],
+                     pz = s->p[RBTP_P_POS_Z][p];
+        int lo[3], hi[3];
+        if (!mark_box(s, px, py, pz, lo, hi)) continue;
+        for (int iz = lo[2]; iz <= hi[2]; iz++)
+            for (int iy = lo[1]; iy <= hi[1]; iy++)
+                for (int ix = lo[0]; ix <= hi[0]; ix++) {
+                    const long idx = (long)iz * LXY + (long)iy * LX + ix;
+                    if (s->owner[idx] != -1) continue;
+                    const double dx = (static_cast<double>(ix) + 0.5) - px;
+                    const double dy = (static_cast<double>(iy) + 0.5) - py;
+                    const double dz = (static_cast<double>(iz) + 0.5) - pz;
+                    if (dx * dx + dy * dy + dz * dz <= radius_sq) {
+                        s->ibnode[idx] = 1;
+                        s->owner[idx] = p;
+                    }
+                }
     }
     s->ibnode_prev = s->ibnode;
     s->owner_prev = s->owner;
@@ -1150,8 +1197,8 @@ int or_init(OrState *s, const double *temperature, const double *ppos, const dou
         initial_ibnode(s);
     }
     init_fluid(s, temperature);
-    s->f_collide = s->f;
-    s->g_collide = s->g;
+    copy_par(s->f, s->f_collide);
+    copy_par(s->g, s->g_collide);
     if (s->particles()) or_build_links(s);
     s->step = 0;
     return 0;

@@ -192,8 +192,13 @@ class Oracle:
         p = lib.or_array(self._h, which)
         return np.ctypeslib.as_array(p, shape=(count,)).copy()
 
-    def populations(self, name: str) -> np.ndarray:
+    def populations(self, name: str, copy: bool = True) -> np.ndarray:
+        """27 x n populations; copy=False gives a view of the oracle's own memory (valid while
+        the oracle lives and until its next stage), for lattices too large to copy."""
         which = {"f": 0, "g": 1, "f_collide": 2, "g_collide": 3}[name]
+        if not copy:
+            p = lib.or_array(self._h, which)
+            return np.ctypeslib.as_array(p, shape=(27 * self.n,)).reshape(27, self.n)
         return self._view(which, 27 * self.n).reshape(27, self.n)
 
     def field(self, name: str) -> np.ndarray:
