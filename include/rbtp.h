@@ -144,6 +144,37 @@ int rbtp_synchronize(rbtp_sim *s);
 int rbtp_current_step(const rbtp_sim *s, int *step);
 
 /*
+ * One stage of the time step at a time, as main.cu:35-84 launches them (declared at
+ * include/lbm.h:59-78 and include/particle.h:110-121), for callers that drive the stages
+ * themselves and for localising a divergence to one stage.  FP64 form only (RBTP_E_STATE for the
+ * reduced-precision forms and for serving-batch slots).  The stages must be called in main.cu
+ * order, starting at a step boundary (RBTP_E_STATE otherwise); each runs as its own kernels on
+ * the reference's arrays (d_f, d_f_collide, d_g, d_g_collide, rho/u/phi, F_x, ibnode/owner and
+ * their previous generation, the particle arrays), which the accessors then return.  After
+ * RBTP_STAGE_REFILL the step counter advances.  rbtp_step continues from the staged state once
+ * the staged step is complete.  The particle stages do nothing without particles.
+ */
+#define RBTP_STAGE_MACROVAR 0     /* macrovar (macrovar.cu:3-34) */
+#define RBTP_STAGE_FORCING 1      /* forcing (macrovar.cu:36-47) */
+#define RBTP_STAGE_COLLIDE_F 2    /* collision_BGK (collision.cu:3-33) */
+#define RBTP_STAGE_STREAM_F 3     /* d_f_collide <- d_f, streaming (streaming.cu:3-23), main.cu:45-48 */
+#define RBTP_STAGE_IBB_F 4        /* bounce_back_particles (ibb.cu:5-82, 209-221) */
+#define RBTP_STAGE_COLLIDE_G 5    /* collision_BGK_scalar (collision.cu:35-89) */
+#define RBTP_STAGE_STREAM_G 6     /* d_g_collide <- d_g, streaming_scalar (streaming.cu:25-63) */
+#define RBTP_STAGE_IBB_G 7        /* bounce_back_thermal_particles (ibb.cu:84-136, 223-235) */
+#define RBTP_STAGE_PREPARE 8      /* prepare_particle_forces_step (pforces.cu:80-116) */
+#define RBTP_STAGE_REPULSION 9    /* apply_repulsive_forces (pforces.cu:6-76, 118-148) */
+#define RBTP_STAGE_FORCES 10      /* compute_particle_forces (ibb.cu:138-188, 237-256) */
+#define RBTP_STAGE_HEAT 11        /* compute_particle_heat (ibb.cu:190-205, 258-271) */
+#define RBTP_STAGE_MOVE 12        /* update_particles (move.cu:5-103) */
+#define RBTP_STAGE_BUILD_LINKS 13 /* build_links (links.cu:143-256) */
+#define RBTP_STAGE_REFILL 14      /* refill_nodes (refill.cu:244-329) */
+#define RBTP_STAGE_COUNT 15
+int rbtp_stage(rbtp_sim *s, int stage);
+/* The stage the next rbtp_stage call must run (0 at a step boundary). */
+int rbtp_next_stage(const rbtp_sim *s, int *stage);
+
+/*
  * Parity accessors (reference semantics at a step boundary).  Each copies to caller-owned host
  * memory.  Fields: which = RBTP_FIELD_*.  Populations: which = RBTP_POP_* (27*N doubles).
  */

@@ -26,6 +26,9 @@ STEP_MACRO_LAST = 1
 
 FIELD = {"rho": 0, "ux": 1, "uy": 2, "uz": 3, "phi": 4, "fx": 5, "fy": 6, "fz": 7}
 POP = {"f": 0, "g": 1, "f_collide": 2, "g_collide": 3}
+# RBTP_STAGE_*: the stages of one time step in main.cu order (main.cu:35-84).
+STAGES = ["macrovar", "forcing", "collide_f", "stream_f", "ibb_f", "collide_g", "stream_g",
+          "ibb_g", "prepare", "repulsion", "forces", "heat", "move", "build_links", "refill"]
 PARTICLE_ARRAYS = [
     "pos_x", "pos_y", "pos_z", "vel_x", "vel_y", "vel_z", "omega_x", "omega_y", "omega_z",
     "theta_x", "theta_y", "theta_z", "force_x", "force_y", "force_z", "torque_x", "torque_y",
@@ -124,6 +127,8 @@ def _load():
         "rbtp_step_async": (I, [P, I, U]),
         "rbtp_synchronize": (I, [P]),
         "rbtp_current_step": (I, [P, C.POINTER(I)]),
+        "rbtp_stage": (I, [P, I]),
+        "rbtp_next_stage": (I, [P, C.POINTER(I)]),
         "rbtp_get_field": (I, [P, I, dp]),
         "rbtp_get_populations": (I, [P, I, dp]),
         "rbtp_set_populations": (I, [P, dp, dp]),
@@ -324,6 +329,16 @@ class Sim:
 
     def synchronize(self) -> None:
         _check(lib.rbtp_synchronize(self._h))
+
+    def stage(self, name: str) -> None:
+        """Run one stage of the time step (rbtp_stage), in main.cu order; FP64 form only."""
+        _check(lib.rbtp_stage(self._h, STAGES.index(name)))
+
+    @property
+    def next_stage(self) -> str:
+        v = C.c_int()
+        _check(lib.rbtp_next_stage(self._h, C.byref(v)))
+        return STAGES[v.value]
 
     @property
     def current_step(self) -> int:

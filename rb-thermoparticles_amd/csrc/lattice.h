@@ -30,6 +30,23 @@ hipError_t launch_init(S *f, S *g, typename Enc<S>::A *fx, typename Enc<S>::A *r
                        double beta, const float *gshift, Dims d, double *mr, double *mux,
                        double *muy, double *muz, double *mphi, hipStream_t st);
 
+// Arrays of the per-stage execution (rbtp_stage): FP64, populations in the blocked layout of
+// pop_at.  f/g are d_f/d_g (post-stream, bounced back, refilled), fc/gc d_f_collide/d_g_collide.
+struct StageFluid {
+    double *f, *fc, *g, *gc;
+    double *rho, *ux, *uy, *uz, *phi;
+    double *fx;
+    const double *rho_wall;  // wall rho of the state the staged mode starts from
+    Dims d;
+    FluidK k;
+};
+// One fluid stage (RBTP_STAGE_MACROVAR ... RBTP_STAGE_STREAM_G, not the IBB stages).
+hipError_t launch_stage_fluid(int stage, const StageFluid &a, hipStream_t st);
+// Start of the staged mode from the fused state: fc/gc hold the post-collision populations of
+// the last step (or, when poststream, the post-stream ones); f/g receive the post-stream
+// populations (pull with the wall rules, or a copy).
+hipError_t launch_stage_enter(const StageFluid &a, bool poststream, hipStream_t st);
+
 // rbtp_arith_probe: op RBTP_ARITH_* over n host operand pairs, result into host out.
 int lattice_arith_probe(int op, const double *a, const double *b, size_t n, double *out);
 

@@ -38,9 +38,80 @@ template <class S> struct NodeOut {
     S *gout;
 };
 
+// Pieces of the strict FP64 collisions, shared by the fused kernel and the per-stage kernels.
+// Every rounding is the one of the reference's expression (divisions via div_rn, which returns
+// the IEEE quotient).
+//   eu_i = (c_i . u) / RT and h_i = (eu_i^2 - u.u/RT)/2      (collision.cu:20-22, 64-66)
+__device__ inline void strict_eq(int i, double ux, double uy, double uz, double uv,
+                                 const FluidK &k, double &eu, double &h) {
+    const double RT = 1.0 / 3.0;
+    eu = (i == 0) ? 0.0 : div_rn(cdot<double>(i, ux, uy, uz), RT, k.rt_h, k.rt_l);
+    h = 0.5 * (eu * eu - uv);
+}
+// collision_BGK for one population (collision.cu:16-32); X = (1 + eu) + h, which g's
+// equilibrium uses too.
+__device__ inline double strict_f_new(int i, double fpop, double rho, double ux, double eu,
+                                      double h, double fx, const FluidK &k, double &X) {
+    const double RT = 1.0 / 3.0;
+    const double tp = weight(i);
+    const double feq = tp * ((rho + eu) + h);
+    double fchange = div_rn(-(fpop - feq), k.tau, k.tau_h, k.tau_l);
+    X = (1.0 + eu) + h;
+    const double feq_rho0 = tp * X;
+    const double cuF = ((double)D3Q27::cx[i] - ux) * fx;
+    fchange += div_rn(k.force_coeff * cuF, RT, k.rt_h, k.rt_l) * feq_rho0;
+    return fpop + fchange;
+}
+// geqnew of collision_BGK_scalar (collision.cu:53-56) from X.
+__device__ inline double strict_geqnew(int i, double phi, double rho, double X, const FluidK &k) {
+    const double tp = weight(i);
+    const double geq = tp * phi * X;
+    return (i == 0) ? (geq - k.lambda0 * phi * rho) : (geq + tp * phi * rho);
+}
+// Pass 1 (collision.cu:57-70): G += c_i (g_i - geqnew_i), terms with c = 0 skipped.
+__device__ inline void strict_g_moment(int i, double diff, double &gx, double &gy, double &gz) {
+    if (D3Q27::cx[i] == 1) gx += diff;
+    if (D3Q27::cx[i] == -1) gx -= diff;
+    if (D3Q27::cy[i] == 1) gy += diff;
+    if (D3Q27::cy[i] == -1) gy -= diff;
+    if (D3Q27::cz[i] == 1) gz += diff;
+    if (D3Q27::cz[i] == -1) gz -= diff;
+}
+// Pass 2 (collision.cu:72-87) for one population; gd = grad_denom with its reciprocal pair.
+__device__ inline double strict_g_new(int i, double gpop, double geqnew, double gx, double gy,
+                                      double gz, double fx, double phi, double rho, double gd,
+                                      double gd_h, double gd_l, const FluidK &k) {
+    const double tp = weight(i);
+    double gchange;
+    if (i == 0) {
+        gchange = div_rn(-(gpop - geqnew), k.tauc, k.tauc_h, k.tauc_l);
+    } else {
+        const int cx = D3Q27::cx[i];
+        const double gneq = cdot<double>(i, gx, gy, gz);
+        double gradt, src;
+        if (cx != 0) {
+            const double eb = cx > 0 ? fx : -fx;
+            gradt = div_rn(-(2.0 * gneq + eb * phi), gd, gd_h, gd_l);
+            src = gradt * rho + 3.0 * phi * eb;
+        } else {
+            gradt = div_rn(-(2.0 * gneq), gd, gd_h, gd_l);
+            src = gradt * rho;
+        }
+        const double Ralph = tp * k.ralph_coeff * src;
+        gchange = div_rn(-(gpop - geqnew), k.tauc, k.tauc_h, k.tauc_l) + Ralph;
+    }
+    return gpop + gchange;
+}
+// grad_denom = rho/3 + 2 tauc/3 (collision.cu:73) and its double-word reciprocal.
+__device__ inline double strict_grad_denom(double rho, const FluidK &k, double &h, double &l) {
+    const double gd = div_rn(rho, 3.0, k.three_h, k.three_l) + k.c2t;
+    h = 1.0 / gd;
+    l = __builtin_fma(-gd, h, 1.0) * h;
+    return gd;
+}
+
 // Strict FP64 macrovar + forcing + collision_BGK + collision_BGK_scalar for one node.
-// Returns the new F_x; writes the post-collision populations.  Every rounding is the one of
-// the reference's expression (divisions via div_rn, which returns the IEEE quotient).
+// Returns the new F_x; writes the post-collision populations.
 template <class S>
 __device__ inline double collide_strict(double (&f)[NPOP], double (&g)[NPOP], double fx_old,
                                         int idx, const NodeOut<S> &o, const FluidK &k,
@@ -53,65 +124,24 @@ __device__ inline double collide_strict(double (&f)[NPOP], double (&g)[NPOP], do
     const double u2 = ux * ux + uy * uy + uz * uz;
     const double uv = div_rn(u2, RT, k.rt_h, k.rt_l);
     double X[NPOP];  // (1 + eu) + 0.5 (eu^2 - uv), shared by f's Guo term and g's equilibrium
-    // collision_BGK (collision.cu:16-32)
 #pragma unroll
     for (int i = 0; i < NPOP; i++) {
-        const double tp = weight(i);
-        const double eu = (i == 0) ? 0.0 : div_rn(cdot<double>(i, ux, uy, uz), RT, k.rt_h, k.rt_l);
-        const double h = 0.5 * (eu * eu - uv);
-        const double feq = tp * ((rho + eu) + h);
-        const double fpop = f[i];
-        double fchange = div_rn(-(fpop - feq), k.tau, k.tau_h, k.tau_l);
-        X[i] = (1.0 + eu) + h;
-        const double feq_rho0 = tp * X[i];
-        const double cuF = ((double)D3Q27::cx[i] - ux) * fx;
-        fchange += div_rn(k.force_coeff * cuF, RT, k.rt_h, k.rt_l) * feq_rho0;
-        Enc<S>::st(o.fout + pop_at(i, idx), fpop + fchange);
+        double eu, h;
+        strict_eq(i, ux, uy, uz, uv, k, eu, h);
+        Enc<S>::st(o.fout + pop_at(i, idx), strict_f_new(i, f[i], rho, ux, eu, h, fx, k, X[i]));
     }
-    // collision_BGK_scalar pass 1 (collision.cu:57-70)
     double gx = 0.0, gy = 0.0, gz = 0.0;
 #pragma unroll
     for (int i = 0; i < NPOP; i++) {
-        const double tp = weight(i);
-        const double geq = tp * phi * X[i];
-        const double geqnew = (i == 0) ? (geq - k.lambda0 * phi * rho) : (geq + tp * phi * rho);
-        X[i] = geqnew;
-        const double diff = g[i] - geqnew;
-        if (D3Q27::cx[i] == 1) gx += diff;
-        if (D3Q27::cx[i] == -1) gx -= diff;
-        if (D3Q27::cy[i] == 1) gy += diff;
-        if (D3Q27::cy[i] == -1) gy -= diff;
-        if (D3Q27::cz[i] == 1) gz += diff;
-        if (D3Q27::cz[i] == -1) gz -= diff;
+        X[i] = strict_geqnew(i, phi, rho, X[i], k);
+        strict_g_moment(i, g[i] - X[i], gx, gy, gz);
     }
-    // pass 2 (collision.cu:72-87)
-    const double grad_denom = div_rn(rho, 3.0, k.three_h, k.three_l) + k.c2t;
-    const double gd_h = 1.0 / grad_denom;
-    const double gd_l = __builtin_fma(-grad_denom, gd_h, 1.0) * gd_h;
+    double gd_h, gd_l;
+    const double gd = strict_grad_denom(rho, k, gd_h, gd_l);
 #pragma unroll
-    for (int i = 0; i < NPOP; i++) {
-        const double tp = weight(i);
-        const double gpop = g[i];
-        double gchange;
-        if (i == 0) {
-            gchange = div_rn(-(gpop - X[i]), k.tauc, k.tauc_h, k.tauc_l);
-        } else {
-            const int cx = D3Q27::cx[i];
-            const double gneq = cdot<double>(i, gx, gy, gz);
-            double gradt, src;
-            if (cx != 0) {
-                const double eb = cx > 0 ? fx : -fx;
-                gradt = div_rn(-(2.0 * gneq + eb * phi), grad_denom, gd_h, gd_l);
-                src = gradt * rho + 3.0 * phi * eb;
-            } else {
-                gradt = div_rn(-(2.0 * gneq), grad_denom, gd_h, gd_l);
-                src = gradt * rho;
-            }
-            const double Ralph = tp * k.ralph_coeff * src;
-            gchange = div_rn(-(gpop - X[i]), k.tauc, k.tauc_h, k.tauc_l) + Ralph;
-        }
-        Enc<S>::st(o.gout + pop_at(i, idx), gpop + gchange);
-    }
+    for (int i = 0; i < NPOP; i++)
+        Enc<S>::st(o.gout + pop_at(i, idx),
+                   strict_g_new(i, g[i], X[i], gx, gy, gz, fx, phi, rho, gd, gd_h, gd_l, k));
     return fx;
 }
 
@@ -479,6 +509,125 @@ __global__ __launch_bounds__(BLOCK) void k_shift_reencode(S *__restrict__ g, Dim
     }
 }
 
+// ---- per-stage execution (rbtp_stage) ---------------------------------------------------------
+//
+// One reference stage per launch on the FP64 arrays of StageFluid, with the strict form's
+// arithmetic (the functions above, shared with the fused kernel).
+
+// macrovar (macrovar.cu:3-34): moments of the post-stream populations and the previous F_x.
+__global__ __launch_bounds__(BLOCK) void k_st_macrovar(const StageFluid a) {
+    const int idx = blockIdx.x * BLOCK + threadIdx.x;
+    if (idx >= a.d.n) return;
+    double f[NPOP], g[NPOP];
+#pragma unroll
+    for (int i = 0; i < NPOP; i++) {
+        f[i] = a.f[pop_at(i, idx)];
+        g[i] = a.g[pop_at(i, idx)];
+    }
+    double rho, ux, uy, uz, phi;
+    moments<double>(f, g, rho, ux, uy, uz, phi);
+    a.rho[idx] = rho;
+    a.ux[idx] = ux + 0.5 * a.fx[idx];
+    a.uy[idx] = uy;
+    a.uz[idx] = uz;
+    a.phi[idx] = phi;
+}
+
+// forcing (macrovar.cu:36-47).
+__global__ __launch_bounds__(BLOCK) void k_st_forcing(const StageFluid a) {
+    const int idx = blockIdx.x * BLOCK + threadIdx.x;
+    if (idx >= a.d.n) return;
+    a.fx[idx] = a.k.gb * (a.phi[idx] - a.k.t0);
+}
+
+// collision_BGK (collision.cu:3-33) in place on d_f.
+__global__ __launch_bounds__(BLOCK) void k_st_collide_f(const StageFluid a) {
+    const int idx = blockIdx.x * BLOCK + threadIdx.x;
+    if (idx >= a.d.n) return;
+    const double rho = a.rho[idx], ux = a.ux[idx], uy = a.uy[idx], uz = a.uz[idx];
+    const double fx = a.fx[idx];
+    const double RT = 1.0 / 3.0;
+    const double uv = div_rn(ux * ux + uy * uy + uz * uz, RT, a.k.rt_h, a.k.rt_l);
+#pragma unroll
+    for (int i = 0; i < NPOP; i++) {
+        double eu, h, X;
+        strict_eq(i, ux, uy, uz, uv, a.k, eu, h);
+        double *p = a.f + pop_at(i, idx);
+        *p = strict_f_new(i, *p, rho, ux, eu, h, fx, a.k, X);
+    }
+}
+
+// collision_BGK_scalar (collision.cu:35-89) in place on d_g.
+__global__ __launch_bounds__(BLOCK) void k_st_collide_g(const StageFluid a) {
+    const int idx = blockIdx.x * BLOCK + threadIdx.x;
+    if (idx >= a.d.n) return;
+    const double rho = a.rho[idx], ux = a.ux[idx], uy = a.uy[idx], uz = a.uz[idx];
+    const double phi = a.phi[idx], fx = a.fx[idx];
+    const double RT = 1.0 / 3.0;
+    const double uv = div_rn(ux * ux + uy * uy + uz * uz, RT, a.k.rt_h, a.k.rt_l);
+    double g[NPOP], geqnew[NPOP];
+    double gx = 0.0, gy = 0.0, gz = 0.0;
+#pragma unroll
+    for (int i = 0; i < NPOP; i++) {
+        double eu, h;
+        strict_eq(i, ux, uy, uz, uv, a.k, eu, h);
+        g[i] = a.g[pop_at(i, idx)];
+        geqnew[i] = strict_geqnew(i, phi, rho, (1.0 + eu) + h, a.k);
+        strict_g_moment(i, g[i] - geqnew[i], gx, gy, gz);
+    }
+    double gd_h, gd_l;
+    const double gd = strict_grad_denom(rho, a.k, gd_h, gd_l);
+#pragma unroll
+    for (int i = 0; i < NPOP; i++)
+        a.g[pop_at(i, idx)] =
+            strict_g_new(i, g[i], geqnew[i], gx, gy, gz, fx, phi, rho, gd, gd_h, gd_l, a.k);
+}
+
+__global__ __launch_bounds__(BLOCK) void k_st_copy(const double *__restrict__ src,
+                                                   double *__restrict__ dst, size_t len) {
+    const size_t j = (size_t)blockIdx.x * BLOCK + threadIdx.x;
+    if (j < len) dst[j] = src[j];
+}
+
+// streaming (streaming.cu:3-23, thermal = 0) and streaming_scalar (streaming.cu:25-63,
+// thermal = 1): dst_i(x) = src_i(x - c_i), or the wall rule where x - c_i is outside (z, then y
+// walls reflect; the x walls of g apply -src_opp(x) + 2 w_i T_wall (1 + rho(x)), with rho from
+// the field of this step's macrovar, or from the wall rho of the state the staged mode started
+// from).
+__global__ __launch_bounds__(BLOCK) void k_st_pull(const double *__restrict__ src,
+                                                   double *__restrict__ dst,
+                                                   const double *__restrict__ rho_node,
+                                                   const double *__restrict__ rho_wall,
+                                                   int thermal, Dims d, double thot,
+                                                   double tcold) {
+    const int idx = blockIdx.x * BLOCK + threadIdx.x;
+    if (idx >= d.n) return;
+    const int iz = idx / d.lxy;
+    const int rem = idx - iz * d.lxy;
+    const int iy = rem / d.lx;
+    const int ix = rem - iy * d.lx;
+    double rw = 0.0;
+    if (ix == 0 || ix == d.lx - 1) {
+        if (rho_node) rw = rho_node[idx];
+        else rw = rho_wall[(ix == 0 ? 0 : d.ly * d.lz) + iz * d.ly + iy];
+    }
+#pragma unroll
+    for (int i = 0; i < NPOP; i++) {
+        const int cx = D3Q27::cx[i], cy = D3Q27::cy[i], cz = D3Q27::cz[i];
+        const bool outx = (cx == 1 && ix == 0) || (cx == -1 && ix == d.lx - 1);
+        const bool outy = (cy == 1 && iy == 0) || (cy == -1 && iy == d.ly - 1);
+        const bool outz = (cz == 1 && iz == 0) || (cz == -1 && iz == d.lz - 1);
+        const bool inside = !(outx || outy || outz);
+        const int off = cx + cy * d.lx + cz * d.lxy;
+        double v = inside ? src[pop_at(i, idx - off)] : src[pop_at(D3Q27::opp[i], idx)];
+        if (thermal && outx && !outy && !outz) {
+            const double twall = (cx == 1) ? thot : tcold;
+            v = -v + 2.0 * weight(i) * twall * (1.0 + rw);
+        }
+        dst[pop_at(i, idx)] = v;
+    }
+}
+
 // The FP64 lattice kernel's memory traffic without its arithmetic: the same launch shape, the
 // same pull of 54 populations under the wall rules (gather), the same F_x read and write, and
 // the 54 values stored unchanged into the other buffer.  Its time is the ceiling the access
@@ -699,6 +848,59 @@ hipError_t launch_lattice_step(const StepArgs<S> &a, bool strict, bool nostream,
         else hipLaunchKernelGGL((k_lattice_step<S, float, false, false>), nb, BLOCK, 0, st, a);
     }
     return hipGetLastError();
+}
+
+hipError_t launch_stage_fluid(int stage, const StageFluid &a, hipStream_t st) {
+    const int nb = blocks_for((size_t)a.d.n);
+    const size_t len = pop_buffer_len((size_t)a.d.n);
+    const int nbl = blocks_for(len);
+    switch (stage) {
+        case RBTP_STAGE_MACROVAR:
+            hipLaunchKernelGGL(k_st_macrovar, nb, BLOCK, 0, st, a);
+            break;
+        case RBTP_STAGE_FORCING:
+            hipLaunchKernelGGL(k_st_forcing, nb, BLOCK, 0, st, a);
+            break;
+        case RBTP_STAGE_COLLIDE_F:
+            hipLaunchKernelGGL(k_st_collide_f, nb, BLOCK, 0, st, a);
+            break;
+        case RBTP_STAGE_COLLIDE_G:
+            hipLaunchKernelGGL(k_st_collide_g, nb, BLOCK, 0, st, a);
+            break;
+        case RBTP_STAGE_STREAM_F:
+        case RBTP_STAGE_STREAM_G: {
+            // main.cu:45,61 (the pre-stream copy), then the pull (main.cu:47-48,63-64)
+            const bool g = stage == RBTP_STAGE_STREAM_G;
+            hipLaunchKernelGGL(k_st_copy, nbl, BLOCK, 0, st, (const double *)(g ? a.g : a.f),
+                               g ? a.gc : a.fc, len);
+            hipError_t e = hipGetLastError();
+            if (e != hipSuccess) return e;
+            hipLaunchKernelGGL(k_st_pull, nb, BLOCK, 0, st, (const double *)(g ? a.gc : a.fc),
+                               g ? a.g : a.f, (const double *)a.rho, (const double *)nullptr,
+                               g ? 1 : 0, a.d, a.k.thot, a.k.tcold);
+            break;
+        }
+        default:
+            return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_stage_enter(const StageFluid &a, bool poststream, hipStream_t st) {
+    const int nb = blocks_for((size_t)a.d.n);
+    const size_t len = pop_buffer_len((size_t)a.d.n);
+    for (int which = 0; which < 2; which++) {
+        const double *src = which ? a.gc : a.fc;
+        double *dst = which ? a.g : a.f;
+        if (poststream)
+            hipLaunchKernelGGL(k_st_copy, blocks_for(len), BLOCK, 0, st, src, dst, len);
+        else
+            hipLaunchKernelGGL(k_st_pull, nb, BLOCK, 0, st, src, dst, (const double *)nullptr,
+                               a.rho_wall, which, a.d, a.k.thot, a.k.tcold);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
 }
 
 template <class S>

@@ -111,9 +111,20 @@ int part_fixup_blocks(const PartDev &pd, int per_block);
 // form prec.  The caller then advances each slot's geo_cur and sets special_pending.
 int part_phase(const PartGrid &g, int prec, hipStream_t st);
 int part_check_errors(PartDev &pd);
+// One particle stage of the per-stage execution (RBTP_STAGE_IBB_F, IBB_G and PREPARE ...
+// REFILL) over the slots of g, whose fcur/gcur are d_f_collide/d_g_collide and whose geometry
+// generations are those of the step's links (owner_prev) and of build_links (owner_curr);
+// fo/go are d_f/d_g (FP64, blocked layout).
+int part_stage(int stage, const PartGrid &g, double *fo, double *go, hipStream_t st);
+// part_overwrite_special into populations in the blocked layout of pop_at.
+hipError_t part_overwrite_special_blocked(PartDev &pd, double *f, double *g, const Dims &dims,
+                                          hipStream_t st);
 hipError_t part_overwrite_special(PartDev &pd, double *f, double *g, const Dims &dims,
                                   hipStream_t st);
-int part_collect_links(PartDev &pd, const Dims &dims, hipStream_t st, std::vector<rbtp_link> &out);
+// Link records of geometry generation gen (pd.owner[gen]) in the reference layout, from the
+// current positions or (snapshot) from those before the last update_particles.
+int part_collect_links(PartDev &pd, int gen, bool snapshot, const Dims &dims, hipStream_t st,
+                       std::vector<rbtp_link> &out);
 int part_collect_new_fluid(PartDev &pd, hipStream_t st,
                            std::vector<std::pair<int32_t, int32_t>> &out);
 

@@ -746,6 +746,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     for (int c = 0; c < 7; c++) part[c * plane + e] = v[c];
 }
 
+// Planes C0 .. C0+NC-1 of the node shares (0-2 force, 3-5 torque, 6 heat) onto the particle's
+// sums; the particle phase reduces all seven, the per-stage execution the forces or the heat.
+template <int C0, int NC>
 __global__ __launch_bounds__(256) void k_link_reduce(PartGrid g) {
     const int w = blockIdx.x * (256 / RL) + (threadIdx.x / RL);
     if (w >= g.total_p) return;  // whole waves
@@ -755,7 +758,8 @@ __global__ __launch_bounds__(256) void k_link_reduce(PartGrid g) {
     const PartSlot &sl = g.tab[slot];
     const PartDev &pd = sl.pd;
     const int p = w0 - g.pstart[slot];
-    double s[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    double s[NC];
+    for (int c = 0; c < NC; c++) s[c] = 0.0;
     int lo[3], hi[3];
     if (load_box(sl.box_prev, p, lo, hi)) {
         grow_box(sl.d, lo, hi);
@@ -766,30 +770,27 @@ __global__ __launch_bounds__(256) void k_link_reduce(PartGrid g) {
         constexpr int U = 4;  // shares of up to U nodes loaded together, then added in order
         for (int w1 = 0; w1 < nw; w1 += U) {
             bool has[U];
-            double sh[U][7];
+            double sh[U][NC];
 #pragma unroll
             for (int i = 0; i < U; i++) {
                 const unsigned long long b = (w1 + i < nw) ? bits[w1 + i] : 0ull;
                 has[i] = (b >> lane) & 1ull;
-                for (int c = 0; c < 7; c++)
-                    sh[i][c] = has[i] ? pd.lnk_part[c * plane + base + (size_t)(w1 + i) * RL] : 0.0;
+                for (int c = 0; c < NC; c++)
+                    sh[i][c] = has[i] ? pd.lnk_part[(C0 + c) * plane + base + (size_t)(w1 + i) * RL]
+                                      : 0.0;
             }
 #pragma unroll
             for (int i = 0; i < U; i++)
                 if (has[i])
-                    for (int c = 0; c < 7; c++) s[c] += sh[i][c];
+                    for (int c = 0; c < NC; c++) s[c] += sh[i][c];
         }
     }
     for (int stride = RL / 2; stride >= 1; stride >>= 1)
-        for (int c = 0; c < 7; c++) s[c] = s[c] + __shfl_down(s[c], stride, RL);
+        for (int c = 0; c < NC; c++) s[c] = s[c] + __shfl_down(s[c], stride, RL);
     if (lane == 0) {
-        pd.p[RBTP_P_FORCE_X][p] += s[0];
-        pd.p[RBTP_P_FORCE_Y][p] += s[1];
-        pd.p[RBTP_P_FORCE_Z][p] += s[2];
-        pd.p[RBTP_P_TORQUE_X][p] += s[3];
-        pd.p[RBTP_P_TORQUE_Y][p] += s[4];
-        pd.p[RBTP_P_TORQUE_Z][p] += s[5];
-        pd.p[RBTP_P_HEAT][p] += s[6];
+        const int dst[7] = {RBTP_P_FORCE_X, RBTP_P_FORCE_Y, RBTP_P_FORCE_Z, RBTP_P_TORQUE_X,
+                            RBTP_P_TORQUE_Y, RBTP_P_TORQUE_Z, RBTP_P_HEAT};
+        for (int c = 0; c < NC; c++) pd.p[dst[C0 + c]][p] += s[c];
     }
 }
 
@@ -979,9 +980,11 @@ __global__ __launch_bounds__(128) void k_fixup_links(PartGrid g) {
 // and one lane per population j: refill_new_fluid_kernel (refill.cu:121-215) from the
 // post-collision state at neighbours that were fluid in the previous geometry, with the moved
 // state for the fallback equilibrium; lane 0 then forms rho, u, phi (refill.cu:217-235) in
-// population order and sets F = 0.
-template <class S>
-__global__ __launch_bounds__(128) void k_fixup_refill(PartGrid g) {
+// population order and sets F = 0.  The particle phase stores the populations in the special
+// list; the per-stage execution (STAGED) writes them into d_f and d_g (fo, go: FP64, blocked).
+template <class S, bool STAGED = false>
+__global__ __launch_bounds__(128) void k_fixup_refill(PartGrid g, double *fo = nullptr,
+                                                      double *go = nullptr) {
     const int slot = slot_of(g.sstart, g.nslots, blockIdx.x);
     const PartSlot &sl = g.tab[slot];
     const PartDev &pd = sl.pd;
@@ -1058,8 +1061,13 @@ __global__ __launch_bounds__(128) void k_fixup_refill(PartGrid g) {
             const double inv_count = 1.0 / static_cast<double>(sample_count);
             fv = temp_f * inv_count;
             gv = temp_g * inv_count;
-            out[(size_t)lane * cap + j] = A(fv);
-            out[(size_t)(NPOP + lane) * cap + j] = A(gv);
+            if constexpr (STAGED) {
+                fo[pop_at(lane, idx)] = fv;
+                go[pop_at(lane, idx)] = gv;
+            } else {
+                out[(size_t)lane * cap + j] = A(fv);
+                out[(size_t)(NPOP + lane) * cap + j] = A(gv);
+            }
         }
         // the group's 27 populations, read across its lanes in population order (all 32 lanes
         // of the group are active here)
@@ -1093,8 +1101,9 @@ __global__ __launch_bounds__(128) void k_fixup_refill(PartGrid g) {
 }
 
 // The replaced populations of the special nodes over a plain pull of the whole lattice (the
-// post-stream accessor): f and g are FP64 in the reference's SoA order.
-template <class A>
+// post-stream accessor): f and g are FP64 in the reference's SoA order, or in the blocked order
+// of pop_at (BLOCKED, the start of the per-stage execution).
+template <class A, bool BLOCKED = false>
 __global__ __launch_bounds__(256) void k_overwrite_special(PartDev pd, double *f, double *g, Dims d) {
     const A *pop = (const A *)pd.sp.pop;
     const int count = min(*pd.sp.count, pd.sp.cap);
@@ -1104,9 +1113,198 @@ __global__ __launch_bounds__(256) void k_overwrite_special(PartDev pd, double *f
         const uint32_t m = pd.sp.mask[j];
         for (int i = 0; i < NPOP; i++) {
             if (!(m & (1u << i))) continue;
-            f[(size_t)i * n + idx] = double(pop[(size_t)i * cap + j]);
-            g[(size_t)i * n + idx] = double(pop[(size_t)(NPOP + i) * cap + j]);
+            const size_t at = BLOCKED ? pop_at(i, idx) : (size_t)i * n + idx;
+            f[at] = double(pop[(size_t)i * cap + j]);
+            g[at] = double(pop[(size_t)(NPOP + i) * cap + j]);
         }
+    }
+}
+
+// ---- per-stage execution (rbtp_stage) -------------------------------------------------------
+//
+// The particle stages one at a time on the FP64 arrays of the staged mode: fcur/gcur of the slot
+// are d_f_collide/d_g_collide, fo/go are d_f/d_g (blocked layout), owner_prev/box_prev the
+// geometry of this step's links (ibnode = ibnode_prev until build_links), owner_curr/box_curr the
+// geometry build_links writes.  Links are derived per link-box node as in k_link_nodes.
+
+enum { STL_IBB_F = 0, STL_IBB_G = 1, STL_FORCES = 2, STL_HEAT = 3 };
+
+// IBB_F: bounce_back_particles_kernel (ibb.cu:5-82) writes f_opp(x) into d_f.
+// IBB_G: bounce_back_thermal_kernel (ibb.cu:84-136) writes g_opp(x) into d_g.
+// FORCES: the node's share of compute_particle_forces_kernel (ibb.cu:138-188), reading the
+//         bounced-back f_opp(x) from d_f; HEAT: of compute_particle_heat_kernel (ibb.cu:190-205).
+// The shares are summed by k_link_reduce in the order of the particle phase.
+template <int MODE>
+__global__ __launch_bounds__(256) void k_st_links(PartGrid g, double *fo, double *go) {
+    const int u = blockIdx.x * 256 + threadIdx.x;
+    if (u >= g.total_b) return;  // total_b is a whole number of waves
+    const int u0 = __builtin_amdgcn_readfirstlane(u);
+    const int slot = slot_of(g.bstart, g.nslots, u0);
+    const PartSlot &sl = g.tab[slot];
+    const PartDev &pd = sl.pd;
+    const Dims d = sl.d;
+    const int p = (u0 - g.bstart[slot]) / pd.lnk_stride;
+    const int k = u - g.bstart[slot] - p * pd.lnk_stride;
+    const int32_t *owner = sl.owner_prev;
+    const PView pv = pview_now(pd);
+    int lo[3], hi[3];
+    bool in_box = load_box(sl.box_prev, p, lo, hi);
+    if (in_box) {
+        grow_box(d, lo, hi);
+        in_box = k < box_count(lo, hi);
+    }
+    const double X = pv.px[p], Y = pv.py[p], Z = pv.pz[p];
+    unsigned mask = 0u, fluid_nb = 0u;
+    int ix = 0, iy = 0, iz = 0, idx = 0;
+    double x0 = 0.0, y0 = 0.0, z0 = 0.0;
+    if (in_box) {
+        idx = box_node(d, lo, hi, k, ix, iy, iz);
+        x0 = static_cast<double>(ix) + 0.5;
+        y0 = static_cast<double>(iy) + 0.5;
+        z0 = static_cast<double>(iz) + 0.5;
+        if (owner[idx] == -1) {
+            for (int ip = 1; ip < NPOP; ip++) {
+                const int nx = ix + dir_cx(ip), ny = iy + dir_cy(ip), nz = iz + dir_cz(ip);
+                if (!in_domain(d, nx, ny, nz)) continue;
+                const int own = owner[nz * d.lxy + ny * d.lx + nx];
+                if (own == -1) fluid_nb |= 1u << ip;
+                if (own == p && link_exists(ip, x0, y0, z0, X, Y, Z, pd.radius_sq))
+                    mask |= 1u << ip;
+            }
+        }
+    }
+    if (MODE == STL_FORCES || MODE == STL_HEAT) {
+        const unsigned long long has = __ballot(mask != 0u);
+        if (__lane_id() == 0) pd.lnk_bits[(size_t)p * (pd.lnk_stride / 64) + k / 64] = has;
+    }
+    if (mask == 0u) return;
+    const PopView<double> pop{gbl((const double *)sl.fcur), gbl((const double *)sl.gcur),
+                              (size_t)d.n, gbl(pd.gshift), d.lx};
+    const double density = sl.macro[0][idx];
+    double v[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    while (mask) {
+        const int ip = __builtin_ctz(mask);
+        mask &= mask - 1u;
+        const int opp = dir_opp(ip);
+        const int cx = dir_cx(ip), cy = dir_cy(ip), cz = dir_cz(ip);
+        const bool up_fluid = (fluid_nb >> opp) & 1u;  // x - c_ip = x + c_opp
+        const int idx_up = idx - (cx + cy * d.lx + cz * d.lxy);
+        double q = 0.0;
+        link_q(ip, x0, y0, z0, X, Y, Z, pd.radius_sq, q);
+        const LinkLoads Ld = link_loads<double>(ip, idx, idx_up, up_fluid, q, pop);
+        const LinkOut L = eval_link_at(Ld, ip, ix, iy, iz, up_fluid, q, p, pv, density, pd.rho0);
+        if (MODE == STL_IBB_F) fo[pop_at(opp, idx)] = L.f_out;
+        if (MODE == STL_IBB_G) go[pop_at(opp, idx)] = L.g_out;
+        if (MODE == STL_FORCES) {
+            const double f_out = fo[pop_at(opp, idx)];
+            const double rx = q * static_cast<double>(cx);
+            const double ry = q * static_cast<double>(cy);
+            const double rz = q * static_cast<double>(cz);
+            const double relx = ix + 0.5 + rx - X;
+            const double rely = iy + 0.5 + ry - Y;
+            const double relz = iz + 0.5 + rz - Z;
+            const double Fx = (cx - L.uwx) * L.f_in - (-cx - L.uwx) * f_out;
+            const double Fy = (cy - L.uwy) * L.f_in - (-cy - L.uwy) * f_out;
+            const double Fz = (cz - L.uwz) * L.f_in - (-cz - L.uwz) * f_out;
+            v[0] += Fx;
+            v[1] += Fy;
+            v[2] += Fz;
+            v[3] += rely * Fz - relz * Fy;
+            v[4] += relz * Fx - relx * Fz;
+            v[5] += relx * Fy - rely * Fx;
+        }
+        if (MODE == STL_HEAT) v[6] += L.g_in - go[pop_at(opp, idx)];
+    }
+    const size_t plane = (size_t)pd.np * pd.lnk_stride;
+    const size_t e = (size_t)p * pd.lnk_stride + k;
+    if (MODE == STL_FORCES)
+        for (int c = 0; c < 6; c++) pd.lnk_part[c * plane + e] = v[c];
+    if (MODE == STL_HEAT) pd.lnk_part[6 * plane + e] = v[6];
+}
+
+// prepare_particle_forces_step (pforces.cu:80-116).
+__global__ __launch_bounds__(128) void k_st_prepare(PartGrid g) {
+    const int u = blockIdx.x * 128 + threadIdx.x;
+    if (u >= g.total_p) return;
+    const int s = slot_of(g.pstart, g.nslots, u);
+    double *const *P = g.tab[s].pd.p;
+    const int i = u - g.pstart[s];
+    for (int a = 0; a < 3; a++) {
+        P[RBTP_P_FORCE_PREV2_X + a][i] = P[RBTP_P_FORCE_PREV_X + a][i];
+        P[RBTP_P_TORQUE_PREV2_X + a][i] = P[RBTP_P_TORQUE_PREV_X + a][i];
+    }
+    for (int a = 0; a < 3; a++) {
+        P[RBTP_P_FORCE_PREV_X + a][i] = P[RBTP_P_FORCE_X + a][i];
+        P[RBTP_P_TORQUE_PREV_X + a][i] = P[RBTP_P_TORQUE_X + a][i];
+    }
+    for (int a = 0; a < 3; a++) {
+        P[RBTP_P_FORCE_X + a][i] = 0.0;
+        P[RBTP_P_TORQUE_X + a][i] = 0.0;
+    }
+}
+
+// apply_repulsive_forces (pforces.cu:118-148): every partner in ascending order
+// (repulsive_pair_kernel, pforces.cu:6-38), then the six walls (repulsive_wall_kernel,
+// pforces.cu:40-76), added onto the accumulated force.
+__global__ __launch_bounds__(128) void k_st_repulsion(PartGrid g) {
+    const int u = blockIdx.x * 128 + threadIdx.x;
+    if (u >= g.total_p) return;
+    const int s = slot_of(g.pstart, g.nslots, u);
+    const PartDev &pd = g.tab[s].pd;
+    const Dims d = g.tab[s].d;
+    const int i = u - g.pstart[s];
+    double *const *P = pd.p;
+    const double *X = P[RBTP_P_POS_X], *Y = P[RBTP_P_POS_Y], *Z = P[RBTP_P_POS_Z];
+    const double xi = X[i], yi = Y[i], zi = Z[i];
+    double FX = P[RBTP_P_FORCE_X][i], FY = P[RBTP_P_FORCE_Y][i], FZ = P[RBTP_P_FORCE_Z][i];
+    if (pd.np > 1)
+        for (int j = 0; j < pd.np; j++) {
+            if (j == i) continue;
+            double fx, fy, fz;
+            if (pair_term(pd, X, Y, Z, xi, yi, zi, j, fx, fy, fz)) {
+                FX += fx;
+                FY += fy;
+                FZ += fz;
+            }
+        }
+    double fx = 0.0, fy = 0.0, fz = 0.0;
+    const double mgw = pd.min_gap_w, radius = pd.radius, coeff = pd.coeff;
+    auto wall = [&](double gap, double nx, double ny, double nz) {
+        if (gap >= mgw) return;
+        const double overlap = (gap < 0.0) ? -gap : 0.0;
+        const double factor = (mgw - gap) / mgw;
+        const double lubforce = coeff * (factor * factor / pd.stf0_w + overlap / (mgw * pd.stf1_w));
+        fx += lubforce * nx;
+        fy += lubforce * ny;
+        fz += lubforce * nz;
+    };
+    wall(xi - radius, 1.0, 0.0, 0.0);
+    wall((static_cast<double>(d.lx) - xi) - radius, -1.0, 0.0, 0.0);
+    wall(yi - radius, 0.0, 1.0, 0.0);
+    wall((static_cast<double>(d.ly) - yi) - radius, 0.0, -1.0, 0.0);
+    wall(zi - radius, 0.0, 0.0, 1.0);
+    wall((static_cast<double>(d.lz) - zi) - radius, 0.0, 0.0, -1.0);
+    P[RBTP_P_FORCE_X][i] = FX + fx;
+    P[RBTP_P_FORCE_Y][i] = FY + fy;
+    P[RBTP_P_FORCE_Z][i] = FZ + fz;
+}
+
+// The host part of compute_particle_heat (ibb.cu:260-261): heat_prev <- heat, heat <- 0.
+__global__ __launch_bounds__(128) void k_st_heat_reset(PartGrid g) {
+    const int u = blockIdx.x * 128 + threadIdx.x;
+    if (u >= g.total_p) return;
+    const int s = slot_of(g.pstart, g.nslots, u);
+    double *const *P = g.tab[s].pd.p;
+    const int i = u - g.pstart[s];
+    P[RBTP_P_HEAT_PREV][i] = P[RBTP_P_HEAT][i];
+    P[RBTP_P_HEAT][i] = 0.0;
+}
+
+// An empty special-node list for each slot (the new-fluid nodes of refill_nodes are listed next).
+__global__ void k_st_special_reset(PartGrid g) {
+    for (int s = threadIdx.x; s < g.nslots; s += blockDim.x) {
+        g.tab[s].pd.sp.count[0] = 0;
+        g.tab[s].pd.sp.count[1] = 0;
     }
 }
 
@@ -1114,13 +1312,18 @@ __global__ __launch_bounds__(256) void k_overwrite_special(PartDev pd, double *f
 // A record is listed when the hot path's own test (link_exists, as in k_link_nodes) accepts the
 // link, so the list compared with the reference criterion is the set the coupling uses.
 __global__ __launch_bounds__(PB) void k_collect_links(PartDev pd, const int32_t *owner,
-                                                      const int32_t *box, Dims d,
+                                                      const int32_t *box, Dims d, int snapshot,
                                                       rbtp_link *out, int cap, int *count) {
     const int p = blockIdx.x;
     int lo[3], hi[3];
     if (!load_box(box, p, lo, hi)) return;
     grow_box(d, lo, hi);
-    const double X = pd.p[RBTP_P_POS_X][p], Y = pd.p[RBTP_P_POS_Y][p], Z = pd.p[RBTP_P_POS_Z][p];
+    // the positions the geometry was built from: the current ones, or the snapshot k_move took
+    // before moving the particles (between update_particles and build_links)
+    const double *PX = snapshot ? pd.old[OLD_PX] : pd.p[RBTP_P_POS_X];
+    const double *PY = snapshot ? pd.old[OLD_PY] : pd.p[RBTP_P_POS_Y];
+    const double *PZ = snapshot ? pd.old[OLD_PZ] : pd.p[RBTP_P_POS_Z];
+    const double X = PX[p], Y = PY[p], Z = PZ[p];
     const int cnt = box_count(lo, hi);
     for (int k = threadIdx.x; k < cnt; k += PB) {
         int ix, iy, iz;
@@ -1453,7 +1656,7 @@ int part_phase(const PartGrid &g, int prec, hipStream_t st) {
         hipLaunchKernelGGL((k_link_nodes<S>), (g.total_b + 255) / 256, 256, 0, st, g);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
-        hipLaunchKernelGGL(k_link_reduce, (tp + 256 / RL - 1) / (256 / RL), 256, 0, st, g);
+        hipLaunchKernelGGL((k_link_reduce<0, 7>), (tp + 256 / RL - 1) / (256 / RL), 256, 0, st, g);
         return hipGetLastError();
     }));
     hipLaunchKernelGGL(k_move, (tp + 127) / 128, 128, 0, st, g);
@@ -1473,6 +1676,66 @@ int part_phase(const PartGrid &g, int prec, hipStream_t st) {
         return hipGetLastError();
     }));
     return RBTP_OK;
+}
+
+int part_stage(int stage, const PartGrid &g, double *fo, double *go, hipStream_t st) {
+    if (g.nslots == 0 || g.total_p == 0) return RBTP_OK;
+    const int tp = g.total_p;
+    const int nb = (g.total_b + 255) / 256, np128 = (tp + 127) / 128;
+    const int nred = (tp + 256 / RL - 1) / (256 / RL);
+    switch (stage) {
+        case RBTP_STAGE_IBB_F:
+            hipLaunchKernelGGL(k_st_links<STL_IBB_F>, nb, 256, 0, st, g, fo, go);
+            break;
+        case RBTP_STAGE_IBB_G:
+            hipLaunchKernelGGL(k_st_links<STL_IBB_G>, nb, 256, 0, st, g, fo, go);
+            break;
+        case RBTP_STAGE_PREPARE:
+            hipLaunchKernelGGL(k_st_prepare, np128, 128, 0, st, g);
+            break;
+        case RBTP_STAGE_REPULSION:
+            hipLaunchKernelGGL(k_st_repulsion, np128, 128, 0, st, g);
+            break;
+        case RBTP_STAGE_FORCES:
+            hipLaunchKernelGGL(k_st_links<STL_FORCES>, nb, 256, 0, st, g, fo, go);
+            PHIP(hipGetLastError());
+            hipLaunchKernelGGL((k_link_reduce<0, 6>), nred, 256, 0, st, g);
+            break;
+        case RBTP_STAGE_HEAT:
+            hipLaunchKernelGGL(k_st_heat_reset, np128, 128, 0, st, g);
+            PHIP(hipGetLastError());
+            hipLaunchKernelGGL(k_st_links<STL_HEAT>, nb, 256, 0, st, g, fo, go);
+            PHIP(hipGetLastError());
+            hipLaunchKernelGGL((k_link_reduce<6, 1>), nred, 256, 0, st, g);
+            break;
+        case RBTP_STAGE_MOVE:
+            hipLaunchKernelGGL(k_move, np128, 128, 0, st, g);
+            break;
+        case RBTP_STAGE_BUILD_LINKS:
+            hipLaunchKernelGGL(k_geo_clear, tp, PB, 0, st, g);
+            PHIP(hipGetLastError());
+            hipLaunchKernelGGL(k_geo_mark, tp, PB, 0, st, g);
+            break;
+        case RBTP_STAGE_REFILL:
+            hipLaunchKernelGGL(k_st_special_reset, 1, 64, 0, st, g);
+            PHIP(hipGetLastError());
+            hipLaunchKernelGGL(k_refill_find, tp, LB, 0, st, g);
+            PHIP(hipGetLastError());
+            hipLaunchKernelGGL((k_fixup_refill<double, true>), g.total_s, 128, 0, st, g, fo, go);
+            break;
+        default:
+            return report(RBTP_E_ARG, "not a particle stage");
+    }
+    PHIP(hipGetLastError());
+    return RBTP_OK;
+}
+
+hipError_t part_overwrite_special_blocked(PartDev &pd, double *f, double *g, const Dims &dims,
+                                          hipStream_t st) {
+    if (!pd.special_pending) return hipSuccess;
+    hipLaunchKernelGGL((k_overwrite_special<double, true>), grid_for(pd.sp.cap, 256, 1024), 256, 0,
+                       st, pd, f, g, dims);
+    return hipGetLastError();
 }
 
 int part_check_errors(PartDev &pd) {
@@ -1495,14 +1758,15 @@ hipError_t part_overwrite_special(PartDev &pd, double *f, double *g, const Dims 
     return hipGetLastError();
 }
 
-int part_collect_links(PartDev &pd, const Dims &dims, hipStream_t st, std::vector<rbtp_link> &out) {
+int part_collect_links(PartDev &pd, int gen, bool snapshot, const Dims &dims, hipStream_t st,
+                       std::vector<rbtp_link> &out) {
     int *count = nullptr;
     PHIP(hipMalloc(&count, sizeof(int)));
     int total = 0;
     hipError_t e = hipMemsetAsync(count, 0, sizeof(int), st);
     if (e == hipSuccess) {
-        hipLaunchKernelGGL(k_collect_links, pd.np, PB, 0, st, pd, pd.owner[pd.geo_cur],
-                           pd.box[pd.geo_cur], dims, (rbtp_link *)nullptr, 0, count);
+        hipLaunchKernelGGL(k_collect_links, pd.np, PB, 0, st, pd, pd.owner[gen], pd.box[gen], dims,
+                           snapshot ? 1 : 0, (rbtp_link *)nullptr, 0, count);
         e = hipGetLastError();
     }
     if (e == hipSuccess) e = hipMemcpyAsync(&total, count, sizeof(int), hipMemcpyDeviceToHost, st);
@@ -1512,8 +1776,8 @@ int part_collect_links(PartDev &pd, const Dims &dims, hipStream_t st, std::vecto
     if (e == hipSuccess && total > 0) {
         e = hipMemsetAsync(count, 0, sizeof(int), st);
         if (e == hipSuccess) {
-            hipLaunchKernelGGL(k_collect_links, pd.np, PB, 0, st, pd, pd.owner[pd.geo_cur],
-                               pd.box[pd.geo_cur], dims, buf, total, count);
+            hipLaunchKernelGGL(k_collect_links, pd.np, PB, 0, st, pd, pd.owner[gen], pd.box[gen],
+                               dims, snapshot ? 1 : 0, buf, total, count);
             e = hipGetLastError();
         }
         out.resize(total);

@@ -109,6 +109,16 @@ struct rbtp_sim {
     PartTables ptab;  // this lattice's particle-phase tables (one slot)
     bool ptab_valid;
     int ptab_rel;     // cur ^ pd.geo_cur at the particle phase the tables describe
+    bool batch_slot;  // owned by a serving batch (stepped by it, never staged)
+    // Per-stage execution (rbtp_stage).  While st_active the state is the reference's arrays:
+    // d_f and d_g in buffers 1 - cur, d_f_collide and d_g_collide in buffers cur (FP64, blocked
+    // layout), the fields and the particle arrays as they are; st_next is the stage the next
+    // call must run, st_built that build_links has run and refill not yet (the newest geometry
+    // is then owner[1 - geo_cur]).  st_tab are the particle tables of those arrays.
+    bool st_active;
+    int st_next;
+    bool st_built;
+    PartTables st_tab;
 };
 
 // ---------------------------------------------------------------------------------------------
@@ -220,6 +230,7 @@ extern "C" void rbtp_destroy(rbtp_sim *s) {
     (void)hipFree(s->shift_delta);
     part_free(s->pd);
     part_tables_free(s->ptab);
+    part_tables_free(s->st_tab);
     if (s->ev_start) (void)hipEventDestroy(s->ev_start);
     if (s->ev_stop) (void)hipEventDestroy(s->ev_stop);
     if (s->ev_k0) (void)hipEventDestroy(s->ev_k0);
@@ -372,6 +383,9 @@ static StepArgs<S> step_args(rbtp_sim *s, bool macro_out) {
     return a;
 }
 
+static int stage_exit(rbtp_sim *s);
+static int current_gen(const rbtp_sim *s);
+
 // ---------------------------------------------------------------------------------------------
 // Initial state
 // ---------------------------------------------------------------------------------------------
@@ -410,6 +424,8 @@ extern "C" int rbtp_init(rbtp_sim *s, const double *temperature, const double *p
         tdev = s->temp_dev;
     }
     s->cur = 0;
+    s->st_active = false;
+    s->ptab_valid = false;
     RB_TRY(reset_shift(s));
     HIP_TRY(dispatch(s->prec, [&](auto *tag) {
         using S = std::remove_pointer_t<decltype(tag)>;
@@ -516,6 +532,7 @@ static int step_impl(rbtp_sim *s, int nsteps, unsigned flags, bool sync) {
     if (!s->initialised) return set_error(RBTP_E_STATE, "rbtp_step before rbtp_init");
     if (nsteps < 0) return set_error(RBTP_E_ARG, "nsteps < 0");
     HIP_TRY(hipSetDevice(s->device));
+    RB_TRY(stage_exit(s));
     if (s->kernel_timing) {
         s->kernel_ms_sum = 0.0;
         s->kernel_launches = 0;
@@ -585,6 +602,137 @@ extern "C" int rbtp_kernel_ms(rbtp_sim *s, double *avg_ms, int64_t *launches) {
 }
 
 // ---------------------------------------------------------------------------------------------
+// Per-stage execution
+// ---------------------------------------------------------------------------------------------
+
+static const char *const kStageNames[RBTP_STAGE_COUNT] = {
+    "macrovar", "forcing", "collide_f", "stream_f", "ibb_f", "collide_g", "stream_g", "ibb_g",
+    "prepare", "repulsion", "forces", "heat", "move", "build_links", "refill"};
+
+// Geometry generation that holds the reference's current ibnode/owner.
+static int current_gen(const rbtp_sim *s) {
+    return (s->st_active && s->st_built) ? 1 - s->pd.geo_cur : s->pd.geo_cur;
+}
+
+static StageFluid stage_fluid(rbtp_sim *s) {
+    StageFluid a;
+    a.f = (double *)s->f[1 - s->cur];
+    a.fc = (double *)s->f[s->cur];
+    a.g = (double *)s->g[1 - s->cur];
+    a.gc = (double *)s->g[s->cur];
+    a.rho = s->macro[0];
+    a.ux = s->macro[1];
+    a.uy = s->macro[2];
+    a.uz = s->macro[3];
+    a.phi = s->macro[4];
+    a.fx = (double *)s->fx;
+    a.rho_wall = (const double *)s->rho_wall;
+    a.d = s->dims;
+    a.k = s->k;
+    return a;
+}
+
+// Particle tables over the staged arrays: d_f_collide/d_g_collide in buffer cur, the links of
+// geometry generation geo_cur, build_links writing the other one.
+static int stage_tables(rbtp_sim *s) {
+    std::vector<PartSlot> v[2];
+    for (int g = 0; g < 2; g++) v[g].push_back(part_slot(s, s->cur, s->pd.geo_cur));
+    return part_tables_build(s->st_tab, v, s->stream);
+}
+
+// From the fused state at a step boundary to the reference's arrays: d_f_collide and
+// d_g_collide are the post-collision buffers (or, right after an initialisation, the post-stream
+// ones, which initial.cu:326 copies into them), d_f and d_g the post-stream populations pulled
+// from them with the special nodes' corrected values.
+static int stage_enter(rbtp_sim *s) {
+    if (!s->initialised) return set_error(RBTP_E_STATE, "rbtp_stage before rbtp_init");
+    if (s->prec != RBTP_PREC_FP64)
+        return set_error(RBTP_E_STATE, "per-stage execution runs the FP64 form only");
+    if (s->batch_slot)
+        return set_error(RBTP_E_STATE, "a serving-batch slot is stepped by its batch");
+    const StageFluid a = stage_fluid(s);
+    HIP_TRY(launch_stage_enter(a, s->poststream, s->stream));
+    if (s->particles && !s->poststream)
+        HIP_TRY(part_overwrite_special_blocked(s->pd, a.f, a.g, s->dims, s->stream));
+    if (s->particles) RB_TRY(stage_tables(s));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    s->st_active = true;
+    s->st_next = 0;
+    s->st_built = false;
+    return RBTP_OK;
+}
+
+// Back to the fused state at a step boundary: d_f and d_g become the post-stream state the next
+// lattice kernel reads in place, and the special-node list is empty.
+static int stage_exit(rbtp_sim *s) {
+    if (!s->st_active) return RBTP_OK;
+    if (s->st_next != 0)
+        return set_error(RBTP_E_STATE, "the staged step is not finished (next stage: %s)",
+                         kStageNames[s->st_next]);
+    s->cur = 1 - s->cur;
+    s->poststream = true;
+    s->ptab_valid = false;
+    if (s->particles) {
+        s->pd.special_pending = false;
+        HIP_TRY(hipMemsetAsync(s->pd.sp.count, 0, 2 * sizeof(int), s->stream));
+        HIP_TRY(hipMemsetAsync(s->pd.sp.map, 0xFF, (size_t)s->dims.n * sizeof(int32_t), s->stream));
+        HIP_TRY(hipStreamSynchronize(s->stream));
+    }
+    s->st_active = false;
+    return RBTP_OK;
+}
+
+extern "C" int rbtp_stage(rbtp_sim *s, int stage) {
+    if (!s) return set_error(RBTP_E_ARG, "null handle");
+    if (stage < 0 || stage >= RBTP_STAGE_COUNT) return set_error(RBTP_E_ARG, "unknown stage %d", stage);
+    HIP_TRY(hipSetDevice(s->device));
+    if (!s->st_active) {
+        if (stage != 0)
+            return set_error(RBTP_E_STATE, "the staged step starts with macrovar, not %s",
+                             kStageNames[stage]);
+        RB_TRY(stage_enter(s));
+    }
+    if (stage != s->st_next)
+        return set_error(RBTP_E_STATE, "stage %s called where main.cu runs %s", kStageNames[stage],
+                         kStageNames[s->st_next]);
+    const StageFluid a = stage_fluid(s);
+    switch (stage) {
+        case RBTP_STAGE_MACROVAR:
+        case RBTP_STAGE_FORCING:
+        case RBTP_STAGE_COLLIDE_F:
+        case RBTP_STAGE_STREAM_F:
+        case RBTP_STAGE_COLLIDE_G:
+        case RBTP_STAGE_STREAM_G:
+            HIP_TRY(launch_stage_fluid(stage, a, s->stream));
+            break;
+        default:
+            if (!s->particles) break;
+            RB_TRY(part_stage(stage, s->st_tab.g[0], a.f, a.g, s->stream));
+            if (stage == RBTP_STAGE_BUILD_LINKS) s->st_built = true;
+            if (stage == RBTP_STAGE_REFILL) {
+                // ibnode_prev <- ibnode, owner_prev <- owner (refill.cu:327-328)
+                s->pd.geo_cur ^= 1;
+                s->st_built = false;
+                s->pd.special_pending = true;  // the list holds this step's new-fluid nodes
+                RB_TRY(stage_tables(s));
+            }
+            break;
+    }
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    if (s->particles) RB_TRY(part_check_errors(s->pd));
+    if (stage == RBTP_STAGE_MACROVAR) s->macro_valid = true;
+    s->st_next = (stage + 1) % RBTP_STAGE_COUNT;
+    if (s->st_next == 0) s->step++;
+    return RBTP_OK;
+}
+
+extern "C" int rbtp_next_stage(const rbtp_sim *s, int *stage) {
+    if (!s || !stage) return set_error(RBTP_E_ARG, "null argument");
+    *stage = s->st_active ? s->st_next : 0;
+    return RBTP_OK;
+}
+
+// ---------------------------------------------------------------------------------------------
 // Accessors
 // ---------------------------------------------------------------------------------------------
 
@@ -630,7 +778,13 @@ extern "C" int rbtp_get_populations(rbtp_sim *s, int which, double *out) {
     int rc = RBTP_OK;
     hipError_t e = hipSuccess;
     const bool want_g = (which == RBTP_POP_G || which == RBTP_POP_G_COLLIDE);
-    if (which == RBTP_POP_F_COLLIDE || which == RBTP_POP_G_COLLIDE || s->poststream) {
+    if (s->st_active) {
+        // the staged arrays: d_f, d_g in buffers 1 - cur, d_f_collide, d_g_collide in cur
+        const bool collide = (which == RBTP_POP_F_COLLIDE || which == RBTP_POP_G_COLLIDE);
+        const int b = collide ? s->cur : 1 - s->cur;
+        e = launch_decode<double>((const double *)(want_g ? s->g[b] : s->f[b]), dev, n,
+                                  want_g ? 1 : 0, s->gshift, s->dims.lx, s->stream);
+    } else if (which == RBTP_POP_F_COLLIDE || which == RBTP_POP_G_COLLIDE || s->poststream) {
         // d_f_collide holds f after init (initial.cu:326) and the last collision otherwise.
         e = dispatch(s->prec, [&](auto *tag) {
             using S = std::remove_pointer_t<decltype(tag)>;
@@ -662,6 +816,7 @@ extern "C" int rbtp_set_populations(rbtp_sim *s, const double *f, const double *
     if (!s || !f || !g) return set_error(RBTP_E_ARG, "null argument");
     if (!s->initialised) return set_error(RBTP_E_STATE, "not initialised");
     HIP_TRY(hipSetDevice(s->device));
+    RB_TRY(stage_exit(s));
     const size_t n = (size_t)s->dims.n;
     double *dev = nullptr;
     HIP_TRY(hipMalloc(&dev, 27 * n * sizeof(double)));
@@ -697,7 +852,7 @@ extern "C" int rbtp_get_flags(rbtp_sim *s, int32_t *ibnode, int32_t *owner) {
     std::vector<int32_t> own(n, -1);
     if (s->particles) {
         HIP_TRY(hipStreamSynchronize(s->stream));
-        HIP_TRY(copy_to_host(own.data(), s->pd.owner[s->pd.geo_cur], n * sizeof(int32_t), s->stream));
+        HIP_TRY(copy_to_host(own.data(), s->pd.owner[current_gen(s)], n * sizeof(int32_t), s->stream));
     }
     if (owner) std::copy(own.begin(), own.end(), owner);
     if (ibnode)
@@ -731,7 +886,10 @@ extern "C" int rbtp_get_links(rbtp_sim *s, rbtp_link *out, int64_t capacity, int
     if (!s->particles) return RBTP_OK;
     HIP_TRY(hipSetDevice(s->device));
     std::vector<rbtp_link> links;
-    RB_TRY(part_collect_links(s->pd, s->dims, s->stream, links));
+    // between update_particles and build_links of a staged step the list is still the one built
+    // from the positions before the motion
+    const bool snapshot = s->st_active && s->st_next == RBTP_STAGE_BUILD_LINKS;
+    RB_TRY(part_collect_links(s->pd, current_gen(s), snapshot, s->dims, s->stream, links));
     std::sort(links.begin(), links.end(), [](const rbtp_link &a, const rbtp_link &b) {
         if (a.cell_k != b.cell_k) return a.cell_k < b.cell_k;
         if (a.cell_j != b.cell_j) return a.cell_j < b.cell_j;
@@ -774,7 +932,7 @@ extern "C" int rbtp_nu(rbtp_sim *s, double out[7]) {
     if (!s || !out) return set_error(RBTP_E_ARG, "null argument");
     if (!s->macro_valid) return set_error(RBTP_E_STATE, "run the last step with RBTP_STEP_MACRO_LAST first");
     HIP_TRY(hipSetDevice(s->device));
-    const int32_t *owner = s->particles ? s->pd.owner[s->pd.geo_cur] : nullptr;
+    const int32_t *owner = s->particles ? s->pd.owner[current_gen(s)] : nullptr;
     RB_TRY(diag_nu(s->c, s->d, s->dims, s->macro[1], s->macro[4], owner, s->step, s->stream, out));
     return RBTP_OK;
 }
@@ -993,6 +1151,7 @@ extern "C" int rbtp_restart_load(rbtp_sim *s, const char *dir, int step) {
         // allocate the state the way rbtp_init would, then replace it
         RB_TRY(rbtp_init(s, nullptr, nullptr, nullptr));
     }
+    s->st_active = false;  // the loaded state replaces any staged one
     HIP_TRY(hipSetDevice(s->device));
     const size_t n = (size_t)s->dims.n;
     char path[4096];
@@ -1205,6 +1364,7 @@ extern "C" int rbtp_batch_admit(rbtp_batch *b, const rbtp_case *c, const double 
     HIP_TRY(hipStreamSynchronize(b->stream));
     rbtp_sim *s = nullptr;
     RB_TRY(create_on_stream(c, &s, b->stream));
+    s->batch_slot = true;
     // Batched steps have no per-slot refresh of the shift profile; the admission step, taken
     // through the single-sim path, must not refresh it either.
     s->shift_interval = 0;
