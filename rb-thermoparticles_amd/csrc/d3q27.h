@@ -54,8 +54,10 @@ __host__ __device__ inline int dir_opp(int i) {
 // indices; a run keeps population 0 of its nodes, then population 1, ..., then population 26.
 // A workgroup's access to one population is one contiguous piece and the 27 pieces of a run
 // sit together in HBM, instead of 27 streams N values apart; the reference's order (pop[i*N +
-// idx], initial.cu:276-279) is what the accessors and files see.
-constexpr int POP_RUN_LOG2 = 10;
+// idx], initial.cu:276-279) is what the accessors and files see.  Runs of 4096 nodes: the pull's
+// access pattern alone moved 5.56 TB/s with them at 320x320x160, against 5.29 (1024), 5.40
+// (2048) and 5.49 (8192) on the same device.
+constexpr int POP_RUN_LOG2 = 12;
 constexpr int POP_RUN = 1 << POP_RUN_LOG2;
 __host__ __device__ inline size_t pop_at(int i, int idx) {
     return ((size_t)(idx >> POP_RUN_LOG2) * NPOP + (size_t)i) * POP_RUN +

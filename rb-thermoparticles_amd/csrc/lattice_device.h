@@ -100,6 +100,103 @@ __device__ inline void gather(const S *__restrict__ fin, const S *__restrict__ g
     }
 }
 
+// Exchange of one 64-bit value between neighbouring lanes of a wave (DPP wave_shr:1 and
+// wave_shl:1): shr gives lane l the value of lane l - 1, shl that of lane l + 1; the lane that has
+// no neighbour gets 0 and is filled in by the caller.
+__device__ inline double wave_shr1(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)b, 0x138, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), 0x138, 0xf, 0xf, false);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+__device__ inline double wave_shl1(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)b, 0x130, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), 0x130, 0xf, 0xf, false);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+// Populations pulled with c_x != 0 (18 per distribution) and the two edge values a wave hands to
+// its neighbours for each of them.
+constexpr int NSHIFT = 36;
+struct RowEdges {
+    double last[NSHIFT];   // lane 63's aligned value (lane 0 of the next wave needs it, c_x = +1)
+    double first[NSHIFT];  // lane 0's aligned value (lane 63 of the previous wave, c_x = -1)
+};
+
+// gather() of the FP64 form for lattices whose rows are whole waves (lx a multiple of 64, every
+// workgroup full).  A pull with c_x = +-1 reads element idx - c_x of a row that is 64-aligned, so
+// each wave loads its own aligned 512 bytes (4 cache lines instead of 5) and moves the values one
+// lane over (DPP); the lane at the wave's edge takes its value from the neighbouring wave of the
+// workgroup through LDS, or loads it (first or last wave of the workgroup), or applies the x-wall
+// rule (a row's first and last node are the lanes at the edges).  The y and z walls cover whole
+// rows, so they are decided per wave.  Same values as gather(), in the same order of operations.
+template <int NW>
+__device__ inline void gather_rows(const double *__restrict__ fin,
+                                   const double *__restrict__ gin, int idx, int ix, int iy,
+                                   int iz, const Dims &d, double rho_w, double thot,
+                                   double tcold, RowEdges *edges, double (&f)[NPOP],
+                                   double (&g)[NPOP]) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    bool rowout[NPOP];
+#pragma unroll
+    for (int i = 0; i < NPOP; i++) {
+        const int cy = D3Q27::cy[i], cz = D3Q27::cz[i];
+        const bool outy = (cy == 1 && iy == 0) || (cy == -1 && iy == d.ly - 1);
+        const bool outz = (cz == 1 && iz == 0) || (cz == -1 && iz == d.lz - 1);
+        rowout[i] = outy || outz;
+        const int aidx = rowout[i] ? idx : idx - cy * d.lx - cz * d.lxy;
+        const int sp = rowout[i] ? D3Q27::opp[i] : i;
+        f[i] = fin[pop_at(sp, aidx)];
+        g[i] = gin[pop_at(sp, aidx)];
+    }
+    int k = 0;
+#pragma unroll
+    for (int i = 0; i < NPOP; i++) {
+        if (D3Q27::cx[i] == 0) continue;
+        if (D3Q27::cx[i] == 1 && lane == 63) {
+            edges[w].last[k] = f[i];
+            edges[w].last[k + 1] = g[i];
+        }
+        if (D3Q27::cx[i] == -1 && lane == 0) {
+            edges[w].first[k] = f[i];
+            edges[w].first[k + 1] = g[i];
+        }
+        k += 2;
+    }
+    __syncthreads();
+    k = 0;
+#pragma unroll
+    for (int i = 0; i < NPOP; i++) {
+        const int cx = D3Q27::cx[i], cy = D3Q27::cy[i], cz = D3Q27::cz[i];
+        if (cx == 0) continue;
+        if (!rowout[i]) {
+            double fv = cx == 1 ? wave_shr1(f[i]) : wave_shl1(f[i]);
+            double gv = cx == 1 ? wave_shr1(g[i]) : wave_shl1(g[i]);
+            if (cx == 1 ? lane == 0 : lane == 63) {
+                const int nw = w - cx;
+                if (cx == 1 ? ix == 0 : ix == d.lx - 1) {
+                    // x wall only: bounce-back for f, hot/cold plate for g (streaming.cu:10-21,
+                    // 45-61)
+                    fv = fin[pop_at(D3Q27::opp[i], idx)];
+                    const double twall = (cx == 1) ? thot : tcold;
+                    gv = -gin[pop_at(D3Q27::opp[i], idx)] + 2.0 * weight(i) * twall * (1.0 + rho_w);
+                } else if (nw >= 0 && nw < NW) {
+                    fv = cx == 1 ? edges[nw].last[k] : edges[nw].first[k];
+                    gv = cx == 1 ? edges[nw].last[k + 1] : edges[nw].first[k + 1];
+                } else {
+                    const int sidx = idx - (cx + cy * d.lx + cz * d.lxy);
+                    fv = fin[pop_at(i, sidx)];
+                    gv = gin[pop_at(i, sidx)];
+                }
+            }
+            f[i] = fv;
+            g[i] = gv;
+        }
+        k += 2;
+    }
+}
+
 // gather() for the shifted encodings (float, half, bf16), with g returned in the variables the
 // FP32 collision works in: p_i = g_i / w_i - T_s, T_s = gshift[ix] the shift of the node's own
 // column.  A value pulled from x - c_i is stored relative to the shift of that node's column, so

@@ -38,8 +38,9 @@ def test_init_state_bitwise(rb, O):
         assert np.array_equal(sim.field(name), ora.field(name)), name
 
 
-@pytest.mark.parametrize("shape", [(24, 20, 16), (17, 9, 5), (3, 3, 3)])
+@pytest.mark.parametrize("shape", [(24, 20, 16), (17, 9, 5), (3, 3, 3), (64, 12, 8), (128, 4, 3)])
 def test_one_step_bitwise(rb, O, shape):
+    """(64, 12, 8) and (128, 4, 3) run the FP64 step with the row pull (rows of whole waves)."""
     sim, ora = make_pair(rb, O, single_phase(*shape, ra=1e5))
     sim.step(1)
     ora.step(1)
@@ -52,8 +53,9 @@ def test_one_step_bitwise(rb, O, shape):
         assert np.array_equal(a, b), name
 
 
-def test_multi_step_bitwise(rb, O):
-    sim, ora = make_pair(rb, O, single_phase(32, 24, 16, ra=1e6))
+@pytest.mark.parametrize("shape", [(32, 24, 16), (64, 16, 12)])
+def test_multi_step_bitwise(rb, O, shape):
+    sim, ora = make_pair(rb, O, single_phase(*shape, ra=1e6))
     for n in (1, 4, 25):
         sim.step(n)
         ora.step(n)

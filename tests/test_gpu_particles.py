@@ -132,6 +132,20 @@ def test_moving_spheres_many_steps(rb, O):
     assert ora.unhandled_links(False) == 0 and ora.unhandled_links(True) == 0
 
 
+def test_moving_spheres_row_pull(rb, O):
+    """The moving-sphere case on a lattice 64 nodes wide (the FP64 step's row pull), spheres near
+    the x walls so that special nodes sit at the rows' first and last nodes."""
+    kw = particle_case(64, 16, 16, npart=3, diam=6.0, ra=1e5, beta=0.5)
+    pos = [(3.4, 8.0, 8.0), (60.7, 8.5, 7.0), (32.0, 8.0, 4.0)]
+    sim, ora = make_pair(rb, O, kw, pos, ptemp=np.array([0.3, -0.2, 0.0]))
+    for k in range(3):
+        sim.step(10)
+        ora.step(10)
+        check_particles(sim, ora, f"block {k}")
+        check_geometry(sim, ora, f"block {k}")
+    check_fields(sim, ora, "end")
+
+
 def test_reference_placement(rb, O):
     """glibc srand/rand rejection placement (pinit.cu:97-144) gives the oracle's positions."""
     kw = particle_case(40, 40, 40, npart=12, diam=6.0, ra=1e4, beta=0.05)
