@@ -50,22 +50,28 @@ __host__ __device__ inline int dir_opp(int i) {
     return (int)((w >> sh) & 31u);
 }
 
-// Storage order of a buffer of 27 populations.  Nodes are taken in runs of POP_RUN consecutive
-// indices; a run keeps population 0 of its nodes, then population 1, ..., then population 26.
-// A workgroup's access to one population is one contiguous piece and the 27 pieces of a run
-// sit together in HBM, instead of 27 streams N values apart; the reference's order (pop[i*N +
-// idx], initial.cu:276-279) is what the accessors and files see.  Runs of 4096 nodes: the pull's
-// access pattern alone moved 5.56 TB/s with them at 320x320x160, against 5.29 (1024), 5.40
-// (2048) and 5.49 (8192) on the same device.
+// Storage order of the populations.  f and g of one time level share a buffer (a "pair
+// buffer").  Nodes are taken in runs of POP_RUN consecutive indices; a run keeps population 0
+// of f for its nodes, then population 1, ..., population 26, then population 0 of g, ...,
+// population 26 of g.  The g pointer of a pair is G_OFFSET elements after the f pointer, so
+// pop_at(i, idx) addresses population i of either distribution relative to its own pointer.
+// A workgroup's access to one population is one contiguous piece, and the 54 pieces a node reads
+// sit together in HBM, instead of 27 streams N values apart in each of two arrays; the
+// reference's order (pop[i*N + idx], initial.cu:276-279) is what the accessors and files see.
+// The pull's access pattern alone, 320x320x160 on one MI355X: runs of 1024 nodes with f and g in
+// separate arrays 5.29 TB/s (FP64) and 5.32 (16-bit storage); runs of 4096, separate 5.56 and
+// 5.43; runs of 4096 with f and g interleaved 5.64 (5.78 with the FP64 row pull) and 5.54.
 constexpr int POP_RUN_LOG2 = 12;
 constexpr int POP_RUN = 1 << POP_RUN_LOG2;
+constexpr int RUN_POPS = 2 * NPOP;
+constexpr size_t G_OFFSET = (size_t)NPOP * POP_RUN;
 __host__ __device__ inline size_t pop_at(int i, int idx) {
-    return ((size_t)(idx >> POP_RUN_LOG2) * NPOP + (size_t)i) * POP_RUN +
+    return ((size_t)(idx >> POP_RUN_LOG2) * RUN_POPS + (size_t)i) * POP_RUN +
            (size_t)(idx & (POP_RUN - 1));
 }
-// Values in one buffer of 27 populations over n nodes (the last run padded).
+// Values in one pair buffer over n nodes (the last run padded).
 __host__ __device__ inline size_t pop_buffer_len(size_t n) {
-    return (size_t)NPOP * ((n + POP_RUN - 1) / POP_RUN) * POP_RUN;
+    return (size_t)RUN_POPS * ((n + POP_RUN - 1) / POP_RUN) * POP_RUN;
 }
 
 // Weights exactly as the reference evaluates them (8.0/27.0 etc. in double).

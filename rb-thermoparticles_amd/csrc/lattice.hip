@@ -601,10 +601,14 @@ __global__ __launch_bounds__(BLOCK) void k_st_collide_g(const StageFluid a) {
             strict_g_new(i, g[i], geqnew[i], gx, gy, gz, fx, phi, rho, gd, gd_h, gd_l, a.k);
 }
 
+// dst_i(x) <- src_i(x) for the 27 populations of one distribution (f and g of a pair buffer
+// are interleaved by runs, so a distribution is not one contiguous range).
 __global__ __launch_bounds__(BLOCK) void k_st_copy(const double *__restrict__ src,
-                                                   double *__restrict__ dst, size_t len) {
-    const size_t j = (size_t)blockIdx.x * BLOCK + threadIdx.x;
-    if (j < len) dst[j] = src[j];
+                                                   double *__restrict__ dst, int n) {
+    const int idx = blockIdx.x * BLOCK + threadIdx.x;
+    if (idx >= n) return;
+#pragma unroll
+    for (int i = 0; i < NPOP; i++) dst[pop_at(i, idx)] = src[pop_at(i, idx)];
 }
 
 // streaming (streaming.cu:3-23, thermal = 0) and streaming_scalar (streaming.cu:25-63,
@@ -647,23 +651,28 @@ __global__ __launch_bounds__(BLOCK) void k_st_pull(const double *__restrict__ sr
 }
 
 // The FP64 lattice kernel's memory traffic without its arithmetic: the same launch shape, the
-// same pull of 54 populations under the wall rules (gather), the same F_x read and write, and
-// the 54 values stored unchanged into the other buffer.  Its time is the ceiling the access
-// pattern itself reaches on the device.
+// same pull of 54 populations under the wall rules (gather, or gather_rows where the kernel uses
+// the row pull), the same F_x read and write, and the 54 values stored unchanged into the other
+// pair buffer.  Its time is the ceiling the access pattern itself reaches on the device.
+template <bool ROWS>
 __global__ __launch_bounds__(BLOCK) void k_stencil_probe(const double *__restrict__ fin,
                                                          const double *__restrict__ gin,
                                                          double *__restrict__ fout,
                                                          double *__restrict__ gout,
                                                          double *__restrict__ fx, Dims d) {
+    __shared__ RowEdges edges[ROWS ? BLOCK / 64 : 1];
     const int nblocks = (d.n + BLOCK - 1) / BLOCK;
     const int idx = xcd_block(nblocks) * BLOCK + threadIdx.x;
-    if (idx >= d.n) return;
+    if (!ROWS && idx >= d.n) return;
     const int iz = idx / d.lxy;
     const int rem = idx - iz * d.lxy;
     const int iy = rem / d.lx;
     const int ix = rem - iy * d.lx;
     double f[NPOP], g[NPOP];
-    gather<double, double, false>(fin, gin, idx, ix, iy, iz, d, 0.0, nullptr, 0.0, 0.0, f, g);
+    if constexpr (ROWS)
+        gather_rows<BLOCK / 64>(fin, gin, idx, ix, iy, iz, d, 0.0, 0.0, 0.0, edges, f, g);
+    else
+        gather<double, double, false>(fin, gin, idx, ix, iy, iz, d, 0.0, nullptr, 0.0, 0.0, f, g);
 #pragma unroll
     for (int i = 0; i < NPOP; i++) {
         Enc<double>::st(fout + pop_at(i, idx), f[i]);
@@ -805,13 +814,14 @@ int lattice_streaming_probe(int scheme, int lx, int ly, int lz, int reps, double
         return report(RBTP_E_ARG, "streaming probe: scheme 0 (pull), 1 (AA even) or 2 (AA odd)");
     Dims d;
     d.lx = lx; d.ly = ly; d.lz = lz; d.lxy = lx * ly; d.n = lx * ly * lz;
+    // two pair buffers, as a sim holds them: f at the buffer, g G_OFFSET elements after it
     const size_t len = pop_buffer_len((size_t)d.n);
-    double *buf[4] = {nullptr, nullptr, nullptr, nullptr};
+    double *buf[2] = {nullptr, nullptr};
     double *fx = nullptr;
     hipStream_t st = nullptr;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     hipError_t e = hipSuccess;
-    for (int k = 0; k < 4 && e == hipSuccess; k++) e = hipMalloc(&buf[k], len * sizeof(double));
+    for (int k = 0; k < 2 && e == hipSuccess; k++) e = hipMalloc(&buf[k], len * sizeof(double));
     if (e == hipSuccess) e = hipMalloc(&fx, (size_t)d.n * sizeof(double));
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipEventCreate(&e0);
@@ -819,16 +829,23 @@ int lattice_streaming_probe(int scheme, int lx, int ly, int lz, int reps, double
     for (int k = 0; k < 2 && e == hipSuccess; k++) e = hipMemsetAsync(buf[k], 0, len * sizeof(double), st);
     if (e == hipSuccess) e = hipMemsetAsync(fx, 0, (size_t)d.n * sizeof(double), st);
     const int nb = blocks_for((size_t)d.n);
+    const bool rows = rows_pull(d);
     int cur = 0;
     auto launch = [&]() {
+        double *f0 = buf[0], *g0 = buf[0] + G_OFFSET;
         if (scheme == 1) {
-            hipLaunchKernelGGL(k_aa_even_probe, nb, BLOCK, 0, st, buf[0], buf[1], fx, d);
+            hipLaunchKernelGGL(k_aa_even_probe, nb, BLOCK, 0, st, f0, g0, fx, d);
         } else if (scheme == 2) {
-            hipLaunchKernelGGL(k_aa_odd_probe, nb, BLOCK, 0, st, buf[0], buf[1], fx, d);
+            hipLaunchKernelGGL(k_aa_odd_probe, nb, BLOCK, 0, st, f0, g0, fx, d);
         } else {
-            hipLaunchKernelGGL(k_stencil_probe, nb, BLOCK, 0, st, buf[cur], buf[cur + 1],
-                               buf[2 - cur], buf[3 - cur], fx, d);
-            cur = 2 - cur;
+            double *fi = buf[cur], *fo = buf[1 - cur];
+            if (rows)
+                hipLaunchKernelGGL(k_stencil_probe<true>, nb, BLOCK, 0, st, fi, fi + G_OFFSET, fo,
+                                   fo + G_OFFSET, fx, d);
+            else
+                hipLaunchKernelGGL(k_stencil_probe<false>, nb, BLOCK, 0, st, fi, fi + G_OFFSET,
+                                   fo, fo + G_OFFSET, fx, d);
+            cur = 1 - cur;
         }
         return hipGetLastError();
     };
@@ -871,8 +888,6 @@ hipError_t launch_lattice_step(const StepArgs<S> &a, bool strict, bool nostream,
 
 hipError_t launch_stage_fluid(int stage, const StageFluid &a, hipStream_t st) {
     const int nb = blocks_for((size_t)a.d.n);
-    const size_t len = pop_buffer_len((size_t)a.d.n);
-    const int nbl = blocks_for(len);
     switch (stage) {
         case RBTP_STAGE_MACROVAR:
             hipLaunchKernelGGL(k_st_macrovar, nb, BLOCK, 0, st, a);
@@ -890,8 +905,8 @@ hipError_t launch_stage_fluid(int stage, const StageFluid &a, hipStream_t st) {
         case RBTP_STAGE_STREAM_G: {
             // main.cu:45,61 (the pre-stream copy), then the pull (main.cu:47-48,63-64)
             const bool g = stage == RBTP_STAGE_STREAM_G;
-            hipLaunchKernelGGL(k_st_copy, nbl, BLOCK, 0, st, (const double *)(g ? a.g : a.f),
-                               g ? a.gc : a.fc, len);
+            hipLaunchKernelGGL(k_st_copy, nb, BLOCK, 0, st, (const double *)(g ? a.g : a.f),
+                               g ? a.gc : a.fc, a.d.n);
             hipError_t e = hipGetLastError();
             if (e != hipSuccess) return e;
             hipLaunchKernelGGL(k_st_pull, nb, BLOCK, 0, st, (const double *)(g ? a.gc : a.fc),
@@ -907,12 +922,11 @@ hipError_t launch_stage_fluid(int stage, const StageFluid &a, hipStream_t st) {
 
 hipError_t launch_stage_enter(const StageFluid &a, bool poststream, hipStream_t st) {
     const int nb = blocks_for((size_t)a.d.n);
-    const size_t len = pop_buffer_len((size_t)a.d.n);
     for (int which = 0; which < 2; which++) {
         const double *src = which ? a.gc : a.fc;
         double *dst = which ? a.g : a.f;
         if (poststream)
-            hipLaunchKernelGGL(k_st_copy, blocks_for(len), BLOCK, 0, st, src, dst, len);
+            hipLaunchKernelGGL(k_st_copy, nb, BLOCK, 0, st, src, dst, a.d.n);
         else
             hipLaunchKernelGGL(k_st_pull, nb, BLOCK, 0, st, src, dst, (const double *)nullptr,
                                a.rho_wall, which, a.d, a.k.thot, a.k.tcold);

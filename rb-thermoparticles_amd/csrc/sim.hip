@@ -217,10 +217,7 @@ extern "C" void rbtp_destroy(rbtp_sim *s) {
     if (!s) return;
     (void)hipSetDevice(s->device);
     if (s->stream) (void)hipStreamSynchronize(s->stream);
-    for (int b = 0; b < 2; b++) {
-        (void)hipFree(s->f[b]);
-        (void)hipFree(s->g[b]);
-    }
+    for (int b = 0; b < 2; b++) (void)hipFree(s->f[b]);  // g[b] lies in the same pair buffer
     (void)hipFree(s->fx);
     (void)hipFree(s->rho_wall);
     for (double *m : s->macro) (void)hipFree(m);
@@ -291,10 +288,11 @@ static int create_impl(const rbtp_case *c, rbtp_sim *s, hipStream_t stream) {
     HIP_TRY(hipEventCreate(&s->ev_k0));
     HIP_TRY(hipEventCreate(&s->ev_k1));
     const size_t n = (size_t)s->dims.n;
-    const size_t pop_bytes = pop_buffer_len(n) * storage_bytes(s->prec);
+    // one pair buffer (f and g interleaved by runs) per time level
+    const size_t pair_bytes = pop_buffer_len(n) * storage_bytes(s->prec);
     for (int b = 0; b < 2; b++) {
-        HIP_TRY(hipMalloc(&s->f[b], pop_bytes));
-        HIP_TRY(hipMalloc(&s->g[b], pop_bytes));
+        HIP_TRY(hipMalloc(&s->f[b], pair_bytes));
+        s->g[b] = (char *)s->f[b] + G_OFFSET * storage_bytes(s->prec);
     }
     HIP_TRY(hipMalloc(&s->fx, n * aux_bytes(s->prec)));
     HIP_TRY(hipMalloc(&s->rho_wall, 2 * (size_t)c->ly * c->lz * aux_bytes(s->prec)));
@@ -1262,7 +1260,7 @@ static size_t args_size(int prec) {
 // Device bytes one slot allocates (create_impl, part_alloc).
 static size_t sim_device_bytes(const rbtp_case &c, const rbtp_derived &d) {
     const size_t n = (size_t)c.lx * c.ly * c.lz;
-    size_t b = 4 * pop_buffer_len(n) * storage_bytes(c.precision) + n * aux_bytes(c.precision) +
+    size_t b = 2 * pop_buffer_len(n) * storage_bytes(c.precision) + n * aux_bytes(c.precision) +
                2 * (size_t)c.ly * c.lz * aux_bytes(c.precision) + 5 * n * sizeof(double);
     if (c.activate_particles && d.npart > 0) {
         const size_t np = (size_t)d.npart;
