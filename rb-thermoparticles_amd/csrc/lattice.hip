@@ -279,15 +279,16 @@ __device__ inline void node_step(const StepArgs<S> &a, int idx, RowEdges *edges 
                                        T(a.k.thot), T(a.k.tcold), f, g);
     }
     // At a special node (IBB link node or refilled node) the populations the particle phase
-    // corrected replace the pulled ones; the first step after a (re)load has none.
+    // corrected replace the pulled ones; the first step after a (re)load has none.  The node's
+    // values lie together in its list entry (one or two cache lines).
     const int sj = (!NOSTREAM && a.sp_map) ? a.sp_map[idx] : -1;
     if (sj >= 0) {
         const uint32_t m = a.sp_mask[sj];
 #pragma unroll
         for (int i = 0; i < NPOP; i++) {
             if (m & (1u << i)) {
-                f[i] = T(a.sp_pop[(size_t)i * a.sp_cap + sj]);
-                const T gv = T(a.sp_pop[(size_t)(NPOP + i) * a.sp_cap + sj]);
+                f[i] = T(a.sp_pop[sp_at(sj, i)]);
+                const T gv = T(a.sp_pop[sp_at(sj, NPOP + i)]);
                 if constexpr (STRICT) g[i] = gv;
                 else g[i] = gv * T(1.0 / weight(i)) - ts;
             }
@@ -307,6 +308,19 @@ __global__ __launch_bounds__(BLOCK) void k_lattice_step(const StepArgs<S> a) {
     const int idx = xcd_block(nblocks) * BLOCK + threadIdx.x;
     if (idx >= a.d.n) return;
     node_step<S, T, STRICT, NOSTREAM>(a, idx);
+}
+
+// The step of the FP32-arithmetic forms with the pull: five waves per SIMD (at most 96 VGPRs,
+// no spills; the compiler's own choice, 97, allows four).  The kernel waits on memory most of
+// the time, and at 512x512x256 with 5000 spheres in FP16 storage the fifth wave took its time
+// from 3.66 to 3.36 ms; without spheres it is unchanged.
+template <class S>
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(5))) void k_lattice_step_lp(
+    const StepArgs<S> a) {
+    const int nblocks = (a.d.n + BLOCK - 1) / BLOCK;
+    const int idx = xcd_block(nblocks) * BLOCK + threadIdx.x;
+    if (idx >= a.d.n) return;
+    node_step<S, float, false, false>(a, idx);
 }
 
 // The FP64 step with the row pull, for lx a multiple of 64 and n a multiple of BLOCK: every
@@ -881,7 +895,7 @@ hipError_t launch_lattice_step(const StepArgs<S> &a, bool strict, bool nostream,
     } else {
         (void)strict;
         if (nostream) hipLaunchKernelGGL((k_lattice_step<S, float, false, true>), nb, BLOCK, 0, st, a);
-        else hipLaunchKernelGGL((k_lattice_step<S, float, false, false>), nb, BLOCK, 0, st, a);
+        else hipLaunchKernelGGL((k_lattice_step_lp<S>), nb, BLOCK, 0, st, a);
     }
     return hipGetLastError();
 }

@@ -1,6 +1,7 @@
 // particles.h — particle coupling and diagnostics (particles.hip, diag.hip).
 #pragma once
 
+#include <algorithm>
 #include <utility>
 #include <vector>
 
@@ -55,6 +56,15 @@ struct PartDev {
 inline int link_stride(int box_w) {
     const int b = box_w + 2;
     return (b * b * b + 63) / 64 * 64;
+}
+
+// Entries of the special-node list: a head of one entry per link-box node of every particle
+// (IBB link nodes, at fixed positions) and a tail for the refilled nodes, which lie in the
+// marking boxes and are distinct lattice nodes.
+inline long long special_capacity(int np, int box_w, long long n) {
+    const long long mb = (long long)box_w * box_w * box_w;
+    const long long tail = std::min((long long)np * mb, n);
+    return std::max((long long)np * link_stride(box_w) + tail, 1LL);
 }
 
 

@@ -9,8 +9,9 @@
 //     the particle and stores the node's share of force, torque and heat; k_link_reduce sums
 //     the shares of each particle in a fixed order (64 lane partials + pairwise tree),
 //     replacing the reference's FP64 atomics (ibb.cu:182-187,204) with a reproducible reduction;
-//   * k_link_nodes also lists the IBB link nodes, k_refill_find the refilled nodes: the compact
-//     list of the nodes whose post-stream populations the coupling replaces;
+//   * k_link_nodes also lists the IBB link nodes (one fixed entry per link-box node), and
+//     k_refill_find appends the refilled nodes: the list of the nodes whose post-stream
+//     populations the coupling replaces;
 //   * k_fixup_links / k_fixup_refill: those nodes' corrected post-stream populations (pull +
 //     IBB, or refill), buffered for the next lattice step.
 // Geometry is two owner maps (previous and current generation); each generation is cleared only
@@ -341,10 +342,11 @@ __global__ __launch_bounds__(CW) void k_contacts(PartGrid g) {
     double *const *P = pd.p;
     if (lane == 0) {
         // the special-node list of this step is rebuilt from here on (k_special_unmap, the last
-        // reader of the previous list, ran before this kernel)
+        // reader of the previous list, ran before this kernel): its head is one entry per
+        // link-box node, which k_link_nodes fills or marks empty; refilled nodes follow
         if (i == 0) {
-            pd.sp.count[0] = 0;
-            pd.sp.count[1] = 0;
+            pd.sp.count[0] = pd.np * pd.lnk_stride;
+            pd.sp.count[1] = pd.np * pd.lnk_stride;
         }
         for (int a = 0; a < 3; a++) {
             P[RBTP_P_FORCE_PREV2_X + a][i] = P[RBTP_P_FORCE_PREV_X + a][i];
@@ -448,10 +450,12 @@ __global__ __launch_bounds__(CW) void k_contacts(PartGrid g) {
 
 // ---- special-node list -------------------------------------------------------------------------
 
-// Appends the nodes of the calling wave's lanes that have `want` set, in lane order, with one
-// atomic per wave: consecutive box nodes stay consecutive in the list, which keeps the later
-// passes over the list coalesced.  Every active lane of the wave must call it.  Returns the
-// lane's list position, or -1 (not appended, or the list is full).
+// Appends the nodes of the calling wave's lanes that have `want` set to the list's tail (the
+// refilled nodes), in lane order, with one atomic per wave.  Every active lane of the wave must
+// call it.  Returns the lane's list position, or -1 (not appended, or the list is full).  The
+// IBB link nodes do not come through here: a counter that every wave of k_link_nodes
+// incremented cost more than the rest of that kernel at 5000 spheres, so they have fixed
+// positions in the list's head.
 __device__ inline int push_special(const PartDev &pd, bool want, int idx, int kind) {
     const unsigned long long mask = __ballot(want);
     if (mask == 0) return -1;
@@ -467,7 +471,6 @@ __device__ inline int push_special(const PartDev &pd, bool want, int idx, int ki
     if (j < pd.sp.cap) {
         gbl(pd.sp.node)[j] = idx;
         gbl(pd.sp_kind)[j] = kind;
-        gbl(pd.sp.map)[idx] = j;
         return j;
     }
     atomicOr(pd.err + ERR_SPECIAL_CAP, 1);
@@ -497,6 +500,32 @@ __device__ inline PView pview_now(const PartDev &pd) {
 __device__ inline PView pview_old(const PartDev &pd) {
     return pview(pd.old, OLD_PX, OLD_PY, OLD_PZ, OLD_VX, OLD_VY, OLD_VZ, OLD_WX, OLD_WY, OLD_WZ,
                  OLD_T);
+}
+
+// The state of one particle that the bounce-back of its links reads, loaded once per node
+// rather than once per link (the stores of the link values could alias these arrays, so the
+// compiler would otherwise reload them after every link).
+struct PState {
+    double px, py, pz, vx, vy, vz, wx, wy, wz, temp;
+};
+__device__ inline PState pstate(const PView &pv, int pid) {
+    return PState{pv.px[pid], pv.py[pid], pv.pz[pid], pv.vx[pid], pv.vy[pid],
+                  pv.vz[pid], pv.wx[pid], pv.wy[pid], pv.wz[pid], pv.temp[pid]};
+}
+// A double that every active lane holds the same value of, moved to scalar registers.
+__device__ inline double wave_uniform(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readfirstlane((int)b);
+    const int hi = __builtin_amdgcn_readfirstlane((int)(b >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+// pstate of a particle that is the same for the whole wave.
+__device__ inline PState pstate_uniform(const PView &pv, int pid) {
+    const PState s = pstate(pv, pid);
+    return PState{wave_uniform(s.px), wave_uniform(s.py), wave_uniform(s.pz),
+                  wave_uniform(s.vx), wave_uniform(s.vy), wave_uniform(s.vz),
+                  wave_uniform(s.wx), wave_uniform(s.wy), wave_uniform(s.wz),
+                  wave_uniform(s.temp)};
 }
 
 template <class S>
@@ -555,7 +584,7 @@ __device__ inline LinkLoads link_loads(int ip, int idx, int idx_up, bool up_flui
 // L; returns the values they write into f[opp][idx] and g[opp][idx] plus what the force and
 // heat sums read (ibb.cu:160-180,202-203).
 __device__ inline LinkOut eval_link_at(const LinkLoads &L, int ip, int ix, int iy, int iz,
-                                       bool up_fluid, double q, int pid, const PView &pv,
+                                       bool up_fluid, double q, const PState &ps,
                                        double density, double rho0) {
     LinkOut o;
     const double tp = wt(ip);
@@ -563,11 +592,11 @@ __device__ inline LinkOut eval_link_at(const LinkLoads &L, int ip, int ix, int i
     const double cx = static_cast<double>(ix) + 0.5 + q * dir_cx(ip);
     const double cy = static_cast<double>(iy) + 0.5 + q * dir_cy(ip);
     const double cz = static_cast<double>(iz) + 0.5 + q * dir_cz(ip);
-    const double relx = cx - pv.px[pid], rely = cy - pv.py[pid], relz = cz - pv.pz[pid];
-    const double wx = pv.wx[pid], wy = pv.wy[pid], wz = pv.wz[pid];
-    const double uwx = pv.vx[pid] + (relz * wy - rely * wz);
-    const double uwy = pv.vy[pid] + (relx * wz - relz * wx);
-    const double uwz = pv.vz[pid] + (rely * wx - relx * wy);
+    const double relx = cx - ps.px, rely = cy - ps.py, relz = cz - ps.pz;
+    const double wx = ps.wx, wy = ps.wy, wz = ps.wz;
+    const double uwx = ps.vx + (relz * wy - rely * wz);
+    const double uwy = ps.vy + (relx * wz - relz * wx);
+    const double uwz = ps.vz + (rely * wx - relx * wy);
     o.uwx = uwx; o.uwy = uwy; o.uwz = uwz;
     const double evel = dir_cx(ip) * uwx + dir_cy(ip) * uwy + dir_cz(ip) * uwz;
     const double RT = 1.0 / 3.0;
@@ -586,7 +615,7 @@ __device__ inline LinkOut eval_link_at(const LinkLoads &L, int ip, int ix, int i
         o.f_out = f_here - momentum_term;
     }
     // thermal (ibb.cu:109-135)
-    const double Tp = pv.temp[pid];
+    const double Tp = ps.temp;
     const double amp = 2.0 * tp * Tp * (1.0 + density);
     const double g_here = L.g_here;
     o.g_in = g_here;
@@ -644,7 +673,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
         interior = lo[0] > 0 && lo[1] > 0 && lo[2] > 0 && hi[0] < d.lx - 1 &&
                    hi[1] < d.ly - 1 && hi[2] < d.lz - 1;
     }
-    const double X = pv.px[p], Y = pv.py[p], Z = pv.pz[p];
+    const double X = wave_uniform(pv.px[p]), Y = wave_uniform(pv.py[p]), Z = wave_uniform(pv.pz[p]);
     int ix = 0, iy = 0, iz = 0, idx = 0;
     // links of this node to p, as a direction mask (q is evaluated again below), whether it also
     // links to a sphere of lower pid (a node linked to several spheres is listed once, by the
@@ -691,8 +720,19 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
             }
         }
     }
+    // The node's entry in the special list is its box-node slot p * lnk_stride + k (the list
+    // head is one entry per link-box node, no counter): a listed node writes it below, every
+    // other slot is marked empty.
+    const int e = p * pd.lnk_stride + k;
+    const int j = (mask != 0u && !lower) ? e : -1;
+    if (j >= 0) {
+        gbl(pd.sp.node)[j] = idx;
+        gbl(pd.sp_kind)[j] = -1;
+        gbl(pd.sp.map)[idx] = j;
+    } else {
+        gbl(pd.sp.mask)[e] = 0u;
+    }
     // every lane of the wave is here
-    const int j = push_special(pd, mask != 0u && !lower, idx, -1);
     const unsigned long long has = __ballot(mask != 0u);
     if (__lane_id() == 0) gbl(pd.lnk_bits)[(size_t)p * (pd.lnk_stride / 64) + k / 64] = has;
     if (mask == 0u) return;
@@ -700,8 +740,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
                          gbl(pd.gshift), d.lx};
     using A = typename Enc<S>::A;
     const gptr<A> out = gbl((A *)pd.sp.pop);
-    const size_t cap = (size_t)pd.sp.cap;
     const double density = gbl((const double *)sl.macro[0])[idx];
+    const PState ps = pstate_uniform(pv, p);  // p is the same for the whole wave
+    const double radius_sq = wave_uniform(pd.radius_sq), rho0 = wave_uniform(pd.rho0);
     double v[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     uint32_t replaced = 0u;
     // links in ascending direction order; each link's lattice loads are issued together
@@ -713,12 +754,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
         const bool up_fluid = (fluid_nb >> opp) & 1u;  // x - c_ip = x + c_opp
         const int idx_up = idx - (cx + cy * d.lx + cz * d.lxy);
         double q = 0.0;
-        link_q(ip, x0, y0, z0, X, Y, Z, pd.radius_sq, q);
+        link_q(ip, x0, y0, z0, X, Y, Z, radius_sq, q);
         const LinkLoads Ld = link_loads<S>(ip, idx, idx_up, up_fluid, q, pop);
-        const LinkOut L = eval_link_at(Ld, ip, ix, iy, iz, up_fluid, q, p, pv, density, pd.rho0);
+        const LinkOut L = eval_link_at(Ld, ip, ix, iy, iz, up_fluid, q, ps, density, rho0);
         if (j >= 0) {
-            out[(size_t)opp * cap + j] = A(L.f_out);
-            out[(size_t)(NPOP + opp) * cap + j] = A(L.g_out);
+            out[sp_at(j, opp)] = A(L.f_out);
+            out[sp_at(j, NPOP + opp)] = A(L.g_out);
             replaced |= 1u << opp;
         }
         // Force uses the link record's r = q c (links.cu:134-136, ibb.cu:160-168).
@@ -741,7 +782,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     }
     if (j >= 0) gbl(pd.sp.mask)[j] = replaced | (higher ? SP_MULTI : 0u);
     const size_t plane = (size_t)pd.np * pd.lnk_stride;
-    const size_t e = (size_t)p * pd.lnk_stride + k;
     const gptr<double> part = gbl(pd.lnk_part);
     for (int c = 0; c < 7; c++) part[c * plane + e] = v[c];
 }
@@ -796,18 +836,20 @@ __global__ __launch_bounds__(256) void k_link_reduce(PartGrid g) {
 
 // ---- motion -------------------------------------------------------------------------------------
 
-// update_particles_kernel (move.cu:5-103), after saving the state the IBB of this step used.
 // Entries of the previous step's special-node list leave the node map (the lattice kernel that
-// read them ran before this kernel); slot s uses workgroups sstart[s] .. sstart[s+1]-1.  The
-// list itself is cleared by k_contacts, which runs next.
+// read them ran before this kernel); slot s uses workgroups sstart[s] .. sstart[s+1]-1.  Empty
+// entries of the list head have a zero mask.  The list itself is rebuilt from k_contacts on.
 __global__ __launch_bounds__(128) void k_special_unmap(PartGrid g) {
     const int slot = slot_of(g.sstart, g.nslots, blockIdx.x);
     const PartDev &pd = g.tab[slot].pd;
     const int blk = blockIdx.x - g.sstart[slot];
     const int nblk = g.sstart[slot + 1] - g.sstart[slot];
-    const int count = min(*pd.sp.count, pd.sp.cap);
-    for (int j = blk * 128 + threadIdx.x; j < count; j += nblk * 128) pd.sp.map[pd.sp.node[j]] = -1;
+    const int count = min(pd.sp.count[0], pd.sp.cap);
+    for (int j = blk * 128 + threadIdx.x; j < count; j += nblk * 128)
+        if (pd.sp.mask[j] != 0u) pd.sp.map[pd.sp.node[j]] = -1;
 }
+
+// update_particles_kernel (move.cu:5-103), after saving the state the IBB of this step used.
 
 __global__ __launch_bounds__(128) void k_move(PartGrid g) {
     const int u = blockIdx.x * 128 + threadIdx.x;
@@ -815,8 +857,6 @@ __global__ __launch_bounds__(128) void k_move(PartGrid g) {
     const int s = slot_of(g.pstart, g.nslots, u);
     const PartDev &pd = g.tab[s].pd;
     const int pid = u - g.pstart[s];
-    // k_link_nodes, which ran before, listed the IBB link nodes: they head the list
-    if (pid == 0) pd.sp.count[1] = pd.sp.count[0];
     double *const *P = pd.p;
     const int src[OLD_COUNT] = {RBTP_P_POS_X, RBTP_P_POS_Y, RBTP_P_POS_Z, RBTP_P_VEL_X,
                                 RBTP_P_VEL_Y, RBTP_P_VEL_Z, RBTP_P_OMEGA_X, RBTP_P_OMEGA_Y,
@@ -901,7 +941,7 @@ __global__ __launch_bounds__(LB) void k_refill_find(PartGrid g) {
     }
 }
 
-// Bounce-back values of the IBB link nodes that link to more than one sphere (SP_MULTI entries at
+// Bounce-back values of the IBB link nodes that link to more than one sphere (SP_MULTI entries in
 // the head of the special list; k_link_nodes wrote every other link node): for every link
 // x -> x + c_ip (x + c_ip solid and linked, with the state before this step's motion) the values
 // bounce_back_particles / bounce_back_thermal_particles write into f[opp][x] and g[opp][x]
@@ -924,7 +964,6 @@ __global__ __launch_bounds__(128) void k_fixup_links(PartGrid g) {
     const int wave = (blockIdx.x - g.lstart[slot]) * 2 + (threadIdx.x >> 6);
     const int nwave = (g.lstart[slot + 1] - g.lstart[slot]) * 2;
     const int nlink = min(pd.sp.count[1], pd.sp.cap);
-    const size_t cap = (size_t)pd.sp.cap;
     const gptr<A> out = gbl((A *)pd.sp.pop);
     const gptr<uint32_t> spmask = gbl(pd.sp.mask);
     const PView pv = pview_old(pd);
@@ -961,10 +1000,10 @@ __global__ __launch_bounds__(128) void k_fixup_links(PartGrid g) {
                     const int idx_up = idx - (dir_cx(ip) + dir_cy(ip) * d.lx + dir_cz(ip) * d.lxy);
                     const double density = macro_rho[idx];
                     const LinkLoads Ld = link_loads<S>(ip, idx, idx_up, up_fluid, q, pop);
-                    const LinkOut L = eval_link_at(Ld, ip, ix, iy, iz, up_fluid, q, own, pv,
+                    const LinkOut L = eval_link_at(Ld, ip, ix, iy, iz, up_fluid, q, pstate(pv, own),
                                                    density, pd.rho0);
-                    out[(size_t)opp * cap + j] = A(L.f_out);
-                    out[(size_t)(NPOP + opp) * cap + j] = A(L.g_out);
+                    out[sp_at(j, opp)] = A(L.f_out);
+                    out[sp_at(j, NPOP + opp)] = A(L.g_out);
                     bit = 1u << opp;
                 }
             }
@@ -1001,7 +1040,6 @@ __global__ __launch_bounds__(128) void k_fixup_refill(PartGrid g, double *fo = n
     const int grp = threadIdx.x >> 5, lane = threadIdx.x & 31;
     const int count = min(pd.sp.count[0], pd.sp.cap);
     const int first = min(pd.sp.count[1], pd.sp.cap);
-    const size_t cap = (size_t)pd.sp.cap;
     A *out = (A *)pd.sp.pop;
     const double lambda0 = 1.0 - wt(0);
     const double RT = 1.0 / 3.0;
@@ -1065,8 +1103,8 @@ __global__ __launch_bounds__(128) void k_fixup_refill(PartGrid g, double *fo = n
                 fo[pop_at(lane, idx)] = fv;
                 go[pop_at(lane, idx)] = gv;
             } else {
-                out[(size_t)lane * cap + j] = A(fv);
-                out[(size_t)(NPOP + lane) * cap + j] = A(gv);
+                out[sp_at(j, lane)] = A(fv);
+                out[sp_at(j, NPOP + lane)] = A(gv);
             }
         }
         // the group's 27 populations, read across its lanes in population order (all 32 lanes
@@ -1096,6 +1134,7 @@ __global__ __launch_bounds__(128) void k_fixup_refill(PartGrid g, double *fo = n
             macro_phi[idx] = phi_loc;
             fx_field[idx] = A(0.0);
             pd.sp.mask[j] = (1u << NPOP) - 1u;
+            if constexpr (!STAGED) pd.sp.map[idx] = j;
         }
     }
 }
@@ -1107,15 +1146,15 @@ template <class A, bool BLOCKED = false>
 __global__ __launch_bounds__(256) void k_overwrite_special(PartDev pd, double *f, double *g, Dims d) {
     const A *pop = (const A *)pd.sp.pop;
     const int count = min(*pd.sp.count, pd.sp.cap);
-    const size_t cap = (size_t)pd.sp.cap, n = (size_t)d.n;
+    const size_t n = (size_t)d.n;
     for (int j = blockIdx.x * 256 + threadIdx.x; j < count; j += gridDim.x * 256) {
         const int idx = pd.sp.node[j];
         const uint32_t m = pd.sp.mask[j];
         for (int i = 0; i < NPOP; i++) {
             if (!(m & (1u << i))) continue;
             const size_t at = BLOCKED ? pop_at(i, idx) : (size_t)i * n + idx;
-            f[at] = double(pop[(size_t)i * cap + j]);
-            g[at] = double(pop[(size_t)(NPOP + i) * cap + j]);
+            f[at] = double(pop[sp_at(j, i)]);
+            g[at] = double(pop[sp_at(j, NPOP + i)]);
         }
     }
 }
@@ -1153,7 +1192,7 @@ __global__ __launch_bounds__(256) void k_st_links(PartGrid g, double *fo, double
         grow_box(d, lo, hi);
         in_box = k < box_count(lo, hi);
     }
-    const double X = pv.px[p], Y = pv.py[p], Z = pv.pz[p];
+    const double X = wave_uniform(pv.px[p]), Y = wave_uniform(pv.py[p]), Z = wave_uniform(pv.pz[p]);
     unsigned mask = 0u, fluid_nb = 0u;
     int ix = 0, iy = 0, iz = 0, idx = 0;
     double x0 = 0.0, y0 = 0.0, z0 = 0.0;
@@ -1181,6 +1220,7 @@ __global__ __launch_bounds__(256) void k_st_links(PartGrid g, double *fo, double
     const PopView<double> pop{gbl((const double *)sl.fcur), gbl((const double *)sl.gcur),
                               (size_t)d.n, gbl(pd.gshift), d.lx};
     const double density = sl.macro[0][idx];
+    const PState ps = pstate_uniform(pv, p);  // p is the same for the whole wave
     double v[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     while (mask) {
         const int ip = __builtin_ctz(mask);
@@ -1192,7 +1232,7 @@ __global__ __launch_bounds__(256) void k_st_links(PartGrid g, double *fo, double
         double q = 0.0;
         link_q(ip, x0, y0, z0, X, Y, Z, pd.radius_sq, q);
         const LinkLoads Ld = link_loads<double>(ip, idx, idx_up, up_fluid, q, pop);
-        const LinkOut L = eval_link_at(Ld, ip, ix, iy, iz, up_fluid, q, p, pv, density, pd.rho0);
+        const LinkOut L = eval_link_at(Ld, ip, ix, iy, iz, up_fluid, q, ps, density, pd.rho0);
         if (MODE == STL_IBB_F) fo[pop_at(opp, idx)] = L.f_out;
         if (MODE == STL_IBB_G) go[pop_at(opp, idx)] = L.g_out;
         if (MODE == STL_FORCES) {
@@ -1413,26 +1453,25 @@ int part_alloc(PartDev &pd, const rbtp_case &c, const rbtp_derived &d, const Dim
     }
     // Marking boxes span at most ceil(2R)+1 nodes per axis; link boxes two more.
     pd.box_w = (int)std::ceil(2.0 * pd.radius) + 1;
-    const long long lb = (long long)(pd.box_w + 2) * (pd.box_w + 2) * (pd.box_w + 2);
-    const long long mb = (long long)pd.box_w * pd.box_w * pd.box_w;
-    long long cap = (long long)np * (lb + mb);
-    if (cap > dims.n) cap = dims.n;
-    if (cap < 1) cap = 1;
+    pd.lnk_stride = link_stride(pd.box_w);
+    if ((long long)pd.np * pd.lnk_stride > INT_MAX)
+        return report(RBTP_E_CAPACITY, "too many particles for the link-box thread grid");
+    const long long cap = special_capacity(pd.np, pd.box_w, dims.n);
+    if (cap > INT_MAX) return report(RBTP_E_CAPACITY, "special-node list beyond 2^31 entries");
     pd.sp.cap = (int)cap;
     PHIP(hipMalloc(&pd.sp.node, cap * sizeof(int32_t)));
+    PHIP(hipMemset(pd.sp.node, 0, cap * sizeof(int32_t)));
     PHIP(hipMalloc(&pd.sp_kind, cap * sizeof(int32_t)));
     pd.sp_elem = c.precision == RBTP_PREC_FP64 ? sizeof(double) : sizeof(float);
     PHIP(hipMalloc(&pd.sp.pop, (size_t)2 * NPOP * cap * pd.sp_elem));
     PHIP(hipMalloc(&pd.sp.mask, cap * sizeof(uint32_t)));
+    PHIP(hipMemset(pd.sp.mask, 0, cap * sizeof(uint32_t)));
     PHIP(hipMalloc(&pd.sp.count, 2 * sizeof(int)));
     PHIP(hipMemset(pd.sp.count, 0, 2 * sizeof(int)));
     PHIP(hipMalloc(&pd.sp.map, (size_t)dims.n * sizeof(int32_t)));
     PHIP(hipMemset(pd.sp.map, 0xFF, (size_t)dims.n * sizeof(int32_t)));
     PHIP(hipMalloc(&pd.err, ERR_COUNT * sizeof(int32_t)));
     PHIP(hipMemset(pd.err, 0, ERR_COUNT * sizeof(int32_t)));
-    pd.lnk_stride = link_stride(pd.box_w);
-    if ((long long)pd.np * pd.lnk_stride > INT_MAX)
-        return report(RBTP_E_CAPACITY, "too many particles for the link-box thread grid");
     PHIP(hipMalloc(&pd.lnk_part, 7 * np * (size_t)pd.lnk_stride * sizeof(double)));
     PHIP(hipMalloc(&pd.lnk_bits, np * (size_t)(pd.lnk_stride / 64) * sizeof(unsigned long long)));
     // Pair-search grid: cells 1% wider than the pair range so that rounding in the binning
@@ -1794,16 +1833,19 @@ int part_collect_links(PartDev &pd, int gen, bool snapshot, const Dims &dims, hi
 int part_collect_new_fluid(PartDev &pd, hipStream_t st, std::vector<std::pair<int32_t, int32_t>> &out) {
     out.clear();
     if (!pd.special_pending) return RBTP_OK;
-    int count = 0;
-    PHIP(hipMemcpyAsync(&count, pd.sp.count, sizeof(int), hipMemcpyDeviceToHost, st));
+    // the refilled nodes are the list's tail, entries count[1] .. count[0]-1
+    int cnt[2] = {0, 0};
+    PHIP(hipMemcpyAsync(cnt, pd.sp.count, sizeof(cnt), hipMemcpyDeviceToHost, st));
     PHIP(hipStreamSynchronize(st));
-    count = std::min(count, pd.sp.cap);
-    std::vector<int32_t> nodes(count), kinds(count);
-    if (count > 0) {
-        PHIP(copy_to_host(nodes.data(), pd.sp.node, count * sizeof(int32_t), st));
-        PHIP(copy_to_host(kinds.data(), pd.sp_kind, count * sizeof(int32_t), st));
+    const int count = std::min(cnt[0], pd.sp.cap);
+    const int first = std::min(cnt[1], count);
+    const int m = count - first;
+    std::vector<int32_t> nodes(m), kinds(m);
+    if (m > 0) {
+        PHIP(copy_to_host(nodes.data(), pd.sp.node + first, m * sizeof(int32_t), st));
+        PHIP(copy_to_host(kinds.data(), pd.sp_kind + first, m * sizeof(int32_t), st));
     }
-    for (int j = 0; j < count; j++)
+    for (int j = 0; j < m; j++)
         if (kinds[j] >= 0) out.emplace_back(nodes[j], kinds[j]);
     return RBTP_OK;
 }

@@ -120,12 +120,11 @@ template <class S> struct StepArgs {
     double *rho_full;              // rho of this step at every node (particle coupling), or null
     // Special nodes (particle coupling), or null: sp_map[idx] is the node's entry j in the
     // special list (-1 elsewhere); bit i of sp_mask[j] marks population i as replaced by its
-    // corrected post-stream value (f: sp_pop[i*sp_cap + j], g: sp_pop[(27+i)*sp_cap + j], in the
+    // corrected post-stream value (f: sp_pop[sp_at(j, i)], g: sp_pop[sp_at(j, 27 + i)], in the
     // auxiliary type).
     const int32_t *sp_map;
     const uint32_t *sp_mask;
     const typename Enc<S>::A *sp_pop;
-    size_t sp_cap;
     double *macro_rho, *macro_ux, *macro_uy, *macro_uz, *macro_phi;  // output fields, or null
     const float *gshift;  // [lx] shift profile of the reduced-precision g encoding
     Dims d;
@@ -139,14 +138,19 @@ template <class S> struct StepArgs {
 // node replaces all 27 (refill.cu:214-215).
 struct SpecialBuf {
     int *node;       // lattice index, [cap]
-    void *pop;       // [54][cap]: f_0..f_26 then g_0..g_26 (post-stream, post-IBB/refill), in
-                     // the auxiliary type of the precision form (FP64 only for the FP64 form;
-                     // the other forms round them to FP32 when their lattice kernel reads them)
+    void *pop;       // [cap][54]: per entry f_0..f_26 then g_0..g_26 (post-stream,
+                     // post-IBB/refill; sp_at), in the auxiliary type of the precision form
+                     // (FP64 for the FP64 form, FP32 for the others).  Entry-major: an entry's
+                     // values share one or two cache lines, which its writer and its reader
+                     // (one lattice thread) each touch once.
     uint32_t *mask;  // [cap]: bit i set when population i (f and g) is replaced
     int *count;      // device counters: [0] list length, [1] IBB link nodes at its head
     int cap;
     int32_t *map;    // [n]: list entry of each node, -1 when it is not special
 };
+
+// Position of value v (f_i: v = i, g_i: v = 27 + i) of special-list entry j in SpecialBuf::pop.
+__host__ __device__ inline size_t sp_at(int j, int v) { return (size_t)j * (2 * NPOP) + v; }
 
 // Synchronous copies between caller-owned host memory and the device on stream st, staged
 // through library-owned pinned buffers (xfer.hip).

@@ -366,7 +366,6 @@ static StepArgs<S> step_args(rbtp_sim *s, bool macro_out) {
     a.sp_map = s->particles ? s->pd.sp.map : nullptr;
     a.sp_pop = s->particles ? (const typename Enc<S>::A *)s->pd.sp.pop : nullptr;
     a.sp_mask = s->particles ? s->pd.sp.mask : nullptr;
-    a.sp_cap = s->particles ? (size_t)s->pd.sp.cap : 0;
     if (macro_out) {
         a.macro_rho = s->macro[0];
         a.macro_ux = s->macro[1];
@@ -1265,8 +1264,7 @@ static size_t sim_device_bytes(const rbtp_case &c, const rbtp_derived &d) {
     if (c.activate_particles && d.npart > 0) {
         const size_t np = (size_t)d.npart;
         const long long w = (long long)std::ceil(2.0 * d.particle_radius) + 1;
-        long long cap = (long long)np * ((w + 2) * (w + 2) * (w + 2) + w * w * w);
-        if (cap > (long long)n) cap = (long long)n;
+        const long long cap = special_capacity((int)np, (int)w, (long long)n);
         b += (RBTP_P_COUNT + OLD_COUNT) * np * sizeof(double) + 3 * n * sizeof(int32_t) +
              12 * np * sizeof(int32_t) +
              (size_t)cap * (2 * 27 * aux_bytes(c.precision) + sizeof(double) + 2 * sizeof(int32_t)) +

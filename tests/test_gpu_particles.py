@@ -138,7 +138,20 @@ def test_moving_spheres_row_pull(rb, O):
     kw = particle_case(64, 16, 16, npart=3, diam=6.0, ra=1e5, beta=0.5)
     pos = [(3.4, 8.0, 8.0), (60.7, 8.5, 7.0), (32.0, 8.0, 4.0)]
     sim, ora = make_pair(rb, O, kw, pos, ptemp=np.array([0.3, -0.2, 0.0]))
-    for k in range(3):
+    # the first steps refill nodes of the first and last column (ix = 0, 63): the populations
+    # those nodes pull through the thermal x wall come from the special list itself
+    wall_refills = 0
+    for s in range(10):
+        sim.step(1)
+        ora.step(1)
+        nf_g, _ = sim.new_fluid()
+        nf_c, _ = ora.new_fluid()
+        assert np.array_equal(nf_g, nf_c), f"step {s + 1}: new fluid"
+        wall_refills += int(np.count_nonzero(np.isin(nf_c % 64, (0, 63))))
+        if wall_refills:
+            check_fields(sim, ora, f"step {s + 1}")
+    assert wall_refills > 0
+    for k in range(1, 3):
         sim.step(10)
         ora.step(10)
         check_particles(sim, ora, f"block {k}")
