@@ -15,6 +15,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -1243,6 +1244,7 @@ struct rbtp_batch {
     hipGraph_t graph;
     hipGraphExec_t exec;
     bool graph_valid;
+    bool capturing;  // batch_step_eager is being recorded into the graph
     double last_ms;
     int64_t graph_launches;
 };
@@ -1361,9 +1363,6 @@ extern "C" int rbtp_batch_admit(rbtp_batch *b, const rbtp_case *c, const double 
     rbtp_sim *s = nullptr;
     RB_TRY(create_on_stream(c, &s, b->stream));
     s->batch_slot = true;
-    // Batched steps have no per-slot refresh of the shift profile; the admission step, taken
-    // through the single-sim path, must not refresh it either.
-    s->shift_interval = 0;
     int r = rbtp_init(s, temperature, ppos, ptemp);
     if (r == RBTP_OK) r = step_impl(s, 1, 0, true);  // first step reads the post-stream start
     if (r != RBTP_OK) {
@@ -1489,9 +1488,32 @@ static int batch_step_eager(rbtp_batch *b, bool macro) {
             s->pd.special_pending = true;
         }
     }
-    for (int k : b->active) b->slot[k]->step++;
+    for (int k : b->active) {
+        rbtp_sim *s = b->slot[k];
+        s->step++;
+        // each slot's own shift refresh, at the same steps and in the same place of the step as
+        // in rbtp_step (after the particle phase); never inside a captured graph, whose replays
+        // rbtp_batch_step keeps clear of every slot's refresh steps
+        if (!b->capturing && s->shift_interval > 0 && s->step % s->shift_interval == 0)
+            HIP_TRY(dispatch(s->prec, [&](auto *tag) {
+                using S = std::remove_pointer_t<decltype(tag)>;
+                return launch_shift_refresh<S>((S *)s->g[s->cur], s->dims, s->gshift,
+                                               s->shift_partial, s->shift_delta, b->stream);
+            }));
+    }
     b->phase ^= 1;
     return RBTP_OK;
+}
+
+// Batched steps from now until the first one after which some slot refreshes its shift profile
+// (that step included); INT_MAX when no slot refreshes.
+static int steps_to_refresh(const rbtp_batch *b) {
+    int room = INT_MAX;
+    for (int k : b->active) {
+        const rbtp_sim *s = b->slot[k];
+        if (s->shift_interval > 0) room = std::min(room, s->shift_interval - s->step % s->shift_interval);
+    }
+    return room;
 }
 
 static int batch_capture(rbtp_batch *b) {
@@ -1500,7 +1522,9 @@ static int batch_capture(rbtp_batch *b) {
     for (int k : b->active) steps.push_back(b->slot[k]->step);
     HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
     int r = RBTP_OK;
+    b->capturing = true;
     for (int j = 0; j < b->graph_steps && r == RBTP_OK; j++) r = batch_step_eager(b, false);
+    b->capturing = false;
     hipGraph_t g = nullptr;
     const hipError_t e = hipStreamEndCapture(b->stream, &g);
     for (size_t j = 0; j < b->active.size(); j++) b->slot[b->active[j]]->step = steps[j];
@@ -1530,20 +1554,21 @@ extern "C" int rbtp_batch_step(rbtp_batch *b, int nsteps, unsigned flags) {
     const bool macro_last = (flags & RBTP_STEP_MACRO_LAST) != 0;
     int body = macro_last ? nsteps - 1 : nsteps;
     HIP_TRY(hipEventRecord(b->ev0, b->stream));
-    if (b->use_graph && body >= b->graph_steps + b->phase) {
-        if (b->phase == 1) {
-            RB_TRY(batch_step_eager(b, false));
-            body--;
-        }
-        if (!b->graph_valid) RB_TRY(batch_capture(b));
-        while (body >= b->graph_steps) {
+    // Graph replays (captured from phase 0, an even number of steps) where a whole one fits and
+    // no slot refreshes its shift profile inside it; single eager steps elsewhere.
+    while (body > 0) {
+        if (b->use_graph && b->phase == 0 && body >= b->graph_steps &&
+            steps_to_refresh(b) > b->graph_steps) {
+            if (!b->graph_valid) RB_TRY(batch_capture(b));
             HIP_TRY(hipGraphLaunch(b->exec, b->stream));
             b->graph_launches++;
             for (int k : b->active) b->slot[k]->step += b->graph_steps;
             body -= b->graph_steps;
+        } else {
+            RB_TRY(batch_step_eager(b, false));
+            body--;
         }
     }
-    while (body-- > 0) RB_TRY(batch_step_eager(b, false));
     if (macro_last) RB_TRY(batch_step_eager(b, true));
     HIP_TRY(hipEventRecord(b->ev1, b->stream));
     HIP_TRY(hipEventSynchronize(b->ev1));

@@ -248,3 +248,45 @@ def test_particles_enabled_with_none(rb, O):
     ora.step(6)
     check_fields(sim, ora, "no particles")
     assert len(sim.links()) == 0
+
+
+# Bounds about 10x above the deviations from FP64 measured after 300 steps of the moving-sphere
+# case below (FP32: position 3.7e-6, velocity 9.1e-7, temperature 2.2e-7, force 1.4e-6, phi
+# 9.9e-7; FP16 storage: 3.6e-5, 1.8e-5, 8.3e-6, 5.8e-4, 8.6e-5; BF16 storage: 6.1e-4, 1.8e-4,
+# 1.2e-4, 3.1e-3, 8.3e-4).  Position in lattice units, the rest relative to the FP64 maximum.
+LOW_PRECISION_PARTICLE_BOUNDS = {
+    1: dict(pos=4e-5, vel=1e-5, temp=3e-6, force=2e-5, phi=1e-5),
+    2: dict(pos=4e-4, vel=2e-4, temp=1e-4, force=6e-3, phi=1e-3),
+    3: dict(pos=6e-3, vel=2e-3, temp=1.2e-3, force=3e-2, phi=8e-3),
+}
+
+
+@pytest.mark.parametrize("prec", [1, 2, 3])
+def test_low_precision_particles_track_fp64(rb, prec):
+    """FP32 arithmetic with FP32, FP16 or BF16 storage against the FP64 form on four moving,
+    heated spheres: the particle phase (IBB, sums, motion, refill) runs in FP64 in every form,
+    so the deviations come from the stored populations alone and stay within the bounds above
+    over 300 steps."""
+    kw = particle_case(40, 32, 32, npart=4, diam=6.0, ra=1e5, beta=0.5)
+    pos = [(10.0, 10.0, 10.0), (28.5, 16.0, 15.0), (20.0, 22.0, 22.0), (12.0, 24.0, 9.0)]
+    ppos = np.asarray(pos, dtype=np.float64).T.reshape(-1)
+    ptemp = np.array([0.3, -0.2, 0.0, 0.1])
+    sims = {}
+    for p in (0, prec):
+        case = rb.default_case(**kw, precision=p)
+        sims[p] = rb.Sim(case)
+        sims[p].init(temperature=rb.linear_temperature(case, 3, 1e-2), ppos=ppos, ptemp=ptemp)
+        sims[p].step(300)
+    ref, low = sims[0], sims[prec]
+    bound = LOW_PRECISION_PARTICLE_BOUNDS[prec]
+    dpos = max(float(np.max(np.abs(low.particles(k) - ref.particles(k))))
+               for k in ("pos_x", "pos_y", "pos_z"))
+    assert dpos <= bound["pos"], dpos
+    vref = [ref.particles(k) for k in ("vel_x", "vel_y", "vel_z")]
+    vmax = max(float(np.max(np.abs(v))) for v in vref)
+    dvel = max(float(np.max(np.abs(low.particles(k) - v)))
+               for k, v in zip(("vel_x", "vel_y", "vel_z"), vref)) / vmax
+    assert dvel <= bound["vel"], dvel
+    assert rel_err(low.particles("temp"), ref.particles("temp")) <= bound["temp"]
+    assert rel_err(low.particles("force_x"), ref.particles("force_x")) <= bound["force"]
+    assert rel_err(low.field("phi"), ref.field("phi")) <= bound["phi"]

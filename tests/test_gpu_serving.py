@@ -63,7 +63,10 @@ def test_alone_equals_mixed(rb, prec):
 
 
 def test_batch_matches_single_sim(rb):
-    """A batched slot and a standalone rbtp_sim of the same case agree bitwise."""
+    """A batched slot and a standalone rbtp_sim of the same case agree bitwise, also past the
+    FP16 form's shift refreshes (every 500 steps): the batch refreshes each slot at the same
+    steps, between graph replays, as rbtp_step does.  A standalone run without refreshes
+    differs, so the comparison sees them."""
     kw = dict(REQS[2], precision=2)
     case = rb.default_case(**kw)
     T = rb.linear_temperature(case, 12, 1e-3)
@@ -76,6 +79,21 @@ def test_batch_matches_single_sim(rb):
     b = batch.sim(s)
     for k in ("f", "g"):
         assert np.array_equal(b.populations(k), sim.populations(k)), k
+    for k in ("pos_x", "vel_x", "temp", "force_x"):
+        assert np.array_equal(b.particles(k), sim.particles(k)), k
+    # past the refreshes after steps 500 and 1000, in uneven chunks of graph replays and
+    # single steps
+    batch.step(437)
+    batch.step(590)
+    batch.step(33)
+    sim.step(1060)
+    plain = rb.Sim(case)
+    plain.set_shift_interval(0)
+    plain.init(temperature=T)
+    plain.step(1101)
+    for k in ("f", "g"):
+        assert np.array_equal(b.populations(k), sim.populations(k)), f"{k} after refreshes"
+    assert not np.array_equal(plain.populations("g"), sim.populations("g"))
     for k in ("pos_x", "vel_x", "temp", "force_x"):
         assert np.array_equal(b.particles(k), sim.particles(k)), k
 
