@@ -239,6 +239,12 @@ def main():
     aa_odd = rb.streaming_probe(2, kw64["lx"], kw64["ly"], kw64["lz"], 20)
     aa_pair = 2.0 / (1.0 / aa_even + 1.0 / aa_odd)  # mean rate over an even + odd step pair
     log(f"AA-pattern probes: even {aa_even:.0f}, odd {aa_odd:.0f}, pair {aa_pair:.0f} GB/s")
+    # the other candidate pull patterns: FP64 with plain 8-byte loads, and the 16-bit storage
+    # forms' plain pull (their kernels' pattern), row pull with lane shifts, two nodes per thread
+    probes = {name: rb.streaming_probe(scheme, kw64["lx"], kw64["ly"], kw64["lz"], 20)
+              for name, scheme in (("fp64_plain_pull", 3), ("p16_plain_pull", 4),
+                                   ("p16_row_pull", 5), ("p16_pair_pull", 6))}
+    log("pattern probes: " + ", ".join(f"{k} {v:.0f}" for k, v in probes.items()) + " GB/s")
     r64 = run_form(rb, kw64, args.steps, args.warmup, args.timing_steps, log, args.nu_window)
     n = r64["n"]
     mlups = n * args.steps / (r64["wall_ms"] * 1e-3) / 1e6
@@ -257,6 +263,10 @@ def main():
                 nu_hot_rel_err_vs_fp64=abs(r["nu"][4] - r64["nu"][4]) / abs(r64["nu"][4]),
                 ke_rel_err_vs_fp64=abs(r["ke"] - r64["ke"]) / abs(r64["ke"]),
                 bytes_per_node=BYTES_PER_NODE[prec])
+            if prec in (2, 3):
+                # the 16-bit kernels pull with plain 2-byte loads: their pattern's ceiling
+                form["pattern_probe_gbs"] = probes["p16_plain_pull"]
+                form["frac_of_pattern_probe"] = form["kernel_gbs"] / probes["p16_plain_pull"]
             if window:
                 err = abs(r["nu_avg"][4] - r64["nu_avg"][4]) / abs(r64["nu_avg"][4])
                 form.update(
@@ -305,6 +315,7 @@ def main():
                      "stencil_probe_gbs": stencil_gbs,
                      "frac_of_stencil_probe": achieved / stencil_gbs,
                      "aa_pattern_probe_gbs": {"even": aa_even, "odd": aa_odd, "pair": aa_pair},
+                     "pattern_probes_gbs": probes,
                      "kernel_ms": r64["kernel_ms"], "bytes_per_node": BYTES_PER_NODE[0]},
         "cpu_baseline": cpu,
         "forms": forms,

@@ -299,10 +299,15 @@ int rbtp_stencil_probe(int lx, int ly, int lz, int reps, double *gbs);
 #define RBTP_ARITH_DIV_RT 3
 #define RBTP_ARITH_SQRT 4
 int rbtp_arith_probe(int op, const double *a, const double *b, int64_t n, double *out);
-/* Streaming schemes as access patterns, same sizes and units as rbtp_stencil_probe: scheme 0 is
- * the pull between two buffers (rbtp_stencil_probe), 1 and 2 the even and odd steps of in-place
- * AA streaming on one buffer per distribution.  No reference counterpart; it measures the
- * alternative to the pull on this device. */
+/* Streaming schemes as access patterns on the lattice kernels' layout, no arithmetic, same sizes
+ * as rbtp_stencil_probe.  FP64 (GB/s at 880 B per node): 0 the pull the FP64 kernel uses
+ * (rbtp_stencil_probe: the row pull where lx is a multiple of 64), 1 and 2 the even and odd
+ * steps of in-place AA streaming on one buffer per distribution, 3 the plain pull (every
+ * population one 8-byte load).  16-bit storage (GB/s at 224 B per node: 54 2-byte populations
+ * and a 4-byte F_x, read and written): 4 the plain pull the FP16/BF16 kernels use, 5 the row pull
+ * with lane shifts (lx a multiple of 64), 6 two nodes per thread with 4-byte words (lx even,
+ * nodes a multiple of 512).  No reference counterpart; it measures candidate patterns on this
+ * device. */
 int rbtp_streaming_probe(int scheme, int lx, int ly, int lz, int reps, double *gbs);
 
 /*

@@ -205,8 +205,10 @@ def bandwidth_probe(nbytes: int = 2 << 30, reps: int = 20) -> float:
 
 
 def streaming_probe(scheme: int, lx: int, ly: int, lz: int, reps: int = 20) -> float:
-    """GB/s at 880 B per node of a streaming scheme's access pattern (rbtp_streaming_probe):
-    0 the pull between two buffers, 1 and 2 the even and odd steps of in-place AA streaming."""
+    """GB/s of a streaming scheme's access pattern (rbtp_streaming_probe).  FP64, at 880 B per
+    node: 0 the pull of the FP64 kernel, 1 and 2 the even and odd steps of in-place AA
+    streaming, 3 the plain pull.  16-bit storage, at 224 B per node: 4 the plain pull of the
+    FP16/BF16 kernels, 5 the row pull with lane shifts, 6 two nodes per thread."""
     v = C.c_double()
     _check(lib.rbtp_streaming_probe(int(scheme), int(lx), int(ly), int(lz), int(reps), C.byref(v)))
     return v.value

@@ -817,6 +817,222 @@ __global__ __launch_bounds__(BLOCK) void k_aa_odd_probe(double *__restrict__ f,
     fx[idx] = fx[idx] + fv[0];
 }
 
+// ---- 16-bit storage patterns (FP16/BF16 forms): 2-byte populations, F_x as float ----------
+// Each stores what it pulled plus one (so that no store rewrites the value it read), like the
+// FP64 probes; 216 B of populations and 8 B of F_x move per node.
+
+// The pull of gather_shifted: one node per thread, every population one 2-byte load.
+__global__ __launch_bounds__(BLOCK) void k_pull16_probe(const unsigned short *__restrict__ fin,
+                                                        const unsigned short *__restrict__ gin,
+                                                        unsigned short *__restrict__ fout,
+                                                        unsigned short *__restrict__ gout,
+                                                        float *__restrict__ fx, Dims d) {
+    const int nblocks = (d.n + BLOCK - 1) / BLOCK;
+    const int idx = xcd_block(nblocks) * BLOCK + threadIdx.x;
+    if (idx >= d.n) return;
+    const int iz = idx / d.lxy;
+    const int rem = idx - iz * d.lxy;
+    const int iy = rem / d.lx;
+    const int ix = rem - iy * d.lx;
+    unsigned f[NPOP], g[NPOP];
+#pragma unroll
+    for (int i = 0; i < NPOP; i++) {
+        const int cx = D3Q27::cx[i], cy = D3Q27::cy[i], cz = D3Q27::cz[i];
+        const bool outx = (cx == 1 && ix == 0) || (cx == -1 && ix == d.lx - 1);
+        const bool outy = (cy == 1 && iy == 0) || (cy == -1 && iy == d.ly - 1);
+        const bool outz = (cz == 1 && iz == 0) || (cz == -1 && iz == d.lz - 1);
+        const bool inside = !(outx || outy || outz);
+        const size_t src = inside ? pop_at(i, idx - (cx + cy * d.lx + cz * d.lxy))
+                                  : pop_at(D3Q27::opp[i], idx);
+        f[i] = fin[src];
+        g[i] = gin[src];
+    }
+#pragma unroll
+    for (int i = 0; i < NPOP; i++) {
+        __builtin_nontemporal_store((unsigned short)(f[i] + 1u), fout + pop_at(i, idx));
+        __builtin_nontemporal_store((unsigned short)(g[i] + 1u), gout + pop_at(i, idx));
+    }
+    fx[idx] = fx[idx] + (float)f[0];
+}
+
+__device__ inline unsigned lane_shr1(unsigned v) {
+    return (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x138, 0xf, 0xf, false);
+}
+__device__ inline unsigned lane_shl1(unsigned v) {
+    return (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x130, 0xf, 0xf, false);
+}
+
+// The row pull of gather_rows on 2-byte populations (lx a multiple of 64, n of BLOCK): aligned
+// loads, the +-1 x shift across lanes (DPP), the wave's edge values through LDS, or a load at
+// the workgroup's edge, or the x-wall copy.
+__global__ __launch_bounds__(BLOCK) void k_rows16_probe(const unsigned short *__restrict__ fin,
+                                                        const unsigned short *__restrict__ gin,
+                                                        unsigned short *__restrict__ fout,
+                                                        unsigned short *__restrict__ gout,
+                                                        float *__restrict__ fx, Dims d) {
+    constexpr int NW = BLOCK / 64;
+    __shared__ unsigned last[NW][NSHIFT], first[NW][NSHIFT];
+    const int idx = xcd_block(d.n / BLOCK) * BLOCK + threadIdx.x;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int iz = idx / d.lxy;
+    const int rem = idx - iz * d.lxy;
+    const int iy = rem / d.lx;
+    const int ix = rem - iy * d.lx;
+    unsigned fa[NPOP], ga[NPOP];
+    bool yz_out[NPOP];
+#pragma unroll
+    for (int i = 0; i < NPOP; i++) {
+        const int cy = D3Q27::cy[i], cz = D3Q27::cz[i];
+        const bool outy = (cy == 1 && iy == 0) || (cy == -1 && iy == d.ly - 1);
+        const bool outz = (cz == 1 && iz == 0) || (cz == -1 && iz == d.lz - 1);
+        yz_out[i] = outy || outz;
+        const size_t a = yz_out[i] ? pop_at(D3Q27::opp[i], idx)
+                                   : pop_at(i, idx - cy * d.lx - cz * d.lxy);
+        fa[i] = fin[a];
+        ga[i] = gin[a];
+    }
+    int k = 0;
+#pragma unroll
+    for (int i = 0; i < NPOP; i++) {
+        if (D3Q27::cx[i] == 0) continue;
+        if (D3Q27::cx[i] == 1 && lane == 63) { last[w][k] = fa[i]; last[w][k + 1] = ga[i]; }
+        if (D3Q27::cx[i] == -1 && lane == 0) { first[w][k] = fa[i]; first[w][k + 1] = ga[i]; }
+        k += 2;
+    }
+    __syncthreads();
+    unsigned f[NPOP], g[NPOP];
+    k = 0;
+#pragma unroll
+    for (int i = 0; i < NPOP; i++) {
+        const int cx = D3Q27::cx[i], cy = D3Q27::cy[i], cz = D3Q27::cz[i];
+        if (cx == 0 || yz_out[i]) {
+            f[i] = fa[i];
+            g[i] = ga[i];
+            if (cx) k += 2;
+            continue;
+        }
+        unsigned fv = cx == 1 ? lane_shr1(fa[i]) : lane_shl1(fa[i]);
+        unsigned gv = cx == 1 ? lane_shr1(ga[i]) : lane_shl1(ga[i]);
+        if (cx == 1 ? lane == 0 : lane == 63) {
+            const int nw = cx == 1 ? w - 1 : w + 1;
+            if (cx == 1 ? ix == 0 : ix == d.lx - 1) {
+                fv = fin[pop_at(D3Q27::opp[i], idx)];
+                gv = gin[pop_at(D3Q27::opp[i], idx)];
+            } else if (nw >= 0 && nw < NW) {
+                fv = cx == 1 ? last[nw][k] : first[nw][k];
+                gv = cx == 1 ? last[nw][k + 1] : first[nw][k + 1];
+            } else {
+                const size_t a = pop_at(i, idx - (cx + cy * d.lx + cz * d.lxy));
+                fv = fin[a];
+                gv = gin[a];
+            }
+        }
+        f[i] = fv;
+        g[i] = gv;
+        k += 2;
+    }
+#pragma unroll
+    for (int i = 0; i < NPOP; i++) {
+        __builtin_nontemporal_store((unsigned short)(f[i] + 1u), fout + pop_at(i, idx));
+        __builtin_nontemporal_store((unsigned short)(g[i] + 1u), gout + pop_at(i, idx));
+    }
+    fx[idx] = fx[idx] + (float)f[0];
+}
+
+// Two nodes of a row per thread (lx even, n a multiple of 2 BLOCK): each population one aligned
+// 4-byte word; an x-shifted one takes the neighbouring lane's word (DPP, the edge lane's through
+// LDS or a load) and byte-aligns the pair; a wall node copies its own opposite value.  Where two
+// rows meet inside a wave, the lane holding a row's first (last) node is a wall lane, so no
+// value crosses rows.
+__device__ inline unsigned shifted_pair(unsigned own, unsigned nb, int cx) {
+    // c_x = +1: nodes (2t, 2t+1) need elements (2t-1, 2t); c_x = -1: (2t+1, 2t+2)
+    return cx == 1 ? __builtin_amdgcn_alignbyte(own, nb, 2) : __builtin_amdgcn_alignbyte(nb, own, 2);
+}
+__global__ __launch_bounds__(BLOCK) void k_pairs16_probe(const unsigned short *__restrict__ fin,
+                                                         const unsigned short *__restrict__ gin,
+                                                         unsigned short *__restrict__ fout,
+                                                         unsigned short *__restrict__ gout,
+                                                         float *__restrict__ fx, Dims d) {
+    constexpr int NW = BLOCK / 64;
+    __shared__ unsigned last[NW][NSHIFT], first[NW][NSHIFT];
+    const int t = xcd_block(d.n / (2 * BLOCK)) * BLOCK + threadIdx.x;
+    const int idx = 2 * t;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int iz = idx / d.lxy;
+    const int rem = idx - iz * d.lxy;
+    const int iy = rem / d.lx;
+    const int ix = rem - iy * d.lx;
+    unsigned fa[NPOP], ga[NPOP];
+    bool yz_out[NPOP];
+#pragma unroll
+    for (int i = 0; i < NPOP; i++) {
+        const int cy = D3Q27::cy[i], cz = D3Q27::cz[i];
+        const bool outy = (cy == 1 && iy == 0) || (cy == -1 && iy == d.ly - 1);
+        const bool outz = (cz == 1 && iz == 0) || (cz == -1 && iz == d.lz - 1);
+        yz_out[i] = outy || outz;
+        const size_t a = yz_out[i] ? pop_at(D3Q27::opp[i], idx)
+                                   : pop_at(i, idx - cy * d.lx - cz * d.lxy);
+        fa[i] = *(const unsigned *)(fin + a);
+        ga[i] = *(const unsigned *)(gin + a);
+    }
+    int k = 0;
+#pragma unroll
+    for (int i = 0; i < NPOP; i++) {
+        if (D3Q27::cx[i] == 0) continue;
+        if (D3Q27::cx[i] == 1 && lane == 63) { last[w][k] = fa[i]; last[w][k + 1] = ga[i]; }
+        if (D3Q27::cx[i] == -1 && lane == 0) { first[w][k] = fa[i]; first[w][k + 1] = ga[i]; }
+        k += 2;
+    }
+    __syncthreads();
+    unsigned f[NPOP], g[NPOP];
+    k = 0;
+#pragma unroll
+    for (int i = 0; i < NPOP; i++) {
+        const int cx = D3Q27::cx[i], cy = D3Q27::cy[i], cz = D3Q27::cz[i];
+        if (cx == 0 || yz_out[i]) {
+            f[i] = fa[i];
+            g[i] = ga[i];
+            if (cx) k += 2;
+            continue;
+        }
+        unsigned fnb = cx == 1 ? lane_shr1(fa[i]) : lane_shl1(fa[i]);
+        unsigned gnb = cx == 1 ? lane_shr1(ga[i]) : lane_shl1(ga[i]);
+        // rows meet only at a pair holding a row's first or last node
+        const bool wall = cx == 1 ? ix == 0 : ix + 1 == d.lx - 1;
+        if ((cx == 1 ? lane == 0 : lane == 63) && !wall) {
+            const int nw = cx == 1 ? w - 1 : w + 1;
+            if (nw >= 0 && nw < NW) {
+                fnb = cx == 1 ? last[nw][k] : first[nw][k];
+                gnb = cx == 1 ? last[nw][k + 1] : first[nw][k + 1];
+            } else {
+                const size_t a = pop_at(i, idx - 2 * cx - cy * d.lx - cz * d.lxy);
+                fnb = *(const unsigned *)(fin + a);
+                gnb = *(const unsigned *)(gin + a);
+            }
+        }
+        unsigned fv = shifted_pair(fa[i], fnb, cx), gv = shifted_pair(ga[i], gnb, cx);
+        if (wall) {
+            const int wid = cx == 1 ? idx : idx + 1;
+            const unsigned fw = fin[pop_at(D3Q27::opp[i], wid)], gw = gin[pop_at(D3Q27::opp[i], wid)];
+            fv = cx == 1 ? ((fv & 0xffff0000u) | fw) : ((fv & 0x0000ffffu) | (fw << 16));
+            gv = cx == 1 ? ((gv & 0xffff0000u) | gw) : ((gv & 0x0000ffffu) | (gw << 16));
+        }
+        f[i] = fv;
+        g[i] = gv;
+        k += 2;
+    }
+#pragma unroll
+    for (int i = 0; i < NPOP; i++) {
+        __builtin_nontemporal_store(f[i] + 0x00010001u, (unsigned *)(fout + pop_at(i, idx)));
+        __builtin_nontemporal_store(g[i] + 0x00010001u, (unsigned *)(gout + pop_at(i, idx)));
+    }
+    float2 *fx2 = (float2 *)fx;
+    float2 v = fx2[t];
+    v.x += (float)(f[0] & 0xffffu);
+    v.y += (float)(f[0] >> 16);
+    fx2[t] = v;
+}
+
 int lattice_stencil_probe(int lx, int ly, int lz, int reps, double *gbs) {
     return lattice_streaming_probe(0, lx, ly, lz, reps, gbs);
 }
@@ -824,24 +1040,31 @@ int lattice_stencil_probe(int lx, int ly, int lz, int reps, double *gbs) {
 int lattice_streaming_probe(int scheme, int lx, int ly, int lz, int reps, double *gbs) {
     if (lx < 2 || ly < 2 || lz < 2 || reps < 1 || (long long)lx * ly * lz > 0x7fffffffLL)
         return report(RBTP_E_ARG, "stencil probe: lattice of 2..2^31 nodes, reps >= 1");
-    if (scheme < 0 || scheme > 2)
-        return report(RBTP_E_ARG, "streaming probe: scheme 0 (pull), 1 (AA even) or 2 (AA odd)");
+    if (scheme < 0 || scheme > 6)
+        return report(RBTP_E_ARG, "streaming probe: scheme 0 .. 6");
     Dims d;
     d.lx = lx; d.ly = ly; d.lz = lz; d.lxy = lx * ly; d.n = lx * ly * lz;
+    if (scheme == 5 && (lx % 64 != 0 || d.n % BLOCK != 0))
+        return report(RBTP_E_ARG, "streaming probe 5: lx a multiple of 64, nodes a multiple of 256");
+    if (scheme == 6 && (lx % 2 != 0 || d.n % (2 * BLOCK) != 0))
+        return report(RBTP_E_ARG, "streaming probe 6: lx even, nodes a multiple of 512");
+    const bool p16 = scheme >= 4;
     // two pair buffers, as a sim holds them: f at the buffer, g G_OFFSET elements after it
     const size_t len = pop_buffer_len((size_t)d.n);
+    const size_t elem = p16 ? sizeof(unsigned short) : sizeof(double);
+    const size_t fx_elem = p16 ? sizeof(float) : sizeof(double);
     double *buf[2] = {nullptr, nullptr};
     double *fx = nullptr;
     hipStream_t st = nullptr;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     hipError_t e = hipSuccess;
-    for (int k = 0; k < 2 && e == hipSuccess; k++) e = hipMalloc(&buf[k], len * sizeof(double));
-    if (e == hipSuccess) e = hipMalloc(&fx, (size_t)d.n * sizeof(double));
+    for (int k = 0; k < 2 && e == hipSuccess; k++) e = hipMalloc(&buf[k], len * elem);
+    if (e == hipSuccess) e = hipMalloc(&fx, (size_t)d.n * fx_elem);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipEventCreate(&e0);
     if (e == hipSuccess) e = hipEventCreate(&e1);
-    for (int k = 0; k < 2 && e == hipSuccess; k++) e = hipMemsetAsync(buf[k], 0, len * sizeof(double), st);
-    if (e == hipSuccess) e = hipMemsetAsync(fx, 0, (size_t)d.n * sizeof(double), st);
+    for (int k = 0; k < 2 && e == hipSuccess; k++) e = hipMemsetAsync(buf[k], 0, len * elem, st);
+    if (e == hipSuccess) e = hipMemsetAsync(fx, 0, (size_t)d.n * fx_elem, st);
     const int nb = blocks_for((size_t)d.n);
     const bool rows = rows_pull(d);
     int cur = 0;
@@ -851,9 +1074,22 @@ int lattice_streaming_probe(int scheme, int lx, int ly, int lz, int reps, double
             hipLaunchKernelGGL(k_aa_even_probe, nb, BLOCK, 0, st, f0, g0, fx, d);
         } else if (scheme == 2) {
             hipLaunchKernelGGL(k_aa_odd_probe, nb, BLOCK, 0, st, f0, g0, fx, d);
+        } else if (p16) {
+            using U = unsigned short;
+            U *fi = (U *)buf[cur], *fo = (U *)buf[1 - cur];
+            if (scheme == 4)
+                hipLaunchKernelGGL(k_pull16_probe, nb, BLOCK, 0, st, fi, fi + G_OFFSET, fo,
+                                   fo + G_OFFSET, (float *)fx, d);
+            else if (scheme == 5)
+                hipLaunchKernelGGL(k_rows16_probe, d.n / BLOCK, BLOCK, 0, st, fi, fi + G_OFFSET,
+                                   fo, fo + G_OFFSET, (float *)fx, d);
+            else
+                hipLaunchKernelGGL(k_pairs16_probe, d.n / (2 * BLOCK), BLOCK, 0, st, fi,
+                                   fi + G_OFFSET, fo, fo + G_OFFSET, (float *)fx, d);
+            cur = 1 - cur;
         } else {
             double *fi = buf[cur], *fo = buf[1 - cur];
-            if (rows)
+            if (rows && scheme == 0)
                 hipLaunchKernelGGL(k_stencil_probe<true>, nb, BLOCK, 0, st, fi, fi + G_OFFSET, fo,
                                    fo + G_OFFSET, fx, d);
             else
@@ -870,7 +1106,7 @@ int lattice_streaming_probe(int scheme, int lx, int ly, int lz, int reps, double
     if (e == hipSuccess) e = hipEventSynchronize(e1);
     float ms = 0.f;
     if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
-    if (e == hipSuccess) *gbs = 880.0 * (double)d.n * reps / (ms * 1e-3) / 1e9;
+    if (e == hipSuccess) *gbs = (p16 ? 224.0 : 880.0) * (double)d.n * reps / (ms * 1e-3) / 1e9;
     for (double *p : buf) (void)hipFree(p);
     (void)hipFree(fx);
     if (e0) (void)hipEventDestroy(e0);
