@@ -35,6 +35,7 @@ extern "C" {
 #define RBTP_E_STATE (-5)    /* call not valid in the current state */
 #define RBTP_E_CAPACITY (-6) /* a device-side capacity bound was exceeded */
 #define RBTP_E_PLACEMENT (-7)/* particle placement gave up (pinit.cu:126-131) */
+#define RBTP_E_DIVERGED (-8) /* the divergence watchdog flagged a node (rbtp_set_watchdog) */
 
 /* Storage/arithmetic forms of the lattice. */
 #define RBTP_PREC_FP64 0   /* FP64 storage and arithmetic, operation order of the reference */
@@ -140,6 +141,13 @@ int rbtp_step(rbtp_sim *s, int nsteps, unsigned flags);
 /* Same, without waiting for the stream (errors are reported by the next synchronising call). */
 int rbtp_step_async(rbtp_sim *s, int nsteps, unsigned flags);
 int rbtp_synchronize(rbtp_sim *s);
+/* Optional divergence watchdog; no reference counterpart (the reference checks neither NaN nor
+ * velocity).  With umax > 0 every lattice step flags a node whose density or velocity is not
+ * finite or whose |u| exceeds umax, at no cost to the values computed; rbtp_step (and
+ * rbtp_synchronize after rbtp_step_async) then returns RBTP_E_DIVERGED, naming the first
+ * flagged step and the number of flags, and clears them.  The state is left as the steps made
+ * it.  umax = 0 (the default) turns the check off; batch slots are not watched. */
+int rbtp_set_watchdog(rbtp_sim *s, double umax);
 /* Number of completed time steps since rbtp_init (restart_step counts). */
 int rbtp_current_step(const rbtp_sim *s, int *step);
 

@@ -164,6 +164,7 @@ def _load():
         "rbtp_set_kernel_timing": (I, [P, I]),
         "rbtp_kernel_ms": (I, [P, C.POINTER(D), C.POINTER(C.c_int64)]),
         "rbtp_set_shift_interval": (I, [P, I]),
+        "rbtp_set_watchdog": (I, [P, D]),
         "rbtp_streaming_probe": (I, [I, I, I, I, I, C.POINTER(D)]),
         "rbtp_arith_probe": (I, [I, dp, P, C.c_int64, dp]),
         "rbtp_place_particles": (I, [C.POINTER(Case), dp]),
@@ -457,6 +458,12 @@ class Sim:
     # -- timing ------------------------------------------------------------------------------------
     def set_kernel_timing(self, enable: bool) -> None:
         _check(lib.rbtp_set_kernel_timing(self._h, 1 if enable else 0))
+
+    def set_watchdog(self, umax: float) -> None:
+        """Divergence watchdog (rbtp_set_watchdog): with umax > 0, step() raises RbtpError
+        (code -8, RBTP_E_DIVERGED) after steps in which a node's density or velocity was not
+        finite or |u| exceeded umax; 0 turns it off."""
+        _check(lib.rbtp_set_watchdog(self._h, float(umax)))
 
     def set_shift_interval(self, steps: int) -> None:
         """Steps between refreshes of the reduced-precision shift profile (0: never)."""

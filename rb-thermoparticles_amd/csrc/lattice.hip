@@ -299,6 +299,14 @@ __device__ inline void node_step(const StepArgs<S> &a, int idx, RowEdges *edges 
     T rho, ux, uy, uz, phi;
     const T fx = collide_node<S, T, STRICT>(f, g, fx_old, idx, ts, o, a.k, rho, ux, uy, uz, phi);
     write_node_aux<S, T>(a, idx, ix, iy, iz, fx, rho, ux, uy, uz, phi);
+    if (a.watch) {
+        // a NaN fails the comparison; an infinite rho is caught by the finiteness test
+        const T u2 = ux * ux + uy * uy + uz * uz;
+        if (!__builtin_isfinite(rho) || !(u2 <= T(a.watch_u2))) {
+            atomicMin(a.watch, a.watch_step);
+            atomicAdd(a.watch + 1, 1);
+        }
+    }
 }
 
 // One time step for every node.

@@ -127,6 +127,11 @@ template <class S> struct StepArgs {
     const typename Enc<S>::A *sp_pop;
     double *macro_rho, *macro_ux, *macro_uy, *macro_uz, *macro_phi;  // output fields, or null
     const float *gshift;  // [lx] shift profile of the reduced-precision g encoding
+    // Divergence watchdog, or null: a node with a non-finite rho or u, or |u|^2 > watch_u2, takes
+    // watch[0] down to watch_step (atomic min) and adds one to watch[1].
+    int32_t *watch;
+    int watch_step;
+    double watch_u2;
     Dims d;
     FluidK k;
 };

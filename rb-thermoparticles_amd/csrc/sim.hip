@@ -86,6 +86,9 @@ struct rbtp_sim {
     double kernel_ms_sum;
     int64_t kernel_launches;
     double last_step_ms;
+    // divergence watchdog: watch_dev[0] the first flagged step (INT_MAX: none), [1] the flags
+    double watch_umax;
+    int32_t *watch_dev;
     // lattice state
     void *f[2], *g[2];
     void *fx;        // F_x (aux type)
@@ -226,6 +229,7 @@ extern "C" void rbtp_destroy(rbtp_sim *s) {
     (void)hipFree(s->gshift);
     (void)hipFree(s->shift_partial);
     (void)hipFree(s->shift_delta);
+    (void)hipFree(s->watch_dev);
     part_free(s->pd);
     part_tables_free(s->ptab);
     part_tables_free(s->st_tab);
@@ -288,6 +292,11 @@ static int create_impl(const rbtp_case *c, rbtp_sim *s, hipStream_t stream) {
     HIP_TRY(hipEventCreate(&s->ev_stop));
     HIP_TRY(hipEventCreate(&s->ev_k0));
     HIP_TRY(hipEventCreate(&s->ev_k1));
+    {
+        const int32_t none[2] = {INT_MAX, 0};
+        HIP_TRY(hipMalloc(&s->watch_dev, sizeof(none)));
+        HIP_TRY(hipMemcpy(s->watch_dev, none, sizeof(none), hipMemcpyHostToDevice));
+    }
     const size_t n = (size_t)s->dims.n;
     // one pair buffer (f and g interleaved by runs) per time level
     const size_t pair_bytes = pop_buffer_len(n) * storage_bytes(s->prec);
@@ -376,6 +385,9 @@ static StepArgs<S> step_args(rbtp_sim *s, bool macro_out) {
     } else {
         a.macro_rho = a.macro_ux = a.macro_uy = a.macro_uz = a.macro_phi = nullptr;
     }
+    a.watch = (s->watch_umax > 0.0 && !s->batch_slot) ? s->watch_dev : nullptr;
+    a.watch_step = s->step + 1;
+    a.watch_u2 = s->watch_umax * s->watch_umax;
     a.d = s->dims;
     a.k = s->k;
     return a;
@@ -383,6 +395,7 @@ static StepArgs<S> step_args(rbtp_sim *s, bool macro_out) {
 
 static int stage_exit(rbtp_sim *s);
 static int current_gen(const rbtp_sim *s);
+static int watch_check(rbtp_sim *s);
 
 // ---------------------------------------------------------------------------------------------
 // Initial state
@@ -549,7 +562,29 @@ static int step_impl(rbtp_sim *s, int nsteps, unsigned flags, bool sync) {
         HIP_TRY(hipEventElapsedTime(&ms, s->ev_start, s->ev_stop));
         s->last_step_ms = ms;
         if (s->particles) RB_TRY(part_check_errors(s->pd));
+        RB_TRY(watch_check(s));
     }
+    return RBTP_OK;
+}
+
+// The watchdog's verdict on the steps run so far (after a synchronisation); clears it.
+static int watch_check(rbtp_sim *s) {
+    if (!(s->watch_umax > 0.0) || s->batch_slot) return RBTP_OK;
+    int32_t w[2];
+    HIP_TRY(hipMemcpy(w, s->watch_dev, sizeof(w), hipMemcpyDeviceToHost));
+    if (w[0] == INT_MAX) return RBTP_OK;
+    const int32_t none[2] = {INT_MAX, 0};
+    HIP_TRY(hipMemcpy(s->watch_dev, none, sizeof(none), hipMemcpyHostToDevice));
+    return set_error(RBTP_E_DIVERGED,
+                     "step %d: %d node update(s) with a non-finite density or velocity or "
+                     "|u| > %g", w[0], w[1], s->watch_umax);
+}
+
+extern "C" int rbtp_set_watchdog(rbtp_sim *s, double umax) {
+    if (!s) return set_error(RBTP_E_ARG, "null handle");
+    if (!(umax >= 0.0) || !std::isfinite(umax))
+        return set_error(RBTP_E_ARG, "watchdog umax must be finite and >= 0");
+    s->watch_umax = umax;
     return RBTP_OK;
 }
 
@@ -569,6 +604,7 @@ extern "C" int rbtp_synchronize(rbtp_sim *s) {
         s->step_events = false;
     }
     if (s->particles) RB_TRY(part_check_errors(s->pd));
+    RB_TRY(watch_check(s));
     return RBTP_OK;
 }
 extern "C" int rbtp_current_step(const rbtp_sim *s, int *step) {

@@ -187,3 +187,53 @@ def test_synchronize_before_and_after_async_steps(rb, O):
     ora.step(5)
     assert np.array_equal(sim.populations("f"), ora.populations("f"))
     assert np.array_equal(sim.populations("g"), ora.populations("g"))
+
+
+@pytest.mark.parametrize("prec", [0, 2])
+def test_watchdog_flags_injected_nan(rb, prec):
+    """A NaN planted in one population from the host reaches the next step's moments at that
+    node: the watchdog (rbtp_set_watchdog) turns the step into RBTP_E_DIVERGED naming it, then
+    clears its flag.  The kernels themselves run to completion either way."""
+    case = rb.default_case(**single_phase(24, 20, 16, precision=prec))
+    sim = rb.Sim(case)
+    sim.init(temperature=rb.linear_temperature(case, 3, 1e-2))
+    sim.set_watchdog(0.5)
+    sim.step(3)  # a healthy flow passes
+    f, g = sim.populations("f"), sim.populations("g")
+    node = (7 * 20 + 9) * 24 + 11
+    f[5, node] = np.nan
+    sim.set_populations(f, g)
+    with pytest.raises(rb.RbtpError) as exc:
+        sim.step(2)
+    assert exc.value.code == -8
+    assert "step 4:" in str(exc.value)
+    sim.set_watchdog(0.0)
+    sim.step(1)  # off: no verdict, even though the NaN has spread
+    assert not np.isfinite(sim.field("rho")).all()
+
+
+def test_watchdog_velocity_bound_and_no_effect_on_values(rb):
+    """With a bound above every speed the watched run is bit for bit the unwatched one; a bound
+    below the flow's speed (1e-12 against the noise-driven flow) is flagged at the first step."""
+    kw = single_phase(24, 20, 16)
+    runs = []
+    for umax in (0.0, 0.5):
+        case = rb.default_case(**kw)
+        sim = rb.Sim(case)
+        sim.init(temperature=rb.linear_temperature(case, 3, 1e-2))
+        sim.set_watchdog(umax)
+        sim.step(20)
+        runs.append((sim.populations("f"), sim.populations("g")))
+    for a, b in zip(*runs):
+        assert np.array_equal(a, b)
+    case = rb.default_case(**kw)
+    sim = rb.Sim(case)
+    sim.init(temperature=rb.linear_temperature(case, 3, 1e-2))
+    sim.step(5)
+    sim.set_watchdog(1e-12)
+    with pytest.raises(rb.RbtpError) as exc:
+        sim.step(3)
+    assert exc.value.code == -8 and "step 6:" in str(exc.value)
+    with pytest.raises(rb.RbtpError) as exc:
+        sim.set_watchdog(-1.0)
+    assert exc.value.code == -1

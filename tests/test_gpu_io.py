@@ -227,7 +227,7 @@ def test_kinetic_energy_over_fluid_nodes(rb, O):
 
 
 def test_measurement_probes(rb):
-    """The copy and stencil-pattern probes return a positive rate and reject bad arguments."""
+    """The copy and access-pattern probes return a positive rate and reject bad arguments."""
     copy = rb.bandwidth_probe(64 << 20, 2)
     assert 0.0 < copy < 1e5
     sten = rb.stencil_probe(64, 48, 40, 2)
@@ -236,7 +236,12 @@ def test_measurement_probes(rb):
         rb.stencil_probe(1, 48, 40, 2)
     with pytest.raises(rb.RbtpError):
         rb.stencil_probe(64, 48, 40, 0)
-    for scheme in (0, 1, 2):  # pull, AA even, AA odd
-        assert 0.0 < rb.streaming_probe(scheme, 64, 48, 40, 2) < 1e5
+    # FP64 pull, AA even, AA odd, plain pull; 16-bit plain pull, row pull, two nodes per thread
+    for scheme in range(7):
+        assert 0.0 < rb.streaming_probe(scheme, 64, 48, 40, 2) < 1e5, scheme
     with pytest.raises(rb.RbtpError):
-        rb.streaming_probe(3, 64, 48, 40, 2)
+        rb.streaming_probe(7, 64, 48, 40, 2)
+    with pytest.raises(rb.RbtpError):
+        rb.streaming_probe(5, 48, 48, 40, 2)  # rows of whole waves only
+    with pytest.raises(rb.RbtpError):
+        rb.streaming_probe(6, 63, 48, 40, 2)  # node pairs within rows only

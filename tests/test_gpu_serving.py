@@ -153,3 +153,30 @@ def test_request_loop_small(rb):
     assert all(r.steps >= 201 for r in reqs)
     assert summary["p99_s"] >= summary["p50_s"] > 0
     assert summary["aggregate_mlups"] > 0
+
+
+def test_full_size_requests_match_oracle(rb, O):
+    """The largest requests the request generator makes (LX 96 and 128, 64 spheres each, the
+    reference's rejection placement) served together in one FP64 batch, with a third small
+    request beside them, agree with the oracle run of each case alone: populations bitwise,
+    the particle state within 1e-12."""
+    kws = [case_kw(128, 1e6, 0.71, 64, 5), case_kw(96, 3e5, 4.3, 64, 6)]
+    batch = rb.Batch(precision=0, max_slots=4)
+    slots = [admit(rb, batch, kw, 40 + i) for i, kw in enumerate(kws)]
+    admit(rb, batch, case_kw(32, 3e4, 0.71, 0, 1), 39)
+    batch.step(3)
+    for kw, slot, seed in zip(kws, slots, (40, 41)):
+        case = O.default_case(**kw)
+        ora = O.Oracle(case)
+        ora.init(temperature=O.linear_temperature(case, seed, 1e-3))
+        ora.step(4)  # the admission's step and the three batched ones
+        b = batch.sim(slot)
+        assert b.np == 64
+        for name in ("f", "g"):
+            assert np.array_equal(b.populations(name), ora.populations(name)), (kw["lx"], name)
+        for name in ("pos_x", "pos_y", "pos_z", "vel_x", "omega_z", "temp", "force_x", "heat"):
+            a, r = b.particles(name), ora.particles(name)
+            scale = max(float(np.max(np.abs(r))), 1e-300)
+            assert float(np.max(np.abs(a - r))) / scale <= 1e-12, (kw["lx"], name)
+        ora.close()
+    batch.close()
