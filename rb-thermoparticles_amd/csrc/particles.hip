@@ -563,20 +563,63 @@ struct LinkLoads {
     double f_here, f_pair, g_here, g_pair;
 };
 
+// A population value as stored (the 2-byte forms as their bits), loaded without decoding.
 template <class S>
-__device__ inline LinkLoads link_loads(int ip, int idx, int idx_up, bool up_fluid, double q,
-                                       const PopView<S> &pop) {
+__device__ inline auto raw_pop(gptr<const S> p) {
+    if constexpr (sizeof(S) == 2) return uint32_t(*(gptr<const unsigned short>)p);
+    else return *p;
+}
+template <class S, class R>
+__device__ inline S raw_as(R r) {
+    if constexpr (std::is_same<S, __half>::value) return __ushort_as_half((unsigned short)r);
+    else if constexpr (std::is_same<S, __hip_bfloat16>::value)
+        return __ushort_as_bfloat16((unsigned short)r);
+    else return r;
+}
+
+// The two loads of x's own values, which do not depend on q (issued before link_q's square
+// root), and the pair loads of the branch q selects.
+template <class S> struct LinkRaw {
+    decltype(raw_pop<S>(nullptr)) fh, fp, gh, gp;
+    float sh_here, sh_pair;  // shift profile at x and at the g partner (reduced-precision g)
+};
+template <class S>
+__device__ inline void link_raw_here(LinkRaw<S> &r, int ip, int idx, const PopView<S> &pop) {
+    r.fh = raw_pop<S>(pop.f + pop_at(ip, idx));
+    r.gh = raw_pop<S>(pop.g + pop_at(ip, idx));
+    r.sh_here = 0.0f;
+    if constexpr (Enc<S>::scaled) r.sh_here = pop.gshift[idx % pop.lx];
+}
+template <class S>
+__device__ inline LinkLoads link_raw_pair(LinkRaw<S> &r, int ip, int idx, int idx_up,
+                                          bool up_fluid, double q, const PopView<S> &pop) {
     const int opp = dir_opp(ip);
     const int fi = up_fluid ? (q < 0.5 ? ip : opp) : ip;
     const int fn = up_fluid && q < 0.5 ? idx_up : idx;
     const int gi = up_fluid ? (q <= 0.5 ? ip : opp) : ip;
     const int gn = up_fluid && q <= 0.5 ? idx_up : idx;
+    r.fp = raw_pop<S>(pop.f + pop_at(fi, fn));
+    r.gp = raw_pop<S>(pop.g + pop_at(gi, gn));
+    r.sh_pair = 0.0f;
+    if constexpr (Enc<S>::scaled) r.sh_pair = pop.gshift[gn % pop.lx];
+    // All four values are in flight here and the link waits for them once: without this point
+    // the compiler sinks each pair load into the branch of eval_link_at that reads it, and the
+    // link then waits for memory two or three times in turn.
+    asm volatile("" ::"v"(r.fh), "v"(r.fp), "v"(r.gh), "v"(r.gp));
     LinkLoads L;
-    L.f_here = pop.F(ip, idx);
-    L.f_pair = pop.F(fi, fn);
-    L.g_here = pop.G(ip, idx);
-    L.g_pair = pop.G(gi, gn);
+    L.f_here = dec_f<S, double>(raw_as<S>(r.fh), ip);
+    L.f_pair = dec_f<S, double>(raw_as<S>(r.fp), fi);
+    L.g_here = dec_g<S, double>(raw_as<S>(r.gh), ip, double(r.sh_here));
+    L.g_pair = dec_g<S, double>(raw_as<S>(r.gp), gi, double(r.sh_pair));
     return L;
+}
+
+template <class S>
+__device__ inline LinkLoads link_loads(int ip, int idx, int idx_up, bool up_fluid, double q,
+                                       const PopView<S> &pop) {
+    LinkRaw<S> r;
+    link_raw_here<S>(r, ip, idx, pop);
+    return link_raw_pair<S>(r, ip, idx, idx_up, up_fluid, q, pop);
 }
 
 // bounce_back_particles_kernel (ibb.cu:19-81) and bounce_back_thermal_kernel (ibb.cu:94-135)
@@ -753,9 +796,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
         const int cx = dir_cx(ip), cy = dir_cy(ip), cz = dir_cz(ip);
         const bool up_fluid = (fluid_nb >> opp) & 1u;  // x - c_ip = x + c_opp
         const int idx_up = idx - (cx + cy * d.lx + cz * d.lxy);
+        LinkRaw<S> raw;
+        link_raw_here<S>(raw, ip, idx, pop);
         double q = 0.0;
         link_q(ip, x0, y0, z0, X, Y, Z, radius_sq, q);
-        const LinkLoads Ld = link_loads<S>(ip, idx, idx_up, up_fluid, q, pop);
+        const LinkLoads Ld = link_raw_pair<S>(raw, ip, idx, idx_up, up_fluid, q, pop);
         const LinkOut L = eval_link_at(Ld, ip, ix, iy, iz, up_fluid, q, ps, density, rho0);
         if (j >= 0) {
             out[sp_at(j, opp)] = A(L.f_out);
