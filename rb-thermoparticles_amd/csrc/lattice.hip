@@ -237,7 +237,7 @@ __device__ inline void write_node_aux(const StepArgs<S> &a, int idx, int ix, int
     a.fx[idx] = A(fx);
     if (ix == 0) a.rho_wall[iz * a.d.ly + iy] = A(rho);
     if (ix == a.d.lx - 1) a.rho_wall[a.d.ly * a.d.lz + iz * a.d.ly + iy] = A(rho);
-    if (a.rho_full) a.rho_full[idx] = double(rho);
+    if (a.rho_full) a.rho_full[idx] = A(rho);
     if (a.macro_rho) {
         a.macro_rho[idx] = double(rho);
         a.macro_ux[idx] = double(ux);

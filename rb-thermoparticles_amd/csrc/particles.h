@@ -88,7 +88,8 @@ struct PartSlot {
     const void *fcur, *gcur;               // this step's post-collision populations
     void *fx;                              // F_x (Enc<S>::A)
     const void *rho_wall;                  // wall rho (Enc<S>::A)
-    double *macro[5];                      // rho (also the full rho field), ux, uy, uz, phi
+    double *macro[5];                      // output fields rho, ux, uy, uz, phi
+    const void *rho_full;                  // rho of the last lattice step (Enc<S>::A)
 };
 
 // Launch shape of a particle phase over a table of slots.  The prefix arrays map a flat index

@@ -783,7 +783,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
                          gbl(pd.gshift), d.lx};
     using A = typename Enc<S>::A;
     const gptr<A> out = gbl((A *)pd.sp.pop);
-    const double density = gbl((const double *)sl.macro[0])[idx];
+    const double density = double(gbl((const typename Enc<S>::A *)sl.rho_full)[idx]);
     const PState ps = pstate_uniform(pv, p);  // p is the same for the whole wave
     const double radius_sq = wave_uniform(pd.radius_sq), rho0 = wave_uniform(pd.rho0);
     double v[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
@@ -1004,7 +1004,7 @@ __global__ __launch_bounds__(128) void k_fixup_links(PartGrid g) {
     const PopView<S> pop{gbl((const S *)sl.fcur), gbl((const S *)sl.gcur), (size_t)d.n,
                          gbl(pd.gshift), d.lx};
     const gptr<const int32_t> owner_prev = gbl(sl.owner_prev);
-    const gptr<const double> macro_rho = gbl((const double *)sl.macro[0]);
+    const gptr<const typename Enc<S>::A> rho_full = gbl((const typename Enc<S>::A *)sl.rho_full);
     const int lane = __lane_id();
     const int wave = (blockIdx.x - g.lstart[slot]) * 2 + (threadIdx.x >> 6);
     const int nwave = (g.lstart[slot + 1] - g.lstart[slot]) * 2;
@@ -1043,7 +1043,7 @@ __global__ __launch_bounds__(128) void k_fixup_links(PartGrid g) {
                     const int opp = dir_opp(ip);
                     const bool up_fluid = own_up == -1;  // x - c_ip = x + c_opp, inside and fluid
                     const int idx_up = idx - (dir_cx(ip) + dir_cy(ip) * d.lx + dir_cz(ip) * d.lxy);
-                    const double density = macro_rho[idx];
+                    const double density = double(rho_full[idx]);
                     const LinkLoads Ld = link_loads<S>(ip, idx, idx_up, up_fluid, q, pop);
                     const LinkOut L = eval_link_at(Ld, ip, ix, iy, iz, up_fluid, q, pstate(pv, own),
                                                    density, pd.rho0);

@@ -117,7 +117,7 @@ template <class S> struct StepArgs {
     S *gout;
     typename Enc<S>::A *fx;        // F_x of the previous forcing, updated in place
     typename Enc<S>::A *rho_wall;  // rho at ix=0 (first ly*lz) and ix=lx-1 (next ly*lz), in place
-    double *rho_full;              // rho of this step at every node (particle coupling), or null
+    typename Enc<S>::A *rho_full;  // rho of this step at every node (particle coupling), or null
     // Special nodes (particle coupling), or null: sp_map[idx] is the node's entry j in the
     // special list (-1 elsewhere); bit i of sp_mask[j] marks population i as replaced by its
     // corrected post-stream value (f: sp_pop[sp_at(j, i)], g: sp_pop[sp_at(j, 27 + i)], in the

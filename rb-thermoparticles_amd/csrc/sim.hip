@@ -94,6 +94,10 @@ struct rbtp_sim {
     void *fx;        // F_x (aux type)
     void *rho_wall;  // 2*ly*lz (aux type)
     double *macro[5];  // rho, ux, uy, uz, phi
+    // rho of the last step at every node for the particle coupling (aux type): macro[0] in FP64;
+    // an array of its own in the FP32-arithmetic forms, whose rho is an FP32 value (4 B per node
+    // and step less than FP64).  Null without particles.
+    void *rho_full;
     double *temp_dev;  // initial temperature upload
     // Shift profile of the reduced-precision g encoding ([lx], the conduction profile after
     // init/restart), refreshed to the plane-mean temperature every shift_interval steps
@@ -224,6 +228,7 @@ extern "C" void rbtp_destroy(rbtp_sim *s) {
     for (int b = 0; b < 2; b++) (void)hipFree(s->f[b]);  // g[b] lies in the same pair buffer
     (void)hipFree(s->fx);
     (void)hipFree(s->rho_wall);
+    if (s->rho_full != (void *)s->macro[0]) (void)hipFree(s->rho_full);
     for (double *m : s->macro) (void)hipFree(m);
     (void)hipFree(s->temp_dev);
     (void)hipFree(s->gshift);
@@ -316,6 +321,8 @@ static int create_impl(const rbtp_case *c, rbtp_sim *s, hipStream_t stream) {
     if (s->particles) {
         RB_TRY(part_alloc(s->pd, *c, s->d, s->dims));
         s->pd.gshift = s->gshift;
+        if (s->prec == RBTP_PREC_FP64) s->rho_full = s->macro[0];
+        else HIP_TRY(hipMalloc(&s->rho_full, n * sizeof(float)));
     }
     return RBTP_OK;
 }
@@ -371,7 +378,7 @@ static StepArgs<S> step_args(rbtp_sim *s, bool macro_out) {
     a.gout = (S *)s->g[1 - s->cur];
     a.fx = (typename Enc<S>::A *)s->fx;
     a.rho_wall = (typename Enc<S>::A *)s->rho_wall;
-    a.rho_full = s->particles ? s->macro[0] : nullptr;
+    a.rho_full = (typename Enc<S>::A *)s->rho_full;
     a.gshift = s->gshift;
     a.sp_map = s->particles ? s->pd.sp.map : nullptr;
     a.sp_pop = s->particles ? (const typename Enc<S>::A *)s->pd.sp.pop : nullptr;
@@ -476,6 +483,7 @@ static PartSlot part_slot(const rbtp_sim *s, int cur, int geo) {
     sl.fx = s->fx;
     sl.rho_wall = s->rho_wall;
     for (int k = 0; k < 5; k++) sl.macro[k] = s->macro[k];
+    sl.rho_full = s->rho_full;
     return sl;
 }
 
@@ -1301,6 +1309,7 @@ static size_t sim_device_bytes(const rbtp_case &c, const rbtp_derived &d) {
                2 * (size_t)c.ly * c.lz * aux_bytes(c.precision) + 5 * n * sizeof(double);
     if (c.activate_particles && d.npart > 0) {
         const size_t np = (size_t)d.npart;
+        if (c.precision != RBTP_PREC_FP64) b += n * sizeof(float);  // rho_full
         const long long w = (long long)std::ceil(2.0 * d.particle_radius) + 1;
         const long long cap = special_capacity((int)np, (int)w, (long long)n);
         b += (RBTP_P_COUNT + OLD_COUNT) * np * sizeof(double) + 3 * n * sizeof(int32_t) +
