@@ -778,57 +778,116 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     // every lane of the wave is here
     const unsigned long long has = __ballot(mask != 0u);
     if (__lane_id() == 0) gbl(pd.lnk_bits)[(size_t)p * (pd.lnk_stride / 64) + k / 64] = has;
-    if (mask == 0u) return;
+    if (has == 0ull) return;  // the whole wave
+    // the populations a listed node replaces are its links' opposites
+    if (j >= 0) {
+        uint32_t replaced = 0u;
+        for (uint32_t m = mask; m; m &= m - 1u) replaced |= 1u << dir_opp(__builtin_ctz(m));
+        gbl(pd.sp.mask)[j] = replaced | (higher ? SP_MULTI : 0u);
+    }
+    // The wave's links are numbered in lane order, and in ascending direction order within a
+    // lane: lane L holds links off .. off + cnt - 1.  They are evaluated one per lane, 64 at a
+    // time (a wave holds about 90), instead of each node walking its own links while the lanes
+    // of nodes with fewer links wait.
+    const int lane = __lane_id();
+    const int cnt = __popc(mask);
+    int incl = cnt;
+#pragma unroll
+    for (int s = 1; s < 64; s <<= 1) {
+        const int t = __shfl_up(incl, s);
+        if (lane >= s) incl += t;
+    }
+    const int off = incl - cnt;
+    const int total = __shfl(incl, 63);
     const PopView<S> pop{gbl((const S *)sl.fcur), gbl((const S *)sl.gcur), (size_t)d.n,
                          gbl(pd.gshift), d.lx};
     using A = typename Enc<S>::A;
     const gptr<A> out = gbl((A *)pd.sp.pop);
-    const double density = double(gbl((const typename Enc<S>::A *)sl.rho_full)[idx]);
+    const double density =
+        mask != 0u ? double(gbl((const typename Enc<S>::A *)sl.rho_full)[idx]) : 0.0;
     const PState ps = pstate_uniform(pv, p);  // p is the same for the whole wave
     const double radius_sq = wave_uniform(pd.radius_sq), rho0 = wave_uniform(pd.rho0);
     double v[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-    uint32_t replaced = 0u;
-    // links in ascending direction order; each link's lattice loads are issued together
-    while (mask) {
-        const int ip = __builtin_ctz(mask);
-        mask &= mask - 1u;
-        const int opp = dir_opp(ip);
-        const int cx = dir_cx(ip), cy = dir_cy(ip), cz = dir_cz(ip);
-        const bool up_fluid = (fluid_nb >> opp) & 1u;  // x - c_ip = x + c_opp
-        const int idx_up = idx - (cx + cy * d.lx + cz * d.lxy);
-        LinkRaw<S> raw;
-        link_raw_here<S>(raw, ip, idx, pop);
-        double q = 0.0;
-        link_q(ip, x0, y0, z0, X, Y, Z, radius_sq, q);
-        const LinkLoads Ld = link_raw_pair<S>(raw, ip, idx, idx_up, up_fluid, q, pop);
-        const LinkOut L = eval_link_at(Ld, ip, ix, iy, iz, up_fluid, q, ps, density, rho0);
-        if (j >= 0) {
-            out[sp_at(j, opp)] = A(L.f_out);
-            out[sp_at(j, NPOP + opp)] = A(L.g_out);
-            replaced |= 1u << opp;
+    for (int base = 0; base < total; base += 64) {
+        const int t = base + lane;
+        const bool act = t < total;
+        // the node of link t: the last lane whose first link is at or before t (that lane has
+        // links, since the next lane's first link is past t)
+        int n = 0;
+#pragma unroll
+        for (int st = 32; st >= 1; st >>= 1)
+            if (__shfl(off, n + st) <= t) n += st;
+        uint32_t m = __shfl(mask, n);
+        const int kth = t - __shfl(off, n);
+        for (int r = 0; r < kth; r++) m &= m - 1u;  // drop the node's earlier links
+        const int nidx = __shfl(idx, n), nj = __shfl(j, n);
+        const uint32_t nfluid = __shfl(fluid_nb, n);
+        const int nix = __shfl(ix, n), niy = __shfl(iy, n), niz = __shfl(iz, n);
+        const double ndens = __shfl(density, n);
+        double c[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+        if (act) {
+            const int ip = __builtin_ctz(m);
+            const int opp = dir_opp(ip);
+            const int cx = dir_cx(ip), cy = dir_cy(ip), cz = dir_cz(ip);
+            const bool up_fluid = (nfluid >> opp) & 1u;  // x - c_ip = x + c_opp
+            const int idx_up = nidx - (cx + cy * d.lx + cz * d.lxy);
+            const double lx0 = static_cast<double>(nix) + 0.5;
+            const double ly0 = static_cast<double>(niy) + 0.5;
+            const double lz0 = static_cast<double>(niz) + 0.5;
+            LinkRaw<S> raw;
+            link_raw_here<S>(raw, ip, nidx, pop);
+            double q = 0.0;
+            link_q(ip, lx0, ly0, lz0, X, Y, Z, radius_sq, q);
+            const LinkLoads Ld = link_raw_pair<S>(raw, ip, nidx, idx_up, up_fluid, q, pop);
+            const LinkOut L = eval_link_at(Ld, ip, nix, niy, niz, up_fluid, q, ps, ndens, rho0);
+            if (nj >= 0) {
+                out[sp_at(nj, opp)] = A(L.f_out);
+                out[sp_at(nj, NPOP + opp)] = A(L.g_out);
+            }
+            // Force uses the link record's r = q c (links.cu:134-136, ibb.cu:160-168).
+            const double rx = q * static_cast<double>(cx);
+            const double ry = q * static_cast<double>(cy);
+            const double rz = q * static_cast<double>(cz);
+            const double relx = nix + 0.5 + rx - X;
+            const double rely = niy + 0.5 + ry - Y;
+            const double relz = niz + 0.5 + rz - Z;
+            const double Fx = (cx - L.uwx) * L.f_in - (-cx - L.uwx) * L.f_out;
+            const double Fy = (cy - L.uwy) * L.f_in - (-cy - L.uwy) * L.f_out;
+            const double Fz = (cz - L.uwz) * L.f_in - (-cz - L.uwz) * L.f_out;
+            c[0] = Fx;
+            c[1] = Fy;
+            c[2] = Fz;
+            c[3] = rely * Fz - relz * Fy;
+            c[4] = relz * Fx - relx * Fz;
+            c[5] = relx * Fy - rely * Fx;
+            c[6] = L.g_in - L.g_out;
         }
-        // Force uses the link record's r = q c (links.cu:134-136, ibb.cu:160-168).
-        const double rx = q * static_cast<double>(cx);
-        const double ry = q * static_cast<double>(cy);
-        const double rz = q * static_cast<double>(cz);
-        const double relx = ix + 0.5 + rx - X;
-        const double rely = iy + 0.5 + ry - Y;
-        const double relz = iz + 0.5 + rz - Z;
-        const double Fx = (cx - L.uwx) * L.f_in - (-cx - L.uwx) * L.f_out;
-        const double Fy = (cy - L.uwy) * L.f_in - (-cy - L.uwy) * L.f_out;
-        const double Fz = (cz - L.uwz) * L.f_in - (-cz - L.uwz) * L.f_out;
-        v[0] += Fx;
-        v[1] += Fy;
-        v[2] += Fz;
-        v[3] += rely * Fz - relz * Fy;
-        v[4] += relz * Fx - relx * Fz;
-        v[5] += relx * Fy - rely * Fx;
-        v[6] += L.g_in - L.g_out;
+        // each node adds its links' shares of this batch in ascending direction order, the
+        // order of its own walk over them
+        const int first = off > base ? off : base;
+        const int last = (off + cnt < base + 64) ? off + cnt : base + 64;
+        const int here = last - first;
+        int rounds = here;
+#pragma unroll
+        for (int s = 32; s >= 1; s >>= 1) {
+            const int o = __shfl_xor(rounds, s);
+            rounds = o > rounds ? o : rounds;
+        }
+        for (int r = 0; r < rounds; r++) {
+            const int src = (first - base + r) & 63;
+            double sh[7];
+#pragma unroll
+            for (int cc = 0; cc < 7; cc++) sh[cc] = __shfl(c[cc], src);
+            if (r < here) {
+#pragma unroll
+                for (int cc = 0; cc < 7; cc++) v[cc] += sh[cc];
+            }
+        }
     }
-    if (j >= 0) gbl(pd.sp.mask)[j] = replaced | (higher ? SP_MULTI : 0u);
+    if (mask == 0u) return;
     const size_t plane = (size_t)pd.np * pd.lnk_stride;
     const gptr<double> part = gbl(pd.lnk_part);
-    for (int c = 0; c < 7; c++) part[c * plane + e] = v[c];
+    for (int cc = 0; cc < 7; cc++) part[cc * plane + e] = v[cc];
 }
 
 // Planes C0 .. C0+NC-1 of the node shares (0-2 force, 3-5 torque, 6 heat) onto the particle's
