@@ -16,9 +16,9 @@ state resident in HBM before the timed region starts.  The line also carries:
   forms         the FP16-storage (FP32 math), FP32 and BF16-storage forms on the same workload,
                 with their Nu and KE deviations from FP64: at the end of the timed run and as the
                 mean over an untimed window (10 000 steps by default) after it, checked against
-                the Nu +-1 % bar; and the 300-sphere FP64 case;
+                the Nu +-1 % bar; and the 300-sphere FP64 case, 5 000 timed steps by default;
   serving       the served batch (FP16 storage) under Poisson arrivals at half its measured
-                capacity: time-to-result p50/p99 over a bounded request set.
+                capacity: time-to-result p50/p99 over 256 requests by default.
 """
 import argparse
 import json
@@ -183,6 +183,9 @@ def main():
     ap.add_argument("--no-forms", action="store_true",
                     help="skip the FP16-, FP32- and BF16-storage forms")
     ap.add_argument("--no-particles", action="store_true", help="skip the particle-laden case")
+    ap.add_argument("--particle-steps", type=int, default=5000,
+                    help="timed steps of the 300-sphere case (after a tenth as many warm-up "
+                         "steps), independent of --steps")
     ap.add_argument("--quiet", action="store_true")
     ap.add_argument("--batch-probe", type=int, default=0, metavar="K",
                     help="K eager (not graph-replayed) steps of a 16-request serving batch, "
@@ -190,7 +193,7 @@ def main():
     ap.add_argument("--counter-run", action="store_true",
                     help="minimal run for memory-counter passes: 3 FP64 lattice steps at "
                          "320x320x160 and one 1 GiB copy-probe launch, no JSON")
-    ap.add_argument("--serve", type=int, default=24,
+    ap.add_argument("--serve", type=int, default=256,
                     help="requests of the served-batch workload (0: skip it)")
     args = ap.parse_args()
 
@@ -282,8 +285,8 @@ def main():
                 f"({'within' if form['nu_bar']['meets'] else 'FAILS'} the {NU_BAR:.0%} bar)")
             forms[name] = form
     if not args.no_particles:
-        forms["fp64_particle_laden"] = run_particles(rb, max(args.steps // 2, 10),
-                                                     max(args.warmup // 2, 5), log)
+        forms["fp64_particle_laden"] = run_particles(rb, max(args.particle_steps, 1),
+                                                     max(args.particle_steps // 10, 1), log)
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "lattice_fp64_320x320x160_traffic.json")
     if os.path.exists(tpath):
