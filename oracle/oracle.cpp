@@ -34,6 +34,24 @@
 #include <omp.h>
 #endif
 
+// Planted mistakes.  ORACLE_MUTANT = k > 0 builds the oracle with mistake k of the list below;
+// tests/test_oracle_mutants.py builds each one and shows that the pin meant to catch it fails.
+// Every normal build has ORACLE_MUTANT = 0, and every MUT(k) is then a constant false.
+//    1-4  bounce_back_particles: x and x - c swapped (q < 1/2), i and opp swapped (q < 1/2),
+//         i and opp swapped (q >= 1/2), the momentum term dropped
+//    5-8  bounce_back_thermal: x and x - c swapped (q <= 1/2), i and opp swapped (q > 1/2),
+//         the (1 + rho) factor dropped, ibnode in place of ibnode_prev
+//    9-11 new-fluid predicate: < for <=, the current owner for the previous one, the previous
+//         and current flags swapped
+//   12-14 output_nu: the previous flags, the wall sums normalised by the fluid count, the
+//         actual solid fraction in place of PARTICLE_FRACTION
+//   15-17 output fields: compute_Ked without the force term, compute_ted's denominator with
+//         rho + 1, the particle-node overwrite without the rotation
+#ifndef ORACLE_MUTANT
+#define ORACLE_MUTANT 0
+#endif
+#define MUT(k) (ORACLE_MUTANT == (k))
+
 namespace {
 
 constexpr int NPOP = 27;
@@ -739,20 +757,23 @@ void or_ibb_f(OrState *s) {
         const double uwz = V[2][pid] + (rely * W[0][pid] - relx * W[1][pid]);
         const double evel = CIX[ip] * uwx + CIY[ip] * uwy + CIZ[ip] * uwz;
         const double RT = 1.0 / 3.0;
-        const double momentum_term = 2.0 * s->tp[ip] * RHO0 * evel / RT;
+        const double momentum_term = MUT(4) ? 0.0 : 2.0 * s->tp[ip] * RHO0 * evel / RT;
         bool handled = false;
         if (ix1 >= 0 && ix1 < LX && iy1 >= 0 && iy1 < LY && iz1 >= 0 && iz1 < LZ) {
             const long idx1 = (long)iz1 * LXY + (long)iy1 * LX + ix1;
             if (s->ibnode[idx1] == 0) {
                 const double f_here = s->f_collide[ip * N + idx];
                 if (q < 0.5) {
-                    const double f_upstream = s->f_collide[ip * N + idx1];
-                    s->f[opp * N + idx] = 2.0 * q * f_here + (1.0 - 2.0 * q) * f_upstream - momentum_term;
+                    const double f_upstream = s->f_collide[(MUT(2) ? opp : ip) * N + idx1];
+                    s->f[opp * N + idx] = MUT(1) ? 2.0 * q * f_upstream + (1.0 - 2.0 * q) * f_here - momentum_term
+                                                 : 2.0 * q * f_here + (1.0 - 2.0 * q) * f_upstream - momentum_term;
                 } else {
                     const double f_opposite_collide = s->f_collide[opp * N + idx];
                     const double inv_2q = 0.5 / q;
-                    s->f[opp * N + idx] = (f_here - momentum_term) * inv_2q +
-                                          (2.0 * q - 1.0) * inv_2q * f_opposite_collide;
+                    s->f[opp * N + idx] = MUT(3) ? (f_opposite_collide - momentum_term) * inv_2q +
+                                                       (2.0 * q - 1.0) * inv_2q * f_here
+                                                 : (f_here - momentum_term) * inv_2q +
+                                                       (2.0 * q - 1.0) * inv_2q * f_opposite_collide;
                 }
                 handled = true;
             }
@@ -785,17 +806,17 @@ void or_ibb_g(OrState *s) {
         const long idx1 = (long)iz1 * LXY + (long)iy1 * LX + ix1;
         const double density = s->rho[idx];
         const double Tp = s->p[RBTP_P_TEMP][pid];
-        const double amp = 2.0 * s->tp[ip] * Tp * (1.0 + density);
+        const double amp = 2.0 * s->tp[ip] * Tp * (MUT(7) ? 1.0 : 1.0 + density);
         double value = 0.0;
         if (ix1 >= 0 && ix1 < LX && iy1 >= 0 && iy1 < LY && iz1 >= 0 && iz1 < LZ &&
-            s->ibnode_prev[idx1] == 0) {
+            (MUT(8) ? s->ibnode[idx1] : s->ibnode_prev[idx1]) == 0) {
             if (q <= 0.5) {
-                const double g_here = s->g_collide[ip * N + idx];
-                const double g_upstream = s->g_collide[ip * N + idx1];
+                const double g_here = s->g_collide[ip * N + (MUT(5) ? idx1 : idx)];
+                const double g_upstream = s->g_collide[ip * N + (MUT(5) ? idx : idx1)];
                 value = -(2.0 * q * g_here + (1.0 - 2.0 * q) * g_upstream) + amp;
             } else {
-                const double g_here = s->g_collide[ip * N + idx];
-                const double g_opp_here = s->g_collide[opp * N + idx];
+                const double g_here = s->g_collide[(MUT(6) ? opp : ip) * N + idx];
+                const double g_opp_here = s->g_collide[(MUT(6) ? ip : opp) * N + idx];
                 const double inv_2q = 0.5 / q;
                 value = -g_here * inv_2q + amp * inv_2q + (2.0 * q - 1.0) * inv_2q * g_opp_here;
             }
@@ -1068,8 +1089,10 @@ void or_refill(OrState *s) {
     s->nf_nodes.clear();
     s->nf_pids.clear();
     for (long idx = 0; idx < N; idx++) {
-        if (s->ibnode_prev[idx] != 1 || s->ibnode[idx] == 1) continue;
-        const int pid = s->owner_prev[idx];
+        if (MUT(11) ? (s->ibnode[idx] != 1 || s->ibnode_prev[idx] == 1)
+                    : (s->ibnode_prev[idx] != 1 || s->ibnode[idx] == 1))
+            continue;
+        const int pid = MUT(10) ? s->owner[idx] : s->owner_prev[idx];
         if (pid < 0 || pid >= np) continue;
         const int iz = (int)(idx / LXY);
         const int rem = (int)(idx - (long)iz * LXY);
@@ -1081,7 +1104,7 @@ void or_refill(OrState *s) {
         const double dx = x0 - X[pid], dy = y0 - Y[pid], dz = z0 - Z[pid];
         const double dist = std::sqrt(dx * dx + dy * dy + dz * dz);
         const double close_thresh = 1.7606816861659007;
-        if (!(dist - R <= close_thresh)) continue;
+        if (!(MUT(9) ? dist - R < close_thresh : dist - R <= close_thresh)) continue;
         s->nf_nodes.push_back((int)idx);
         s->nf_pids.push_back(pid);
     }
@@ -1246,7 +1269,13 @@ int or_step(OrState *s, int nsteps) {
 void or_nu(const OrState *s, double out[7]) {
     const int LX = s->LX, LY = s->LY, LZ = s->LZ, LXY = s->LXY;
     const bool parts = s->particles();
-    const double volume_fraction = s->c.particle_fraction;
+    const std::vector<int> &ibnode = MUT(12) ? s->ibnode_prev : s->ibnode;
+    double volume_fraction = s->c.particle_fraction;
+    if (MUT(14) && parts) {
+        volume_fraction = 0.0;
+        for (int v : ibnode) volume_fraction += v != 0;
+        volume_fraction /= static_cast<double>(s->N);
+    }
     const double fluid_fraction = std::fmax(0.0, 1.0 - volume_fraction);
     double plane_avg_uT_fluid = 0.0, plane_avg_uT_part = 0.0;
     for (int ix = 0; ix < LX; ++ix) {
@@ -1256,7 +1285,7 @@ void or_nu(const OrState *s, double out[7]) {
             for (int iy = 0; iy < LY; ++iy) {
                 const long idx = (long)iz * LXY + (long)iy * LX + ix;
                 bool is_fluid = true;
-                if (parts) is_fluid = (s->ibnode[idx] == 0);
+                if (parts) is_fluid = (ibnode[idx] == 0);
                 if (is_fluid) {
                     plane_sum_fluid += s->ux[idx] * s->phi[idx];
                     fluid_count++;
@@ -1284,14 +1313,14 @@ void or_nu(const OrState *s, double out[7]) {
         for (int iy = 0; iy < LY; ++iy) {
             const long idx_bottom = (long)iz * LXY + (long)iy * LX;
             bool bottom_fluid = true;
-            if (parts) bottom_fluid = (s->ibnode[idx_bottom] == 0);
+            if (parts) bottom_fluid = (ibnode[idx_bottom] == 0);
             if (bottom_fluid) {
                 bottom += (tHot - s->phi[idx_bottom]) / 0.5;
                 bottom_count++;
             }
             const long idx_top = (long)iz * LXY + (long)iy * LX + (LX - 1);
             bool top_fluid = true;
-            if (parts) top_fluid = (s->ibnode[idx_top] == 0);
+            if (parts) top_fluid = (ibnode[idx_top] == 0);
             if (top_fluid) {
                 top += (s->phi[idx_top] - tCold) / 0.5;
                 top_count++;
@@ -1300,8 +1329,8 @@ void or_nu(const OrState *s, double out[7]) {
     const double section_area = static_cast<double>(LY) * static_cast<double>(LZ);
     const double wall_scale = static_cast<double>(LX) / (tHot - tCold);
     double Tbottom = 0.0, Ttop = 0.0;
-    if (bottom_count > 0) Tbottom = (bottom / section_area) * wall_scale;
-    if (top_count > 0) Ttop = (top / section_area) * wall_scale;
+    if (bottom_count > 0) Tbottom = (bottom / (MUT(13) ? bottom_count : section_area)) * wall_scale;
+    if (top_count > 0) Ttop = (top / (MUT(13) ? top_count : section_area)) * wall_scale;
     out[0] = (double)s->step;
     out[1] = volume_fraction;
     out[2] = Nu_bulk;
@@ -1363,7 +1392,7 @@ void or_output_fields(const OrState *s, double *out) {
             double feq = s->tp[ip] * (rho_local + eu + 0.5 * (eu * eu - uv));
             double fneq = s->f_collide[ip * N + idx] - feq;
             double cidotF = CIX[ip] * fxv + CIY[ip] * fyv + CIZ[ip] * fzv;
-            double feq_term = 0.5 * (cidotF - bu) * feq / RT;
+            double feq_term = MUT(15) ? 0.0 : 0.5 * (cidotF - bu) * feq / RT;
             tau_xx += CIX[ip] * CIX[ip] * (fneq + feq_term);
             tau_xy += CIX[ip] * CIY[ip] * (fneq + feq_term);
             tau_xz += CIX[ip] * CIZ[ip] * (fneq + feq_term);
@@ -1391,7 +1420,7 @@ void or_output_fields(const OrState *s, double *out) {
             gneqz += CIZ[ip] * dg;
         }
         double p = rho_local * RT;
-        double denominator = 2.0 * tauc * RT + p;
+        double denominator = 2.0 * tauc * RT + (MUT(16) ? (1.0 + rho_local) * RT : p);
         double tx = -(2.0 * gneqx + phi_local * fxv) / denominator;
         double ty = -(2.0 * gneqy + phi_local * fyv) / denominator;
         double tz = -(2.0 * gneqz + phi_local * fzv) / denominator;
@@ -1409,9 +1438,10 @@ void or_output_fields(const OrState *s, double *out) {
                 const double wx = s->p[RBTP_P_OMEGA_X][pid], wy = s->p[RBTP_P_OMEGA_Y][pid],
                              wz = s->p[RBTP_P_OMEGA_Z][pid];
                 r = s->c.rho0;
-                u = s->p[RBTP_P_VEL_X][pid] + (relz * wy - rely * wz);
-                v = s->p[RBTP_P_VEL_Y][pid] + (relx * wz - relz * wx);
-                w = s->p[RBTP_P_VEL_Z][pid] + (rely * wx - relx * wy);
+                const double rot = MUT(17) ? 0.0 : 1.0;
+                u = s->p[RBTP_P_VEL_X][pid] + rot * (relz * wy - rely * wz);
+                v = s->p[RBTP_P_VEL_Y][pid] + rot * (relx * wz - relz * wx);
+                w = s->p[RBTP_P_VEL_Z][pid] + rot * (rely * wx - relx * wy);
                 T = s->p[RBTP_P_TEMP][pid];
                 ted = 0.0;
                 Ked = 0.0;

@@ -12,7 +12,9 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "liboracle.so")
+# RBTP_ORACLE_LIB names another build of oracle.cpp: tests/test_oracle_mutants.py points it at
+# builds with a planted mistake (ORACLE_MUTANT) to show that the pins catch it.
+LIB_PATH = os.environ.get("RBTP_ORACLE_LIB") or os.path.join(HERE, "liboracle.so")
 ROOT = os.path.dirname(HERE)
 
 
