@@ -47,6 +47,19 @@
 //         actual solid fraction in place of PARTICLE_FRACTION
 //   15-17 output fields: compute_Ked without the force term, compute_ted's denominator with
 //         rho + 1, the particle-node overwrite without the rotation
+//      18 macrovar: u without F_old/2
+//   19-20 collision_BGK: the Guo term's (c_x - u_x) reversed, its (1 - 1/(2 tau)) left out
+//   21-23 collision_BGK_scalar: the lambda0 term of geqnew_0 with + for -, grad_denom with
+//         rho + 1, the 3 phi c.F source term dropped
+//      24 streaming: a wall returns population i instead of opp(i)
+//   25-27 streaming_scalar: the x walls without (1 + rho), the hot and cold walls swapped, the
+//         x walls before the z and y walls
+//   28-29 geometry: compute_link_q's larger root, the highest pid keeping a shared node
+//   30-31 momentum exchange with (c_opp + u_w), heat with g*_i + g_opp
+//      32 repulsion: the gap measured from one radius
+//   33-34 update_particles: force history weights 1, 1, 0, buoyancy with the wrong sign
+//   35-37 refill: e.u divided by RT in the fallback, first-order samples only, F kept
+//      38 init_g: the lambda0 term of g_0 with + for -
 #ifndef ORACLE_MUTANT
 #define ORACLE_MUTANT 0
 #endif
@@ -317,7 +330,7 @@ void or_macrovar(OrState *s) {
         for (long j = 0; j < m; j++) {
             const long idx = base + j;
             s->rho[idx] = rho_local[j];
-            s->ux[idx] = ux_local[j] + 0.5 * s->fx[idx];
+            s->ux[idx] = ux_local[j] + (MUT(18) ? 0.0 : 0.5 * s->fx[idx]);
             s->uy[idx] = uy_local[j] + 0.5 * s->fy[idx];
             s->uz[idx] = uz_local[j] + 0.5 * s->fz[idx];
             s->phi[idx] = phi_local[j];
@@ -342,7 +355,7 @@ void or_collision_f(OrState *s) {
     const long N = s->N;
     const double tau = s->d.tau;
     const double RT = 1.0 / 3.0;
-    const double force_coeff = (1.0 - 0.5 / tau);
+    const double force_coeff = MUT(20) ? 1.0 : (1.0 - 0.5 / tau);
 #pragma omp parallel for schedule(static)
     for (long base = 0; base < N; base += CH) {
         const long m = std::min(CH, N - base);
@@ -363,7 +376,7 @@ void or_collision_f(OrState *s) {
                 double fpop = fp[j];
                 double fchange = -(fpop - feq) / tau;
                 double feq_rho0 = tp * (1.0 + eu + 0.5 * (eu * eu - uv[j]));
-                double cuF = (cx - ux_local[j]) * fxv[j];
+                double cuF = (MUT(19) ? ux_local[j] - cx : cx - ux_local[j]) * fxv[j];
                 fchange += force_coeff * cuF / RT * feq_rho0;
                 fp[j] = fpop + fchange;
             }
@@ -389,7 +402,7 @@ void or_collision_g(OrState *s) {
             const double u2 = ux_local[j] * ux_local[j] + uy_local[j] * uy_local[j] + uz_local[j] * uz_local[j];
             uv[j] = u2 / RT;
             gneqx[j] = 0.0; gneqy[j] = 0.0; gneqz[j] = 0.0;
-            grad_denom[j] = rho_local[j] / 3.0 + 2.0 * tauc / 3.0;
+            grad_denom[j] = (MUT(22) ? rho_local[j] + 1.0 : rho_local[j]) / 3.0 + 2.0 * tauc / 3.0;
         }
         for (int ip = 0; ip < NPOP; ip++) {
             const int cx = CIX[ip], cy = CIY[ip], cz = CIZ[ip];
@@ -398,7 +411,8 @@ void or_collision_g(OrState *s) {
             for (long j = 0; j < m; j++) {
                 double eu = (cx * ux_local[j] + cy * uy_local[j] + cz * uz_local[j]) / RT;
                 double geq = tp * phi_local[j] * (1.0 + eu + 0.5 * (eu * eu - uv[j]));
-                double geqnew = (ip == 0) ? (geq - lambda0 * phi_local[j] * rho_local[j])
+                double geqnew = (ip == 0) ? (MUT(21) ? geq + lambda0 * phi_local[j] * rho_local[j]
+                                                     : geq - lambda0 * phi_local[j] * rho_local[j])
                                           : (geq + tp * phi_local[j] * rho_local[j]);
                 double diff = gp[j] - geqnew;
                 gneqx[j] += cx * diff;
@@ -414,10 +428,11 @@ void or_collision_g(OrState *s) {
                 double eb = cx * fxv[j] + cy * fyv[j] + cz * fzv[j];
                 double gneq = cx * gneqx[j] + cy * gneqy[j] + cz * gneqz[j];
                 double gradt = -(2.0 * gneq + eb * phi_local[j]) / grad_denom[j];
-                double Ralph = tp * Ralph_coeff * (gradt * rho_local[j] + 3.0 * phi_local[j] * eb);
+                double Ralph = tp * Ralph_coeff * (gradt * rho_local[j] + (MUT(23) ? 0.0 : 3.0 * phi_local[j] * eb));
                 double eu = (cx * ux_local[j] + cy * uy_local[j] + cz * uz_local[j]) / RT;
                 double geq = tp * phi_local[j] * (1.0 + eu + 0.5 * (eu * eu - uv[j]));
-                double geqnew = (ip == 0) ? (geq - lambda0 * phi_local[j] * rho_local[j])
+                double geqnew = (ip == 0) ? (MUT(21) ? geq + lambda0 * phi_local[j] * rho_local[j]
+                                                     : geq - lambda0 * phi_local[j] * rho_local[j])
                                           : (geq + tp * phi_local[j] * rho_local[j]);
                 double gpop = gp[j];
                 double gchange = -(gpop - geqnew) / tauc + Ralph;
@@ -450,6 +465,11 @@ static void stream_into(OrState *s, const Field &src, Field &dst,
                 const int sy = iy - cy, sz = iz - cz;
                 if (!(sy >= 0 && sy < LY && sz >= 0 && sz < LZ)) {
                     for (int ix = 0; ix < LX; ix++) dp[row + ix] = so[row + ix];
+                    if (MUT(27) && thermal && cx != 0) {
+                        const int iw = cx > 0 ? 0 : LX - 1;
+                        dp[row + iw] = -so[row + iw] + 2.0 * tpw * (cx > 0 ? tHot : tCold) *
+                                                           (1.0 + s->rho[row + iw]);
+                    }
                     continue;
                 }
                 const long srow = (long)sz * LXY + (long)sy * LX;
@@ -458,13 +478,14 @@ static void stream_into(OrState *s, const Field &src, Field &dst,
                     if (sx >= 0 && sx < LX) {
                         dp[row + ix] = sp[srow + sx];
                     } else if (!thermal) {
-                        dp[row + ix] = so[row + ix];
+                        dp[row + ix] = MUT(24) ? sp[row + ix] : so[row + ix];
                     } else {
-                        const double density = s->rho[row + ix];
+                        const double density = MUT(25) ? 0.0 : s->rho[row + ix];
+                        const double t_in = MUT(26) ? tCold : tHot, t_out = MUT(26) ? tHot : tCold;
                         if (sx < 0 && cx > 0)
-                            dp[row + ix] = -so[row + ix] + 2.0 * tpw * tHot * (1.0 + density);
+                            dp[row + ix] = -so[row + ix] + 2.0 * tpw * t_in * (1.0 + density);
                         else if (sx >= LX && cx < 0)
-                            dp[row + ix] = -so[row + ix] + 2.0 * tpw * tCold * (1.0 + density);
+                            dp[row + ix] = -so[row + ix] + 2.0 * tpw * t_out * (1.0 + density);
                         else
                             dp[row + ix] = so[row + ix];
                     }
@@ -530,7 +551,8 @@ static void init_fluid(OrState *s, const double *temperature) {
             const double eu = (CIX[ip] * u + CIY[ip] * v + CIZ[ip] * w) / RT;
             double geq = s->tp[ip] * tempT * (1.0 + eu + 0.5 * (eu * eu - uv));
             if (ip == 0) {
-                geq -= lambda0 * tempT * r;
+                if (MUT(38)) geq += lambda0 * tempT * r;
+                else geq -= lambda0 * tempT * r;
             } else {
                 geq += s->tp[ip] * tempT * r;
             }
@@ -557,7 +579,7 @@ static bool compute_link_q(int ip, double x0, double y0, double z0, double px, d
     const double c = dx * dx + dy * dy + dz * dz - radius_sq;
     const double disc = b * b - 4.0 * a * c;
     if (disc <= 0.0) return false;
-    q = (-b - std::sqrt(disc)) / (2.0 * a);
+    q = (MUT(28) ? -b + std::sqrt(disc) : -b - std::sqrt(disc)) / (2.0 * a);
     return (q >= 0.0 && q <= 1.0);
 }
 
@@ -603,7 +625,7 @@ void or_build_links(OrState *s) {
                     const double dx = cx - PX[p], dy = cy - PY[p], dz = cz - PZ[p];
                     if (dx * dx + dy * dy + dz * dz > radius_sq) continue;
                     const long idx = (long)iz * LXY + (long)iy * LX + ix;
-                    if (s->owner[idx] != -1) continue;
+                    if (!MUT(29) && s->owner[idx] != -1) continue;
                     s->owner[idx] = p;
                     s->ibnode[idx] = 1;
                 }
@@ -672,7 +694,7 @@ static void initial_ibnode(OrState *s) {
             for (int iy = lo[1]; iy <= hi[1]; iy++)
                 for (int ix = lo[0]; ix <= hi[0]; ix++) {
                     const long idx = (long)iz * LXY + (long)iy * LX + ix;
-                    if (s->owner[idx] != -1) continue;
+                    if (!MUT(29) && s->owner[idx] != -1) continue;
                     const double dx = (static_cast<double>(ix) + 0.5) - px;
                     const double dy = (static_cast<double>(iy) + 0.5) - py;
                     const double dz = (static_cast<double>(iz) + 0.5) - pz;
@@ -870,7 +892,7 @@ void or_repulsion(OrState *s) {
                 double dist_sq = dx * dx + dy * dy + dz * dz;
                 if (dist_sq <= 1.0e-12) continue;
                 double dist = std::sqrt(dist_sq);
-                double hgap = dist - 2.0 * radius;
+                double hgap = dist - (MUT(32) ? 1.0 : 2.0) * radius;
                 if (hgap >= min_gap) continue;
                 double overlap = (hgap < 0.0) ? -hgap : 0.0;
                 double factor = (min_gap - hgap) / min_gap;
@@ -957,9 +979,10 @@ void or_forces(OrState *s) {
         const double uwz = s->p[RBTP_P_VEL_Z][pid] + (rely * wx - relx * wy);
         const double f_in = s->f_collide[ip * N + idx];
         const double f_out = s->f[opp * N + idx];
-        const double Fx = (CIX[ip] - uwx) * f_in - (CIX[opp] - uwx) * f_out;
-        const double Fy = (CIY[ip] - uwy) * f_in - (CIY[opp] - uwy) * f_out;
-        const double Fz = (CIZ[ip] - uwz) * f_in - (CIZ[opp] - uwz) * f_out;
+        const double sw = MUT(30) ? -1.0 : 1.0;
+        const double Fx = (CIX[ip] - uwx) * f_in - (CIX[opp] - sw * uwx) * f_out;
+        const double Fy = (CIY[ip] - uwy) * f_in - (CIY[opp] - sw * uwy) * f_out;
+        const double Fz = (CIZ[ip] - uwz) * f_in - (CIZ[opp] - sw * uwz) * f_out;
         std::array<double, 6> &v = share[{pid, link_slot(s, link)}];
         v[0] += Fx;
         v[1] += Fy;
@@ -1001,7 +1024,7 @@ void or_heat(OrState *s) {
         const long idx = (long)link.cell_k * LXY + (long)link.cell_j * LX + link.cell_i;
         const double Gi = s->g_collide[ip * N + idx];
         const double Go = s->g[opp * N + idx];
-        share[{link.pid, link_slot(s, link)}] += Gi - Go;
+        share[{link.pid, link_slot(s, link)}] += MUT(31) ? Gi + Go : Gi - Go;
     }
     for (const auto &kv : share)
         part[(size_t)kv.first.first * RED_LANES + kv.first.second % RED_LANES] += kv.second;
@@ -1032,12 +1055,12 @@ void or_move(OrState *s) {
                      tpz = P(RBTP_P_TORQUE_PREV_Z)[pid];
         const double tp2x = P(RBTP_P_TORQUE_PREV2_X)[pid], tp2y = P(RBTP_P_TORQUE_PREV2_Y)[pid],
                      tp2z = P(RBTP_P_TORQUE_PREV2_Z)[pid];
-        const double buoyancy = (1.0 - 1.0 / rho_ratio) * s->d.grav0;
+        const double buoyancy = (MUT(34) ? -1.0 : 1.0) * (1.0 - 1.0 / rho_ratio) * s->d.grav0;
         const double vx_old = P(RBTP_P_VEL_X)[pid], vy_old = P(RBTP_P_VEL_Y)[pid],
                      vz_old = P(RBTP_P_VEL_Z)[pid];
         const double omega_x_old = P(RBTP_P_OMEGA_X)[pid], omega_y_old = P(RBTP_P_OMEGA_Y)[pid],
                      omega_z_old = P(RBTP_P_OMEGA_Z)[pid];
-        const double combo_fx = 0.5 * forceX + fx_prev + 0.5 * fx_prev2;
+        const double combo_fx = MUT(33) ? forceX + fx_prev : 0.5 * forceX + fx_prev + 0.5 * fx_prev2;
         const double combo_fy = 0.5 * forceY + fy_prev + 0.5 * fy_prev2;
         const double combo_fz = 0.5 * forceZ + fz_prev + 0.5 * fz_prev2;
         const double combo_torque_x = 0.5 * torqueX + tpx + 0.5 * tp2x;
@@ -1133,7 +1156,7 @@ void or_refill(OrState *s) {
                 if (s->ibnode_prev[idx1] != 0) continue;
                 int ix2 = ix + 2 * CIX[ip2], iy2 = iy + 2 * CIY[ip2], iz2 = iz + 2 * CIZ[ip2];
                 bool inside2 = (ix2 >= 0 && ix2 < LX && iy2 >= 0 && iy2 < LY && iz2 >= 0 && iz2 < LZ);
-                if (inside2) {
+                if (inside2 && !MUT(36)) {
                     const long idx2 = (long)iz2 * LXY + (long)iy2 * LX + ix2;
                     if (s->ibnode_prev[idx2] == 0) {
                         temp_f += 2.0 * s->f_collide[ip * N + idx1] - s->f_collide[ip * N + idx2];
@@ -1166,7 +1189,7 @@ void or_refill(OrState *s) {
                 double w = vz + (rely * omega_x - relx * omega_y);
                 if (pid < 0) { u = 0.0; v = 0.0; w = 0.0; }
                 const double r = RHO0;
-                const double eu = CIX[ip] * u + CIY[ip] * v + CIZ[ip] * w;
+                const double eu = (CIX[ip] * u + CIY[ip] * v + CIZ[ip] * w) / (MUT(35) ? RT : 1.0);
                 const double uv = (u * u + v * v + w * w) / RT;
                 temp_f = s->tp[ip] * (r + eu + 0.5 * (eu * eu - uv));
                 temp_g = s->tp[ip] * tempT * (1.0 + eu + 0.5 * (eu * eu - uv));
@@ -1198,7 +1221,7 @@ void or_refill(OrState *s) {
             s->ux[idx] = 0.0; s->uy[idx] = 0.0; s->uz[idx] = 0.0;
         }
         s->phi[idx] = phi_loc;
-        s->fx[idx] = 0.0; s->fy[idx] = 0.0; s->fz[idx] = 0.0;
+        if (!MUT(37)) { s->fx[idx] = 0.0; s->fy[idx] = 0.0; s->fz[idx] = 0.0; }
     }
     s->ibnode_prev = s->ibnode;
     s->owner_prev = s->owner;

@@ -70,10 +70,12 @@
  * or i and opp swapped, the (1 + rho) factor dropped, ibnode for ibnode_prev, < for <=, the old
  * centre or the current owner in the new-fluid test, the previous flags or a fluid-count wall
  * normalisation in Nu, the force term of compute_Ked, rho + 1 in compute_ted, the rotation in the
- * overwrite) move the result far beyond its tolerance.  Seventeen of them are planted in
- * oracle.cpp behind ORACLE_MUTANT (off in every normal build); tests/test_oracle_mutants.py builds
- * the oracle once per mistake and shows that the targeted pin fails on that build, while the
- * unmutated build passes.
+ * overwrite) move the result far beyond its tolerance.  Thirty-eight plausible mistakes, these
+ * and others across macrovar, both collisions, both streamings, link q, marking, the momentum
+ * exchange, heat, repulsion, the integrator, refill and init_g, are planted in oracle.cpp behind
+ * ORACLE_MUTANT (off in every normal build); tests/test_oracle_mutants.py builds the oracle once
+ * per mistake and shows that the targeted pin fails on that build, while the unmutated build
+ * passes.
  *
  * Summation order: see the header of oracle.cpp (deterministic per-particle reductions).
  *

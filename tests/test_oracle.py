@@ -614,21 +614,29 @@ def test_refill_extrapolation_exact_for_linear_fields(O):
 
 def test_refill_writes_consistent_moments(O):
     """refill_new_fluid_kernel (refill.cu:214-239): at every refilled node rho = sum f,
-    phi = sum g of the values written, and F = 0."""
+    phi = sum g of the values written, and F = 0.  With buoyancy on, this step's forcing set F
+    from phi at every node, solid ones included, so F is not zero there before the refill."""
     L = (20, 14, 14)
-    kw = dict(single_phase(*L, beta=0.0), activate_particles=1, npart_override=1,
+    kw = dict(single_phase(*L), activate_particles=1, npart_override=1,
               particle_diam_frac=8.0 / 20)
     ora = O.Oracle(O.default_case(**kw))
-    ora.init(ppos=np.array([8.0, 7.0, 7.0]), ptemp=np.array([0.2]))
+    T = O.linear_temperature(ora.case, 5, 1e-3)
+    ora.init(temperature=T, ppos=np.array([8.0, 7.0, 7.0]), ptemp=np.array([0.2]))
     ora.set_particles("vel_x", [0.9])
+    stages = ("macrovar", "forcing", "collide_f", "stream_f", "ibb_f", "collide_g", "stream_g",
+              "ibb_g", "prepare", "repulsion", "forces", "heat", "move", "build_links")
     seen = 0
     for _ in range(4):
-        ora.step(1)
+        for st in stages:
+            ora.stage(st)
+        fx_before = ora.field("fx")
+        ora.stage("refill")
         nodes, pids = ora.new_fluid()
         f, g = ora.populations("f"), ora.populations("g")
         for idx in nodes:
-            assert ora.field("rho")[idx] == pytest.approx(f[:, idx].sum(), abs=1e-15)
-            assert ora.field("phi")[idx] == pytest.approx(g[:, idx].sum(), abs=1e-15)
+            assert ora.field("rho")[idx] == pytest.approx(f[:, idx].sum(), rel=1e-14, abs=1e-15)
+            assert ora.field("phi")[idx] == pytest.approx(g[:, idx].sum(), rel=1e-14, abs=1e-15)
+            assert fx_before[idx] != 0.0
             assert ora.field("fx")[idx] == 0.0
             seen += 1
     assert seen > 0

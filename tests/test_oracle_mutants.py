@@ -1,9 +1,10 @@
 """The oracle's pins catch the mistakes they are meant to catch.
 
 oracle.cpp carries numbered planted mistakes (ORACLE_MUTANT = k, none in a normal build).  Each
-test here builds the oracle with one of them and runs the pin in tests/test_oracle_pins.py that
-targets it against that build: the pin must fail.  The unmutated oracle built the same way must
-pass the same pins (the control), so a failure comes from the mistake and not from the build.
+test here builds the oracle with one of them and runs the pin that targets it (in
+tests/test_oracle_pins.py or tests/test_oracle.py) against that build: the pin must fail.  The
+unmutated oracle built the same way must pass the same pins (the control), so a failure comes
+from the mistake and not from the build.
 """
 import os
 import shutil
@@ -16,13 +17,29 @@ import pytest
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 ORACLE = os.path.join(ROOT, "oracle", "oracle.cpp")
 PINS = os.path.join(ROOT, "tests", "test_oracle_pins.py")
+SELF = os.path.join(ROOT, "tests", "test_oracle.py")
 
-IBB_F = "test_ibb_f_bouzidi_linear_fields"
-IBB_G = "test_ibb_g_bouzidi_linear_fields_with_density"
-NEW_FLUID = "test_new_fluid_predicate_brute_force"
-NU = "test_nu_with_particle_nodes_closed_form"
-KED = "test_ked_ted_closed_form"
-OVERWRITE = "test_output_overwrite_means_profile_closed_form"
+IBB_F = f"{PINS}::test_ibb_f_bouzidi_linear_fields"
+IBB_G = f"{PINS}::test_ibb_g_bouzidi_linear_fields_with_density"
+NEW_FLUID = f"{PINS}::test_new_fluid_predicate_brute_force"
+NU = f"{PINS}::test_nu_with_particle_nodes_closed_form"
+KED = f"{PINS}::test_ked_ted_closed_form"
+OVERWRITE = f"{PINS}::test_output_overwrite_means_profile_closed_form"
+MACROVAR = f"{SELF}::test_macrovar_moments"
+COLLIDE_F = f"{SELF}::test_collision_f_conserves_mass_and_adds_force"
+COLLIDE_G = f"{SELF}::test_collision_g_moments"
+STREAM_F = f"{SELF}::test_streaming_delta_and_walls"
+STREAM_G = f"{SELF}::test_thermal_wall_rules"
+LINKS_Q = f"{SELF}::test_links_axis_aligned_q"
+LINKS_BF = f"{SELF}::test_links_brute_force"
+FORCE = f"{SELF}::test_moving_sphere_first_step_force_closed_form"
+HEAT = f"{SELF}::test_sphere_heat_closed_form"
+REPULSION = f"{SELF}::test_repulsion_closed_form"
+MOVE = f"{SELF}::test_move_closed_form"
+REFILL_FALLBACK = f"{SELF}::test_refill_fallback_moments_closed_form"
+REFILL_LINEAR = f"{SELF}::test_refill_extrapolation_exact_for_linear_fields"
+REFILL_MOMENTS = f"{SELF}::test_refill_writes_consistent_moments"
+INIT = f"{SELF}::test_init_uniform_temperature"
 
 # mutant number (oracle.cpp) -> (the mistake, the pin that must catch it)
 MUTANTS = {
@@ -43,6 +60,27 @@ MUTANTS = {
     15: ("compute_Ked: force term dropped", KED),
     16: ("compute_ted: denominator with rho + 1", KED),
     17: ("overwrite_particle_nodes: rotation dropped", OVERWRITE),
+    18: ("macrovar: u without F_old/2", MACROVAR),
+    19: ("collision_BGK: Guo term (c_x - u_x) reversed", COLLIDE_F),
+    20: ("collision_BGK: Guo coefficient 1 - 1/(2 tau) left out", COLLIDE_F),
+    21: ("collision_BGK_scalar: lambda0 term of geqnew_0 with + for -", COLLIDE_G),
+    22: ("collision_BGK_scalar: grad_denom with rho + 1", COLLIDE_G),
+    23: ("collision_BGK_scalar: 3 phi c.F source term dropped", COLLIDE_G),
+    24: ("streaming: a wall returns i instead of opp(i)", STREAM_F),
+    25: ("streaming_scalar: x walls without (1 + rho)", STREAM_G),
+    26: ("streaming_scalar: hot and cold walls swapped", STREAM_G),
+    27: ("streaming_scalar: x walls before the z and y walls", STREAM_G),
+    28: ("compute_link_q: larger root", LINKS_Q),
+    29: ("marking: the highest pid keeps a shared node", LINKS_BF),
+    30: ("momentum exchange with (c_opp + u_w)", FORCE),
+    31: ("heat with g*_i + g_opp", HEAT),
+    32: ("repulsion: gap measured from one radius", REPULSION),
+    33: ("update_particles: force history weights 1, 1, 0", MOVE),
+    34: ("update_particles: buoyancy with the wrong sign", MOVE),
+    35: ("refill fallback: e.u divided by RT", REFILL_FALLBACK),
+    36: ("refill: first-order samples only", REFILL_LINEAR),
+    37: ("refill: F kept at the refilled node", REFILL_MOMENTS),
+    38: ("init_g: lambda0 term of g_0 with + for -", INIT),
 }
 PINNED = sorted({t for _, t in MUTANTS.values()})
 
@@ -61,8 +99,7 @@ def _build(k, out_dir):
 def _run_pins(lib, tests):
     """Runs the named pins against the oracle build lib; returns (returncode, output)."""
     env = dict(os.environ, RBTP_ORACLE_LIB=lib, OMP_NUM_THREADS="2")
-    ids = [f"{PINS}::{t}" for t in tests]
-    r = subprocess.run([sys.executable, "-m", "pytest", "-q", "-p", "no:cacheprovider", *ids],
+    r = subprocess.run([sys.executable, "-m", "pytest", "-q", "-p", "no:cacheprovider", *tests],
                        capture_output=True, text=True, env=env, cwd=ROOT)
     return r.returncode, r.stdout + r.stderr
 
