@@ -9,8 +9,9 @@ Prints one JSON line.  value = lattice updates per second of the whole job (MLUP
 state resident in HBM before the timed region starts.  The line also carries:
   roofline      the fused lattice kernel's algorithmic HBM bytes per launch (880 B/node: 54 FP64
                 populations read and written plus F_x read and written) over its average launch
-                time measured with HIP events on the library's stream, beside two measured
-                ceilings: a streaming copy and the same access pattern without arithmetic;
+                time, measured with HIP events around every launch of the timed steps on the
+                library's stream, beside two measured ceilings: a streaming copy and the same
+                access pattern without arithmetic;
   cpu_baseline  the FP64 host oracle (OpenMP) timed on this machine's cores on a bounded sample,
                 with the CPU model;
   forms         the FP16-storage (FP32 math), FP32 and BF16-storage forms on the same workload,
@@ -61,7 +62,7 @@ def load_rbtp():
     return mod
 
 
-def run_form(rb, kw, steps, warmup, timing_steps, log, nu_window=0):
+def run_form(rb, kw, steps, warmup, log, nu_window=0):
     case = rb.default_case(**kw)
     sim = rb.Sim(case)
     T = rb.linear_temperature(case, 2024, 1e-3)
@@ -69,11 +70,16 @@ def run_form(rb, kw, steps, warmup, timing_steps, log, nu_window=0):
     n = sim.n
     sim.step(warmup, macro_last=False)
     sim.synchronize()
+    # HIP events around every lattice launch of the timed steps, on the library's stream; they
+    # are recorded without waiting and read after the run's synchronisation
+    sim.set_kernel_timing(True)
     t0 = time.perf_counter()
     sim.step(steps, macro_last=True)
     t1 = time.perf_counter()
     wall_ms = (t1 - t0) * 1e3
-    dev_ms = sim.timing().step_ms
+    tm = sim.timing()
+    sim.set_kernel_timing(False)
+    dev_ms = tm.step_ms
     nu = sim.nu()
     ke = sim.kinetic_energy()
     # untimed continuation: Nu sampled every 100 steps for a window average
@@ -82,10 +88,6 @@ def run_form(rb, kw, steps, warmup, timing_steps, log, nu_window=0):
         sim.step(100, macro_last=True)
         nu_samples.append(sim.nu())
     nu_avg = np.mean(nu_samples, axis=0) if nu_samples else nu
-    # kernel-only timing (events around every lattice launch)
-    sim.set_kernel_timing(True)
-    sim.step(timing_steps, macro_last=False)
-    tm = sim.timing()
     sim.close()
     log(f"form precision={kw.get('precision', 0)}: {steps} steps in {wall_ms:.1f} ms wall "
         f"({dev_ms:.1f} ms device), kernel avg {tm.kernel_ms:.4f} ms over {tm.launches}")
@@ -175,7 +177,6 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10000)
     ap.add_argument("--warmup", type=int, default=1000)
-    ap.add_argument("--timing-steps", type=int, default=100)
     ap.add_argument("--nu-window", type=int, default=10000,
                     help="untimed steps after the timed run over which Nu is averaged, for "
                          "FP64 and every reduced-precision form")
@@ -248,7 +249,7 @@ def main():
               for name, scheme in (("fp64_plain_pull", 3), ("p16_plain_pull", 4),
                                    ("p16_row_pull", 5), ("p16_pair_pull", 6))}
     log("pattern probes: " + ", ".join(f"{k} {v:.0f}" for k, v in probes.items()) + " GB/s")
-    r64 = run_form(rb, kw64, args.steps, args.warmup, args.timing_steps, log, args.nu_window)
+    r64 = run_form(rb, kw64, args.steps, args.warmup, log, args.nu_window)
     n = r64["n"]
     mlups = n * args.steps / (r64["wall_ms"] * 1e-3) / 1e6
     achieved = BYTES_PER_NODE[0] * n / (r64["kernel_ms"] * 1e-3) / 1e9
@@ -256,8 +257,7 @@ def main():
     if not args.no_forms:
         for name, prec in (("fp16s", 2), ("fp32", 1), ("bf16s", 3)):
             window = args.nu_window
-            r = run_form(rb, config2(precision=prec), args.steps, args.warmup, args.timing_steps,
-                         log, window)
+            r = run_form(rb, config2(precision=prec), args.steps, args.warmup, log, window)
             m = n * args.steps / (r["wall_ms"] * 1e-3) / 1e6
             form = dict(
                 mlups=m, ratio_vs_fp64=m / mlups, kernel_ms=r["kernel_ms"],

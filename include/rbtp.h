@@ -372,7 +372,9 @@ int rbtp_batch_info(rbtp_batch *b, int *active, int64_t *used_bytes, int64_t *fr
  * handle's stream (milliseconds). */
 int rbtp_last_step_ms(rbtp_sim *s, double *ms);
 /* Average duration (ms) of the fused lattice kernel over the last rbtp_step* call, measured
- * with HIP events around every launch when enabled (0 disables, the default). */
+ * with HIP events around every launch when enabled (0 disables, the default).  The events are
+ * recorded without waiting and read at the call's synchronisation (rbtp_step, rbtp_synchronize,
+ * rbtp_kernel_ms), so timed steps are enqueued as untimed ones are. */
 int rbtp_set_kernel_timing(rbtp_sim *s, int enable);
 int rbtp_kernel_ms(rbtp_sim *s, double *avg_ms, int64_t *launches);
 

@@ -80,9 +80,14 @@ struct rbtp_sim {
     int device;
     hipStream_t stream;
     bool own_stream;
-    hipEvent_t ev_start, ev_stop, ev_k0, ev_k1;
+    hipEvent_t ev_start, ev_stop;
     bool step_events;  // ev_start/ev_stop bracket an asynchronous run not yet timed
+    // Per-launch timing of the lattice kernel (rbtp_set_kernel_timing): a pair of events around
+    // each launch, recorded without waiting and read after the run's own synchronisation (or when
+    // all KT_CAP pairs are in use), so that the timed steps run as they do untimed.
     bool kernel_timing;
+    std::vector<hipEvent_t> kt_ev;  // 2 * KT_CAP events, created on first use
+    int kt_pending;                 // pairs recorded and not yet read
     double kernel_ms_sum;
     int64_t kernel_launches;
     double last_step_ms;
@@ -240,8 +245,7 @@ extern "C" void rbtp_destroy(rbtp_sim *s) {
     part_tables_free(s->st_tab);
     if (s->ev_start) (void)hipEventDestroy(s->ev_start);
     if (s->ev_stop) (void)hipEventDestroy(s->ev_stop);
-    if (s->ev_k0) (void)hipEventDestroy(s->ev_k0);
-    if (s->ev_k1) (void)hipEventDestroy(s->ev_k1);
+    for (hipEvent_t e : s->kt_ev) (void)hipEventDestroy(e);
     if (s->stream && s->own_stream) (void)hipStreamDestroy(s->stream);
     delete s;
 }
@@ -295,8 +299,6 @@ static int create_impl(const rbtp_case *c, rbtp_sim *s, hipStream_t stream) {
     }
     HIP_TRY(hipEventCreate(&s->ev_start));
     HIP_TRY(hipEventCreate(&s->ev_stop));
-    HIP_TRY(hipEventCreate(&s->ev_k0));
-    HIP_TRY(hipEventCreate(&s->ev_k1));
     {
         const int32_t none[2] = {INT_MAX, 0};
         HIP_TRY(hipMalloc(&s->watch_dev, sizeof(none)));
@@ -518,31 +520,48 @@ static int after_lattice(rbtp_sim *s) {
     return RBTP_OK;
 }
 
+// Pairs of timing events in the pool of a sim.
+constexpr int KT_CAP = 1024;
+
+// Adds the recorded launch times to the sums (waits for the last recorded launch).
+static int kt_drain(rbtp_sim *s) {
+    if (s->kt_pending == 0) return RBTP_OK;
+    HIP_TRY(hipEventSynchronize(s->kt_ev[2 * (size_t)s->kt_pending - 1]));
+    for (int j = 0; j < s->kt_pending; j++) {
+        float ms = 0.f;
+        HIP_TRY(hipEventElapsedTime(&ms, s->kt_ev[2 * (size_t)j], s->kt_ev[2 * (size_t)j + 1]));
+        s->kernel_ms_sum += ms;
+        s->kernel_launches++;
+    }
+    s->kt_pending = 0;
+    return RBTP_OK;
+}
+
 static int one_step(rbtp_sim *s, bool macro_out) {
     const bool nostream = s->poststream;
     const hipError_t pending = hipGetLastError();
     if (pending != hipSuccess)
         return set_error(RBTP_E_HIP, "step %d: HIP error left by earlier work: %s", s->step,
                          hipGetErrorString(pending));
+    hipEvent_t k0 = nullptr, k1 = nullptr;
+    if (s->kernel_timing) {
+        if (s->kt_pending == KT_CAP) RB_TRY(kt_drain(s));
+        k0 = s->kt_ev[2 * (size_t)s->kt_pending];
+        k1 = s->kt_ev[2 * (size_t)s->kt_pending + 1];
+    }
     HIP_TRY(dispatch(s->prec, [&](auto *tag) -> hipError_t {
         using S = std::remove_pointer_t<decltype(tag)>;
         const StepArgs<S> a = step_args<S>(s, macro_out);
-        if (s->kernel_timing) {
-            hipError_t e = hipEventRecord(s->ev_k0, s->stream);
+        if (k0) {
+            hipError_t e = hipEventRecord(k0, s->stream);
             if (e != hipSuccess) return e;
         }
         hipError_t e = launch_lattice_step<S>(a, s->strict, nostream, s->stream);
         if (e != hipSuccess) return e;
-        if (s->kernel_timing) e = hipEventRecord(s->ev_k1, s->stream);
+        if (k1) e = hipEventRecord(k1, s->stream);
         return e;
     }));
-    if (s->kernel_timing) {
-        HIP_TRY(hipEventSynchronize(s->ev_k1));
-        float ms = 0.f;
-        HIP_TRY(hipEventElapsedTime(&ms, s->ev_k0, s->ev_k1));
-        s->kernel_ms_sum += ms;
-        s->kernel_launches++;
-    }
+    if (k0) s->kt_pending++;
     return after_lattice(s);
 }
 
@@ -553,6 +572,7 @@ static int step_impl(rbtp_sim *s, int nsteps, unsigned flags, bool sync) {
     HIP_TRY(hipSetDevice(s->device));
     RB_TRY(stage_exit(s));
     if (s->kernel_timing) {
+        RB_TRY(kt_drain(s));
         s->kernel_ms_sum = 0.0;
         s->kernel_launches = 0;
     }
@@ -569,6 +589,7 @@ static int step_impl(rbtp_sim *s, int nsteps, unsigned flags, bool sync) {
         float ms = 0.f;
         HIP_TRY(hipEventElapsedTime(&ms, s->ev_start, s->ev_stop));
         s->last_step_ms = ms;
+        RB_TRY(kt_drain(s));
         if (s->particles) RB_TRY(part_check_errors(s->pd));
         RB_TRY(watch_check(s));
     }
@@ -611,6 +632,7 @@ extern "C" int rbtp_synchronize(rbtp_sim *s) {
         s->last_step_ms = ms;
         s->step_events = false;
     }
+    RB_TRY(kt_drain(s));
     if (s->particles) RB_TRY(part_check_errors(s->pd));
     RB_TRY(watch_check(s));
     return RBTP_OK;
@@ -627,6 +649,12 @@ extern "C" int rbtp_last_step_ms(rbtp_sim *s, double *ms) {
 }
 extern "C" int rbtp_set_kernel_timing(rbtp_sim *s, int enable) {
     if (!s) return set_error(RBTP_E_ARG, "null handle");
+    HIP_TRY(hipSetDevice(s->device));
+    RB_TRY(kt_drain(s));
+    if (enable && s->kt_ev.empty()) {
+        s->kt_ev.assign(2 * (size_t)KT_CAP, nullptr);
+        for (hipEvent_t &e : s->kt_ev) HIP_TRY(hipEventCreate(&e));
+    }
     s->kernel_timing = enable != 0;
     return RBTP_OK;
 }
@@ -638,6 +666,8 @@ extern "C" int rbtp_set_shift_interval(rbtp_sim *s, int steps) {
 }
 extern "C" int rbtp_kernel_ms(rbtp_sim *s, double *avg_ms, int64_t *launches) {
     if (!s) return set_error(RBTP_E_ARG, "null handle");
+    HIP_TRY(hipSetDevice(s->device));
+    RB_TRY(kt_drain(s));
     if (avg_ms) *avg_ms = s->kernel_launches ? s->kernel_ms_sum / s->kernel_launches : 0.0;
     if (launches) *launches = s->kernel_launches;
     return RBTP_OK;

@@ -245,3 +245,31 @@ def test_measurement_probes(rb):
         rb.streaming_probe(5, 48, 48, 40, 2)  # rows of whole waves only
     with pytest.raises(rb.RbtpError):
         rb.streaming_probe(6, 63, 48, 40, 2)  # node pairs within rows only
+
+
+def test_kernel_timing_over_the_timed_steps(rb):
+    """rbtp_set_kernel_timing: one pair of events per lattice launch, read at the run's
+    synchronisation.  Over more steps than the library's pool of event pairs (1024) every launch
+    is counted, the average launch lies within the step time, and the timed run ends in the same
+    state, bit for bit, as an untimed one."""
+    kw = single_phase(32, 24, 16, ra=1e5)
+    case = rb.default_case(**kw)
+    T = rb.linear_temperature(case, 11, 1e-2)
+    sims = [rb.Sim(case), rb.Sim(case)]
+    for s in sims:
+        s.init(temperature=T)
+    sims[0].set_kernel_timing(True)
+    for s in sims:
+        s.step(1100, macro_last=True)
+    tm = sims[0].timing()
+    assert tm.launches == 1100
+    assert 0.0 < tm.kernel_ms <= tm.step_ms / 1100 * 1.05
+    for name in ("f", "g"):
+        assert np.array_equal(sims[0].populations(name), sims[1].populations(name))
+    # a second timed call counts its own launches only
+    sims[0].step(7)
+    assert sims[0].timing().launches == 7
+    sims[0].set_kernel_timing(False)
+    sims[0].step(3)
+    for s in sims:
+        s.close()
