@@ -98,8 +98,9 @@ struct PartGrid {
     const PartSlot *tab;
     const int *pstart;  // [nslots + 1] particles
     const int *cstart;  // [nslots + 1] pair-search cells
-    const int *sstart;  // [nslots + 1] workgroups of the special-node kernels (4 nodes each)
-    const int *lstart;  // [nslots + 1] workgroups of the link-node fix-up (128 nodes each)
+    const int *sstart;  // [nslots + 1] workgroups of the refill (4 refilled nodes each)
+    const int *lstart;  // [nslots + 1] workgroups over the special list's head (128 entries
+                        // each): the unmap and the link-node fix-up
     const int *bstart;  // [nslots + 1] link-box node threads (np * lnk_stride per slot)
     int nslots, total_p, total_c, total_s, total_l, total_b;
 };
@@ -114,10 +115,12 @@ struct PartTables {
 
 int part_tables_build(PartTables &t, const std::vector<PartSlot> (&slots)[2], hipStream_t st);
 void part_tables_free(PartTables &t);
-// Workgroups of 128 threads the special-node kernels give to one lattice, at `per_block` nodes
-// per workgroup: from an estimate of its special-node count, which the grid-stride loops only
-// use for parallelism.
-int part_fixup_blocks(const PartDev &pd, int per_block);
+// Workgroups of 128 threads the special-list kernels give to one lattice.  Their loops stride
+// over the grid, so these counts set only the parallelism: the list head (one entry per
+// link-box node) at 128 entries per workgroup, and the refilled nodes of a step at one
+// workgroup of four per sphere (a sphere uncovers a few nodes per step at most).
+int part_head_blocks(const PartDev &pd);
+int part_refill_blocks(const PartDev &pd);
 // The particle phase (main.cu:52-84 minus the lattice passes) of every slot of g, in precision
 // form prec.  The caller then advances each slot's geo_cur and sets special_pending.
 int part_phase(const PartGrid &g, int prec, hipStream_t st);

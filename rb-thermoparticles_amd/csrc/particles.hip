@@ -941,13 +941,13 @@ __global__ __launch_bounds__(256) void k_link_reduce(PartGrid g) {
 // ---- motion -------------------------------------------------------------------------------------
 
 // Entries of the previous step's special-node list leave the node map (the lattice kernel that
-// read them ran before this kernel); slot s uses workgroups sstart[s] .. sstart[s+1]-1.  Empty
+// read them ran before this kernel); slot s uses workgroups lstart[s] .. lstart[s+1]-1.  Empty
 // entries of the list head have a zero mask.  The list itself is rebuilt from k_contacts on.
 __global__ __launch_bounds__(128) void k_special_unmap(PartGrid g) {
-    const int slot = slot_of(g.sstart, g.nslots, blockIdx.x);
+    const int slot = slot_of(g.lstart, g.nslots, blockIdx.x);
     const PartDev &pd = g.tab[slot].pd;
-    const int blk = blockIdx.x - g.sstart[slot];
-    const int nblk = g.sstart[slot + 1] - g.sstart[slot];
+    const int blk = blockIdx.x - g.lstart[slot];
+    const int nblk = g.lstart[slot + 1] - g.lstart[slot];
     const int count = min(pd.sp.count[0], pd.sp.cap);
     for (int j = blk * 128 + threadIdx.x; j < count; j += nblk * 128)
         if (pd.sp.mask[j] != 0u) pd.sp.map[pd.sp.node[j]] = -1;
@@ -1155,29 +1155,63 @@ __global__ __launch_bounds__(128) void k_fixup_refill(PartGrid g, double *fo = n
         const int rem = idx - iz * d.lxy;
         const int iy = rem / d.lx;
         const int ix = rem - iy * d.lx;
+        // Which directions d give a sample (refill.cu:121-153): x + c_d inside and fluid in the
+        // previous geometry, and whether x + 2c_d is too (the second-order sample).  This depends
+        // only on the geometry, so lane d decides direction d for the whole group, with its two
+        // owner loads in flight together, and the group shares the answers as bit masks.
+        bool s1 = false, s2 = false;
+        if (lane >= 1 && lane < NPOP) {
+            const int cx = dir_cx(lane), cy = dir_cy(lane), cz = dir_cz(lane);
+            const int off = cx + cy * d.lx + cz * d.lxy;
+            const bool in1 = in_domain(d, ix + cx, iy + cy, iz + cz);
+            const bool in2 = in_domain(d, ix + 2 * cx, iy + 2 * cy, iz + 2 * cz);
+            const int o1 = in1 ? owner_prev[idx + off] : 0;
+            const int o2 = in2 ? owner_prev[idx + 2 * off] : 0;
+            s1 = in1 && o1 == -1;
+            s2 = s1 && in2 && o2 == -1;
+        }
+        const int half = threadIdx.x & 32;  // the group's half of its wave
+        const uint32_t samp = (uint32_t)(__ballot(s1) >> half);  // bit d: x + c_d is sampled
+        const uint32_t ext = (uint32_t)(__ballot(s2) >> half);   // bit d: and x + 2c_d
         double fv = 0.0, gv = 0.0;
         if (lane < NPOP) {
             const int ipop = lane;
             double temp_f = 0.0, temp_g = 0.0;
-            int sample_count = 0;
-            for (int ip2 = 1; ip2 < NPOP; ++ip2) {
-                const int ix1 = ix + dir_cx(ip2), iy1 = iy + dir_cy(ip2), iz1 = iz + dir_cz(ip2);
-                if (!in_domain(d, ix1, iy1, iz1)) continue;
-                const int idx1 = iz1 * d.lxy + iy1 * d.lx + ix1;
-                if (owner_prev[idx1] != -1) continue;
-                const int ix2 = ix + 2 * dir_cx(ip2), iy2 = iy + 2 * dir_cy(ip2), iz2 = iz + 2 * dir_cz(ip2);
-                if (in_domain(d, ix2, iy2, iz2)) {
-                    const int idx2 = iz2 * d.lxy + iy2 * d.lx + ix2;
-                    if (owner_prev[idx2] == -1) {
-                        temp_f += 2.0 * pop.F(ipop, idx1) - pop.F(ipop, idx2);
-                        temp_g += 2.0 * pop.G(ipop, idx1) - pop.G(ipop, idx2);
-                        ++sample_count;
-                        continue;
+            int sample_count = __popc(samp);
+            // The samples in increasing d, as the reference adds them.  The loads of a chunk of
+            // directions go out before its sums, so a node waits for memory a few times rather
+            // than once or twice per direction.
+            constexpr int CHUNK = 9;
+#pragma unroll
+            for (int d0 = 1; d0 < NPOP; d0 += CHUNK) {
+                double f1[CHUNK], f2[CHUNK], g1[CHUNK], g2[CHUNK];
+#pragma unroll
+                for (int k = 0; k < CHUNK; k++) {
+                    const int dd = d0 + k;
+                    f1[k] = f2[k] = g1[k] = g2[k] = 0.0;
+                    if (dd < NPOP && ((samp >> dd) & 1u)) {
+                        const int off = dir_cx(dd) + dir_cy(dd) * d.lx + dir_cz(dd) * d.lxy;
+                        f1[k] = pop.F(ipop, idx + off);
+                        g1[k] = pop.G(ipop, idx + off);
+                        if ((ext >> dd) & 1u) {
+                            f2[k] = pop.F(ipop, idx + 2 * off);
+                            g2[k] = pop.G(ipop, idx + 2 * off);
+                        }
                     }
                 }
-                temp_f += pop.F(ipop, idx1);
-                temp_g += pop.G(ipop, idx1);
-                ++sample_count;
+#pragma unroll
+                for (int k = 0; k < CHUNK; k++) {
+                    const int dd = d0 + k;
+                    if (dd < NPOP && ((samp >> dd) & 1u)) {
+                        if ((ext >> dd) & 1u) {
+                            temp_f += 2.0 * f1[k] - f2[k];
+                            temp_g += 2.0 * g1[k] - g2[k];
+                        } else {
+                            temp_f += f1[k];
+                            temp_g += g1[k];
+                        }
+                    }
+                }
             }
             if (sample_count == 0) {
                 const double px = pd.p[RBTP_P_POS_X][pid], py = pd.p[RBTP_P_POS_Y][pid], pz = pd.p[RBTP_P_POS_Z][pid];
@@ -1714,13 +1748,11 @@ int part_init(PartDev &pd, const rbtp_case &c, const rbtp_derived &d, const Dims
     return part_load_state(pd, host.data(), dims, st);
 }
 
-int part_fixup_blocks(const PartDev &pd, int per_block) {
-    // link nodes lie in a shell about one lattice spacing thick around each sphere
-    const double r = pd.radius + 1.0;
-    const double est = (double)pd.np * (4.0 * 3.141592653589793 * r * r * 2.0) + 64.0;
-    const double capped = std::min(est, (double)pd.sp.cap);
-    return (int)std::min(std::max(std::ceil(capped / per_block), 1.0), 16384.0);
+int part_head_blocks(const PartDev &pd) {
+    const long long head = (long long)pd.np * pd.lnk_stride;
+    return (int)std::min(std::max((head + 127) / 128, 1LL), 16384LL);
 }
+int part_refill_blocks(const PartDev &pd) { return std::min(std::max(pd.np, 1), 4096); }
 
 void part_tables_free(PartTables &t) {
     (void)hipFree(t.mem);
@@ -1749,8 +1781,8 @@ int part_tables_build(PartTables &t, const std::vector<PartSlot> (&slots)[2], hi
         const PartDev &pd = slots[0][j].pd;
         pre[0][j + 1] = pre[0][j] + pd.np;
         pre[1][j + 1] = pre[1][j] + (pd.use_cells ? pd.ncell[0] * pd.ncell[1] * pd.ncell[2] : 0);
-        pre[2][j + 1] = pre[2][j] + part_fixup_blocks(pd, 4);
-        pre[3][j + 1] = pre[3][j] + part_fixup_blocks(pd, 128);
+        pre[2][j + 1] = pre[2][j] + part_refill_blocks(pd);
+        pre[3][j + 1] = pre[3][j] + part_head_blocks(pd);
         pre[4][j + 1] = pre[4][j] + pd.np * pd.lnk_stride;
     }
     std::vector<char> host(need, 0);
@@ -1790,7 +1822,7 @@ int part_phase(const PartGrid &g, int prec, hipStream_t st) {
     }
     hipLaunchKernelGGL(k_cell_bin, (tp + 255) / 256, 256, 0, st, g);
     PHIP(hipGetLastError());
-    hipLaunchKernelGGL(k_special_unmap, g.total_s, 128, 0, st, g);
+    hipLaunchKernelGGL(k_special_unmap, g.total_l, 128, 0, st, g);
     PHIP(hipGetLastError());
     hipLaunchKernelGGL(k_contacts, tp, CW, 0, st, g);
     PHIP(hipGetLastError());
