@@ -203,11 +203,14 @@ __device__ inline int slot_of(const int *start, int n, int u) {
 
 // ---- geometry --------------------------------------------------------------------------------
 
-__device__ inline void geo_clear_body(int32_t *owner, const int32_t *box, const Dims &d, int p) {
+// The owner map generation that is about to be rebuilt goes back to -1 inside particle p's box
+// (the threads of one workgroup of nthreads share the box).
+__device__ inline void geo_clear_body(int32_t *owner, const int32_t *box, const Dims &d, int p,
+                                      int nthreads) {
     int lo[3], hi[3];
     if (!load_box(box, p, lo, hi)) return;
     const int cnt = box_count(lo, hi);
-    for (int k = threadIdx.x; k < cnt; k += PB) {
+    for (int k = threadIdx.x; k < cnt; k += nthreads) {
         int ix, iy, iz;
         owner[box_node(d, lo, hi, k, ix, iy, iz)] = -1;
     }
@@ -216,7 +219,7 @@ __device__ inline void geo_clear_body(int32_t *owner, const int32_t *box, const 
 __global__ __launch_bounds__(PB) void k_geo_clear(PartGrid g) {
     const int s = slot_of(g.pstart, g.nslots, blockIdx.x);
     const PartSlot &sl = g.tab[s];
-    geo_clear_body(sl.owner_curr, sl.box_curr, sl.d, blockIdx.x - g.pstart[s]);
+    geo_clear_body(sl.owner_curr, sl.box_curr, sl.d, blockIdx.x - g.pstart[s], PB);
 }
 
 // mark_ibnode_kernel (links.cu:25-48) for every particle at once.  The reference launches the
@@ -340,6 +343,10 @@ __global__ __launch_bounds__(CW) void k_contacts(PartGrid g) {
     const Dims d = g.tab[s].d;
     const int i = u - g.pstart[s];
     double *const *P = pd.p;
+    // The owner map generation rebuilt at the end of this step (k_geo_mark) is cleared here in
+    // particle i's old box: nothing reads that generation before k_geo_mark, and the stores need
+    // no launch of their own.
+    geo_clear_body(g.tab[s].owner_curr, g.tab[s].box_curr, d, i, CW);
     if (lane == 0) {
         // the special-node list of this step is rebuilt from here on (k_special_unmap, the last
         // reader of the previous list, ran before this kernel): its head is one entry per
@@ -890,9 +897,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     for (int cc = 0; cc < 7; cc++) part[cc * plane + e] = v[cc];
 }
 
+__device__ inline void move_particle(const PartDev &pd, int pid);
+
 // Planes C0 .. C0+NC-1 of the node shares (0-2 force, 3-5 torque, 6 heat) onto the particle's
-// sums; the particle phase reduces all seven, the per-stage execution the forces or the heat.
-template <int C0, int NC>
+// sums; the particle phase reduces all seven and, with MOVE, steps the particle right after
+// (update_particles reads only its own sums); the per-stage execution reduces the forces or
+// the heat and moves the particles in a launch of its own.
+template <int C0, int NC, bool MOVE = false>
 __global__ __launch_bounds__(256) void k_link_reduce(PartGrid g) {
     const int w = blockIdx.x * (256 / RL) + (threadIdx.x / RL);
     if (w >= g.total_p) return;  // whole waves
@@ -911,22 +922,28 @@ __global__ __launch_bounds__(256) void k_link_reduce(PartGrid g) {
         const unsigned long long *bits = pd.lnk_bits + (size_t)p * (pd.lnk_stride / 64);
         const size_t base = (size_t)p * pd.lnk_stride + lane;
         const size_t plane = (size_t)pd.np * pd.lnk_stride;
-        constexpr int U = 4;  // shares of up to U nodes loaded together, then added in order
-        for (int w1 = 0; w1 < nw; w1 += U) {
-            bool has[U];
-            double sh[U][NC];
+        constexpr int U = 7;  // shares of up to U nodes loaded together, then added in order
+        // up to 64 bit words at once, one per lane, handed to every lane by shuffles
+        for (int wb = 0; wb < nw; wb += RL) {
+            const unsigned long long mine = (wb + lane < nw) ? bits[wb + lane] : 0ull;
+            const int nwb = min(RL, nw - wb);
+            for (int w1 = 0; w1 < nwb; w1 += U) {
+                bool has[U];
+                double sh[U][NC];
 #pragma unroll
-            for (int i = 0; i < U; i++) {
-                const unsigned long long b = (w1 + i < nw) ? bits[w1 + i] : 0ull;
-                has[i] = (b >> lane) & 1ull;
-                for (int c = 0; c < NC; c++)
-                    sh[i][c] = has[i] ? pd.lnk_part[(C0 + c) * plane + base + (size_t)(w1 + i) * RL]
-                                      : 0.0;
+                for (int i = 0; i < U; i++) {
+                    const unsigned long long b = __shfl(mine, (w1 + i) & (RL - 1));
+                    has[i] = w1 + i < nwb && ((b >> lane) & 1ull);
+                    for (int c = 0; c < NC; c++)
+                        sh[i][c] = has[i] ? pd.lnk_part[(C0 + c) * plane + base +
+                                                        (size_t)(wb + w1 + i) * RL]
+                                          : 0.0;
+                }
+#pragma unroll
+                for (int i = 0; i < U; i++)
+                    if (has[i])
+                        for (int c = 0; c < NC; c++) s[c] += sh[i][c];
             }
-#pragma unroll
-            for (int i = 0; i < U; i++)
-                if (has[i])
-                    for (int c = 0; c < NC; c++) s[c] += sh[i][c];
         }
     }
     for (int stride = RL / 2; stride >= 1; stride >>= 1)
@@ -935,6 +952,8 @@ __global__ __launch_bounds__(256) void k_link_reduce(PartGrid g) {
         const int dst[7] = {RBTP_P_FORCE_X, RBTP_P_FORCE_Y, RBTP_P_FORCE_Z, RBTP_P_TORQUE_X,
                             RBTP_P_TORQUE_Y, RBTP_P_TORQUE_Z, RBTP_P_HEAT};
         for (int c = 0; c < NC; c++) pd.p[dst[C0 + c]][p] += s[c];
+        // the particle step needs only this particle's sums: the same lane takes it
+        if (MOVE) move_particle(pd, p);
     }
 }
 
@@ -953,14 +972,9 @@ __global__ __launch_bounds__(128) void k_special_unmap(PartGrid g) {
         if (pd.sp.mask[j] != 0u) pd.sp.map[pd.sp.node[j]] = -1;
 }
 
-// update_particles_kernel (move.cu:5-103), after saving the state the IBB of this step used.
-
-__global__ __launch_bounds__(128) void k_move(PartGrid g) {
-    const int u = blockIdx.x * 128 + threadIdx.x;
-    if (u >= g.total_p) return;
-    const int s = slot_of(g.pstart, g.nslots, u);
-    const PartDev &pd = g.tab[s].pd;
-    const int pid = u - g.pstart[s];
+// update_particles_kernel (move.cu:5-103) for particle pid of lattice pd, after saving the state
+// the IBB of this step used.
+__device__ inline void move_particle(const PartDev &pd, int pid) {
     double *const *P = pd.p;
     const int src[OLD_COUNT] = {RBTP_P_POS_X, RBTP_P_POS_Y, RBTP_P_POS_Z, RBTP_P_VEL_X,
                                 RBTP_P_VEL_Y, RBTP_P_VEL_Z, RBTP_P_OMEGA_X, RBTP_P_OMEGA_Y,
@@ -1009,6 +1023,13 @@ __global__ __launch_bounds__(128) void k_move(PartGrid g) {
     double T = P[RBTP_P_TEMP][pid];
     T += 0.5 * dt * (heat + heat_prev) / heat_capacity;
     P[RBTP_P_TEMP][pid] = T;
+}
+
+__global__ __launch_bounds__(128) void k_move(PartGrid g) {
+    const int u = blockIdx.x * 128 + threadIdx.x;
+    if (u >= g.total_p) return;
+    const int s = slot_of(g.pstart, g.nslots, u);
+    move_particle(g.tab[s].pd, u - g.pstart[s]);
 }
 
 // ---- special nodes ------------------------------------------------------------------------------
@@ -1831,13 +1852,11 @@ int part_phase(const PartGrid &g, int prec, hipStream_t st) {
         hipLaunchKernelGGL((k_link_nodes<S>), (g.total_b + 255) / 256, 256, 0, st, g);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
-        hipLaunchKernelGGL((k_link_reduce<0, 7>), (tp + 256 / RL - 1) / (256 / RL), 256, 0, st, g);
+        // the hydrodynamic sums and the particle step (update_particles) in one launch
+        hipLaunchKernelGGL((k_link_reduce<0, 7, true>), (tp + 256 / RL - 1) / (256 / RL), 256, 0,
+                           st, g);
         return hipGetLastError();
     }));
-    hipLaunchKernelGGL(k_move, (tp + 127) / 128, 128, 0, st, g);
-    PHIP(hipGetLastError());
-    hipLaunchKernelGGL(k_geo_clear, tp, PB, 0, st, g);
-    PHIP(hipGetLastError());
     hipLaunchKernelGGL(k_geo_mark, tp, PB, 0, st, g);
     PHIP(hipGetLastError());
     hipLaunchKernelGGL(k_refill_find, tp, LB, 0, st, g);
