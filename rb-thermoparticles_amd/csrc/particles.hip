@@ -922,7 +922,7 @@ __global__ __launch_bounds__(256) void k_link_reduce(PartGrid g) {
         const unsigned long long *bits = pd.lnk_bits + (size_t)p * (pd.lnk_stride / 64);
         const size_t base = (size_t)p * pd.lnk_stride + lane;
         const size_t plane = (size_t)pd.np * pd.lnk_stride;
-        constexpr int U = 7;  // shares of up to U nodes loaded together, then added in order
+        constexpr int U = 4;  // shares of up to U nodes loaded together, then added in order
         // up to 64 bit words at once, one per lane, handed to every lane by shuffles
         for (int wb = 0; wb < nw; wb += RL) {
             const unsigned long long mine = (wb + lane < nw) ? bits[wb + lane] : 0ull;
