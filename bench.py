@@ -6,7 +6,10 @@ Ra=1e8, Pr=0.71, FP64 (the plain drop-in form), synthetic initial state (linear 
 seeded noise).  A "step" is one full time step of main.cu's sequence over all nodes.
 
 Prints one JSON line.  value = lattice updates per second of the whole job (MLUPS), with the
-state resident in HBM before the timed region starts.  The line also carries:
+state resident in HBM before the timed region starts.  Every form's lattice is created and
+initialised before the probes run, so that each form's warm-up and timed steps follow device
+work rather than the host's preparation (the device lowers its clocks when idle).  The line also
+carries:
   roofline      the fused lattice kernel's algorithmic HBM bytes per launch (880 B/node: 54 FP64
                 populations read and written plus F_x read and written) over its average launch
                 time, measured with HIP events around every launch of the timed steps on the
@@ -63,11 +66,15 @@ def load_rbtp():
     return mod
 
 
-def run_form(rb, kw, steps, warmup, log, nu_window=0):
+def prepare_form(rb, kw):
+    """A lattice of one precision form, created and initialised (host work: the device idles)."""
     case = rb.default_case(**kw)
     sim = rb.Sim(case)
-    T = rb.linear_temperature(case, 2024, 1e-3)
-    sim.init(temperature=T)
+    sim.init(temperature=rb.linear_temperature(case, 2024, 1e-3))
+    return sim
+
+
+def run_form(sim, steps, warmup, log, nu_window=0):
     n = sim.n
     sim.step(warmup, macro_last=False)
     sim.synchronize()
@@ -96,8 +103,8 @@ def run_form(rb, kw, steps, warmup, log, nu_window=0):
         sim.step(100, macro_last=False)
         steady = sim.timing().kernel_ms
         sim.set_kernel_timing(False)
-    sim.close()
-    log(f"form precision={kw.get('precision', 0)}: {steps} steps in {wall_ms:.1f} ms wall "
+    prec = sim.case.precision
+    log(f"form precision={prec}: {steps} steps in {wall_ms:.1f} ms wall "
         f"({dev_ms:.1f} ms device), kernel avg {tm.kernel_ms:.4f} ms over {tm.launches}")
     return dict(n=n, wall_ms=wall_ms, dev_ms=dev_ms, kernel_ms=tm.kernel_ms, kernel_ms_steady=steady,
                 kernel_launches=tm.launches, nu=nu.tolist(), ke=ke, nu_avg=np.asarray(nu_avg).tolist(),
@@ -242,6 +249,15 @@ def main():
         rb.bandwidth_probe(1 << 30, 1)
         log("counter run done")
         return
+    # Every lattice is set up first, and freed only after the last form has run: the host-side
+    # preparation and the freeing of a lattice, during which the device idles and lowers its
+    # clocks, then lie outside the forms' runs.  The forms run back to back, FP64 last, so that
+    # each form's warm-up and timed steps follow the previous form's steps (the first, the
+    # probes).
+    form_precs = () if args.no_forms else (("fp16s", 2), ("fp32", 1), ("bf16s", 3))
+    sims = {0: prepare_form(rb, kw64)}
+    for _, prec in form_precs:
+        sims[prec] = prepare_form(rb, config2(precision=prec))
     copy_gbs = rb.bandwidth_probe(2 << 30, 20)
     log(f"copy probe: {copy_gbs:.0f} GB/s")
     stencil_gbs = rb.stencil_probe(kw64["lx"], kw64["ly"], kw64["lz"], 20)
@@ -257,15 +273,21 @@ def main():
               for name, scheme in (("fp64_plain_pull", 3), ("p16_plain_pull", 4),
                                    ("p16_row_pull", 5), ("p16_pair_pull", 6))}
     log("pattern probes: " + ", ".join(f"{k} {v:.0f}" for k, v in probes.items()) + " GB/s")
-    r64 = run_form(rb, kw64, args.steps, args.warmup, log, args.nu_window)
+    runs = {}
+    for prec in (1, 2, 3, 0):
+        if prec in sims:
+            runs[prec] = run_form(sims[prec], args.steps, args.warmup, log, args.nu_window)
+    for sim in sims.values():
+        sim.close()
+    r64 = runs[0]
     n = r64["n"]
     mlups = n * args.steps / (r64["wall_ms"] * 1e-3) / 1e6
     achieved = BYTES_PER_NODE[0] * n / (r64["kernel_ms"] * 1e-3) / 1e9
     forms = {}
-    if not args.no_forms:
-        for name, prec in (("fp16s", 2), ("fp32", 1), ("bf16s", 3)):
+    if form_precs:
+        for name, prec in form_precs:
             window = args.nu_window
-            r = run_form(rb, config2(precision=prec), args.steps, args.warmup, log, window)
+            r = runs[prec]
             m = n * args.steps / (r["wall_ms"] * 1e-3) / 1e6
             form = dict(
                 mlups=m, ratio_vs_fp64=m / mlups, kernel_ms=r["kernel_ms"],
