@@ -509,18 +509,23 @@ class Batch:
         """Admit a case; returns its slot id.  Raises CapacityError when full."""
         case = Case.from_buffer_copy(case)
         case.precision = self.precision
+        n = case.lx * case.ly * case.lz
+        npart = derive(case).npart if case.activate_particles else 0
         keep = []
 
-        def ptr(a):
+        def ptr(a, size, what):
             if a is None:
                 return None
             a = np.ascontiguousarray(a, dtype=np.float64).reshape(-1)
+            if a.size != size:
+                raise ValueError(f"{what} must hold {size} values (got {a.size})")
             keep.append(a)
             return a.ctypes.data_as(C.c_void_p)
 
         slot = C.c_int(-1)
-        rc = lib.rbtp_batch_admit(self._h, C.byref(case), ptr(temperature), ptr(ppos),
-                                  ptr(ptemp), C.byref(slot))
+        rc = lib.rbtp_batch_admit(self._h, C.byref(case), ptr(temperature, n, "temperature"),
+                                  ptr(ppos, 3 * npart, "ppos (x[], y[], z[])"),
+                                  ptr(ptemp, npart, "ptemp"), C.byref(slot))
         if rc == -6:
             raise CapacityError(rc, lib.rbtp_last_error().decode(errors="replace"))
         _check(rc)

@@ -132,6 +132,32 @@ def test_moving_spheres_many_steps(rb, O):
     assert ora.unhandled_links(False) == 0 and ora.unhandled_links(True) == 0
 
 
+RAGGED_POS = [(6.5, 7.0, 8.0), (30.2, 24.5, 18.6), (18.0, 14.5, 3.6), (10.0, 22.0, 12.5)]
+
+
+def test_ragged_lattice_moving_spheres(rb, O):
+    """Four fast, heated spheres on a 37 x 29 x 23 lattice: rows that are not whole waves, a last
+    workgroup of 103 nodes whose last wave holds 39, and a sphere whose link box reaches the last
+    plane and row, so that link, refill and special nodes sit in that partial wave."""
+    kw = particle_case(37, 29, 23, npart=4, diam=6.0, ra=1e5, beta=0.5)
+    sim, ora = make_pair(rb, O, kw, RAGGED_POS, ptemp=np.array([0.3, -0.2, 0.0, 0.1]))
+    total_nf = 0
+    for k in range(3):
+        sim.step(15)
+        ora.step(15)
+        nf_g, _ = sim.new_fluid()
+        nf_c, _ = ora.new_fluid()
+        assert np.array_equal(nf_g, nf_c), f"block {k}: new fluid"
+        total_nf += len(nf_c)
+        check_particles(sim, ora, f"block {k}")
+        check_geometry(sim, ora, f"block {k}")
+    check_fields(sim, ora, "end")
+    assert total_nf > 0
+    # the link boxes reach the lattice's last node (idx n - 1 lies in the top-back row)
+    ib, ow = sim.flags()
+    assert int(np.max(np.nonzero(ow.reshape(23, 29, 37) >= 0)[0])) >= 20
+
+
 def test_moving_spheres_row_pull(rb, O):
     """The moving-sphere case on a lattice 64 nodes wide (the FP64 step's row pull), spheres near
     the x walls so that special nodes sit at the rows' first and last nodes."""

@@ -98,6 +98,49 @@ def test_batch_matches_single_sim(rb):
         assert np.array_equal(b.particles(k), sim.particles(k)), k
 
 
+@pytest.mark.parametrize("prec", [2, 0])
+def test_ragged_slots_match_single_sims(rb, prec):
+    """Slots whose node counts are not whole tiles (37 x 29 x 23 with four moving spheres,
+    25 x 23 x 21 without), so that tiles of the batched launch end inside a slot: each slot
+    equals a standalone rbtp_sim of its case bit for bit."""
+    from cases import particle_case, single_phase
+    cases = [
+        (particle_case(37, 29, 23, npart=4, diam=6.0, ra=1e5, beta=0.5, precision=prec),
+         np.asarray([(6.5, 7.0, 8.0), (30.2, 24.5, 18.6), (18.0, 14.5, 3.6),
+                     (10.0, 22.0, 12.5)], dtype=np.float64).T.reshape(-1),
+         np.array([0.3, -0.2, 0.0, 0.1])),
+        (single_phase(25, 23, 21, ra=1e6, precision=prec), None, None),
+    ]
+    batch = rb.Batch(precision=prec, max_slots=4)
+    sims, slots = [], []
+    for i, (kw, ppos, ptemp) in enumerate(cases):
+        case = rb.default_case(**kw)
+        T = rb.linear_temperature(case, 30 + i, 1e-2)
+        # arrays of the wrong length are refused before the library reads them
+        with pytest.raises(ValueError):
+            batch.admit(case, temperature=T[:-1], ppos=ppos, ptemp=ptemp)
+        if ppos is not None:
+            with pytest.raises(ValueError):
+                batch.admit(case, temperature=T, ppos=ppos[:-1], ptemp=ptemp)
+        slots.append(batch.admit(case, temperature=T, ppos=ppos, ptemp=ptemp))
+        sim = rb.Sim(case)
+        sim.init(temperature=T, ppos=ppos, ptemp=ptemp)
+        sims.append(sim)
+    batch.step(40)
+    batch.step(9)
+    for sim in sims:
+        sim.step(50)  # the batch's admission ran each slot's first step
+    for s, sim in zip(slots, sims):
+        b = batch.sim(s)
+        for k in ("f", "g"):
+            assert np.array_equal(b.populations(k), sim.populations(k)), k
+        assert np.array_equal(b.field("phi"), sim.field("phi"))
+        if sim.np:
+            for k in ("pos_x", "vel_z", "temp", "force_x", "heat"):
+                assert np.array_equal(b.particles(k), sim.particles(k)), k
+    batch.close()
+
+
 def test_graph_replay_equals_eager(rb):
     res = []
     for graph_steps in (0, 10):
