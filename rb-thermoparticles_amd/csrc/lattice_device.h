@@ -22,14 +22,16 @@ __device__ inline double div_rn(double x, double d, double yh, double yl) {
 
 // Workgroups are dispatched round-robin over the 8 XCDs, each with its own L2.  The remapped id
 // gives each XCD one contiguous range of logical workgroups (a bijection of [0, nblocks)), so
-// that neighbouring workgroups, which share cache lines, share an L2.
+// that neighbouring workgroups, which share cache lines, share an L2.  xcd_remap takes the
+// workgroup's position bid in a range of nblocks workgroups that starts at a multiple of 8 of
+// the grid (its XCD is then bid % 8).
 constexpr int NUM_XCD = 8;
-__device__ inline int xcd_block(int nblocks) {
-    const int bid = blockIdx.x;
+__device__ inline int xcd_remap(int bid, int nblocks) {
     const int q = nblocks / NUM_XCD, r = nblocks % NUM_XCD;
     const int xcd = bid % NUM_XCD, local = bid / NUM_XCD;
     return xcd * q + (xcd < r ? xcd : r) + local;
 }
+__device__ inline int xcd_block(int nblocks) { return xcd_remap(blockIdx.x, nblocks); }
 
 // ---- populations in and out ------------------------------------------------------------------
 

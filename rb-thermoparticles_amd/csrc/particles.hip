@@ -204,13 +204,13 @@ __device__ inline int slot_of(const int *start, int n, int u) {
 // ---- geometry --------------------------------------------------------------------------------
 
 // The owner map generation that is about to be rebuilt goes back to -1 inside particle p's box
-// (the threads of one workgroup of nthreads share the box).
+// (nthreads threads share the box; this is thread tid).
 __device__ inline void geo_clear_body(int32_t *owner, const int32_t *box, const Dims &d, int p,
-                                      int nthreads) {
+                                      int tid, int nthreads) {
     int lo[3], hi[3];
     if (!load_box(box, p, lo, hi)) return;
     const int cnt = box_count(lo, hi);
-    for (int k = threadIdx.x; k < cnt; k += nthreads) {
+    for (int k = tid; k < cnt; k += nthreads) {
         int ix, iy, iz;
         owner[box_node(d, lo, hi, k, ix, iy, iz)] = -1;
     }
@@ -219,7 +219,7 @@ __device__ inline void geo_clear_body(int32_t *owner, const int32_t *box, const 
 __global__ __launch_bounds__(PB) void k_geo_clear(PartGrid g) {
     const int s = slot_of(g.pstart, g.nslots, blockIdx.x);
     const PartSlot &sl = g.tab[s];
-    geo_clear_body(sl.owner_curr, sl.box_curr, sl.d, blockIdx.x - g.pstart[s], PB);
+    geo_clear_body(sl.owner_curr, sl.box_curr, sl.d, blockIdx.x - g.pstart[s], threadIdx.x, PB);
 }
 
 // mark_ibnode_kernel (links.cu:25-48) for every particle at once.  The reference launches the
@@ -283,10 +283,9 @@ __global__ __launch_bounds__(256) void k_cell_clear(PartGrid g) {
     }
 }
 
-// Bins every particle by its centre.  The order inside a cell is arbitrary: the pair sum sorts
-// its candidates by pid.
-__global__ __launch_bounds__(256) void k_cell_bin(PartGrid g) {
-    const int u = blockIdx.x * 256 + threadIdx.x;
+// Bins particle u (flat index) by its centre.  The order inside a cell is arbitrary: the pair sum
+// sorts its candidates by pid.
+__device__ inline void cell_bin_body(const PartGrid &g, int u) {
     if (u >= g.total_p) return;
     const int s = slot_of(g.pstart, g.nslots, u);
     const PartDev &pd = g.tab[s].pd;
@@ -331,13 +330,20 @@ __device__ inline bool pair_term(const PartDev &pd, const double *X, const doubl
 // than CAND_MAX candidates, i falls back to every partner.  The lanes evaluate the terms of 64
 // consecutive partners at a time; every lane then adds them in partner order (shuffles), so the
 // sum is the sequential one.
-constexpr int CW = 64;  // lanes per particle in k_contacts (one wave)
-__global__ __launch_bounds__(CW) void k_contacts(PartGrid g) {
-    __shared__ int32_t cand[CAND_MAX];    // candidates in cell order
-    __shared__ int32_t sorted[CAND_MAX];  // the same, ranked by pid
-    const int u = blockIdx.x;
-    if (u >= g.total_p) return;  // whole workgroup (one wave)
-    const int lane = threadIdx.x;
+constexpr int CW = 64;  // lanes per particle in the contact sums (one wave)
+
+// Orders a wave's LDS stores before its lanes' later LDS loads: the wave's rows of the candidate
+// lists are its own, so no workgroup barrier is needed.
+__device__ inline void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Particle u (flat index) on one wave (lane): cand and sorted are the wave's own LDS rows of
+// CAND_MAX ids (candidates in cell order, then ranked by pid).
+__device__ inline void contacts_body(const PartGrid &g, int u, int lane, int32_t *cand,
+                                     int32_t *sorted) {
     const int s = slot_of(g.pstart, g.nslots, u);
     const PartDev &pd = g.tab[s].pd;
     const Dims d = g.tab[s].d;
@@ -346,7 +352,7 @@ __global__ __launch_bounds__(CW) void k_contacts(PartGrid g) {
     // The owner map generation rebuilt at the end of this step (k_geo_mark) is cleared here in
     // particle i's old box: nothing reads that generation before k_geo_mark, and the stores need
     // no launch of their own.
-    geo_clear_body(g.tab[s].owner_curr, g.tab[s].box_curr, d, i, CW);
+    geo_clear_body(g.tab[s].owner_curr, g.tab[s].box_curr, d, i, lane, CW);
     if (lane == 0) {
         // the special-node list of this step is rebuilt from here on (k_special_unmap, the last
         // reader of the previous list, ran before this kernel): its head is one entry per
@@ -376,30 +382,33 @@ __global__ __launch_bounds__(CW) void k_contacts(PartGrid g) {
             const int cx = cell_coord(xi, pd.cell_scale[0], pd.ncell[0]);
             const int cy = cell_coord(yi, pd.cell_scale[1], pd.ncell[1]);
             const int cz = cell_coord(zi, pd.cell_scale[2], pd.ncell[2]);
-            // the (up to) 27 neighbour cells, their counts and where their ids start in `cand`
-            int cell[27], start[28];
-            int nc = 0;
-            for (int z = max(cz - 1, 0); z <= min(cz + 1, pd.ncell[2] - 1); z++)
-                for (int y = max(cy - 1, 0); y <= min(cy + 1, pd.ncell[1] - 1); y++)
-                    for (int x = max(cx - 1, 0); x <= min(cx + 1, pd.ncell[0] - 1); x++)
-                        cell[nc++] = (z * pd.ncell[1] + y) * pd.ncell[0] + x;
+            // the (up to) 27 neighbour cells, in z, y, x order: their counts, then their ids
+            // into `cand` one cell after another
+            const int z0 = max(cz - 1, 0), z1 = min(cz + 1, pd.ncell[2] - 1);
+            const int y0 = max(cy - 1, 0), y1 = min(cy + 1, pd.ncell[1] - 1);
+            const int x0 = max(cx - 1, 0), x1 = min(cx + 1, pd.ncell[0] - 1);
             bool full = false;
-            start[0] = 0;
-            for (int q = 0; q < nc; q++) {
-                const int c = pd.cell_cnt[cell[q]];
-                full = full || c > CELL_CAP;
-                start[q + 1] = start[q] + c;
-            }
-            const int total = start[nc];
-            // every lane read the same counts: this branch, and the barriers in it, are uniform
-            // over the workgroup (one wave)
+            int total = 0;
+            for (int z = z0; z <= z1; z++)
+                for (int y = y0; y <= y1; y++)
+                    for (int x = x0; x <= x1; x++) {
+                        const int c = pd.cell_cnt[(z * pd.ncell[1] + y) * pd.ncell[0] + x];
+                        full = full || c > CELL_CAP;
+                        total += c;
+                    }
+            // every lane read the same counts: this branch is uniform over the wave
             if (!full && total <= CAND_MAX) {
-                for (int t = lane; t < total; t += CW) {
-                    int q = 0;
-                    while (start[q + 1] <= t) q++;
-                    cand[t] = pd.cell_pid[(size_t)cell[q] * CELL_CAP + (t - start[q])];
-                }
-                __syncthreads();
+                int base = 0;
+                for (int z = z0; z <= z1; z++)
+                    for (int y = y0; y <= y1; y++)
+                        for (int x = x0; x <= x1; x++) {
+                            const int cell = (z * pd.ncell[1] + y) * pd.ncell[0] + x;
+                            const int c = pd.cell_cnt[cell];
+                            for (int t = lane; t < c; t += CW)
+                                cand[base + t] = pd.cell_pid[(size_t)cell * CELL_CAP + t];
+                            base += c;
+                        }
+                wave_lds_sync();
                 // rank by pid (every particle is binned once: the ids are distinct)
                 for (int t = lane; t < total; t += CW) {
                     const int32_t v = cand[t];
@@ -407,7 +416,7 @@ __global__ __launch_bounds__(CW) void k_contacts(PartGrid g) {
                     for (int m = 0; m < total; m++) r += cand[m] < v ? 1 : 0;
                     sorted[r] = v;
                 }
-                __syncthreads();
+                wave_lds_sync();
                 count = total;
                 ranked = true;
             }
@@ -696,13 +705,10 @@ __device__ inline LinkOut eval_link_at(const LinkLoads &L, int ip, int ix, int i
 constexpr int RL = 64;  // lanes of the per-particle reduction (one wave)
 constexpr uint32_t SP_MULTI = 1u << 31;  // special-node mask flag: links to several spheres
 
-// Four waves per SIMD (at most 128 VGPRs; the 16-bit forms spill 28 bytes per lane) measured
-// 1.7 % faster on the whole step with 5000 spheres in FP16 storage than the compiler's own
-// choice of three; five waves spill far more and are 20 % slower.
+// Link-box node threads u = blk * 256 + threadIdx.x of one workgroup of k_contacts_links.
 template <class S>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_link_nodes(PartGrid g) {
-    // a particle's waves are consecutive: on one XCD, they share its L2
-    const int u = xcd_block(gridDim.x) * 256 + threadIdx.x;
+__device__ inline void link_nodes_body(const PartGrid &g, int blk) {
+    const int u = blk * 256 + threadIdx.x;
     if (u >= g.total_b) return;  // total_b is a whole number of waves
     // every wave lies in one particle of one slot: slot and particle are wave-uniform
     const int u0 = __builtin_amdgcn_readfirstlane(u);
@@ -897,6 +903,30 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     for (int cc = 0; cc < 7; cc++) part[cc * plane + e] = v[cc];
 }
 
+// Contact sums and link nodes in one launch: neither reads what the other writes (the contacts
+// write the force, torque and heat history, the force of the contacts, the owner generation
+// being rebuilt and the special list's counters; the link nodes read positions, velocities,
+// temperatures, the other owner generation and the populations, and write the node shares and
+// the list's head), and the wave-per-particle contact sums, latency-bound like the link nodes,
+// then run beside them instead of as a launch of their own.  The first nc workgroups (a multiple
+// of 8) hold four particles' contact sums each, one per wave; the rest are the link nodes',
+// remapped so that a particle's consecutive workgroups share an XCD.  Four waves per SIMD (at
+// most 128 VGPRs) measured 1.7 % faster on the whole step with 5000 spheres in FP16 storage than
+// the compiler's own choice of three for the link nodes; five waves spill far more.
+template <class S>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_contacts_links(
+    PartGrid g, int nc) {
+    __shared__ int32_t cand[4][CAND_MAX];
+    __shared__ int32_t sorted[4][CAND_MAX];
+    if ((int)blockIdx.x < nc) {
+        const int w = threadIdx.x >> 6;
+        const int u = (int)blockIdx.x * 4 + w;
+        if (u < g.total_p) contacts_body(g, u, threadIdx.x & 63, cand[w], sorted[w]);
+        return;
+    }
+    link_nodes_body<S>(g, xcd_remap((int)blockIdx.x - nc, (int)gridDim.x - nc));
+}
+
 __device__ inline void move_particle(const PartDev &pd, int pid);
 
 // Planes C0 .. C0+NC-1 of the node shares (0-2 force, 3-5 torque, 6 heat) onto the particle's
@@ -962,14 +992,26 @@ __global__ __launch_bounds__(256) void k_link_reduce(PartGrid g) {
 // Entries of the previous step's special-node list leave the node map (the lattice kernel that
 // read them ran before this kernel); slot s uses workgroups lstart[s] .. lstart[s+1]-1.  Empty
 // entries of the list head have a zero mask.  The list itself is rebuilt from k_contacts on.
-__global__ __launch_bounds__(128) void k_special_unmap(PartGrid g) {
-    const int slot = slot_of(g.lstart, g.nslots, blockIdx.x);
+// Workgroup b of 128 threads (thread tid) of that walk.
+__device__ inline void special_unmap_body(const PartGrid &g, int b, int tid) {
+    const int slot = slot_of(g.lstart, g.nslots, b);
     const PartDev &pd = g.tab[slot].pd;
-    const int blk = blockIdx.x - g.lstart[slot];
+    const int blk = b - g.lstart[slot];
     const int nblk = g.lstart[slot + 1] - g.lstart[slot];
     const int count = min(pd.sp.count[0], pd.sp.cap);
-    for (int j = blk * 128 + threadIdx.x; j < count; j += nblk * 128)
+    for (int j = blk * 128 + tid; j < count; j += nblk * 128)
         if (pd.sp.mask[j] != 0u) pd.sp.map[pd.sp.node[j]] = -1;
+}
+
+// The cell binning (first nbin workgroups) and the unmap (two of its 128-thread workgroups per
+// workgroup of 256) in one launch: they touch different arrays.
+__global__ __launch_bounds__(256) void k_bin_unmap(PartGrid g, int nbin) {
+    if ((int)blockIdx.x < nbin) {
+        cell_bin_body(g, (int)blockIdx.x * 256 + threadIdx.x);
+        return;
+    }
+    const int b = ((int)blockIdx.x - nbin) * 2 + (threadIdx.x >> 7);
+    if (b < g.total_l) special_unmap_body(g, b, threadIdx.x & 127);
 }
 
 // update_particles_kernel (move.cu:5-103) for particle pid of lattice pd, after saving the state
@@ -1075,8 +1117,8 @@ __global__ __launch_bounds__(LB) void k_refill_find(PartGrid g) {
 // consecutive entries at once and completes their SP_MULTI entries one after another, one lane
 // per direction; the links of a node are independent (no sum), so their order does not matter.
 template <class S>
-__global__ __launch_bounds__(128) void k_fixup_links(PartGrid g) {
-    const int slot = slot_of(g.lstart, g.nslots, blockIdx.x);
+__device__ inline void fixup_links_body(const PartGrid &g, int b) {
+    const int slot = slot_of(g.lstart, g.nslots, b);
     const PartSlot &sl = g.tab[slot];
     const PartDev &pd = sl.pd;
     const Dims d = sl.d;
@@ -1086,7 +1128,7 @@ __global__ __launch_bounds__(128) void k_fixup_links(PartGrid g) {
     const gptr<const int32_t> owner_prev = gbl(sl.owner_prev);
     const gptr<const typename Enc<S>::A> rho_full = gbl((const typename Enc<S>::A *)sl.rho_full);
     const int lane = __lane_id();
-    const int wave = (blockIdx.x - g.lstart[slot]) * 2 + (threadIdx.x >> 6);
+    const int wave = (b - g.lstart[slot]) * 2 + (threadIdx.x >> 6);
     const int nwave = (g.lstart[slot + 1] - g.lstart[slot]) * 2;
     const int nlink = min(pd.sp.count[1], pd.sp.cap);
     const gptr<A> out = gbl((A *)pd.sp.pop);
@@ -1146,10 +1188,9 @@ __global__ __launch_bounds__(128) void k_fixup_links(PartGrid g) {
 // state for the fallback equilibrium; lane 0 then forms rho, u, phi (refill.cu:217-235) in
 // population order and sets F = 0.  The particle phase stores the populations in the special
 // list; the per-stage execution (STAGED) writes them into d_f and d_g (fo, go: FP64, blocked).
-template <class S, bool STAGED = false>
-__global__ __launch_bounds__(128) void k_fixup_refill(PartGrid g, double *fo = nullptr,
-                                                      double *go = nullptr) {
-    const int slot = slot_of(g.sstart, g.nslots, blockIdx.x);
+template <class S, bool STAGED>
+__device__ inline void fixup_refill_body(const PartGrid &g, int b, double *fo, double *go) {
+    const int slot = slot_of(g.sstart, g.nslots, b);
     const PartSlot &sl = g.tab[slot];
     const PartDev &pd = sl.pd;
     const Dims d = sl.d;
@@ -1160,7 +1201,7 @@ __global__ __launch_bounds__(128) void k_fixup_refill(PartGrid g, double *fo = n
     A *fx_field = (A *)sl.fx;
     double *macro_rho = sl.macro[0], *macro_ux = sl.macro[1], *macro_uy = sl.macro[2];
     double *macro_uz = sl.macro[3], *macro_phi = sl.macro[4];
-    const int blk = blockIdx.x - g.sstart[slot];
+    const int blk = b - g.sstart[slot];
     const int nblk = g.sstart[slot + 1] - g.sstart[slot];
     const int grp = threadIdx.x >> 5, lane = threadIdx.x & 31;
     const int count = min(pd.sp.count[0], pd.sp.cap);
@@ -1301,6 +1342,24 @@ __global__ __launch_bounds__(128) void k_fixup_refill(PartGrid g, double *fo = n
 // The replaced populations of the special nodes over a plain pull of the whole lattice (the
 // post-stream accessor): f and g are FP64 in the reference's SoA order, or in the blocked order
 // of pop_at (BLOCKED, the start of the per-stage execution).
+template <class S, bool STAGED = false>
+__global__ __launch_bounds__(128) void k_fixup_refill(PartGrid g, double *fo = nullptr,
+                                                      double *go = nullptr) {
+    fixup_refill_body<S, STAGED>(g, blockIdx.x, fo, go);
+}
+
+// Both fix-ups in one launch (the link nodes' in the first total_l workgroups, the refilled
+// nodes' in the next total_s): they complete different entries of the special list (the head's
+// shared link nodes, the tail's refilled nodes, which were solid in the step's geometry and so
+// are no link nodes) and read only what earlier kernels wrote, so their latency-bound work
+// overlaps.  Four waves per SIMD, as the refill alone runs (the two bodies together would take
+// one register more than that allows).
+template <class S>
+__global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(4))) void k_fixups(PartGrid g) {
+    if ((int)blockIdx.x < g.total_l) fixup_links_body<S>(g, blockIdx.x);
+    else fixup_refill_body<S, false>(g, (int)blockIdx.x - g.total_l, nullptr, nullptr);
+}
+
 template <class A, bool BLOCKED = false>
 __global__ __launch_bounds__(256) void k_overwrite_special(PartDev pd, double *f, double *g, Dims d) {
     const A *pop = (const A *)pd.sp.pop;
@@ -1841,15 +1900,16 @@ int part_phase(const PartGrid &g, int prec, hipStream_t st) {
         hipLaunchKernelGGL(k_cell_clear, std::min((g.total_c + 255) / 256, 1024), 256, 0, st, g);
         PHIP(hipGetLastError());
     }
-    hipLaunchKernelGGL(k_cell_bin, (tp + 255) / 256, 256, 0, st, g);
+    // the cell binning, and the previous list leaving the node map (its last reader, the lattice
+    // kernel, ran before this phase)
+    const int nbin = (tp + 255) / 256;
+    hipLaunchKernelGGL(k_bin_unmap, nbin + (g.total_l + 1) / 2, 256, 0, st, g, nbin);
     PHIP(hipGetLastError());
-    hipLaunchKernelGGL(k_special_unmap, g.total_l, 128, 0, st, g);
-    PHIP(hipGetLastError());
-    hipLaunchKernelGGL(k_contacts, tp, CW, 0, st, g);
-    PHIP(hipGetLastError());
+    // contacts four particles per workgroup, in a multiple of 8 workgroups ahead of the link nodes
+    const int nc = ((tp + 3) / 4 + NUM_XCD - 1) / NUM_XCD * NUM_XCD;
     PHIP(dispatch_prec(prec, [&](auto *tag) {
         using S = std::remove_pointer_t<decltype(tag)>;
-        hipLaunchKernelGGL((k_link_nodes<S>), (g.total_b + 255) / 256, 256, 0, st, g);
+        hipLaunchKernelGGL((k_contacts_links<S>), nc + (g.total_b + 255) / 256, 256, 0, st, g, nc);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
         // the hydrodynamic sums and the particle step (update_particles) in one launch
@@ -1863,10 +1923,7 @@ int part_phase(const PartGrid &g, int prec, hipStream_t st) {
     PHIP(hipGetLastError());
     PHIP(dispatch_prec(prec, [&](auto *tag) {
         using S = std::remove_pointer_t<decltype(tag)>;
-        hipLaunchKernelGGL((k_fixup_links<S>), g.total_l, 128, 0, st, g);
-        hipError_t e = hipGetLastError();
-        if (e != hipSuccess) return e;
-        hipLaunchKernelGGL((k_fixup_refill<S>), g.total_s, 128, 0, st, g);
+        hipLaunchKernelGGL((k_fixups<S>), g.total_l + g.total_s, 128, 0, st, g);
         return hipGetLastError();
     }));
     return RBTP_OK;
