@@ -5,15 +5,16 @@
 // node owned by p, and the ray from x's centre along c_i meets sphere p at q in [0, 1]
 // (link_count_kernel, links.cu:50-87).  Each consumer re-derives the links it needs from the two
 // owner maps with the same bit-exact test:
-//   * k_link_nodes: one thread per node of each particle's link box finds the node's links to
-//     the particle and stores the node's share of force, torque and heat; k_link_reduce sums
-//     the shares of each particle in a fixed order (64 lane partials + pairwise tree),
+//   * the link-node pass (in k_contacts_links): one thread per node of each particle's link box
+//     finds the node's links to the particle and stores the node's share of force, torque and
+//     heat; k_link_reduce sums the shares of each particle in a fixed order (64 lane partials +
+//     pairwise tree),
 //     replacing the reference's FP64 atomics (ibb.cu:182-187,204) with a reproducible reduction;
-//   * k_link_nodes also lists the IBB link nodes (one fixed entry per link-box node), and
+//   * the link-node pass also lists the IBB link nodes (one fixed entry per link-box node), and
 //     k_refill_find appends the refilled nodes: the list of the nodes whose post-stream
 //     populations the coupling replaces;
-//   * k_fixup_links / k_fixup_refill: those nodes' corrected post-stream populations (pull +
-//     IBB, or refill), buffered for the next lattice step.
+//   * k_fixups (the shared link nodes, then the refilled nodes): those nodes' corrected
+//     post-stream populations (pull + IBB, or refill), buffered for the next lattice step.
 // Geometry is two owner maps (previous and current generation); each generation is cleared only
 // inside the marking boxes it was built from, so a step touches O(P R^3) nodes instead of the
 // reference's full-lattice memsets, scans and host round trips (links.cu:147-256,
@@ -354,9 +355,9 @@ __device__ inline void contacts_body(const PartGrid &g, int u, int lane, int32_t
     // no launch of their own.
     geo_clear_body(g.tab[s].owner_curr, g.tab[s].box_curr, d, i, lane, CW);
     if (lane == 0) {
-        // the special-node list of this step is rebuilt from here on (k_special_unmap, the last
-        // reader of the previous list, ran before this kernel): its head is one entry per
-        // link-box node, which k_link_nodes fills or marks empty; refilled nodes follow
+        // the special-node list of this step is rebuilt from here on (the unmap in k_bin_unmap,
+        // the last reader of the previous list, ran before this launch): its head is one entry
+        // per link-box node, which the link-node pass fills or marks empty; refilled nodes follow
         if (i == 0) {
             pd.sp.count[0] = pd.np * pd.lnk_stride;
             pd.sp.count[1] = pd.np * pd.lnk_stride;
@@ -469,7 +470,7 @@ __device__ inline void contacts_body(const PartGrid &g, int u, int lane, int32_t
 // Appends the nodes of the calling wave's lanes that have `want` set to the list's tail (the
 // refilled nodes), in lane order, with one atomic per wave.  Every active lane of the wave must
 // call it.  Returns the lane's list position, or -1 (not appended, or the list is full).  The
-// IBB link nodes do not come through here: a counter that every wave of k_link_nodes
+// IBB link nodes do not come through here: a counter that every wave of the link-node pass
 // incremented cost more than the rest of that kernel at 5000 spheres, so they have fixed
 // positions in the list's head.
 __device__ inline int push_special(const PartDev &pd, bool want, int idx, int kind) {
@@ -693,15 +694,16 @@ __device__ inline LinkOut eval_link_at(const LinkLoads &L, int ip, int ix, int i
 
 // Hydrodynamic force, torque and heat of every particle (compute_particle_forces_kernel,
 // ibb.cu:138-188; compute_particle_heat_kernel, ibb.cu:190-205), in two launches over every
-// particle of every slot.  k_link_nodes gives one thread to each node of a particle's link box:
-// it finds the node's links to the particle, and stores the node's share of the seven sums (its
-// links' terms in direction order, summed from 0) and a bit telling that the node has links.
-// When the particle is the node's lowest linked pid, it also lists the node as a special node
-// and writes the bounce-back values of its links into the node's entry (ibb.cu:5-136); a node
-// that also links to a sphere of higher pid is marked SP_MULTI and completed by k_fixup_links.
+// particle of every slot.  The link-node pass gives one thread to each node of a particle's link
+// box: it finds the node's links to the particle, and stores the node's share of the seven sums
+// (its links' terms in direction order, summed from 0) and a bit telling that the node has
+// links.  When the particle is the node's lowest linked pid, it also lists the node as a special
+// node and writes the bounce-back values of its links into the node's entry (ibb.cu:5-136); a
+// node that also links to a sphere of higher pid is marked SP_MULTI and completed by the
+// shared-link fix-up (k_fixups).
 // k_link_reduce then gives one wave to each particle: lane t adds the shares of the box nodes
 // k = t, t+64, ... in increasing k, and the 64 lane sums are combined pairwise (stride 32 ...
-// 1) and added to the sums left by k_contacts.
+// 1) and added to the sums left by the contact sums.
 constexpr int RL = 64;  // lanes of the per-particle reduction (one wave)
 constexpr uint32_t SP_MULTI = 1u << 31;  // special-node mask flag: links to several spheres
 
@@ -991,7 +993,7 @@ __global__ __launch_bounds__(256) void k_link_reduce(PartGrid g) {
 
 // Entries of the previous step's special-node list leave the node map (the lattice kernel that
 // read them ran before this kernel); slot s uses workgroups lstart[s] .. lstart[s+1]-1.  Empty
-// entries of the list head have a zero mask.  The list itself is rebuilt from k_contacts on.
+// entries of the list head have a zero mask.  The list itself is rebuilt from k_contacts_links on.
 // Workgroup b of 128 threads (thread tid) of that walk.
 __device__ inline void special_unmap_body(const PartGrid &g, int b, int tid) {
     const int slot = slot_of(g.lstart, g.nslots, b);
@@ -1078,7 +1080,7 @@ __global__ __launch_bounds__(128) void k_move(PartGrid g) {
 
 // New-fluid nodes of one particle's marking box (new_fluid_count_kernel, refill.cu:8-41): solid
 // in the previous geometry, fluid now, previous owner valid, and within close_thresh of that
-// owner's new surface.  (IBB link nodes were listed by k_link_nodes.)
+// owner's new surface.  (IBB link nodes were listed by the link-node pass.)
 __global__ __launch_bounds__(LB) void k_refill_find(PartGrid g) {
     const int slot = slot_of(g.pstart, g.nslots, blockIdx.x);
     const PartSlot &sl = g.tab[slot];
@@ -1109,7 +1111,7 @@ __global__ __launch_bounds__(LB) void k_refill_find(PartGrid g) {
 }
 
 // Bounce-back values of the IBB link nodes that link to more than one sphere (SP_MULTI entries in
-// the head of the special list; k_link_nodes wrote every other link node): for every link
+// the head of the special list; the link-node pass wrote every other link node): for every link
 // x -> x + c_ip (x + c_ip solid and linked, with the state before this step's motion) the values
 // bounce_back_particles / bounce_back_thermal_particles write into f[opp][x] and g[opp][x]
 // (ibb.cu:5-136), and the mask of the replaced populations.  The node's other populations are
@@ -1382,7 +1384,7 @@ __global__ __launch_bounds__(256) void k_overwrite_special(PartDev pd, double *f
 // The particle stages one at a time on the FP64 arrays of the staged mode: fcur/gcur of the slot
 // are d_f_collide/d_g_collide, fo/go are d_f/d_g (blocked layout), owner_prev/box_prev the
 // geometry of this step's links (ibnode = ibnode_prev until build_links), owner_curr/box_curr the
-// geometry build_links writes.  Links are derived per link-box node as in k_link_nodes.
+// geometry build_links writes.  Links are derived per link-box node as in the link-node pass.
 
 enum { STL_IBB_F = 0, STL_IBB_G = 1, STL_FORCES = 2, STL_HEAT = 3 };
 
@@ -1567,7 +1569,7 @@ __global__ void k_st_special_reset(PartGrid g) {
 }
 
 // Link records of the newest geometry in the reference layout (links.cu:89-139), unordered.
-// A record is listed when the hot path's own test (link_exists, as in k_link_nodes) accepts the
+// A record is listed when the hot path's own test (link_exists, as in the link-node pass) accepts the
 // link, so the list compared with the reference criterion is the set the coupling uses.
 __global__ __launch_bounds__(PB) void k_collect_links(PartDev pd, const int32_t *owner,
                                                       const int32_t *box, Dims d, int snapshot,
@@ -1594,7 +1596,7 @@ __global__ __launch_bounds__(PB) void k_collect_links(PartDev pd, const int32_t 
             const int nx = ix + dir_cx(ip), ny = iy + dir_cy(ip), nz = iz + dir_cz(ip);
             if (!in_domain(d, nx, ny, nz)) continue;
             if (owner[nz * d.lxy + ny * d.lx + nx] != p) continue;
-            // the decision k_link_nodes takes (link_exists), then q as it computes it
+            // the decision the link-node pass takes (link_exists), then q as it computes it
             if (!link_exists(ip, x0, y0, z0, X, Y, Z, pd.radius_sq)) continue;
             double q = 0.0;
             link_q(ip, x0, y0, z0, X, Y, Z, pd.radius_sq, q);
