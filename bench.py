@@ -207,8 +207,11 @@ def main():
                     help="K eager (not graph-replayed) steps of a 16-request serving batch, "
                          "for kernel traces; no JSON")
     ap.add_argument("--counter-run", action="store_true",
-                    help="minimal run for memory-counter passes: 3 FP64 lattice steps at "
+                    help="minimal run for memory-counter passes: 3 lattice steps at "
                          "320x320x160 and one 1 GiB copy-probe launch, no JSON")
+    ap.add_argument("--counter-precision", type=int, default=0, choices=(0, 1, 2, 3),
+                    help="precision form of the counter run's lattice steps (0 FP64, 1 FP32, "
+                         "2 FP16 storage, 3 BF16 storage)")
     ap.add_argument("--serve", type=int, default=256,
                     help="requests of the served-batch workload (0: skip it)")
     args = ap.parse_args()
@@ -241,7 +244,7 @@ def main():
         batch.close()
         return
     if args.counter_run:
-        case = rb.default_case(**kw64)
+        case = rb.default_case(**config2(precision=args.counter_precision))
         sim = rb.Sim(case)
         sim.init(temperature=rb.linear_temperature(case, 2024, 1e-3))
         sim.step(3, macro_last=False)
