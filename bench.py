@@ -66,6 +66,17 @@ def load_rbtp():
     return mod
 
 
+def measured_traffic(name):
+    """HBM bytes per launch and their ratio to the algorithmic bytes, from the committed PMC
+    summary profiles/lattice_<name>_320x320x160_traffic.json, or (None, None)."""
+    path = os.path.join(ROOT, "profiles", f"lattice_{name}_320x320x160_traffic.json")
+    if not os.path.exists(path):
+        return None, None
+    with open(path) as fh:
+        t = json.load(fh)
+    return t.get("bytes_per_launch"), t.get("traffic_over_algorithmic")
+
+
 def prepare_form(rb, kw):
     """A lattice of one precision form, created and initialised (host work: the device idles)."""
     case = rb.default_case(**kw)
@@ -303,6 +314,9 @@ def main():
                 form.update(kernel_ms_steady=r["kernel_ms_steady"],
                             kernel_gbs_steady=BYTES_PER_NODE[prec] * n /
                             (r["kernel_ms_steady"] * 1e-3) / 1e9)
+            if prec == 2:
+                # HBM bytes per launch from the committed PMC passes of this form
+                form["traffic"], form["traffic_over_algorithmic"] = measured_traffic("fp16s")
             if prec in (2, 3):
                 # the 16-bit kernels pull with plain 2-byte loads: their pattern's ceiling
                 form["pattern_probe_gbs"] = probes["p16_plain_pull"]
@@ -325,11 +339,7 @@ def main():
     if not args.no_particles:
         forms["fp64_particle_laden"] = run_particles(rb, max(args.particle_steps, 1),
                                                      max(args.particle_steps // 10, 1), log)
-    traffic = None
-    tpath = os.path.join(ROOT, "profiles", "lattice_fp64_320x320x160_traffic.json")
-    if os.path.exists(tpath):
-        with open(tpath) as fh:
-            traffic = json.load(fh).get("bytes_per_launch")
+    traffic, _ = measured_traffic("fp64")
     cpu = None if args.no_cpu else cpu_baseline(kw64, log)
     serving = run_serving(rb, args.serve, log) if args.serve > 0 else None
     out = {
